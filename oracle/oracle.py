@@ -1,0 +1,254 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/cy3q_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker. The product (``papers_amd``) never
+imports it.
+
+Every wrapper mirrors one reference function (cited in the C source) and returns numpy
+arrays in the reference layouts: points [n, 10] (re/im interleaved), section values
+[n, 2N], derivatives [n, 5, 2N], h as the 2N x 2N real block.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    """Compiles liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    i64, u64, u32, dbl, vp = C.c_int64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p
+    L.orc_lcg_stream.argtypes = [u64, i64, vp, vp, vp]
+    L.orc_n_sections.argtypes = [u32]
+    L.orc_n_sections.restype = i64
+    L.orc_monomials.argtypes = [u32, C.c_int, vp]
+    L.orc_monomials.restype = i64
+    L.orc_quintic_eval.argtypes = [_dp, i64, dbl, vp, vp]
+    L.orc_sample_sz.argtypes = [i64, u64, C.c_int, dbl, C.c_int, _dp, C.POINTER(i64)]
+    L.orc_sample_sz.restype = i64
+    L.orc_sz_lines.argtypes = [i64, u64, C.c_int, dbl, C.c_int, _dp, _i32p]
+    L.orc_weights_logomega.argtypes = [_dp, i64, dbl, vp, vp, vp]
+    L.orc_sections.argtypes = [_dp, i64, _u32p, i64, _dp, vp]
+    L.orc_fs_gram.argtypes = [_dp, _dp, i64, i64, _dp, _dp]
+    L.orc_cholesky.argtypes = [_dp, _dp, i64, _dp, _dp]
+    L.orc_cholesky.restype = C.c_int
+    L.orc_apply_inv_l.argtypes = [_dp, _dp, i64, i64, _dp, vp]
+    L.orc_lu_invert.argtypes = [_dp, i64, _dp]
+    L.orc_lu_invert.restype = C.c_int
+    L.orc_donaldson_step.argtypes = [_dp, _dp, i64, i64, _dp, _dp, vp, vp, vp]
+    L.orc_donaldson_step.restype = dbl
+    L.orc_eta.argtypes = [_dp, _dp, _dp, _dp, _dp, i64, i64, dbl, _dp]
+    L.orc_sigma_reduce.argtypes = [_dp, _dp, i64, vp, vp]
+    L.orc_sigma_reduce.restype = dbl
+    L.orc_num_threads.restype = C.c_int
+    for name in ("orc_sigma_grad", "orc_adam_split", "orc_laplacian", "orc_yukawa"):
+        if hasattr(L, name):
+            pass
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lcg_stream(seed: int, n: int):
+    states = np.zeros(n, np.uint64)
+    f64s = np.zeros(n, np.float64)
+    normals = np.zeros(n, np.float64)
+    lib().orc_lcg_stream(seed, n, _ptr(states), _ptr(f64s), _ptr(normals))
+    return states, f64s, normals
+
+
+def n_sections(k: int) -> int:
+    return int(lib().orc_n_sections(k))
+
+
+def monomials(k: int, quotient: bool = False) -> np.ndarray:
+    n = lib().orc_monomials(k, int(quotient), None)
+    out = np.zeros((n, 5), np.uint32)
+    lib().orc_monomials(k, int(quotient), _ptr(out))
+    return out
+
+
+def quintic_eval(pts: np.ndarray, psi: float = 0.0):
+    pts = np.ascontiguousarray(pts, np.float64)
+    n = pts.shape[0]
+    f = np.zeros((n, 2))
+    g = np.zeros((n, 10))
+    lib().orc_quintic_eval(pts, n, psi, _ptr(f), _ptr(g))
+    return f, g
+
+
+def sample_sz(n: int, seed: int, psi: float = 0.0, dwork: bool | None = None, dk_iters: int = 50):
+    """dwork=False: sample_quintic_points_sz; dwork=True: sample_deformed_quintic_points_sz.
+    Default: Fermat form at psi == 0, Dwork form otherwise."""
+    if dwork is None:
+        dwork = psi != 0.0
+    out = np.zeros((n, 10))
+    tried = C.c_int64(0)
+    got = lib().orc_sample_sz(n, seed, int(dwork), psi, dk_iters, out, C.byref(tried))
+    return out[:got].copy(), int(tried.value)
+
+
+def sz_lines(n_lines: int, seed: int, psi: float = 0.0, dwork: bool = False, dk_iters: int = 50):
+    zs = np.zeros((n_lines, 5, 10))
+    acc = np.zeros((n_lines, 5), np.int32)
+    lib().orc_sz_lines(n_lines, seed, int(dwork), psi, dk_iters, zs, acc)
+    return zs, acc
+
+
+def weights_logomega(pts: np.ndarray, psi: float = 0.0):
+    pts = np.ascontiguousarray(pts, np.float64)
+    n = pts.shape[0]
+    w = np.zeros(n)
+    lo = np.zeros(n)
+    ce = np.zeros((n, 2), np.int32)
+    lib().orc_weights_logomega(pts, n, psi, _ptr(w), _ptr(lo), _ptr(ce))
+    return w, lo, ce
+
+
+def sections(pts: np.ndarray, mono: np.ndarray, derivs: bool = True):
+    pts = np.ascontiguousarray(pts, np.float64)
+    mono = np.ascontiguousarray(mono, np.uint32)
+    n, nb = pts.shape[0], mono.shape[0]
+    sv = np.zeros((n, 2 * nb))
+    sd = np.zeros((n, 5, 2 * nb)) if derivs else None
+    lib().orc_sections(pts, n, mono, nb, sv, _ptr(sd))
+    return sv, sd
+
+
+def fs_gram(sv: np.ndarray, w: np.ndarray):
+    n, two_n = sv.shape
+    nb = two_n // 2
+    gre = np.zeros((nb, nb))
+    gim = np.zeros((nb, nb))
+    lib().orc_fs_gram(np.ascontiguousarray(sv), np.ascontiguousarray(w), n, nb, gre, gim)
+    return gre, gim
+
+
+def cholesky(gre: np.ndarray, gim: np.ndarray):
+    nb = gre.shape[0]
+    lre = np.zeros((nb, nb))
+    lim = np.zeros((nb, nb))
+    rc = lib().orc_cholesky(np.ascontiguousarray(gre), np.ascontiguousarray(gim), nb, lre, lim)
+    if rc != 0:
+        raise ValueError("FS Gram matrix not positive-definite")
+    return lre, lim
+
+
+def orthonormalise(sv: np.ndarray, sd, w: np.ndarray):
+    """orthonormalise_basis_fs_gram (quintic.rs:3140-3165), in place on copies."""
+    sv = sv.copy()
+    sd = None if sd is None else sd.copy()
+    gre, gim = fs_gram(sv, w)
+    lre, lim = cholesky(gre, gim)
+    n, two_n = sv.shape
+    lib().orc_apply_inv_l(lre, lim, n, two_n // 2, sv, _ptr(sd))
+    return sv, sd, (lre, lim)
+
+
+def lu_invert(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float64)
+    out = np.zeros_like(a)
+    if lib().orc_lu_invert(a, a.shape[0], out) != 0:
+        raise ValueError("singular")
+    return out
+
+
+def identity_block(nb: int) -> np.ndarray:
+    return np.eye(2 * nb)
+
+
+def donaldson_step(sv: np.ndarray, w: np.ndarray, h_block: np.ndarray, want_t: bool = False):
+    n, two_n = sv.shape
+    nb = two_n // 2
+    h_new = np.zeros((two_n, two_n))
+    tre = np.zeros((nb, nb)) if want_t else None
+    tim = np.zeros((nb, nb)) if want_t else None
+    kinv = np.zeros(n) if want_t else None
+    r = lib().orc_donaldson_step(np.ascontiguousarray(sv), np.ascontiguousarray(w), n, nb,
+                                 np.ascontiguousarray(h_block), h_new, _ptr(tre), _ptr(tim),
+                                 _ptr(kinv))
+    if want_t:
+        return h_new, r, (tre, tim, kinv)
+    return h_new, r
+
+
+def donaldson_solve(sv, w, h_block, max_iter: int, tol: float):
+    """donaldson_solve (quintic.rs:433-443): returns (h, residual history)."""
+    res = []
+    h = h_block
+    for _ in range(max_iter):
+        h, r = donaldson_step(sv, w, h)
+        res.append(r)
+        if r < tol:
+            break
+    return h, res
+
+
+def eta(pts, sv, sd, log_om, h_block, psi: float = 0.0):
+    n, two_n = sv.shape
+    out = np.zeros(n)
+    lib().orc_eta(np.ascontiguousarray(pts), np.ascontiguousarray(sv), np.ascontiguousarray(sd),
+                  np.ascontiguousarray(log_om), np.ascontiguousarray(h_block), n, two_n // 2,
+                  psi, out)
+    return out
+
+
+def sigma_reduce(eta_v, w):
+    kap = C.c_double(0)
+    l2 = C.c_double(0)
+    s = lib().orc_sigma_reduce(np.ascontiguousarray(eta_v), np.ascontiguousarray(w), len(w),
+                               C.byref(kap), C.byref(l2))
+    return s, kap.value, l2.value
+
+
+def sigma(pts, sv, sd, log_om, w, h_block, psi: float = 0.0) -> float:
+    return sigma_reduce(eta(pts, sv, sd, log_om, h_block, psi), w)[0]
+
+
+class Problem:
+    """The static inputs of QuinticSolver::new_with_sampler(.., ShiffmanZelditch) or
+    ::new_with_psi_sz (quintic.rs:241-322), computed by the oracle."""
+
+    def __init__(self, k: int, n_pts: int, seed: int, psi: float = 0.0, dwork: bool | None = None,
+                 quotient: bool = False, derivs: bool = True, fs_gram: bool = True):
+        self.k, self.seed, self.psi = k, seed, psi
+        self.mono = monomials(k, quotient)
+        self.nb = self.mono.shape[0]
+        self.pts, self.lines_tried = sample_sz(n_pts, seed, psi, dwork)
+        self.n = self.pts.shape[0]
+        self.w, self.log_om, self.chart_elim = weights_logomega(self.pts, psi)
+        self.sv_raw, self.sd_raw = sections(self.pts, self.mono, derivs)
+        if fs_gram:
+            self.sv, self.sd, self.L = orthonormalise(self.sv_raw, self.sd_raw, self.w)
+        else:
+            self.sv, self.sd, self.L = self.sv_raw, self.sd_raw, None
+        self.h = identity_block(self.nb)
+
+    def sigma(self, h=None) -> float:
+        return sigma(self.pts, self.sv, self.sd, self.log_om, self.w, self.h if h is None else h,
+                     self.psi)
