@@ -1,0 +1,122 @@
+/*
+ * cy3q: MI355X-native quintic Calabi–Yau metric hot path. C ABI.
+ *
+ * The library (papers_amd/libcy3q.so) replaces the reference crate's QuinticSolver path
+ * (gable-digital/papers, substrate_sm_derivation/cy3_solver, cited below as cy3/). That path
+ * covers the SZ sampler, the section basis, FS-Gram orthonormalisation, the Donaldson
+ * T-operator, the sigma-functional and its analytic gradient with Adam. Every compute stage
+ * runs in hand-written HIP kernels for gfx950 on one GPU. The host side only orchestrates.
+ *
+ * Conventions (the same as the reference):
+ *   - points: [n][10] doubles, (re, im) of z_0..z_4, normalised to |z| = 1;
+ *   - section values: [n][2N], interleaved (re, im) per basis element;
+ *   - section derivatives: [n][5][2N];
+ *   - h_block: the 2N x 2N real block of the upper-index Hermitian G, with
+ *     H[2a][2b] = Re G_ab, H[2a][2b+1] = -Im G_ab, H[2a+1][2b] = Im G_ab,
+ *     H[2a+1][2b+1] = Re G_ab (cy3/src/quintic.rs:2148-2161).
+ *
+ * Every function returns CY3Q_OK (0) or a negative status. cy3q_last_error() gives the
+ * message of the calling thread's last failure. Handles are thread-compatible: use one handle
+ * from one thread at a time. Each handle owns one HIP stream on its device.
+ */
+#ifndef CY3Q_H
+#define CY3Q_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CY3Q_OK 0
+#define CY3Q_ERR_ARG (-1)    /* bad argument or unsupported option */
+#define CY3Q_ERR_HIP (-2)    /* HIP runtime failure (allocation, launch) */
+#define CY3Q_ERR_NOT_PD (-3) /* FS Gram not positive definite (quintic.rs:3148-3149) */
+#define CY3Q_ERR_EMPTY (-4)  /* no point sampled: the reference returns None (quintic.rs:203-205) */
+#define CY3Q_ERR_STATE (-5)  /* call out of order */
+
+/* Sampler kinds (cy3/src/quintic.rs:3613-3616; NewtonProjection is not provided). */
+#define CY3Q_SAMPLER_SZ 1       /* sample_quintic_points_sz, quintic.rs:3633-3785 */
+#define CY3Q_SAMPLER_SZ_DWORK 2 /* sample_deformed_quintic_points_sz, quintic.rs:3810-3983 */
+
+/* Arithmetic of the T-operator contractions. */
+#define CY3Q_PREC_FP64 0    /* plain FP64 MFMA */
+#define CY3Q_PREC_SPLIT16 1 /* fp16 hi/lo split MFMA, FP32 accumulate, FP64 across chunks */
+
+typedef struct cy3q_options {
+  int quotient;      /* 0: raw C(k+4,4) basis (quintic.rs:3192-3209); 1: drop e0 >= 5, the
+                        leading-monomial ideal <z0^5> of the quintic (route34/ty_metric.rs:873-893) */
+  int precision;     /* CY3Q_PREC_* for the Donaldson contractions */
+  int device;        /* HIP device ordinal */
+  int dk_iters;      /* Durand–Kerner sweeps per line (reference: 50) */
+  int newton_polish; /* FP64 Newton steps per root after DK (reference: 0) */
+} cy3q_options;
+
+typedef struct cy3q_solver cy3q_solver;
+
+/* Fills the defaults: raw basis, FP64, device 0, 50 DK sweeps, no polish. */
+void cy3q_options_default(cy3q_options *opt);
+const char *cy3q_version(void);
+const char *cy3q_last_error(void);
+
+/* Index tables. n_sections_quintic (quintic.rs:3185-3188) and
+ * build_degree_k_quintic_monomials (quintic.rs:3192-3209). out holds cap rows of 5 exponents. */
+int cy3q_n_sections(uint32_t k, int quotient, int64_t *n);
+int cy3q_monomials(uint32_t k, int quotient, uint32_t *out, int64_t cap, int64_t *n);
+
+/* QuinticSolver::new_with_sampler(k, n, seed, tol, ShiffmanZelditch) when sampler is
+ * CY3Q_SAMPLER_SZ (psi must be 0), QuinticSolver::new_with_psi_sz(k, n, seed, psi) when
+ * CY3Q_SAMPLER_SZ_DWORK (quintic.rs:241-322). Samples, weights, log|Omega|^2 and sections are
+ * computed on the GPU. h starts at the identity. */
+int cy3q_solver_new(uint32_t k, int64_t n_points, uint64_t seed, double psi, int sampler,
+                    const cy3q_options *opt, cy3q_solver **out);
+/* Same workspace built from caller-supplied points ([n][10], host memory). */
+int cy3q_solver_new_from_points(uint32_t k, const double *points, int64_t n, double psi,
+                                const cy3q_options *opt, cy3q_solver **out);
+void cy3q_solver_free(cy3q_solver *s);
+
+int cy3q_solver_info(const cy3q_solver *s, int64_t *n_points, int64_t *n_basis, uint32_t *k,
+                     double *psi);
+int cy3q_solver_set_precision(cy3q_solver *s, int precision);
+
+/* QuinticSolver::orthonormalise_basis_fs_gram (quintic.rs:3140-3165). */
+int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s);
+/* QuinticSolver::donaldson_step (quintic.rs:427-430, 1734-1894): one T-operator iteration.
+ * residual = ||H_new - H_old||_F. */
+int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual);
+/* QuinticSolver::donaldson_solve (quintic.rs:433-443). With tol <= 0 no early stop is taken
+ * and the iterations run back to back with no host synchronisation. residuals: max_iter
+ * entries or NULL. */
+int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
+                                double *residuals);
+/* QuinticSolver::sigma (quintic.rs:423-425, 1301-1499). */
+int cy3q_solver_sigma(cy3q_solver *s, double *sigma);
+/* sigma with its side outputs: kappa, sigma_L2^2 = sum w (eta/kappa-1)^2 / sum w, and the
+ * per-point eta ([n], NaN where skipped) when eta != NULL. */
+int cy3q_solver_sigma_ext(cy3q_solver *s, double *sigma_l1, double *kappa, double *sigma_l2sq,
+                          double *eta);
+
+/* h_block access (4 N^2 doubles). set_h_block takes the Hermitian part. */
+int cy3q_solver_get_h_block(cy3q_solver *s, double *h_block);
+int cy3q_solver_set_h_block(cy3q_solver *s, const double *h_block);
+/* Last trace-normalised T (complex, [N][N][2]) of the previous donaldson_step. */
+int cy3q_solver_get_t(cy3q_solver *s, double *t_cplx);
+
+/* Copies of the workspace's static buffers (host memory). */
+int cy3q_solver_get_points(cy3q_solver *s, double *points);
+int cy3q_solver_get_weights(cy3q_solver *s, double *weights, double *log_omega);
+int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *section_derivs);
+
+/* Per-stage device time (milliseconds, from HIP events on the solver's stream) accumulated
+ * since profiling was switched on: out[i] for stage i, count[i] launches. Stage ids are listed in
+ * cy3q_stage_name(). */
+int cy3q_solver_set_profiling(cy3q_solver *s, int on);
+int cy3q_solver_stage_times(cy3q_solver *s, double *ms, int64_t *count, int cap, int *n_stages);
+const char *cy3q_stage_name(int stage);
+int cy3q_solver_synchronize(cy3q_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CY3Q_H */

@@ -1,0 +1,70 @@
+"""Builds papers_amd/libcy3q.so: the HIP kernels for gfx950 plus the C++ host solver.
+
+Plain hipcc, one object per source compiled in parallel, then one shared-library link.
+The sampler and per-point geometry kernels compile with -ffp-contract=off (as the
+reference's Rust never fuses multiply-add), the contraction kernels with the default.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libcy3q.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(HERE, "..", "include"),
+]
+SOURCES = {
+    "k_sampler.hip": ["-ffp-contract=off"],
+    "k_sections.hip": ["-ffp-contract=off"],
+    "k_gemm_f64.hip": [],
+    "k_dense.hip": [],
+    "k_sigma.hip": ["-ffp-contract=off"],
+    "solver.cpp": ["-x", "hip"],
+}
+OPTIONAL = ["k_split16.hip", "k_adjoint.hip", "k_galerkin.hip"]
+
+
+def _compile(src: str, extra: list[str]) -> tuple[str, str]:
+    obj = os.path.join(BUILD, src + ".o")
+    cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj, r.stderr
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = dict(SOURCES)
+    for s in OPTIONAL:
+        if os.path.exists(os.path.join(CSRC, s)):
+            srcs[s] = []
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        futs = [ex.submit(_compile, s, e) for s, e in srcs.items()]
+        objs = []
+        for f in futs:
+            obj, err = f.result()
+            objs.append(obj)
+            if verbose and err.strip():
+                print(err, file=sys.stderr)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,124 @@
+// Internal declarations shared by the cy3q HIP kernels and the C++ host solver.
+// Written for gfx950 (CDNA4, 64-wide wavefronts, MFMA); no other target is built.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+
+namespace cy3q {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------------------
+// Layout constants
+// ---------------------------------------------------------------------------------------
+// A section row is stored real-ified and interleaved, exactly as the reference's
+// section_values row (re_0, im_0, re_1, im_1, ...), zero-padded from 2N to M columns.
+// M is a multiple of 32 so every MFMA column group of 16 and every 32-wide tile is whole.
+inline int64_t padded_cols(int64_t n_basis) { return (2 * n_basis + 31) / 32 * 32; }
+
+// ---------------------------------------------------------------------------------------
+// Status codes (mirrored in include/cy3q.h)
+// ---------------------------------------------------------------------------------------
+enum {
+  ST_OK = 0,
+  ST_ERR_ARG = -1,
+  ST_ERR_HIP = -2,
+  ST_ERR_NOT_PD = -3,
+  ST_ERR_EMPTY = -4,
+  ST_ERR_STATE = -5,
+};
+
+// ---------------------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files; all asynchronous on `st`)
+// ---------------------------------------------------------------------------------------
+
+// Sampler (k_sampler.hip). Lines [line0, line0 + n_lines) of the SZ stream for `seed`.
+// cand: [n_lines][5][10] doubles, acc: [n_lines][5] int32 accept flags.
+void launch_sz_lines(hipStream_t st, uint64_t seed, int64_t line0, int64_t n_lines, int dwork,
+                     double psi, int dk_iters, int newton_polish, double *cand, int32_t *acc);
+// Counts accepted candidates per block of `block` lines; writes block sums.
+void launch_scan_accept(hipStream_t st, const int32_t *acc, int64_t n_cand, int64_t *pos,
+                        int64_t *block_sums, int64_t *scratch_sums, int64_t *total);
+void launch_compact_points(hipStream_t st, const double *cand, const int32_t *acc,
+                           const int64_t *pos, int64_t n_cand, int64_t out_offset,
+                           int64_t n_max, double *pts);
+
+// Per-point geometry (k_sections.hip).
+void launch_weights_logomega(hipStream_t st, const double *pts, int64_t n, double psi,
+                             double *w, double *log_om, int32_t *chart_elim);
+void launch_sections(hipStream_t st, const double *pts, int64_t n, const uint32_t *mono,
+                     int nb, int kmax, int64_t m_cols, double *sv, double *sd);
+void launch_row_scale_f32(hipStream_t st, const double *sv, int64_t n, int64_t m_cols,
+                          float *x32, float *row_scale2);
+
+// FP64 MFMA contractions (k_gemm_f64.hip).
+// y[r][:] = x[r][:] * B for r < rows (B: [M][M] row-major, block-upper-triangular), with
+// mode 0: store y (ld = m_cols); mode 1: rowsumsq partial per 32-col part into dpart.
+void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m_cols,
+                        const double *bmat, double *y, int mode, int full);
+// Weighted symmetric rank-k: partial lower tiles of X^T diag(c) X over point chunks.
+struct SyrkPlan;
+void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, const double *c,
+                     int64_t n, double *partials);
+void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *partials,
+                        double *xx_full);
+
+// Split-fp16 MFMA contractions (k_split16.hip).
+void launch_dpass_split16(hipStream_t st, const float *x32, int64_t n, int64_t m_cols,
+                          const half8_t *bfrag, double *dsum);
+void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const float *x32,
+                         const float *u, int64_t n, double *partials);
+
+// Small N x N FP64 kernels and per-point coefficients (k_dense.hip).
+void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpart, int n_parts,
+                             int64_t n, double *c);
+void launch_valid_weights(hipStream_t st, const double *w, int64_t n, double *c);
+void launch_sum_f64(hipStream_t st, const double *v, int64_t n, double *out);
+void launch_finish_t(hipStream_t st, const double *xx, int64_t m_cols, int nb,
+                     const double *sum_w, int trace_normalise, double scale, double *t_cplx);
+void launch_cholesky(hipStream_t st, double *a_cplx, int nb, int32_t *status);
+void launch_tri_inverse(hipStream_t st, const double *l_cplx, int nb, double *linv_cplx);
+void launch_build_bmat(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
+                       int conj_v, double *bmat);
+void launch_gram_from_w(hipStream_t st, const double *w_cplx, int nb, double *g_cplx);
+void launch_residual_and_swap(hipStream_t st, double *g_new, double *w_new, double *g_old,
+                              int nb, const int32_t *chol_status, double *residual_out);
+void launch_cplx_to_block(hipStream_t st, const double *g_cplx, int nb, double *h_block);
+void launch_block_to_cplx(hipStream_t st, const double *h_block, int nb, double *g_cplx);
+void launch_identity_cplx(hipStream_t st, int nb, double *g_cplx);
+void launch_hermitian_part(hipStream_t st, double *g_cplx, int nb);
+
+// sigma (k_sigma.hip).
+void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const double *yd,
+                      const double *log_om, int64_t n, int64_t m_cols, double psi,
+                      double *eta);
+void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
+                         double *out5);
+
+// ---------------------------------------------------------------------------------------
+// SYRK work partition: lower 16x16 (FP64) or 32x32 (split16) output sub-tiles of the
+// M x M symmetric result, grouped into "parts" (one workgroup each per point chunk), each
+// part split over the waves of the workgroup. Built on the host (see solver.cpp).
+// ---------------------------------------------------------------------------------------
+struct SyrkPlan {
+  int64_t m_cols = 0;     // M
+  int tile = 16;          // sub-tile edge (16 for FP64 MFMA, 32 for split16)
+  int n_groups = 0;       // M / tile
+  int n_parts = 0;        // workgroups per point chunk
+  int waves = 8;          // waves per workgroup
+  int max_tiles = 0;      // max sub-tiles per wave (accumulator budget)
+  int64_t chunk = 0;      // points per workgroup
+  int64_t n_chunks = 0;
+  // Per (part, wave): list of sub-tile (row group, col group) pairs, padded with -1.
+  // Layout: [n_parts][waves][max_tiles][2] int16.
+  int16_t *d_tiles = nullptr;
+  // Row-group order of the per-wave lists lets a wave reuse the row fragment.
+};
+
+}  // namespace cy3q

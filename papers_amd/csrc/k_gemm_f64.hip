@@ -1,0 +1,249 @@
+// FP64 MFMA contractions of the Donaldson T-operator (the plain FP64 GPU path), also used
+// for FS-Gram orthonormalisation and the sigma pre-contraction.
+//
+// The reference computes K_p = s^T H s with a 2N x 2N quadratic form per point
+// (cy3/src/quintic.rs:1745-1759). It then accumulates T = sum_p (w_p/K_p) conj(s_p) s_p^T
+// sequentially (quintic.rs:1771-1789). Here both are dense contractions on
+// v_mfma_f64_16x16x4f64:
+//   * k_trigemm_f64: Y = X * B, with X the real-ified section rows and B the real-ified
+//     transpose of a lower-triangular complex V (B is block upper triangular, so the K loop
+//     stops at the column block). With V = L^-1 and T = L L^H, the row sum of squares of Y is
+//     D_p = s_p^H T^-1 s_p = K_p. With V = conj(L_gram)^-1, Y is the FS-orthonormalised basis.
+//   * k_syrk_f64: the lower 16x16 tiles of X^T diag(c) X, split over point chunks, with one
+//     FP64 partial per chunk. k_syrk_reduce sums the chunks in a fixed order, so the result
+//     is deterministic run to run.
+#include "cy3q_internal.h"
+
+namespace cy3q {
+
+// ----------------------------------------------------------------------------------------
+// Y = X * B. Workgroup tile: 128 rows x 32 columns, 4 waves. Wave w owns rows
+// [32w, 32w+32) as a 2x2 grid of 16x16 MFMA tiles. LDS: A [128][KT+1], B [KT][32].
+// MFMA f64 16x16x4 operand layout: A[i][k] at lane i + 16k; B[k][j] at lane j + 16k.
+// D[i][j] sits at lane j + 16*(i/4), element i%4.
+// ----------------------------------------------------------------------------------------
+static constexpr int TG_ROWS = 128;
+static constexpr int TG_COLS = 32;
+static constexpr int TG_KT = 16;
+
+__global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ x, int64_t rows,
+                                                     int64_t m_cols, const double *__restrict__ bm,
+                                                     double *__restrict__ y, int mode, int full,
+                                                     int64_t n_rowtiles) {
+  __shared__ double sa[TG_ROWS][TG_KT + 1];
+  __shared__ double sb[TG_KT][TG_COLS];
+  const int n_cb = (int)(m_cols / TG_COLS);
+  // XCD-aware order: consecutive workgroups are dealt round-robin over the 8 XCDs, so the
+  // column blocks of one row tile are grouped on one XCD and share its L2.
+  const int64_t g = blockIdx.x;
+  const int64_t xcd = g & 7, q = g >> 3;
+  const int cb = (int)(q % n_cb);
+  const int64_t rt = (q / n_cb) * 8 + xcd;
+  if (rt >= n_rowtiles) return;
+  const int64_t r0 = rt * TG_ROWS;
+  const int c0 = cb * TG_COLS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t kend = full ? m_cols : (int64_t)(c0 + TG_COLS);
+
+  double4_t acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = 0; k0 < kend; k0 += TG_KT) {
+    // A: 128 x 16 doubles, 8 per thread as 4 double2.
+    for (int e = tid; e < TG_ROWS * TG_KT / 2; e += 256) {
+      int r = e / (TG_KT / 2), kk = (e % (TG_KT / 2)) * 2;
+      int64_t gr = r0 + r;
+      double2 v = {0.0, 0.0};
+      if (gr < rows) v = *reinterpret_cast<const double2 *>(x + gr * m_cols + k0 + kk);
+      sa[r][kk] = v.x;
+      sa[r][kk + 1] = v.y;
+    }
+    for (int e = tid; e < TG_KT * TG_COLS / 2; e += 256) {
+      int kr = e / (TG_COLS / 2), cc = (e % (TG_COLS / 2)) * 2;
+      double2 v = *reinterpret_cast<const double2 *>(bm + (k0 + kr) * m_cols + c0 + cc);
+      sb[kr][cc] = v.x;
+      sb[kr][cc + 1] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < TG_KT / 4; ks++) {
+      const int kk = ks * 4 + (lane >> 4);
+      double a0 = sa[wave * 32 + (lane & 15)][kk];
+      double a1 = sa[wave * 32 + 16 + (lane & 15)][kk];
+      double b0 = sb[kk][lane & 15];
+      double b1 = sb[kk][16 + (lane & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  if (mode == 0) {
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++)
+        for (int e = 0; e < 4; e++) {
+          int64_t gr = r0 + wave * 32 + i * 16 + (lane >> 4) * 4 + e;
+          int gc = c0 + j * 16 + (lane & 15);
+          if (gr < rows) y[gr * m_cols + gc] = acc[i][j][e];
+        }
+  } else {
+    // Row sum of squares over this block's 32 columns -> y[cb * rows + row].
+    for (int i = 0; i < 2; i++)
+      for (int e = 0; e < 4; e++) {
+        double s = acc[i][0][e] * acc[i][0][e] + acc[i][1][e] * acc[i][1][e];
+        for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
+        int64_t gr = r0 + wave * 32 + i * 16 + (lane >> 4) * 4 + e;
+        if ((lane & 15) == 0 && gr < rows) y[(int64_t)cb * rows + gr] = s;
+      }
+  }
+}
+
+void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m_cols,
+                        const double *bmat, double *y, int mode, int full) {
+  if (rows <= 0) return;
+  int64_t n_rt = (rows + TG_ROWS - 1) / TG_ROWS;
+  int64_t n_rt8 = (n_rt + 7) / 8 * 8;
+  int64_t n_cb = m_cols / TG_COLS;
+  int64_t grid = n_rt8 * n_cb;
+  hipLaunchKernelGGL(k_trigemm_f64, dim3((unsigned)grid), dim3(256), 0, st, x, rows, m_cols, bmat,
+                     y, mode, full, n_rt);
+}
+
+// ----------------------------------------------------------------------------------------
+// Weighted SYRK partials. Each workgroup (8 waves) handles one point chunk and one "part" (a
+// set of lower 16x16 sub-tiles of the M x M result). Wave w of part q takes its own list of
+// up to max_tiles sub-tiles, ordered by row group so the A fragment is reused.
+// Per stage, KP points of X are staged in LDS (row-major [KP][M]) with their weights c.
+// Output: partials[chunk][part][wave][t][256], one 16x16 tile per (wave, t) slot.
+// ----------------------------------------------------------------------------------------
+static constexpr int SY_KP = 16;
+static constexpr int SY_WAVES = 8;
+static constexpr int SY_MAXT = 24;
+
+__global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
+                                                  const double *__restrict__ cw, int64_t n,
+                                                  int64_t m_cols, int64_t chunk, int n_parts,
+                                                  int max_tiles, int64_t n_chunks,
+                                                  const int16_t *__restrict__ tiles,
+                                                  double *__restrict__ partials) {
+  extern __shared__ double sx[];  // [SY_KP][m_cols + 1] then c[SY_KP]
+  const int64_t ld = m_cols + 1;
+  double *sc = sx + SY_KP * ld;
+  // XCD-aware: the parts of one chunk share an XCD (and its L2).
+  const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
+  const int part = (int)(qq % n_parts);
+  const int64_t chunk_id = (qq / n_parts) * 8 + xcd;
+  if (chunk_id >= n_chunks) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int16_t *tl = tiles + ((int64_t)(part * SY_WAVES + wave) * max_tiles) * 2;
+  int ntile = 0;
+  for (int t = 0; t < max_tiles; t++)
+    if (tl[2 * t] >= 0) ntile = t + 1;
+
+  double4_t acc[SY_MAXT];
+#pragma unroll
+  for (int t = 0; t < SY_MAXT; t++) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  const int64_t p_begin = chunk_id * chunk;
+  const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
+  for (int64_t p0 = p_begin; p0 < p_end; p0 += SY_KP) {
+    for (int64_t e = tid; e < SY_KP * m_cols / 2; e += 512) {
+      int r = (int)(e / (m_cols / 2));
+      int64_t cc = (e % (m_cols / 2)) * 2;
+      int64_t p = p0 + r;
+      double2 v = {0.0, 0.0};
+      if (p < p_end) v = *reinterpret_cast<const double2 *>(x + p * m_cols + cc);
+      sx[r * ld + cc] = v.x;
+      sx[r * ld + cc + 1] = v.y;
+    }
+    if (tid < SY_KP) {
+      int64_t p = p0 + tid;
+      sc[tid] = (p < p_end) ? cw[p] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < SY_KP / 4; ks++) {
+      const int kp = ks * 4 + (lane >> 4);
+      const double cpt = sc[kp];
+      const double *row = sx + kp * ld;
+      int last_rg = -1;
+      double a = 0.0;
+#pragma unroll
+      for (int t = 0; t < SY_MAXT; t++) {
+        if (t < ntile) {
+          int rg = tl[2 * t], cg = tl[2 * t + 1];
+          if (rg != last_rg) {
+            a = row[rg * 16 + (lane & 15)];
+            last_rg = rg;
+          }
+          double b = cpt * row[cg * 16 + (lane & 15)];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double *out = partials + ((chunk_id * n_parts + part) * SY_WAVES + wave) * (int64_t)max_tiles * 256;
+#pragma unroll
+  for (int t = 0; t < SY_MAXT; t++) {
+    if (t < ntile) {
+      for (int e = 0; e < 4; e++) out[t * 256 + e * 64 + lane] = acc[t][e];
+    }
+  }
+}
+
+void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, const double *c,
+                     int64_t n, double *partials) {
+  int64_t n_ch8 = (plan.n_chunks + 7) / 8 * 8;
+  int64_t grid = n_ch8 * plan.n_parts;
+  size_t lds = sizeof(double) * (SY_KP * (plan.m_cols + 1) + SY_KP);
+  hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)grid), dim3(512), lds, st, x, c, n, plan.m_cols,
+                     plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks, plan.d_tiles,
+                     partials);
+}
+
+// Sum of the chunk partials, in chunk order, scattered into the full M x M matrix (lower tiles
+// and their mirror). One thread per (part, wave, tile slot, element).
+// Element (e, lane) of a 16x16 tile is (row 4*(lane/16)+e, col lane%16), in the MFMA output layout.
+__global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t *__restrict__ tiles,
+                              int n_parts, int waves, int max_tiles, int64_t n_chunks, int tile,
+                              int64_t m_cols, double *__restrict__ xx) {
+  const int64_t per_chunk = (int64_t)n_parts * waves * max_tiles * tile * tile;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per_chunk) return;
+  int64_t slot = idx / (tile * tile);
+  int el = (int)(idx % (tile * tile));
+  int rg = tiles[2 * slot], cg = tiles[2 * slot + 1];
+  if (rg < 0) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < n_chunks; c++) s += partials[c * per_chunk + idx];
+  int r, col;
+  if (tile == 16) {
+    int e = el / 64, ln = el % 64;
+    r = 4 * (ln / 16) + e;
+    col = ln % 16;
+  } else {  // 32x32 MFMA output layout: element e of lane ln -> row 8*(e/4) + 4*(ln/32) + e%4
+    int e = el / 64, ln = el % 64;
+    r = 8 * (e / 4) + 4 * (ln / 32) + (e % 4);
+    col = ln % 32;
+  }
+  // A diagonal sub-tile holds both (r, col) and (col, r); the lower one is kept, so every
+  // element of xx has exactly one writer.
+  if (rg == cg && r < col) return;
+  int64_t gi = (int64_t)rg * tile + r, gj = (int64_t)cg * tile + col;
+  xx[gi * m_cols + gj] = s;
+  xx[gj * m_cols + gi] = s;
+}
+
+void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *partials,
+                        double *xx_full) {
+  int64_t per_chunk = (int64_t)plan.n_parts * plan.waves * plan.max_tiles * plan.tile * plan.tile;
+  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)((per_chunk + 255) / 256)), dim3(256), 0, st,
+                     partials, plan.d_tiles, plan.n_parts, plan.waves, plan.max_tiles,
+                     plan.n_chunks, plan.tile, plan.m_cols, xx_full);
+}
+
+}  // namespace cy3q

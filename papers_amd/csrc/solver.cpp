@@ -1,0 +1,757 @@
+// Host side of cy3q: the QuinticSolver workspace and the C ABI of include/cy3q.h.
+//
+// Mirrors the reference crate's QuinticSolver (cy3/src/quintic.rs:117-165, constructors
+// :174-393, methods :423-514). The host keeps the same workspace: points, weights, log|Omega|^2,
+// section values and derivatives, and h. Every buffer lives on one GPU, and every stage is a
+// HIP kernel on the solver's stream. The host allocates, orders the launches and copies out
+// the few scalars the API returns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cy3q.h"
+#include "cy3q_internal.h"
+
+using namespace cy3q;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(CY3Q_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define ST_TRY(expr)          \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != CY3Q_OK) return rc_; \
+  } while (0)
+
+// Stage ids for the profiling counters.
+enum Stage {
+  STG_SAMPLE = 0,
+  STG_SECTIONS,
+  STG_FSGRAM,
+  STG_DPASS,
+  STG_COEFF,
+  STG_TPASS,
+  STG_TREDUCE,
+  STG_NXN,
+  STG_SIGMA,
+  STG_COUNT
+};
+const char *kStageNames[STG_COUNT] = {"sample",  "sections", "fs_gram",  "d_pass", "coeffs",
+                                      "t_pass",  "t_reduce", "nxn_fp64", "sigma"};
+
+int64_t n_sections(uint32_t k) {
+  int64_t kk = k;
+  return (kk + 1) * (kk + 2) * (kk + 3) * (kk + 4) / 24;
+}
+
+// build_degree_k_quintic_monomials (quintic.rs:3192-3209): e0 up, e1 up, e2 up, e3 up,
+// e4 = k - sum. The quotient basis drops e0 >= 5 in the same order.
+std::vector<uint32_t> monomials(uint32_t k, int quotient) {
+  std::vector<uint32_t> out;
+  int kk = (int)k;
+  for (int e0 = 0; e0 <= kk; e0++)
+    for (int e1 = 0; e1 <= kk - e0; e1++)
+      for (int e2 = 0; e2 <= kk - e0 - e1; e2++)
+        for (int e3 = 0; e3 <= kk - e0 - e1 - e2; e3++) {
+          int e4 = kk - e0 - e1 - e2 - e3;
+          if (e4 < 0) continue;
+          if (quotient && e0 >= 5) continue;
+          out.insert(out.end(), {(uint32_t)e0, (uint32_t)e1, (uint32_t)e2, (uint32_t)e3,
+                                 (uint32_t)e4});
+        }
+  return out;
+}
+
+}  // namespace
+
+struct cy3q_solver {
+  int device = 0;
+  hipStream_t st = nullptr;
+  cy3q_options opt{};
+  uint32_t k = 0;
+  int nb = 0;
+  int64_t n = 0;
+  int64_t m = 0;  // padded real columns
+  double psi = 0.0;
+  int precision = CY3Q_PREC_FP64;
+  std::vector<uint32_t> mono;
+  std::vector<void *> allocs;
+
+  // static workspace
+  double *pts = nullptr, *w = nullptr, *log_om = nullptr, *sv = nullptr, *sd = nullptr;
+  int32_t *chart_elim = nullptr;
+  uint32_t *d_mono = nullptr;
+  double *wvalid = nullptr, *sum_w = nullptr;
+  // h state: G = W^H W, W lower triangular
+  double *g = nullptr, *wmat = nullptr, *t = nullptr, *lfac = nullptr, *wnew = nullptr,
+         *gnew = nullptr;
+  double *bmat = nullptr, *dpart = nullptr, *coef = nullptr, *xx = nullptr, *partials = nullptr;
+  double *resid = nullptr, *sig_out = nullptr;
+  int32_t *status = nullptr;
+  bool bmat_valid = false;
+  SyrkPlan plan64;
+  int64_t partials_cap = 0;
+
+  // profiling
+  bool profiling = false;
+  double stage_ms[STG_COUNT] = {0};
+  int64_t stage_cnt[STG_COUNT] = {0};
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+
+  ~cy3q_solver() {
+    if (device >= 0) hipSetDevice(device);
+    if (st) hipStreamSynchronize(st);
+    for (auto &pe : pending) {
+      hipEventDestroy(pe.second.first);
+      hipEventDestroy(pe.second.second);
+    }
+    for (void *p : allocs) hipFree(p);
+    if (plan64.d_tiles) hipFree(plan64.d_tiles);
+    if (st) hipStreamDestroy(st);
+  }
+
+  template <class T>
+  int alloc(T **p, size_t count) {
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+      return fail(CY3Q_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return CY3Q_OK;
+  }
+  void release(void *p) {
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) {
+      hipFree(p);
+      allocs.erase(it);
+    }
+  }
+
+  // --- profiling helpers: events bracket a stage on the solver's stream ---
+  void stage_begin(int) {
+    if (!profiling) return;
+    hipEvent_t a;
+    hipEventCreate(&a);
+    hipEventRecord(a, st);
+    pending.push_back({-1, {a, nullptr}});
+  }
+  void stage_end(int id) {
+    if (!profiling || pending.empty()) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, st);
+    pending.back().first = id;
+    pending.back().second.second = b;
+  }
+  void collect() {
+    for (auto &pe : pending) {
+      if (pe.first < 0 || !pe.second.second) continue;
+      hipEventSynchronize(pe.second.second);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+      stage_ms[pe.first] += ms;
+      stage_cnt[pe.first] += 1;
+      hipEventDestroy(pe.second.first);
+      hipEventDestroy(pe.second.second);
+    }
+    pending.clear();
+  }
+};
+
+namespace {
+
+// Lower 16x16 sub-tiles of the M x M symmetric product, in row-group-major order, dealt in
+// contiguous runs to (part, wave) slots of at most max_tiles each.
+int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_tiles,
+                    int64_t target_wgs, int64_t min_chunk, int64_t kp) {
+  pl.m_cols = s->m;
+  pl.tile = tile;
+  pl.waves = waves;
+  pl.n_groups = (int)(s->m / tile);
+  std::vector<std::pair<int, int>> tl;
+  for (int r = 0; r < pl.n_groups; r++)
+    for (int c = 0; c <= r; c++) tl.push_back({r, c});
+  const int64_t total = (int64_t)tl.size();
+  pl.n_parts = (int)((total + (int64_t)waves * cap_tiles - 1) / ((int64_t)waves * cap_tiles));
+  const int64_t slots = (int64_t)pl.n_parts * waves;
+  pl.max_tiles = (int)((total + slots - 1) / slots);
+  std::vector<int16_t> h((size_t)slots * pl.max_tiles * 2, (int16_t)-1);
+  int64_t idx = 0;
+  for (int64_t sl = 0; sl < slots; sl++) {
+    int64_t take = std::min<int64_t>(pl.max_tiles, total - idx);
+    for (int64_t t = 0; t < take; t++, idx++) {
+      h[(sl * pl.max_tiles + t) * 2] = (int16_t)tl[idx].first;
+      h[(sl * pl.max_tiles + t) * 2 + 1] = (int16_t)tl[idx].second;
+    }
+  }
+  int64_t want = std::max<int64_t>(1, target_wgs / pl.n_parts);
+  int64_t by_size = std::max<int64_t>(1, (s->n + min_chunk - 1) / min_chunk);
+  pl.n_chunks = std::max<int64_t>(1, std::min(want, by_size));
+  pl.chunk = (s->n + pl.n_chunks - 1) / pl.n_chunks;
+  pl.chunk = (pl.chunk + kp - 1) / kp * kp;
+  pl.n_chunks = (s->n + pl.chunk - 1) / pl.chunk;
+  if (pl.d_tiles) hipFree(pl.d_tiles);
+  HIP_TRY(hipMalloc(&pl.d_tiles, h.size() * sizeof(int16_t)));
+  HIP_TRY(hipMemcpy(pl.d_tiles, h.data(), h.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+  return CY3Q_OK;
+}
+
+int64_t syrk_partials_count(const SyrkPlan &pl) {
+  return pl.n_chunks * pl.n_parts * pl.waves * (int64_t)pl.max_tiles * pl.tile * pl.tile;
+}
+
+// Sections (+ derivatives), weights, log|Omega|^2 and the state buffers for n points in s->pts.
+int finish_workspace(cy3q_solver *s) {
+  const int64_t n = s->n, m = s->m;
+  const int nb = s->nb;
+  ST_TRY(s->alloc(&s->w, n));
+  ST_TRY(s->alloc(&s->log_om, n));
+  ST_TRY(s->alloc(&s->chart_elim, 2 * n));
+  ST_TRY(s->alloc(&s->wvalid, n));
+  ST_TRY(s->alloc(&s->sum_w, 1));
+  ST_TRY(s->alloc(&s->sv, n * m));
+  ST_TRY(s->alloc(&s->sd, n * 5 * m));
+  ST_TRY(s->alloc(&s->d_mono, s->mono.size()));
+  HIP_TRY(hipMemcpyAsync(s->d_mono, s->mono.data(), s->mono.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, s->st));
+  uint32_t kmax = 0;
+  for (uint32_t e : s->mono) kmax = std::max(kmax, e);
+  if (kmax > 12) return fail(CY3Q_ERR_ARG, "degree k > 12 is not supported");
+  s->stage_begin(STG_SECTIONS);
+  launch_weights_logomega(s->st, s->pts, n, s->psi, s->w, s->log_om, s->chart_elim);
+  launch_sections(s->st, s->pts, n, s->d_mono, nb, (int)kmax, m, s->sv, s->sd);
+  launch_valid_weights(s->st, s->w, n, s->wvalid);
+  launch_sum_f64(s->st, s->wvalid, n, s->sum_w);
+  s->stage_end(STG_SECTIONS);
+  HIP_TRY(hipGetLastError());
+  const int64_t nn = (int64_t)nb * nb;
+  ST_TRY(s->alloc(&s->g, 2 * nn));
+  ST_TRY(s->alloc(&s->wmat, 2 * nn));
+  ST_TRY(s->alloc(&s->t, 2 * nn));
+  ST_TRY(s->alloc(&s->lfac, 2 * nn));
+  ST_TRY(s->alloc(&s->wnew, 2 * nn));
+  ST_TRY(s->alloc(&s->gnew, 2 * nn));
+  ST_TRY(s->alloc(&s->bmat, m * m));
+  ST_TRY(s->alloc(&s->xx, m * m));
+  ST_TRY(s->alloc(&s->coef, n));
+  ST_TRY(s->alloc(&s->dpart, (m / 32) * n));
+  ST_TRY(s->alloc(&s->resid, 1));
+  ST_TRY(s->alloc(&s->sig_out, 8));
+  ST_TRY(s->alloc(&s->status, 4));
+  launch_identity_cplx(s->st, nb, s->g);
+  launch_identity_cplx(s->st, nb, s->wmat);
+  launch_identity_cplx(s->st, nb, s->t);
+  s->bmat_valid = false;
+  ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, 24, 1024, 256, 16));
+  s->partials_cap = syrk_partials_count(s->plan64);
+  ST_TRY(s->alloc(&s->partials, s->partials_cap));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int make_solver(uint32_t k, double psi, const cy3q_options *opt, cy3q_solver **out,
+                cy3q_solver **sp) {
+  if (!out) return fail(CY3Q_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (k < 1 || k > 12) return fail(CY3Q_ERR_ARG, "k must be in 1..12");
+  cy3q_solver *s = new cy3q_solver();
+  if (opt)
+    s->opt = *opt;
+  else
+    cy3q_options_default(&s->opt);
+  s->device = s->opt.device;
+  hipError_t e = hipSetDevice(s->device);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    s->st = nullptr;
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  s->k = k;
+  s->psi = psi;
+  s->precision = s->opt.precision;
+  s->mono = monomials(k, s->opt.quotient);
+  s->nb = (int)(s->mono.size() / 5);
+  s->m = padded_cols(s->nb);
+  *sp = s;
+  return CY3Q_OK;
+}
+
+// SZ sampling on the GPU in batches of lines until n points are accepted or the reference's
+// max_tries lines have been drawn (4x the target lines for Fermat, 8x for Dwork).
+int sample_points(cy3q_solver *s, int64_t n_target, uint64_t seed, int dwork) {
+  const int64_t lines_target = (n_target + 4) / 5;
+  const int64_t max_tries = lines_target * (dwork ? 8 : 4);
+  int64_t batch = std::max<int64_t>(1024, lines_target + lines_target / 32 + 64);
+  batch = std::min(batch, max_tries);
+  double *cand = nullptr;
+  int32_t *acc = nullptr;
+  int64_t *pos = nullptr, *bsum = nullptr, *total = nullptr;
+  ST_TRY(s->alloc(&s->pts, n_target * 10));
+  ST_TRY(s->alloc(&cand, batch * 50));
+  ST_TRY(s->alloc(&acc, batch * 5));
+  ST_TRY(s->alloc(&pos, batch * 5));
+  ST_TRY(s->alloc(&bsum, (batch * 5 + 1023) / 1024 + 1));
+  ST_TRY(s->alloc(&total, 1));
+  int64_t have = 0, line0 = 0;
+  s->stage_begin(STG_SAMPLE);
+  while (have < n_target && line0 < max_tries) {
+    int64_t nl = std::min(batch, max_tries - line0);
+    launch_sz_lines(s->st, seed, line0, nl, dwork, s->psi, s->opt.dk_iters, s->opt.newton_polish,
+                    cand, acc);
+    launch_scan_accept(s->st, acc, nl * 5, pos, bsum, nullptr, total);
+    launch_compact_points(s->st, cand, acc, pos, nl * 5, have, n_target, s->pts);
+    HIP_TRY(hipGetLastError());
+    int64_t got = 0;
+    HIP_TRY(hipMemcpyAsync(&got, total, sizeof(int64_t), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    have = std::min(n_target, have + got);
+    line0 += nl;
+  }
+  s->stage_end(STG_SAMPLE);
+  s->release(cand);
+  s->release(acc);
+  s->release(pos);
+  s->release(bsum);
+  s->release(total);
+  s->n = have;
+  return CY3Q_OK;
+}
+
+// T-operator pieces -------------------------------------------------------------------------
+// B(W) for the triangular contraction, rebuilt lazily after W changes.
+void ensure_bmat(cy3q_solver *s) {
+  if (s->bmat_valid) return;
+  launch_build_bmat(s->st, s->wmat, s->nb, s->m, 0, s->bmat);
+  s->bmat_valid = true;
+}
+
+// One Donaldson step in FP64; residual left on the device in s->resid.
+int t_step_fp64(cy3q_solver *s) {
+  const int64_t n = s->n, m = s->m;
+  ensure_bmat(s);
+  s->stage_begin(STG_DPASS);
+  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
+  s->stage_end(STG_DPASS);
+  s->stage_begin(STG_COEFF);
+  launch_point_coeffs_f64(s->st, s->w, s->dpart, (int)(m / 32), n, s->coef);
+  s->stage_end(STG_COEFF);
+  s->stage_begin(STG_TPASS);
+  launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
+  s->stage_end(STG_TPASS);
+  s->stage_begin(STG_TREDUCE);
+  launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
+  s->stage_end(STG_TREDUCE);
+  return CY3Q_OK;
+}
+
+// T from xx, then Cholesky, W = L^-1, G = W^H W, residual, swap.
+int finish_step(cy3q_solver *s, double scale) {
+  const int nb = s->nb;
+  const size_t nn2 = sizeof(double) * 2 * (size_t)nb * nb;
+  s->stage_begin(STG_NXN);
+  launch_finish_t(s->st, s->xx, s->m, nb, s->sum_w, 1, scale, s->t);
+  HIP_TRY(hipMemcpyAsync(s->lfac, s->t, nn2, hipMemcpyDeviceToDevice, s->st));
+  launch_cholesky(s->st, s->lfac, nb, s->status);
+  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
+  launch_gram_from_w(s->st, s->wnew, nb, s->gnew);
+  launch_residual_and_swap(s->st, s->gnew, s->wnew, s->g, nb, s->status, s->resid);
+  std::swap(s->wmat, s->wnew);
+  s->bmat_valid = false;
+  s->stage_end(STG_NXN);
+  HIP_TRY(hipGetLastError());
+  return CY3Q_OK;
+}
+
+int one_step(cy3q_solver *s) {
+  if (s->precision == CY3Q_PREC_FP64) {
+    ST_TRY(t_step_fp64(s));
+    return finish_step(s, 1.0);
+  }
+  return fail(CY3Q_ERR_ARG, "precision not available in this build");
+}
+
+// W from an arbitrary Hermitian PD G: R = chol(G), G^-1 = R^-H R^-1, L = chol(G^-1), W = L^-1.
+int w_from_g(cy3q_solver *s) {
+  const int nb = s->nb;
+  const size_t nn2 = sizeof(double) * 2 * (size_t)nb * nb;
+  HIP_TRY(hipMemcpyAsync(s->lfac, s->g, nn2, hipMemcpyDeviceToDevice, s->st));
+  launch_cholesky(s->st, s->lfac, nb, s->status);
+  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
+  launch_gram_from_w(s->st, s->wnew, nb, s->gnew);  // G^-1
+  launch_cholesky(s->st, s->gnew, nb, s->status + 1);
+  launch_tri_inverse(s->st, s->gnew, nb, s->wmat);
+  s->bmat_valid = false;
+  int32_t st2[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(st2, s->status, sizeof(st2), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  if (st2[0] != 0 || st2[1] != 0) return fail(CY3Q_ERR_ARG, "h_block is not positive definite");
+  return CY3Q_OK;
+}
+
+// Contract W with the section rows and the derivative rows in point chunks, then eta per
+// point and the two-pass reduction.
+int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
+  const int64_t n = s->n, m = s->m;
+  ensure_bmat(s);
+  s->stage_begin(STG_SIGMA);
+  const int64_t chunk = std::min<int64_t>(n, 65536);
+  double *y0 = nullptr, *yd = nullptr, *eta = nullptr;
+  ST_TRY(s->alloc(&y0, chunk * m));
+  ST_TRY(s->alloc(&yd, chunk * 5 * m));
+  ST_TRY(s->alloc(&eta, n));
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    int64_t cn = std::min(chunk, n - p0);
+    launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, y0, 0, 0);
+    launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, yd, 0, 0);
+    launch_sigma_eta(s->st, s->pts + 10 * p0, y0, yd, s->log_om + p0, cn, m, s->psi, eta + p0);
+  }
+  launch_sigma_reduce(s->st, eta, s->w, n, s->sig_out);
+  s->stage_end(STG_SIGMA);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out5, s->sig_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (eta_host)
+    HIP_TRY(hipMemcpyAsync(eta_host, eta, n * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(y0);
+  s->release(yd);
+  s->release(eta);
+  s->collect();
+  return CY3Q_OK;
+}
+
+}  // namespace
+
+// ============================================================================================
+// C ABI
+// ============================================================================================
+extern "C" {
+
+void cy3q_options_default(cy3q_options *opt) {
+  if (!opt) return;
+  opt->quotient = 0;
+  opt->precision = CY3Q_PREC_FP64;
+  opt->device = 0;
+  opt->dk_iters = 50;
+  opt->newton_polish = 0;
+}
+
+const char *cy3q_version(void) { return "cy3q 0.1 (gfx950)"; }
+const char *cy3q_last_error(void) { return g_last_error.c_str(); }
+
+int cy3q_n_sections(uint32_t k, int quotient, int64_t *n) {
+  if (!n) return fail(CY3Q_ERR_ARG, "n is NULL");
+  *n = (int64_t)(monomials(k, quotient).size() / 5);
+  (void)n_sections;
+  return CY3Q_OK;
+}
+
+int cy3q_monomials(uint32_t k, int quotient, uint32_t *out, int64_t cap, int64_t *n) {
+  std::vector<uint32_t> mo = monomials(k, quotient);
+  int64_t cnt = (int64_t)(mo.size() / 5);
+  if (n) *n = cnt;
+  if (out) {
+    if (cap < cnt) return fail(CY3Q_ERR_ARG, "output capacity too small");
+    std::memcpy(out, mo.data(), mo.size() * sizeof(uint32_t));
+  }
+  return CY3Q_OK;
+}
+
+int cy3q_solver_new(uint32_t k, int64_t n_points, uint64_t seed, double psi, int sampler,
+                    const cy3q_options *opt, cy3q_solver **out) {
+  if (n_points <= 0) return fail(CY3Q_ERR_ARG, "n_points must be > 0");
+  if (sampler != CY3Q_SAMPLER_SZ && sampler != CY3Q_SAMPLER_SZ_DWORK)
+    return fail(CY3Q_ERR_ARG, "unsupported sampler (NewtonProjection is not provided)");
+  if (sampler == CY3Q_SAMPLER_SZ && psi != 0.0)
+    return fail(CY3Q_ERR_ARG, "the Fermat SZ sampler takes psi = 0; use CY3Q_SAMPLER_SZ_DWORK");
+  cy3q_solver *s = nullptr;
+  ST_TRY(make_solver(k, psi, opt, out, &s));
+  int rc = sample_points(s, n_points, seed, sampler == CY3Q_SAMPLER_SZ_DWORK ? 1 : 0);
+  if (rc == CY3Q_OK && s->n == 0) rc = fail(CY3Q_ERR_EMPTY, "no point was accepted");
+  if (rc == CY3Q_OK) rc = finish_workspace(s);
+  if (rc != CY3Q_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_new_from_points(uint32_t k, const double *points, int64_t n, double psi,
+                                const cy3q_options *opt, cy3q_solver **out) {
+  if (!points || n <= 0) return fail(CY3Q_ERR_EMPTY, "no points");
+  cy3q_solver *s = nullptr;
+  ST_TRY(make_solver(k, psi, opt, out, &s));
+  s->n = n;
+  int rc = s->alloc(&s->pts, n * 10);
+  if (rc == CY3Q_OK) {
+    hipError_t e = hipMemcpy(s->pts, points, sizeof(double) * 10 * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(CY3Q_ERR_HIP, hipGetErrorString(e));
+  }
+  if (rc == CY3Q_OK) rc = finish_workspace(s);
+  if (rc != CY3Q_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return CY3Q_OK;
+}
+
+void cy3q_solver_free(cy3q_solver *s) { delete s; }
+
+int cy3q_solver_info(const cy3q_solver *s, int64_t *n_points, int64_t *n_basis, uint32_t *k,
+                     double *psi) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_points) *n_points = s->n;
+  if (n_basis) *n_basis = s->nb;
+  if (k) *k = s->k;
+  if (psi) *psi = s->psi;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_precision(cy3q_solver *s, int precision) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (precision != CY3Q_PREC_FP64 && precision != CY3Q_PREC_SPLIT16)
+    return fail(CY3Q_ERR_ARG, "unknown precision");
+  s->precision = precision;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t n = s->n, m = s->m;
+  const int nb = s->nb;
+  s->stage_begin(STG_FSGRAM);
+  // Gram = sum_p w_p conj(s_p) s_p^T / sum w: the weighted SYRK with c = valid w, no trace
+  // normalisation (compute_fs_gram_matrix, quintic.rs:2957-2995).
+  launch_syrk_f64(s->st, s->plan64, s->sv, s->wvalid, n, s->partials);
+  launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
+  launch_finish_t(s->st, s->xx, m, nb, s->sum_w, 0, 1.0, s->lfac);
+  launch_cholesky(s->st, s->lfac, nb, s->status);
+  int32_t stt = 0;
+  HIP_TRY(hipMemcpyAsync(&stt, s->status, sizeof(int32_t), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  if (stt != 0) return fail(CY3Q_ERR_NOT_PD, "FS Gram matrix not positive-definite");
+  // s' = conj(L)^-1 s for the section rows and all derivative rows
+  // (apply_inverse_l_to_sections/derivs, quintic.rs:3095-3133), as triangular contractions.
+  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
+  launch_build_bmat(s->st, s->wnew, nb, m, 1, s->bmat);
+  double *sv2 = nullptr, *sd2 = nullptr;
+  ST_TRY(s->alloc(&sv2, n * m));
+  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, sv2, 0, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(s->sv);
+  s->sv = sv2;
+  ST_TRY(s->alloc(&sd2, n * 5 * m));
+  launch_trigemm_f64(s->st, s->sd, n * 5, m, s->bmat, sd2, 0, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(s->sd);
+  s->sd = sd2;
+  // h <- I in the new basis.
+  launch_identity_cplx(s->st, nb, s->g);
+  launch_identity_cplx(s->st, nb, s->wmat);
+  s->bmat_valid = false;
+  s->stage_end(STG_FSGRAM);
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->collect();
+  return CY3Q_OK;
+}
+
+int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  ST_TRY(one_step(s));
+  double r = 0.0;
+  HIP_TRY(hipMemcpyAsync(&r, s->resid, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->collect();
+  if (residual) *residual = r;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
+                                double *residuals) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  int64_t done = 0;
+  if (tol > 0.0) {
+    for (int64_t it = 0; it < max_iter; it++) {
+      double r = 0.0;
+      ST_TRY(cy3q_solver_donaldson_step(s, &r));
+      if (residuals) residuals[it] = r;
+      done = it + 1;
+      if (r < tol) break;
+    }
+  } else {
+    double *dres = nullptr;
+    ST_TRY(s->alloc(&dres, std::max<int64_t>(max_iter, 1)));
+    for (int64_t it = 0; it < max_iter; it++) {
+      double *keep = s->resid;
+      s->resid = dres + it;
+      int rc = one_step(s);
+      s->resid = keep;
+      if (rc != CY3Q_OK) return rc;
+      done = it + 1;
+    }
+    if (residuals && max_iter > 0)
+      HIP_TRY(hipMemcpyAsync(residuals, dres, sizeof(double) * max_iter, hipMemcpyDeviceToHost,
+                             s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->release(dres);
+    s->collect();
+  }
+  if (iters) *iters = done;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_sigma_ext(cy3q_solver *s, double *sigma_l1, double *kappa, double *sigma_l2sq,
+                          double *eta) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  double o[5];
+  ST_TRY(sigma_impl(s, o, eta));
+  if (sigma_l1) *sigma_l1 = o[0];
+  if (kappa) *kappa = o[1];
+  if (sigma_l2sq) *sigma_l2sq = o[2];
+  return CY3Q_OK;
+}
+
+int cy3q_solver_sigma(cy3q_solver *s, double *sigma) {
+  return cy3q_solver_sigma_ext(s, sigma, nullptr, nullptr, nullptr);
+}
+
+int cy3q_solver_get_h_block(cy3q_solver *s, double *h_block) {
+  if (!s || !h_block) return fail(CY3Q_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t tn = 2 * (int64_t)s->nb;
+  double *d = nullptr;
+  ST_TRY(s->alloc(&d, tn * tn));
+  launch_cplx_to_block(s->st, s->g, s->nb, d);
+  HIP_TRY(hipMemcpyAsync(h_block, d, sizeof(double) * tn * tn, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(d);
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_h_block(cy3q_solver *s, const double *h_block) {
+  if (!s || !h_block) return fail(CY3Q_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t tn = 2 * (int64_t)s->nb;
+  double *d = nullptr;
+  ST_TRY(s->alloc(&d, tn * tn));
+  HIP_TRY(hipMemcpyAsync(d, h_block, sizeof(double) * tn * tn, hipMemcpyHostToDevice, s->st));
+  launch_block_to_cplx(s->st, d, s->nb, s->g);
+  launch_hermitian_part(s->st, s->g, s->nb);
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(d);
+  return w_from_g(s);
+}
+
+int cy3q_solver_get_t(cy3q_solver *s, double *t_cplx) {
+  if (!s || !t_cplx) return fail(CY3Q_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(t_cplx, s->t, sizeof(double) * 2 * s->nb * s->nb, hipMemcpyDeviceToHost,
+                         s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_get_points(cy3q_solver *s, double *points) {
+  if (!s || !points) return fail(CY3Q_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(points, s->pts, sizeof(double) * 10 * s->n, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_get_weights(cy3q_solver *s, double *weights, double *log_omega) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  if (weights)
+    HIP_TRY(hipMemcpyAsync(weights, s->w, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+  if (log_omega)
+    HIP_TRY(hipMemcpyAsync(log_omega, s->log_om, sizeof(double) * s->n, hipMemcpyDeviceToHost,
+                           s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *section_derivs) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t two_n = 2 * (int64_t)s->nb;
+  if (section_values)
+    HIP_TRY(hipMemcpy2DAsync(section_values, two_n * sizeof(double), s->sv, s->m * sizeof(double),
+                             two_n * sizeof(double), s->n, hipMemcpyDeviceToHost, s->st));
+  if (section_derivs)
+    HIP_TRY(hipMemcpy2DAsync(section_derivs, two_n * sizeof(double), s->sd, s->m * sizeof(double),
+                             two_n * sizeof(double), s->n * 5, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_profiling(cy3q_solver *s, int on) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  s->collect();
+  s->profiling = on != 0;
+  for (int i = 0; i < STG_COUNT; i++) {
+    s->stage_ms[i] = 0.0;
+    s->stage_cnt[i] = 0;
+  }
+  return CY3Q_OK;
+}
+
+int cy3q_solver_stage_times(cy3q_solver *s, double *ms, int64_t *count, int cap, int *n_stages) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->collect();
+  if (n_stages) *n_stages = STG_COUNT;
+  for (int i = 0; i < STG_COUNT && i < cap; i++) {
+    if (ms) ms[i] = s->stage_ms[i];
+    if (count) count[i] = s->stage_cnt[i];
+  }
+  return CY3Q_OK;
+}
+
+const char *cy3q_stage_name(int stage) {
+  return (stage >= 0 && stage < STG_COUNT) ? kStageNames[stage] : "";
+}
+
+int cy3q_solver_synchronize(cy3q_solver *s) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->collect();
+  return CY3Q_OK;
+}
+
+}  // extern "C"

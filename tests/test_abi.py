@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every symbol include/cy3q.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(cy3q_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_header_symbols(cy3q):
+    lib = ctypes.CDLL(cy3q.lib_path())
+    names = _declared()
+    assert len(names) > 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_options(cy3q):
+    from papers_amd import quintic as q
+    L = q._load()
+    assert b"gfx950" in L.cy3q_version()
+    o = q._options()
+    assert (o.quotient, o.precision, o.device, o.dk_iters, o.newton_polish) == (0, 0, 0, 50, 0)
+
+
+def test_bad_arguments_fail_loudly(cy3q):
+    # Argument checks run before any device call.
+    from papers_amd import quintic as q
+    import pytest
+    with pytest.raises(q.Cy3qError):
+        q.QuinticSolver.new_with_sampler(3, 0, 1)
+    with pytest.raises(q.Cy3qError):
+        q.QuinticSolver._new(3, 100, 1, 0.5, q.SAMPLER_SZ)

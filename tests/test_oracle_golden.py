@@ -1,0 +1,227 @@
+"""Pins of the CPU oracle against the reference's own published values, tests and closed forms.
+
+The oracle (oracle/cy3q_oracle.c) is the checker for every GPU parity test, so each part of it
+is pinned here by something independent of its own formulas:
+  * published sigma values of the reference's p5_9 run (output/p5_9_ako_apples_to_apples.json);
+  * the reference's unit tests (quintic.rs:5467-5555);
+  * closed forms, invariants and brute force on tiny inputs.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+
+# --------------------------------------------------------------------------------------------
+# LCG (cy3/src/lib.rs:222-245)
+# --------------------------------------------------------------------------------------------
+def test_lcg_integer_recurrence(oracle):
+    states, f64s, normals = oracle.lcg_stream(42, 1000)
+    M = 2 ** 64
+    s = (42 * 6364136223846793005 + 1) % M
+    for i in range(1000):
+        s = (s * 6364136223846793005 + 1442695040888963407) % M
+        assert int(states[i]) == s
+        assert f64s[i] == ((s >> 11) & ((1 << 53) - 1)) / float(1 << 53)
+    # Box-Muller from the same stream: u1, u2 pairs.
+    u = f64s
+    for i in range(0, 20, 2):
+        u1 = max(u[i] + 1e-12, 1e-12)
+        z = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u[i + 1])
+        assert normals[i // 2] == z
+
+
+# --------------------------------------------------------------------------------------------
+# Index tables (quintic.rs:3185-3209, 5472-5484)
+# --------------------------------------------------------------------------------------------
+def test_monomial_counts_and_order(oracle):
+    assert oracle.n_sections(2) == 15 and oracle.n_sections(3) == 35 and oracle.n_sections(4) == 70
+    for k in range(1, 9):
+        m = oracle.monomials(k)
+        assert len(m) == math.comb(k + 4, 4)
+        assert np.all(m.sum(1) == k)
+        # Brute force of the reference's nested loop order: lexicographic in (e0, e1, e2, e3).
+        brute = [e for e in itertools.product(range(k + 1), repeat=4) if sum(e) <= k]
+        brute = [list(e) + [k - sum(e)] for e in sorted(brute)]
+        assert np.array_equal(m, np.array(brute, np.uint32))
+        q = oracle.monomials(k, True)
+        assert len(q) == math.comb(k + 4, 4) - (math.comb(k - 1, 4) if k >= 5 else 0)
+        assert np.array_equal(q, m[m[:, 0] < 5])
+    assert len(oracle.monomials(6, True)) == 205
+    assert len(oracle.monomials(5, True)) == 125
+    assert len(oracle.monomials(8, True)) == 460
+
+
+def test_library_index_tables_bit_exact(oracle, cy3q):
+    for k in range(1, 10):
+        for q in (False, True):
+            assert np.array_equal(cy3q.build_degree_k_quintic_monomials(k, q), oracle.monomials(k, q))
+
+
+# --------------------------------------------------------------------------------------------
+# Quintic, sampler (quintic.rs:3213-3323, 3633-3983, 5487-5494)
+# --------------------------------------------------------------------------------------------
+def test_known_root(oracle):
+    z = np.array([[1.0, 0, 0, 0, 0, 0, 0, 0, -1.0, 0]])
+    f, g = oracle.quintic_eval(z)
+    assert abs(f[0, 0]) < 1e-12 and abs(f[0, 1]) < 1e-12
+    assert np.allclose(g[0], [5, 0, 0, 0, 0, 0, 0, 0, 5, 0])
+
+
+def test_sz_points_on_variety_and_normalised(oracle):
+    pts, tried = oracle.sample_sz(5000, 42)
+    assert pts.shape == (5000, 10) and tried == 1000
+    assert np.allclose(np.sum(pts ** 2, 1), 1.0, atol=1e-14)
+    f, _ = oracle.quintic_eval(pts)
+    assert np.max(np.hypot(f[:, 0], f[:, 1])) < 1e-3
+
+
+def test_sz_roots_match_companion_eigenvalues(oracle):
+    # Each line's 5 candidates are a + t_k b for the 5 roots t_k of f(a + t b). Rebuild a, b from
+    # the LCG stream and compare t_k with numpy's companion-matrix roots (brute force).
+    zs, acc = oracle.sz_lines(30, 123)
+    _, _, normals = oracle.lcg_stream(123, 20 * 30)
+    for l in range(30):
+        nrm = normals[20 * l: 20 * l + 20]
+        a = nrm[0::4] + 1j * nrm[1::4]
+        b = nrm[2::4] + 1j * nrm[3::4]
+        coef = [sum(math.comb(5, j) * a[i] ** (5 - j) * b[i] ** j for i in range(5)) for j in range(6)]
+        roots = np.roots(coef[::-1])
+        for k in range(5):
+            if not acc[l, k]:
+                continue
+            z = zs[l, k, 0::2] + 1j * zs[l, k, 1::2]
+            # z is (a + t b)/|a + t b|: recover t from the projective ratio with the best pair.
+            best = min(abs((z * np.linalg.norm(a + r * b) - (a + r * b))).max() for r in roots)
+            assert best < 1e-9
+
+
+def test_dwork_at_psi0_equals_fermat_points(oracle):
+    # quintic.rs:6048-6073: psi = 0 Dwork == Fermat (same points; only the acceptance bound
+    # differs, 1e-2 vs 1e-3, and no Fermat point sits between the two bounds here).
+    p0, _ = oracle.sample_sz(3000, 5, psi=0.0, dwork=False)
+    p1, _ = oracle.sample_sz(3000, 5, psi=0.0, dwork=True)
+    assert np.array_equal(p0, p1)
+
+
+def test_weight_is_inverse_grad_norm_and_euler(oracle):
+    pts, _ = oracle.sample_sz(2000, 8, psi=0.3, dwork=True)
+    w, lo, ce = oracle.weights_logomega(pts, 0.3)
+    f, g = oracle.quintic_eval(pts, 0.3)
+    gc = g[:, 0::2] + 1j * g[:, 1::2]
+    z = pts[:, 0::2] + 1j * pts[:, 1::2]
+    np.testing.assert_allclose(w, 1.0 / np.sum(np.abs(gc) ** 2, 1), rtol=1e-14)
+    # Euler: z . grad f = 5 f = 0 on the variety.
+    assert np.max(np.abs(np.sum(z * gc, 1))) < 5e-2 * 1.0
+    # log|Omega|^2 = ln|z_chart|^2 - ln|df_elim|^2 with the chart/elim argmax rules.
+    chart = np.argmax(np.abs(z) ** 2, 1)
+    assert np.array_equal(chart, ce[:, 0])
+    np.testing.assert_allclose(lo, np.log(np.abs(z[np.arange(len(z)), ce[:, 0]]) ** 2)
+                               - np.log(np.abs(gc[np.arange(len(z)), ce[:, 1]]) ** 2), rtol=1e-13, atol=1e-14)
+
+
+# --------------------------------------------------------------------------------------------
+# Sections and derivatives (quintic.rs:4021-4071, 1231-1296)
+# --------------------------------------------------------------------------------------------
+def test_sections_closed_form_and_complex_step_derivative(oracle):
+    pts, _ = oracle.sample_sz(50, 4)
+    mono = oracle.monomials(4)
+    sv, sd = oracle.sections(pts, mono)
+    z = pts[:, 0::2] + 1j * pts[:, 1::2]
+    s = np.prod(z[:, None, :] ** mono[None, :, :].astype(int), axis=2)
+    np.testing.assert_allclose(sv[:, 0::2] + 1j * sv[:, 1::2], s, rtol=1e-13, atol=1e-16)
+    # Holomorphic derivative by a complex finite difference along each coordinate.
+    h = 1e-6
+    for i in range(5):
+        zp = z.copy()
+        zp[:, i] += h
+        zm = z.copy()
+        zm[:, i] -= h
+        fd = (np.prod(zp[:, None, :] ** mono.astype(int), 2) - np.prod(zm[:, None, :] ** mono.astype(int), 2)) / (2 * h)
+        np.testing.assert_allclose(sd[:, i, 0::2] + 1j * sd[:, i, 1::2], fd, rtol=1e-7, atol=1e-10)
+
+
+# --------------------------------------------------------------------------------------------
+# FS-Gram, T-step, inversion (quintic.rs:2957-3165, 1734-1894)
+# --------------------------------------------------------------------------------------------
+def test_fs_gram_orthonormalises(oracle):
+    # quintic.rs:5721-5755: after FS-Gram the weighted Gram is the identity within 1e-6.
+    P = oracle.Problem(3, 20000, 42)
+    gre, gim = oracle.fs_gram(P.sv, P.w)
+    assert np.max(np.abs(gre - np.eye(P.nb))) < 1e-10 and np.max(np.abs(gim)) < 1e-10
+
+
+def test_lu_invert_is_inverse(oracle):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((40, 40)) + 40 * np.eye(40)
+    np.testing.assert_allclose(oracle.lu_invert(a) @ a, np.eye(40), atol=1e-13)
+    with pytest.raises(ValueError):
+        oracle.lu_invert(np.zeros((4, 4)))
+
+
+def test_t_step_invariants_and_brute_force(oracle):
+    P = oracle.Problem(2, 3000, 9)
+    h1, r, (tre, tim, kinv) = oracle.donaldson_step(P.sv, P.w, P.h, want_t=True)
+    S = P.sv[:, 0::2] + 1j * P.sv[:, 1::2]
+    # Brute force: K = s^H G s with G = I, T = sum w/K conj(s) s^T / sum w, trace -> N.
+    K = np.sum(np.abs(S) ** 2, 1)
+    T = (S.conj() * (P.w / K)[:, None]).T @ S / P.w.sum()
+    T *= P.nb / np.trace(T).real
+    np.testing.assert_allclose(tre + 1j * tim, T, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(kinv, 1 / K, rtol=1e-13)
+    G = h1[0::2, 0::2] + 1j * h1[1::2, 0::2]
+    np.testing.assert_allclose(G @ T, np.eye(P.nb), atol=1e-10)
+    assert np.allclose(G, G.conj().T, atol=1e-14)
+    assert abs(r - np.linalg.norm(h1 - P.h)) < 1e-13
+
+
+def test_donaldson_residual_decreases(oracle):
+    # quintic.rs:5536-5551 (Donaldson converges).
+    P = oracle.Problem(2, 5000, 17)
+    _, res = oracle.donaldson_solve(P.sv, P.w, P.h, 20, 1e-6)
+    assert res[-1] < res[0]
+
+
+def test_eta_chart_invariance(oracle):
+    # quintic.rs:4894-4909: eta is the same in every (chart, elim) frame at a point. Rescaling a
+    # point by a phase keeps the projective point; eta must not change.
+    P = oracle.Problem(3, 400, 2)
+    eta0 = oracle.eta(P.pts, P.sv, P.sd, P.log_om, P.h)
+    ph = np.exp(1j * 0.7)
+    z = (P.pts[:, 0::2] + 1j * P.pts[:, 1::2]) * ph
+    pts2 = np.empty_like(P.pts)
+    pts2[:, 0::2], pts2[:, 1::2] = z.real, z.imag
+    sv2, sd2 = oracle.sections(pts2, P.mono)
+    sv2, sd2, _ = oracle.orthonormalise(sv2, sd2, P.w)
+    _, lo2, _ = oracle.weights_logomega(pts2)
+    eta1 = oracle.eta(pts2, sv2, sd2, lo2, P.h)
+    np.testing.assert_allclose(eta1, eta0, rtol=1e-9)
+
+
+# --------------------------------------------------------------------------------------------
+# End-to-end pins: the reference's published p5_9 per-seed values
+# --------------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def k3_seed42(oracle):
+    return oracle.Problem(3, 200000, 42)
+
+
+def test_published_sigma_fs_k3(k3_seed42):
+    # output/p5_9_ako_apples_to_apples.json: k=3, 200k, seed 42, sigma_fs_identity.
+    assert abs(k3_seed42.sigma() / 0.19735711129744954 - 1) < 1e-13
+
+
+@pytest.mark.slow
+def test_published_sigma_donaldson_k3(oracle, k3_seed42):
+    # Same record: donaldson_solve(50, 1e-12) runs 28 iterations; sigma_donaldson.
+    P = k3_seed42
+    h, res = oracle.donaldson_solve(P.sv, P.w, P.h, 50, 1e-12)
+    assert len(res) == 28
+    assert abs(P.sigma(h) / 0.19332013761052813 - 1) < 1e-13
+
+
+def test_published_sigma_fs_k4_seed100(oracle):
+    # Same file, k=4 record, seed 100: sigma_fs_identity = 0.1405790096262153.
+    P = oracle.Problem(4, 200000, 100)
+    assert abs(P.sigma() / 0.1405790096262153 - 1) < 1e-12
