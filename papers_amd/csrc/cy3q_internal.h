@@ -70,10 +70,18 @@ void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *part
                         double *xx_full);
 
 // Split-fp16 MFMA contractions (k_split16.hip).
+void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
+                          half8_t *bfrag, double *b_scale);
 void launch_dpass_split16(hipStream_t st, const float *x32, int64_t n, int64_t m_cols,
-                          const half8_t *bfrag, double *dsum);
-void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const float *x32,
-                         const float *u, int64_t n, double *partials);
+                          const half8_t *bfrag, const int16_t *cglist, double *dsum);
+void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
+                           const double *bscale, int64_t n, float *u, unsigned int *umax_bits,
+                           float *zscale, double *tscale);
+void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const float *x32, const float *u,
+                         const float *zscale, int64_t n, float *partials);
+void launch_syrk_reduce_f32(hipStream_t st, const SyrkPlan &plan, const float *partials,
+                            double *xx_full);
+static constexpr int kDpassMaxCg = 5;  // column groups per wave in the split D pass
 
 // Small N x N FP64 kernels and per-point coefficients (k_dense.hip).
 void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpart, int n_parts,
@@ -81,7 +89,8 @@ void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpar
 void launch_valid_weights(hipStream_t st, const double *w, int64_t n, double *c);
 void launch_sum_f64(hipStream_t st, const double *v, int64_t n, double *out);
 void launch_finish_t(hipStream_t st, const double *xx, int64_t m_cols, int nb,
-                     const double *sum_w, int trace_normalise, double scale, double *t_cplx);
+                     const double *sum_w, int trace_normalise, double scale,
+                     const double *dscale, double *t_cplx);
 void launch_cholesky(hipStream_t st, double *a_cplx, int nb, int32_t *status);
 void launch_tri_inverse(hipStream_t st, const double *l_cplx, int nb, double *linv_cplx);
 void launch_build_bmat(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,

@@ -71,13 +71,15 @@ void launch_sum_f64(hipStream_t st, const double *v, int64_t n, double *out) {
 // xx = X^T diag(c) X (M x M, full). T_ab = xx[2a][2b] + xx[2a+1][2b+1]
 //                                       + i (xx[2a][2b+1] - xx[2a+1][2b])   (quintic.rs:1783-1786)
 // then T /= sum_w (if sum_w > 1e-12, else T = 0) and, if trace_normalise, T *= N / tr T when
-// tr T > 1e-10 (quintic.rs:1790-1812). `scale` multiplies xx first (exact power of two from
-// the split path's global scaling; 1 for FP64).
+// tr T > 1e-10 (quintic.rs:1790-1812). xx is first multiplied by scale (and by dscale[0]
+// when given): the exact power of two of the split path's global scaling, 1 for FP64.
 __global__ void __launch_bounds__(1024) k_finish_t(const double *__restrict__ xx, int64_t m,
                                                    int nb, const double *__restrict__ sum_w,
                                                    int trace_normalise, double scale,
+                                                   const double *__restrict__ dscale,
                                                    double *__restrict__ t) {
   __shared__ double sh[1024];
+  if (dscale) scale *= dscale[0];
   const double sw = *sum_w;
   const double inv_w = sw > 1e-12 ? 1.0 / sw : 0.0;
   double tr = 0.0;
@@ -106,9 +108,10 @@ __global__ void __launch_bounds__(1024) k_finish_t(const double *__restrict__ xx
 }
 
 void launch_finish_t(hipStream_t st, const double *xx, int64_t m_cols, int nb,
-                     const double *sum_w, int trace_normalise, double scale, double *t_cplx) {
+                     const double *sum_w, int trace_normalise, double scale,
+                     const double *dscale, double *t_cplx) {
   hipLaunchKernelGGL(k_finish_t, dim3(1), dim3(1024), 0, st, xx, m_cols, nb, sum_w,
-                     trace_normalise, scale, t_cplx);
+                     trace_normalise, scale, dscale, t_cplx);
 }
 
 // ---- Cholesky -------------------------------------------------------------------------------

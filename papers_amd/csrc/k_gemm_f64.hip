@@ -20,7 +20,8 @@ namespace cy3q {
 // Y = X * B. Workgroup tile: 128 rows x 32 columns, 4 waves. Wave w owns rows
 // [32w, 32w+32) as a 2x2 grid of 16x16 MFMA tiles. LDS: A [128][KT+1], B [KT][32].
 // MFMA f64 16x16x4 operand layout: A[i][k] at lane i + 16k; B[k][j] at lane j + 16k.
-// D[i][j] sits at lane j + 16*(i/4), element i%4.
+// D[i][j] sits at lane j + 16*(i%4), element i/4 (so element e of lane l is row 4e + l/16;
+// this differs from the f32-output MFMAs and was checked with a one-hot probe).
 // ----------------------------------------------------------------------------------------
 static constexpr int TG_ROWS = 128;
 static constexpr int TG_COLS = 32;
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++)
         for (int e = 0; e < 4; e++) {
-          int64_t gr = r0 + wave * 32 + i * 16 + (lane >> 4) * 4 + e;
+          int64_t gr = r0 + wave * 32 + i * 16 + 4 * e + (lane >> 4);
           int gc = c0 + j * 16 + (lane & 15);
           if (gr < rows) y[gr * m_cols + gc] = acc[i][j][e];
         }
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
       for (int e = 0; e < 4; e++) {
         double s = acc[i][0][e] * acc[i][0][e] + acc[i][1][e] * acc[i][1][e];
         for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
-        int64_t gr = r0 + wave * 32 + i * 16 + (lane >> 4) * 4 + e;
+        int64_t gr = r0 + wave * 32 + i * 16 + 4 * e + (lane >> 4);
         if ((lane & 15) == 0 && gr < rows) y[(int64_t)cb * rows + gr] = s;
       }
   }
@@ -207,7 +208,7 @@ void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, cons
 
 // Sum of the chunk partials, in chunk order, scattered into the full M x M matrix (lower tiles
 // and their mirror). One thread per (part, wave, tile slot, element).
-// Element (e, lane) of a 16x16 tile is (row 4*(lane/16)+e, col lane%16), in the MFMA output layout.
+// Element (e, lane) of a 16x16 f64 tile is (row 4e + lane/16, col lane%16), the MFMA output layout.
 __global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t *__restrict__ tiles,
                               int n_parts, int waves, int max_tiles, int64_t n_chunks, int tile,
                               int64_t m_cols, double *__restrict__ xx) {
@@ -223,7 +224,7 @@ __global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t
   int r, col;
   if (tile == 16) {
     int e = el / 64, ln = el % 64;
-    r = 4 * (ln / 16) + e;
+    r = 4 * e + ln / 16;
     col = ln % 16;
   } else {  // 32x32 MFMA output layout: element e of lane ln -> row 8*(e/4) + 4*(ln/32) + e%4
     int e = el / 64, ln = el % 64;

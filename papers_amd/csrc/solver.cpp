@@ -109,6 +109,14 @@ struct cy3q_solver {
   bool bmat_valid = false;
   SyrkPlan plan64;
   int64_t partials_cap = 0;
+  // split16 path (allocated on first use)
+  bool x32_valid = false, bfrag_valid = false;
+  float *x32 = nullptr, *rs2 = nullptr, *u = nullptr, *zscale = nullptr, *partials32 = nullptr;
+  double *dsum = nullptr, *bscale = nullptr, *tscale = nullptr;
+  half8_t *bfrag = nullptr;
+  unsigned int *umax_bits = nullptr;
+  int16_t *cglist = nullptr;
+  SyrkPlan plan16;
 
   // profiling
   bool profiling = false;
@@ -126,6 +134,7 @@ struct cy3q_solver {
     }
     for (void *p : allocs) hipFree(p);
     if (plan64.d_tiles) hipFree(plan64.d_tiles);
+    if (plan16.d_tiles) hipFree(plan16.d_tiles);
     if (st) hipStreamDestroy(st);
   }
 
@@ -371,11 +380,11 @@ int t_step_fp64(cy3q_solver *s) {
 }
 
 // T from xx, then Cholesky, W = L^-1, G = W^H W, residual, swap.
-int finish_step(cy3q_solver *s, double scale) {
+int finish_step(cy3q_solver *s, double scale, const double *dscale) {
   const int nb = s->nb;
   const size_t nn2 = sizeof(double) * 2 * (size_t)nb * nb;
   s->stage_begin(STG_NXN);
-  launch_finish_t(s->st, s->xx, s->m, nb, s->sum_w, 1, scale, s->t);
+  launch_finish_t(s->st, s->xx, s->m, nb, s->sum_w, 1, scale, dscale, s->t);
   HIP_TRY(hipMemcpyAsync(s->lfac, s->t, nn2, hipMemcpyDeviceToDevice, s->st));
   launch_cholesky(s->st, s->lfac, nb, s->status);
   launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
@@ -388,12 +397,84 @@ int finish_step(cy3q_solver *s, double scale) {
   return CY3Q_OK;
 }
 
+// Split path set-up: the row-scaled FP32 copy of the (orthonormalised) sections, the D-pass
+// column-group lists and the T-pass plan.
+int ensure_split16(cy3q_solver *s) {
+  const int64_t n = s->n, m = s->m;
+  const int n_cg = (int)(m / 16);
+  if (n_cg > 8 * kDpassMaxCg)
+    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= " +
+                                  std::to_string(8 * kDpassMaxCg * 16) + " (use FP64)");
+  if (!s->x32) {
+    ST_TRY(s->alloc(&s->x32, n * m));
+    ST_TRY(s->alloc(&s->rs2, n));
+    ST_TRY(s->alloc(&s->u, n));
+    ST_TRY(s->alloc(&s->dsum, n));
+    ST_TRY(s->alloc(&s->bscale, 1));
+    ST_TRY(s->alloc(&s->tscale, 1));
+    ST_TRY(s->alloc(&s->zscale, 1));
+    ST_TRY(s->alloc(&s->umax_bits, 1));
+    HIP_TRY(hipMemsetAsync(s->umax_bits, 0, sizeof(unsigned int), s->st));
+    ST_TRY(s->alloc(&s->bfrag, (size_t)(m / 32) * n_cg * 2 * 64));
+    // Column groups to waves: longest-processing-time first on the triangular k-step counts.
+    std::vector<int16_t> cl(8 * kDpassMaxCg, (int16_t)-1);
+    std::vector<int> load(8, 0), cnt(8, 0);
+    for (int cg = n_cg - 1; cg >= 0; cg--) {
+      int best = -1;
+      for (int w = 0; w < 8; w++)
+        if (cnt[w] < kDpassMaxCg && (best < 0 || load[w] < load[best])) best = w;
+      cl[best * kDpassMaxCg + cnt[best]] = (int16_t)cg;
+      cnt[best]++;
+      load[best] += cg / 2 + 1;
+    }
+    ST_TRY(s->alloc(&s->cglist, cl.size()));
+    HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
+                           hipMemcpyHostToDevice, s->st));
+    int64_t target = 512;
+    ST_TRY(build_syrk_plan(s, s->plan16, 32, 8, 12, target, 1024, 16));
+    ST_TRY(s->alloc(&s->partials32, syrk_partials_count(s->plan16)));
+    HIP_TRY(hipStreamSynchronize(s->st));
+  }
+  if (!s->x32_valid) {
+    launch_row_scale_f32(s->st, s->sv, n, m, s->x32, s->rs2);
+    s->x32_valid = true;
+  }
+  return CY3Q_OK;
+}
+
+int t_step_split16(cy3q_solver *s) {
+  const int64_t n = s->n, m = s->m;
+  ST_TRY(ensure_split16(s));
+  if (!s->bfrag_valid) {
+    launch_build_bfrag16(s->st, s->wmat, s->nb, m, s->bfrag, s->bscale);
+    s->bfrag_valid = true;
+  }
+  s->stage_begin(STG_DPASS);
+  launch_dpass_split16(s->st, s->x32, n, m, s->bfrag, s->cglist, s->dsum);
+  s->stage_end(STG_DPASS);
+  s->stage_begin(STG_COEFF);
+  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->umax_bits,
+                        s->zscale, s->tscale);
+  s->stage_end(STG_COEFF);
+  s->stage_begin(STG_TPASS);
+  launch_syrk_split16(s->st, s->plan16, s->x32, s->u, s->zscale, n, s->partials32);
+  s->stage_end(STG_TPASS);
+  s->stage_begin(STG_TREDUCE);
+  launch_syrk_reduce_f32(s->st, s->plan16, s->partials32, s->xx);
+  s->stage_end(STG_TREDUCE);
+  return CY3Q_OK;
+}
+
 int one_step(cy3q_solver *s) {
   if (s->precision == CY3Q_PREC_FP64) {
     ST_TRY(t_step_fp64(s));
-    return finish_step(s, 1.0);
+    ST_TRY(finish_step(s, 1.0, nullptr));
+  } else {
+    ST_TRY(t_step_split16(s));
+    ST_TRY(finish_step(s, 1.0, s->tscale));
   }
-  return fail(CY3Q_ERR_ARG, "precision not available in this build");
+  s->bfrag_valid = false;
+  return CY3Q_OK;
 }
 
 // W from an arbitrary Hermitian PD G: R = chol(G), G^-1 = R^-H R^-1, L = chol(G^-1), W = L^-1.
@@ -407,6 +488,7 @@ int w_from_g(cy3q_solver *s) {
   launch_cholesky(s->st, s->gnew, nb, s->status + 1);
   launch_tri_inverse(s->st, s->gnew, nb, s->wmat);
   s->bmat_valid = false;
+  s->bfrag_valid = false;
   int32_t st2[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(st2, s->status, sizeof(st2), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
@@ -552,7 +634,7 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s) {
   // normalisation (compute_fs_gram_matrix, quintic.rs:2957-2995).
   launch_syrk_f64(s->st, s->plan64, s->sv, s->wvalid, n, s->partials);
   launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
-  launch_finish_t(s->st, s->xx, m, nb, s->sum_w, 0, 1.0, s->lfac);
+  launch_finish_t(s->st, s->xx, m, nb, s->sum_w, 0, 1.0, nullptr, s->lfac);
   launch_cholesky(s->st, s->lfac, nb, s->status);
   int32_t stt = 0;
   HIP_TRY(hipMemcpyAsync(&stt, s->status, sizeof(int32_t), hipMemcpyDeviceToHost, s->st));
@@ -579,6 +661,8 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s) {
   launch_identity_cplx(s->st, nb, s->g);
   launch_identity_cplx(s->st, nb, s->wmat);
   s->bmat_valid = false;
+  s->bfrag_valid = false;
+  s->x32_valid = false;
   s->stage_end(STG_FSGRAM);
   HIP_TRY(hipStreamSynchronize(s->st));
   s->collect();
