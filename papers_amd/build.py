@@ -28,6 +28,7 @@ SOURCES = {
     "k_sections.hip": ["-ffp-contract=off"],
     "k_gemm_f64.hip": [],
     "k_dense.hip": [],
+    "k_hsolve.hip": [],
     "k_sigma.hip": ["-ffp-contract=off"],
     "solver.cpp": ["-x", "hip"],
 }

@@ -54,8 +54,6 @@ void launch_weights_logomega(hipStream_t st, const double *pts, int64_t n, doubl
                              double *w, double *log_om, int32_t *chart_elim);
 void launch_sections(hipStream_t st, const double *pts, int64_t n, const uint32_t *mono,
                      int nb, int kmax, int64_t m_cols, double *sv, double *sd);
-void launch_row_scale_f32(hipStream_t st, const double *sv, int64_t n, int64_t m_cols,
-                          float *x32, float *row_scale2);
 
 // FP64 MFMA contractions (k_gemm_f64.hip).
 // y[r][:] = x[r][:] * B for r < rows (B: [M][M] row-major, block-upper-triangular), with
@@ -64,44 +62,52 @@ void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m
                         const double *bmat, double *y, int mode, int full);
 // Weighted symmetric rank-k: partial lower tiles of X^T diag(c) X over point chunks.
 struct SyrkPlan;
+int syrk_f64_max_tiles(int64_t m_cols);  // accumulator tiles per wave of k_syrk_f64
 void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, const double *c,
                      int64_t n, double *partials);
 void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *partials,
                         double *xx_full);
 
 // Split-fp16 MFMA contractions (k_split16.hip).
+void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
+                         int *ex, float *rs2, _Float16 *xd, _Float16 *xt);
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale);
-void launch_dpass_split16(hipStream_t st, const float *x32, int64_t n, int64_t m_cols,
+int dpass_split16_waves();
+int dpass_split16_max_cg();
+void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, double *dsum);
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *u, unsigned int *umax_bits,
+                           const double *bscale, int64_t n, float *u, float *block_max,
                            float *zscale, double *tscale);
-void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const float *x32, const float *u,
+int syrk_split16_tiles_per_wave();
+int syrk_split16_waves();
+void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const half8_t *xt, const float *u,
                          const float *zscale, int64_t n, float *partials);
 void launch_syrk_reduce_f32(hipStream_t st, const SyrkPlan &plan, const float *partials,
                             double *xx_full);
-static constexpr int kDpassMaxCg = 5;  // column groups per wave in the split D pass
 
-// Small N x N FP64 kernels and per-point coefficients (k_dense.hip).
+// Per-point coefficients, reductions, B operand, layout conversions (k_dense.hip).
 void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpart, int n_parts,
                              int64_t n, double *c);
 void launch_valid_weights(hipStream_t st, const double *w, int64_t n, double *c);
 void launch_sum_f64(hipStream_t st, const double *v, int64_t n, double *out);
-void launch_finish_t(hipStream_t st, const double *xx, int64_t m_cols, int nb,
-                     const double *sum_w, int trace_normalise, double scale,
-                     const double *dscale, double *t_cplx);
-void launch_cholesky(hipStream_t st, double *a_cplx, int nb, int32_t *status);
-void launch_tri_inverse(hipStream_t st, const double *l_cplx, int nb, double *linv_cplx);
 void launch_build_bmat(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                        int conj_v, double *bmat);
-void launch_gram_from_w(hipStream_t st, const double *w_cplx, int nb, double *g_cplx);
-void launch_residual_and_swap(hipStream_t st, double *g_new, double *w_new, double *g_old,
-                              int nb, const int32_t *chol_status, double *residual_out);
 void launch_cplx_to_block(hipStream_t st, const double *g_cplx, int nb, double *h_block);
 void launch_block_to_cplx(hipStream_t st, const double *h_block, int nb, double *g_cplx);
 void launch_identity_cplx(hipStream_t st, int nb, double *g_cplx);
 void launch_hermitian_part(hipStream_t st, double *g_cplx, int nb);
+
+// N x N solve (k_hsolve.hip): T from xx; inverse Cholesky factor W = L^-1; G = W^H W.
+void launch_t_from_xx(hipStream_t st, const double *xx, int64_t m_cols, int nb,
+                      const double *sum_w, double hscale, const double *dscale,
+                      int trace_normalise, double *scale_buf, double *t, double *a, double *r);
+void launch_elim_init(hipStream_t st, const double *src, int nb, double *a, double *r);
+void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv, double *w, int nb,
+                             int32_t *status);
+void launch_gram_residual(hipStream_t st, double *w, int nb, double *g, const int32_t *status,
+                          double *partial, double *residual);
 
 // sigma (k_sigma.hip).
 void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const double *yd,

@@ -18,7 +18,8 @@ namespace cy3q {
 
 // ----------------------------------------------------------------------------------------
 // Y = X * B. Workgroup tile: 128 rows x 32 columns, 4 waves. Wave w owns rows
-// [32w, 32w+32) as a 2x2 grid of 16x16 MFMA tiles. LDS: A [128][KT+1], B [KT][32].
+// [32w, 32w+32) as a 2x2 grid of 16x16 MFMA tiles. LDS: A [128][KT+1], B [KT][32], double
+// buffered; the next k-block is fetched into registers while the current one runs.
 // MFMA f64 16x16x4 operand layout: A[i][k] at lane i + 16k; B[k][j] at lane j + 16k.
 // D[i][j] sits at lane j + 16*(i%4), element i/4 (so element e of lane l is row 4e + l/16;
 // this differs from the f32-output MFMAs and was checked with a one-hot probe).
@@ -31,8 +32,8 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
                                                      int64_t m_cols, const double *__restrict__ bm,
                                                      double *__restrict__ y, int mode, int full,
                                                      int64_t n_rowtiles) {
-  __shared__ double sa[TG_ROWS][TG_KT + 1];
-  __shared__ double sb[TG_KT][TG_COLS];
+  __shared__ double sa[2][TG_ROWS][TG_KT + 1];
+  __shared__ double sb[2][TG_KT][TG_COLS];
   const int n_cb = (int)(m_cols / TG_COLS);
   // XCD-aware order: consecutive workgroups are dealt round-robin over the 8 XCDs, so the
   // column blocks of one row tile are grouped on one XCD and share its L2.
@@ -45,40 +46,63 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
   const int c0 = cb * TG_COLS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t kend = full ? m_cols : (int64_t)(c0 + TG_COLS);
+  const int nk = (int)(kend / TG_KT);
 
   double4_t acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
-  for (int64_t k0 = 0; k0 < kend; k0 += TG_KT) {
-    // A: 128 x 16 doubles, 8 per thread as 4 double2.
-    for (int e = tid; e < TG_ROWS * TG_KT / 2; e += 256) {
-      int r = e / (TG_KT / 2), kk = (e % (TG_KT / 2)) * 2;
-      int64_t gr = r0 + r;
+  // Register staging: A 128 x 16 (4 double2 per thread), B 16 x 32 (1 double2 per thread).
+  double2 ra[4], rb;
+  auto fetch = [&](int kk) {
+    const int64_t k0 = (int64_t)kk * TG_KT;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u;
+      const int r = e >> 3, c2 = (e & 7) * 2;
+      const int64_t gr = r0 + r;
       double2 v = {0.0, 0.0};
-      if (gr < rows) v = *reinterpret_cast<const double2 *>(x + gr * m_cols + k0 + kk);
-      sa[r][kk] = v.x;
-      sa[r][kk + 1] = v.y;
+      if (gr < rows) v = *reinterpret_cast<const double2 *>(x + gr * m_cols + k0 + c2);
+      ra[u] = v;
     }
-    for (int e = tid; e < TG_KT * TG_COLS / 2; e += 256) {
-      int kr = e / (TG_COLS / 2), cc = (e % (TG_COLS / 2)) * 2;
-      double2 v = *reinterpret_cast<const double2 *>(bm + (k0 + kr) * m_cols + c0 + cc);
-      sb[kr][cc] = v.x;
-      sb[kr][cc + 1] = v.y;
+    {
+      const int kr = tid >> 4, c2 = (tid & 15) * 2;
+      rb = *reinterpret_cast<const double2 *>(bm + (k0 + kr) * m_cols + c0 + c2);
     }
-    __syncthreads();
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u;
+      const int r = e >> 3, c2 = (e & 7) * 2;
+      sa[buf][r][c2] = ra[u].x;
+      sa[buf][r][c2 + 1] = ra[u].y;
+    }
+    const int kr = tid >> 4, c2 = (tid & 15) * 2;
+    sb[buf][kr][c2] = rb.x;
+    sb[buf][kr][c2 + 1] = rb.y;
+  };
+  if (nk > 0) {
+    fetch(0);
+    put(0);
+  }
+  __syncthreads();
+  for (int kk = 0; kk < nk; kk++) {
+    const int buf = kk & 1;
+    if (kk + 1 < nk) fetch(kk + 1);
 #pragma unroll
     for (int ks = 0; ks < TG_KT / 4; ks++) {
-      const int kk = ks * 4 + (lane >> 4);
-      double a0 = sa[wave * 32 + (lane & 15)][kk];
-      double a1 = sa[wave * 32 + 16 + (lane & 15)][kk];
-      double b0 = sb[kk][lane & 15];
-      double b1 = sb[kk][16 + (lane & 15)];
+      const int k = ks * 4 + (lane >> 4);
+      double a0 = sa[buf][wave * 32 + (lane & 15)][k];
+      double a1 = sa[buf][wave * 32 + 16 + (lane & 15)][k];
+      double b0 = sb[buf][k][lane & 15];
+      double b1 = sb[buf][k][16 + (lane & 15)];
       acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
+    if (kk + 1 < nk) put(buf ^ 1);
     __syncthreads();
   }
 
@@ -94,10 +118,10 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
     // Row sum of squares over this block's 32 columns -> y[cb * rows + row].
     for (int i = 0; i < 2; i++)
       for (int e = 0; e < 4; e++) {
-        double s = acc[i][0][e] * acc[i][0][e] + acc[i][1][e] * acc[i][1][e];
-        for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
+        double sq = acc[i][0][e] * acc[i][0][e] + acc[i][1][e] * acc[i][1][e];
+        for (int off = 1; off < 16; off <<= 1) sq += __shfl_xor(sq, off);
         int64_t gr = r0 + wave * 32 + i * 16 + 4 * e + (lane >> 4);
-        if ((lane & 15) == 0 && gr < rows) y[(int64_t)cb * rows + gr] = s;
+        if ((lane & 15) == 0 && gr < rows) y[(int64_t)cb * rows + gr] = sq;
       }
   }
 }
@@ -120,29 +144,37 @@ void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m
 // Per stage, KP points of X are staged in LDS (row-major [KP][M]) with their weights c.
 // Output: partials[chunk][part][wave][t][256], one 16x16 tile per (wave, t) slot.
 // ----------------------------------------------------------------------------------------
-static constexpr int SY_KP = 16;
 static constexpr int SY_WAVES = 8;
-static constexpr int SY_MAXT = 24;
+// Register budget: MAXT 16x16 accumulators (4 doubles each) plus STG staged double2 per thread;
+// STG * 1024 >= KP * m_cols must hold: <22, 7, 16> for m_cols <= 448, <12, 8, 8> up to 1024.
 
+template <int SY_MAXT, int SY_STG, int SY_KP>
 __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
                                                   const double *__restrict__ cw, int64_t n,
                                                   int64_t m_cols, int64_t chunk, int n_parts,
                                                   int max_tiles, int64_t n_chunks,
                                                   const int16_t *__restrict__ tiles,
                                                   double *__restrict__ partials) {
-  extern __shared__ double sx[];  // [SY_KP][m_cols + 1] then c[SY_KP]
+  extern __shared__ double sx[];  // 2 buffers of [SY_KP][m_cols + 1] then c[2][SY_KP]
   const int64_t ld = m_cols + 1;
-  double *sc = sx + SY_KP * ld;
+  const int64_t bufsz = SY_KP * ld;
+  double *sc = sx + 2 * bufsz;
   // XCD-aware: the parts of one chunk share an XCD (and its L2).
   const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
   const int part = (int)(qq % n_parts);
   const int64_t chunk_id = (qq / n_parts) * 8 + xcd;
   if (chunk_id >= n_chunks) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int16_t *tl = tiles + ((int64_t)(part * SY_WAVES + wave) * max_tiles) * 2;
+  int trg[SY_MAXT], tcg[SY_MAXT];
   int ntile = 0;
-  for (int t = 0; t < max_tiles; t++)
-    if (tl[2 * t] >= 0) ntile = t + 1;
+#pragma unroll
+  for (int t = 0; t < SY_MAXT; t++) {
+    trg[t] = t < max_tiles ? tl[2 * t] : -1;
+    tcg[t] = t < max_tiles ? tl[2 * t + 1] : -1;
+    if (trg[t] >= 0) ntile = t + 1;
+  }
 
   double4_t acc[SY_MAXT];
 #pragma unroll
@@ -150,41 +182,69 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
 
   const int64_t p_begin = chunk_id * chunk;
   const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
-  for (int64_t p0 = p_begin; p0 < p_end; p0 += SY_KP) {
-    for (int64_t e = tid; e < SY_KP * m_cols / 2; e += 512) {
-      int r = (int)(e / (m_cols / 2));
-      int64_t cc = (e % (m_cols / 2)) * 2;
-      int64_t p = p0 + r;
+  const int n_stage = (int)((p_end - p_begin + SY_KP - 1) / SY_KP);
+  const int64_t half = m_cols / 2;
+  const int64_t per = SY_KP * half;  // double2 per stage
+  double2 rx[SY_STG];
+  double rc = 0.0;
+  auto fetch = [&](int s) {
+    const int64_t p0 = p_begin + (int64_t)s * SY_KP;
+#pragma unroll
+    for (int u = 0; u < SY_STG; u++) {
+      const int64_t e = tid + 512 * u;
       double2 v = {0.0, 0.0};
-      if (p < p_end) v = *reinterpret_cast<const double2 *>(x + p * m_cols + cc);
-      sx[r * ld + cc] = v.x;
-      sx[r * ld + cc + 1] = v.y;
+      if (e < per) {
+        const int r = (int)(e / half);
+        const int64_t cc = (e % half) * 2;
+        if (p0 + r < p_end) v = *reinterpret_cast<const double2 *>(x + (p0 + r) * m_cols + cc);
+      }
+      rx[u] = v;
     }
-    if (tid < SY_KP) {
-      int64_t p = p0 + tid;
-      sc[tid] = (p < p_end) ? cw[p] : 0.0;
+    if (tid < SY_KP) rc = (p0 + tid < p_end) ? cw[p0 + tid] : 0.0;
+  };
+  auto put = [&](int buf) {
+    double *b = sx + buf * bufsz;
+#pragma unroll
+    for (int u = 0; u < SY_STG; u++) {
+      const int64_t e = tid + 512 * u;
+      if (e < per) {
+        const int r = (int)(e / half);
+        const int64_t cc = (e % half) * 2;
+        b[r * ld + cc] = rx[u].x;
+        b[r * ld + cc + 1] = rx[u].y;
+      }
     }
-    __syncthreads();
+    if (tid < SY_KP) sc[buf * SY_KP + tid] = rc;
+  };
+  if (n_stage > 0) {
+    fetch(0);
+    put(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < n_stage; s++) {
+    const int buf = s & 1;
+    if (s + 1 < n_stage) fetch(s + 1);
+    const double *b = sx + buf * bufsz;
 #pragma unroll
     for (int ks = 0; ks < SY_KP / 4; ks++) {
       const int kp = ks * 4 + (lane >> 4);
-      const double cpt = sc[kp];
-      const double *row = sx + kp * ld;
+      const double cpt = sc[buf * SY_KP + kp];
+      const double *row = b + kp * ld;
       int last_rg = -1;
       double a = 0.0;
 #pragma unroll
       for (int t = 0; t < SY_MAXT; t++) {
         if (t < ntile) {
-          int rg = tl[2 * t], cg = tl[2 * t + 1];
-          if (rg != last_rg) {
-            a = row[rg * 16 + (lane & 15)];
-            last_rg = rg;
+          if (trg[t] != last_rg) {
+            a = row[trg[t] * 16 + (lane & 15)];
+            last_rg = trg[t];
           }
-          double b = cpt * row[cg * 16 + (lane & 15)];
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+          double bv = cpt * row[tcg[t] * 16 + (lane & 15)];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
         }
       }
     }
+    if (s + 1 < n_stage) put(buf ^ 1);
     __syncthreads();
   }
   double *out = partials + ((chunk_id * n_parts + part) * SY_WAVES + wave) * (int64_t)max_tiles * 256;
@@ -196,14 +256,23 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
   }
 }
 
+int syrk_f64_max_tiles(int64_t m_cols) { return m_cols <= 448 ? 22 : 12; }
+
 void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, const double *c,
                      int64_t n, double *partials) {
   int64_t n_ch8 = (plan.n_chunks + 7) / 8 * 8;
   int64_t grid = n_ch8 * plan.n_parts;
-  size_t lds = sizeof(double) * (SY_KP * (plan.m_cols + 1) + SY_KP);
-  hipLaunchKernelGGL(k_syrk_f64, dim3((unsigned)grid), dim3(512), lds, st, x, c, n, plan.m_cols,
-                     plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks, plan.d_tiles,
-                     partials);
+  if (plan.m_cols <= 448) {
+    size_t lds = sizeof(double) * 2 * (16 * (plan.m_cols + 1) + 16);
+    hipLaunchKernelGGL((k_syrk_f64<22, 7, 16>), dim3((unsigned)grid), dim3(512), lds, st, x, c, n,
+                       plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks,
+                       plan.d_tiles, partials);
+  } else {  // m_cols <= 1024: 8-point stages keep the two LDS buffers under 160 KB
+    size_t lds = sizeof(double) * 2 * (8 * (plan.m_cols + 1) + 8);
+    hipLaunchKernelGGL((k_syrk_f64<12, 8, 8>), dim3((unsigned)grid), dim3(512), lds, st, x, c, n,
+                       plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks,
+                       plan.d_tiles, partials);
+  }
 }
 
 // Sum of the chunk partials, in chunk order, scattered into the full M x M matrix (lower tiles

@@ -1,0 +1,434 @@
+// The N x N FP64 part of a Donaldson step, on the GPU: T from the contraction, then the
+// inverse Cholesky factor W = L^-1 of T (T = L L^H), then G = W^H W and the step residual.
+//
+// Reference (cy3/src/quintic.rs:1790-1894): T /= sum w; T *= N / tr T; G_next = T^-1 by LU of the
+// 2N x 2N real block; Hermitian projection; identity on failure; residual ||H_next - H||_F.
+// Here T is Hermitian positive definite, so T^-1 = W^H W with W = L^-1 lower triangular. W is
+// what the next step's contractions consume (D_p = |W s_p|^2). G = W^H W is formed for the
+// residual and the caller's h_block. It is exactly Hermitian: G_ab and G_ba are summed in the
+// same order.
+//
+// W comes from one blocked elimination of the augmented matrix [T | I] in 32 x 32 complex tiles.
+// For each block column J:
+//   k_elim_panel  (1 workgroup): factors the diagonal tile and inverts it in the same 32-step
+//                 sweep (one barrier per column). It then forms row block J of W,
+//                 W_JK = L_JJ^-1 R_JK for K <= J, where R is the updated right half.
+//   k_elim_update (one workgroup per tile): L_IJ = A_IJ L_JJ^-H, then A_IK -= L_IJ L_KJ^H for
+//                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J.
+// A non-positive or non-finite pivot sets status = ST_ERR_NOT_PD. Later kernels then return
+// early, and the gram kernel installs the reference's identity fallback (G = W = I).
+// Complex matrices are row-major [n][n] of (re, im).
+#include "cy3q_internal.h"
+
+namespace cy3q {
+
+static constexpr int TB = 32;  // tile edge
+
+struct Cz {
+  double r, i;
+};
+
+__device__ inline Cz cmul(Cz a, Cz b) { return Cz{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+// a * conj(b)
+__device__ inline Cz cmulc(Cz a, Cz b) { return Cz{a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }
+
+// Loads tile (r0, c0) of an n x n complex matrix into LDS, zero outside the matrix (or the
+// identity on the diagonal when pad_identity).
+__device__ inline void load_tile(Cz (*dst)[TB + 1], const double *__restrict__ src, int n, int r0,
+                                 int c0, bool pad_identity, int tid, int nthr) {
+  for (int q = tid; q < TB * TB; q += nthr) {
+    const int r = q / TB, c = q % TB;
+    Cz z{(pad_identity && r == c) ? 1.0 : 0.0, 0.0};
+    if (r0 + r < n && c0 + c < n) {
+      const double2 v = *reinterpret_cast<const double2 *>(src + 2 * ((int64_t)(r0 + r) * n + c0 + c));
+      z = Cz{v.x, v.y};
+    }
+    dst[r][c] = z;
+  }
+}
+
+__device__ inline void store_tile(double *__restrict__ dst, int n, int r0, int c0,
+                                  const Cz (*src)[TB + 1], int tid, int nthr) {
+  for (int q = tid; q < TB * TB; q += nthr) {
+    const int r = q / TB, c = q % TB;
+    if (r0 + r < n && c0 + c < n) {
+      double2 v;
+      v.x = src[r][c].r;
+      v.y = src[r][c].i;
+      *reinterpret_cast<double2 *>(dst + 2 * ((int64_t)(r0 + r) * n + c0 + c)) = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// T from xx = X^T diag(c) X:  T_ab = s * (xx[2a][2b] + xx[2a+1][2b+1] + i (xx[2a][2b+1] -
+// xx[2a+1][2b])), with s = hscale * dscale * inv_w * (N / tr) (quintic.rs:1783-1812;
+// inv_w = 1/sum_w if sum_w > 1e-12 else 0; the trace factor only if trace_normalise and
+// tr > 1e-10). One small workgroup computes s; the fill is spread over many.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_t_scale(const double *__restrict__ xx, int64_t m, int nb,
+                                                 const double *__restrict__ sum_w, double hscale,
+                                                 const double *__restrict__ dscale,
+                                                 int trace_normalise, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double base = hscale;
+  if (dscale) base *= dscale[0];
+  const double sw = *sum_w;
+  const double inv_w = sw > 1e-12 ? 1.0 / sw : 0.0;
+  double tr = 0.0;
+  for (int a = threadIdx.x; a < nb; a += 256)
+    tr += (xx[(2 * a) * m + 2 * a] * base + xx[(2 * a + 1) * m + 2 * a + 1] * base) * inv_w;
+  sh[threadIdx.x] = tr;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double trace = sh[0];
+    const double s = (trace_normalise && trace > 1e-10) ? (double)nb / trace : 1.0;
+    out[0] = base * inv_w * s;
+  }
+}
+
+// t (the caller-visible T), a (the elimination's left half), r (its right half, = I).
+__global__ void k_t_fill(const double *__restrict__ xx, int64_t m, int nb,
+                         const double *__restrict__ scale, double *__restrict__ t,
+                         double *__restrict__ a, double *__restrict__ r) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nb * nb) return;
+  const int ia = (int)(e / nb), ib = (int)(e % nb);
+  const double s = scale[0];
+  const double rr = xx[(2 * ia) * m + 2 * ib], ii = xx[(2 * ia + 1) * m + 2 * ib + 1];
+  const double ri = xx[(2 * ia) * m + 2 * ib + 1], ir = xx[(2 * ia + 1) * m + 2 * ib];
+  const double re = (rr + ii) * s, im = (ri - ir) * s;
+  if (t) {
+    t[2 * e] = re;
+    t[2 * e + 1] = im;
+  }
+  a[2 * e] = re;
+  a[2 * e + 1] = im;
+  r[2 * e] = ia == ib ? 1.0 : 0.0;
+  r[2 * e + 1] = 0.0;
+}
+
+void launch_t_from_xx(hipStream_t st, const double *xx, int64_t m_cols, int nb,
+                      const double *sum_w, double hscale, const double *dscale,
+                      int trace_normalise, double *scale_buf, double *t, double *a, double *r) {
+  hipLaunchKernelGGL(k_t_scale, dim3(1), dim3(256), 0, st, xx, m_cols, nb, sum_w, hscale, dscale,
+                     trace_normalise, scale_buf);
+  const int64_t tot = (int64_t)nb * nb;
+  hipLaunchKernelGGL(k_t_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, xx, m_cols,
+                     nb, scale_buf, t, a, r);
+}
+
+// Copies a complex matrix into the elimination buffers (a = src, r = I).
+__global__ void k_elim_init(const double *__restrict__ src, int nb, double *__restrict__ a,
+                            double *__restrict__ r) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nb * nb) return;
+  a[2 * e] = src[2 * e];
+  a[2 * e + 1] = src[2 * e + 1];
+  r[2 * e] = (e / nb == e % nb) ? 1.0 : 0.0;
+  r[2 * e + 1] = 0.0;
+}
+
+void launch_elim_init(hipStream_t st, const double *src, int nb, double *a, double *r) {
+  const int64_t tot = (int64_t)nb * nb;
+  hipLaunchKernelGGL(k_elim_init, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, src, nb,
+                     a, r);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Panel of block column J.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a,
+                                                    const double *__restrict__ r, int n, int jblk,
+                                                    double *__restrict__ einv,
+                                                    double *__restrict__ w,
+                                                    int32_t *__restrict__ status) {
+  __shared__ Cz d[TB][TB + 1];   // A_JJ, factored in place (lower part)
+  __shared__ Cz rr[TB][TB + 1];  // right half of the tile, starts as I
+  __shared__ Cz e[TB][TB + 1];   // L_JJ^-1
+  __shared__ Cz t2[TB][TB + 1];  // R_JK tiles for W_JK
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  const int j0 = jblk * TB;
+  if (jblk == 0) {
+    if (tid == 0) *status = ST_OK;
+  } else if (*status != ST_OK) {
+    return;
+  }
+  if (tid == 0) bad = 0;
+  load_tile(d, a, n, j0, j0, true, tid, 256);
+  for (int q = tid; q < TB * TB; q += 256) {
+    const int i = q / TB, k = q % TB;
+    rr[i][k] = Cz{i == k ? 1.0 : 0.0, 0.0};
+    e[i][k] = Cz{0.0, 0.0};
+  }
+  __syncthreads();
+  // Step j: l_jj = sqrt(d_jj); row j of L^-1 is rr_j / l_jj; for i > j the trailing left part
+  // loses d_ij conj(d_kj) / d_jj and the right part loses d_ij rr_j / d_jj. Column j of d and
+  // row j of rr are only read in step j, so one barrier per step suffices.
+  for (int j = 0; j < TB; j++) {
+    const double piv = d[j][j].r;
+    if (!(piv > 0.0) || !isfinite(piv)) {
+      if (tid == 0) bad = 1;
+      break;
+    }
+    const double inv_l = 1.0 / sqrt(piv), ipiv = 1.0 / piv;
+    if (tid <= j) e[j][tid] = Cz{rr[j][tid].r * inv_l, rr[j][tid].i * inv_l};
+    const int m = TB - j - 1;
+    for (int q = tid; q < m * m; q += 256) {
+      const int ii = q / m, kk = q % m;
+      if (kk > ii) continue;
+      const int i = j + 1 + ii, k = j + 1 + kk;
+      const Cz p = cmulc(d[i][j], d[k][j]);
+      d[i][k].r -= p.r * ipiv;
+      d[i][k].i -= p.i * ipiv;
+    }
+    for (int q = tid; q < m * (j + 1); q += 256) {
+      const int ii = q / (j + 1), k = q % (j + 1);
+      const int i = j + 1 + ii;
+      const Cz p = cmul(d[i][j], rr[j][k]);
+      rr[i][k].r -= p.r * ipiv;
+      rr[i][k].i -= p.i * ipiv;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) *status = ST_ERR_NOT_PD;
+    return;
+  }
+  // L_JJ^-1 for the update kernel (a 32 x 32 tile per block column), and W_JJ = L_JJ^-1.
+  for (int q = tid; q < TB * TB; q += 256) {
+    const int i = q / TB, k = q % TB;
+    double2 v;
+    v.x = e[i][k].r;
+    v.y = e[i][k].i;
+    *reinterpret_cast<double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q)) = v;
+  }
+  store_tile(w, n, j0, j0, e, tid, 256);
+  // W_JK = L_JJ^-1 R_JK for K < J.
+  for (int kb = 0; kb < jblk; kb++) {
+    __syncthreads();
+    load_tile(t2, r, n, j0, kb * TB, false, tid, 256);
+    __syncthreads();
+    for (int q = tid; q < TB * TB; q += 256) {
+      const int i = q / TB, k = q % TB;
+      double sr = 0.0, si = 0.0;
+      for (int mm = 0; mm <= i; mm++) {
+        const Cz p = cmul(e[i][mm], t2[mm][k]);
+        sr += p.r;
+        si += p.i;
+      }
+      const int gi = j0 + i, gk = kb * TB + k;
+      if (gi < n && gk < n) {
+        double2 v;
+        v.x = sr;
+        v.y = si;
+        *reinterpret_cast<double2 *>(w + 2 * ((int64_t)gi * n + gk)) = v;
+      }
+    }
+  }
+}
+
+// L_IJ = A_IJ L_JJ^-H into dst (x_ik = sum_{m <= k} a_im conj(e_km)).
+__device__ inline void panel_row(Cz (*dst)[TB + 1], const Cz (*aij)[TB + 1],
+                                 const Cz (*e)[TB + 1], int tid) {
+  for (int q = tid; q < TB * TB; q += 256) {
+    const int i = q / TB, k = q % TB;
+    double sr = 0.0, si = 0.0;
+    for (int mm = 0; mm <= k; mm++) {
+      const Cz p = cmulc(aij[i][mm], e[k][mm]);
+      sr += p.r;
+      si += p.i;
+    }
+    dst[i][k] = Cz{sr, si};
+  }
+}
+
+__global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
+                                                     double *__restrict__ r,
+                                                     const double *__restrict__ einv,
+                                                     const double *__restrict__ w, int n, int jblk,
+                                                     const int32_t *__restrict__ status) {
+  __shared__ Cz e[TB][TB + 1], x[TB][TB + 1], li[TB][TB + 1], y[TB][TB + 1];
+  if (*status != ST_OK) return;
+  const int tid = threadIdx.x;
+  const int nblk = (n + TB - 1) / TB;
+  const int mrem = nblk - jblk - 1;
+  const int n_type1 = mrem * (mrem + 1) / 2;
+  int b = blockIdx.x;
+  int I, K;
+  bool type1;
+  if (b < n_type1) {  // (ii, kk), 0 <= kk <= ii < mrem
+    int ii = 0;
+    while (b > ii) {
+      b -= ii + 1;
+      ii++;
+    }
+    I = jblk + 1 + ii;
+    K = jblk + 1 + b;
+    type1 = true;
+  } else {
+    b -= n_type1;
+    I = jblk + 1 + b / (jblk + 1);
+    K = b % (jblk + 1);
+    type1 = false;
+  }
+  const int j0 = jblk * TB;
+  for (int q = tid; q < TB * TB; q += 256) {
+    const double2 v = *reinterpret_cast<const double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q));
+    e[q / TB][q % TB] = Cz{v.x, v.y};
+  }
+  load_tile(x, a, n, I * TB, j0, false, tid, 256);
+  __syncthreads();
+  panel_row(li, x, e, tid);  // L_IJ
+  __syncthreads();
+  if (type1) {
+    // L_KJ into x (I == K reuses L_IJ), then A_IK -= L_IJ L_KJ^H.
+    if (K != I) {
+      load_tile(y, a, n, K * TB, j0, false, tid, 256);
+      __syncthreads();
+      panel_row(x, y, e, tid);
+    } else {
+      for (int q = tid; q < TB * TB; q += 256) x[q / TB][q % TB] = li[q / TB][q % TB];
+    }
+    __syncthreads();
+    load_tile(y, a, n, I * TB, K * TB, false, tid, 256);
+    __syncthreads();
+    for (int q = tid; q < TB * TB; q += 256) {
+      const int i = q / TB, k = q % TB;
+      if (I == K && k > i) continue;
+      double sr = 0.0, si = 0.0;
+      for (int mm = 0; mm < TB; mm++) {
+        const Cz p = cmulc(li[i][mm], x[k][mm]);
+        sr += p.r;
+        si += p.i;
+      }
+      y[i][k].r -= sr;
+      y[i][k].i -= si;
+    }
+    __syncthreads();
+    store_tile(a, n, I * TB, K * TB, y, tid, 256);
+  } else {
+    // R_IK -= L_IJ W_JK.
+    load_tile(x, w, n, j0, K * TB, false, tid, 256);
+    load_tile(y, r, n, I * TB, K * TB, false, tid, 256);
+    __syncthreads();
+    for (int q = tid; q < TB * TB; q += 256) {
+      const int i = q / TB, k = q % TB;
+      double sr = 0.0, si = 0.0;
+      for (int mm = 0; mm < TB; mm++) {
+        const Cz p = cmul(li[i][mm], x[mm][k]);
+        sr += p.r;
+        si += p.i;
+      }
+      y[i][k].r -= sr;
+      y[i][k].i -= si;
+    }
+    __syncthreads();
+    store_tile(r, n, I * TB, K * TB, y, tid, 256);
+  }
+}
+
+// Runs the elimination on (a, r) prepared by launch_t_from_xx or launch_elim_init; W -> w.
+void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv, double *w, int nb,
+                             int32_t *status) {
+  const int nblk = (nb + TB - 1) / TB;
+  for (int j = 0; j < nblk; j++) {
+    hipLaunchKernelGGL(k_elim_panel, dim3(1), dim3(256), 0, st, a, r, nb, j, einv, w, status);
+    const int mrem = nblk - j - 1;
+    const int tiles = mrem * (mrem + 1) / 2 + mrem * (j + 1);
+    if (tiles > 0)
+      hipLaunchKernelGGL(k_elim_update, dim3(tiles), dim3(256), 0, st, a, r, einv, w, nb, j,
+                         status);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// G = W^H W per 32 x 32 tile: G_IK = sum_{J >= max(I, K)} W_JI^H W_JK. With residual != nullptr
+// the tile also yields 2 sum |G_new - G_old|^2 into partial[tile] and G_new replaces g in place;
+// otherwise G_new is written to g. If status shows a failed factorisation, G = W = I (the
+// reference's identity fallback, quintic.rs:1874-1883). Each tile reads and writes only its own
+// G entries, so the in-place update has no race.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gram_tiles(double *__restrict__ w, int n,
+                                                    double *__restrict__ g,
+                                                    const int32_t *__restrict__ status,
+                                                    double *__restrict__ partial) {
+  __shared__ Cz wi[TB][TB + 1], wk[TB][TB + 1];
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int nblk = (n + TB - 1) / TB;
+  const int I = blockIdx.x / nblk, K = blockIdx.x % nblk;
+  const bool fail = status && *status != ST_OK;
+  double acc[4][2];
+  for (int u = 0; u < 4; u++) acc[u][0] = acc[u][1] = 0.0;
+  if (!fail) {
+    for (int J = max(I, K); J < nblk; J++) {
+      load_tile(wi, w, n, J * TB, I * TB, false, tid, 256);
+      load_tile(wk, w, n, J * TB, K * TB, false, tid, 256);
+      __syncthreads();
+      for (int u = 0; u < 4; u++) {
+        const int q = tid + 256 * u, i = q / TB, k = q % TB;
+        for (int mm = 0; mm < TB; mm++) {
+          const Cz p = cmulc(wk[mm][k], wi[mm][i]);  // conj(W_mi) W_mk
+          acc[u][0] += p.r;
+          acc[u][1] += p.i;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double res = 0.0;
+  for (int u = 0; u < 4; u++) {
+    const int q = tid + 256 * u, i = q / TB, k = q % TB;
+    const int gi = I * TB + i, gk = K * TB + k;
+    if (gi >= n || gk >= n) continue;
+    double nr = acc[u][0], ni = acc[u][1];
+    if (fail) {
+      nr = gi == gk ? 1.0 : 0.0;
+      ni = 0.0;
+      w[2 * ((int64_t)gi * n + gk)] = nr;
+      w[2 * ((int64_t)gi * n + gk) + 1] = 0.0;
+    }
+    double *pg = g + 2 * ((int64_t)gi * n + gk);
+    if (partial) {
+      const double dr = nr - pg[0], di = ni - pg[1];
+      res += 2.0 * (dr * dr + di * di);
+    }
+    pg[0] = nr;
+    pg[1] = ni;
+  }
+  if (partial) {
+    red[tid] = res;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (tid < off) red[tid] += red[tid + off];
+      __syncthreads();
+    }
+    if (tid == 0) partial[blockIdx.x] = red[0];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_residual_final(const double *__restrict__ partial, int cnt,
+                                                       double *__restrict__ residual) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < cnt; i++) s += partial[i];
+    *residual = sqrt(s);
+  }
+}
+
+void launch_gram_residual(hipStream_t st, double *w, int nb, double *g, const int32_t *status,
+                          double *partial, double *residual) {
+  const int nblk = (nb + TB - 1) / TB;
+  hipLaunchKernelGGL(k_gram_tiles, dim3(nblk * nblk), dim3(256), 0, st, w, nb, g, status, partial);
+  if (partial && residual)
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, st, partial, nblk * nblk, residual);
+}
+
+}  // namespace cy3q

@@ -1,6 +1,5 @@
 // Per-point geometry on the quintic: CY-measure weights, log|Omega|^2 with the chart/elimination
-// choice, degree-k section values and their z-derivatives, and the row-scaled FP32 copy that the
-// split-precision contraction reads.
+// choice, and degree-k section values and their z-derivatives.
 //
 // Operation order follows the reference so the FP64 values agree to rounding:
 //   cy_measure_weights(_psi)      cy3/src/quintic.rs:3507-3563
@@ -172,36 +171,6 @@ void launch_sections(hipStream_t st, const double *pts, int64_t n, const uint32_
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sections, dim3((unsigned)((n + PTS_PER_WG - 1) / PTS_PER_WG)), dim3(256), 0,
                      st, pts, n, mono, nb, kmax, m_cols, sv, sd);
-}
-
-// Row-scaled FP32 copy for the split-precision path: x32[p] = sv[p] * 2^-e_p with e_p chosen
-// so that max |x32[p][j]| lies in (0.5, 1]. row_scale2[p] = 4^-e_p, the factor by which D_p
-// and T's point weight change (exact powers of two, so the unscaling is exact).
-// One wavefront per row.
-__global__ void __launch_bounds__(256) k_row_scale_f32(const double *__restrict__ sv, int64_t n,
-                                                       int64_t m_cols, float *__restrict__ x32,
-                                                       float *__restrict__ row_scale2) {
-  int lane = threadIdx.x & 63;
-  int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= n) return;
-  const double *row = sv + p * m_cols;
-  double mx = 0.0;
-  for (int64_t j = lane; j < m_cols; j += 64) mx = fmax(mx, fabs(row[j]));
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-  int e = 0;
-  if (mx > 0.0) {
-    frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
-  }
-  double sc = ldexp(1.0, -e);
-  for (int64_t j = lane; j < m_cols; j += 64) x32[p * m_cols + j] = (float)(row[j] * sc);
-  if (lane == 0) row_scale2[p] = (float)ldexp(1.0, -2 * e);
-}
-
-void launch_row_scale_f32(hipStream_t st, const double *sv, int64_t n, int64_t m_cols, float *x32,
-                          float *row_scale2) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_row_scale_f32, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, sv, n,
-                     m_cols, x32, row_scale2);
 }
 
 }  // namespace cy3q

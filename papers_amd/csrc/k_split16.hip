@@ -4,7 +4,8 @@
 // fp16 MFMA with FP32 accumulation. Each FP32 operand x is split as x = hi + lo with
 // hi = fp16(x) and lo = fp16(x - hi); a product is hi*hi' + hi*lo' + lo*hi', about 22 bits per
 // product. Overflow and subnormal loss are avoided by exact power-of-two scalings:
-//   * every section row is stored scaled by 2^-e_p (max |x| in [0.5, 1)), once per workspace;
+//   * every section row is stored scaled by 2^-e_p (max |x| in [0.5, 1)) and split into fp16
+//     hi/lo planes once per workspace;
 //   * W (the D-pass operand) is scaled globally by 2^-f;
 //   * z_p = sqrt(c_p) x_p (the T-pass operand) is scaled globally by 2^g.
 // The host-side bookkeeping undoes each scaling exactly. D_p and the chunk sums are accumulated in
@@ -13,7 +14,7 @@
 // D pass  (k_dpass_split16): v_mfma_f32_16x16x32_f16, 128 points x all M columns per workgroup,
 //                            so D_p = sum_j Y_pj^2 completes inside the workgroup.
 // T pass  (k_syrk_split16):  v_mfma_f32_32x32x16_f16 on the lower 32x32 sub-tiles of Z^T Z,
-//                            one workgroup per (point chunk, part).
+//                            one workgroup per point chunk.
 #include "cy3q_internal.h"
 
 namespace cy3q {
@@ -28,8 +29,8 @@ __device__ inline void split_f32(float x, _Float16 &h, _Float16 &l) {
 // (see k_build_bmat): B[2a][2b] = Re W_ba, B[2a+1][2b] = -Im W_ba, B[2a][2b+1] = Im W_ba,
 // B[2a+1][2b+1] = Re W_ba for a <= b. Fragment (ks, cg) holds B[32ks + 8(l/16) + j][16cg + l%16],
 // j = 0..7, for lane l: the B operand layout of v_mfma_f32_16x16x32_f16.
-// Layout: bfrag[((ks * n_cg + cg) * 2 + {0: hi, 1: lo}) * 64 + lane]. bscale[0] = 4^f, the factor
-// that restores D from the scaled contraction.
+// Layout: bfrag[((ks * n_cg + cg) * 2 + {0: hi, 1: lo}) * 64 + lane]. bscale[0] = 4^f restores D
+// from the scaled contraction; bscale[1] = 2^-f is the scale applied to W.
 // ---------------------------------------------------------------------------------------------
 __device__ inline double bmat_entry(const double *__restrict__ w, int nb, int64_t i, int64_t j) {
   int64_t a = i / 2, b = j / 2;
@@ -42,9 +43,9 @@ __device__ inline double bmat_entry(const double *__restrict__ w, int nb, int64_
   return vr;
 }
 
-__global__ void __launch_bounds__(1024) k_build_bfrag16(const double *__restrict__ w, int nb,
-                                                        int64_t m, half8_t *__restrict__ bfrag,
-                                                        double *__restrict__ bscale) {
+// Scale: bscale[0] = 4^f and bscale[1] = 2^-f with 2^f > max |W| (one workgroup).
+__global__ void __launch_bounds__(1024) k_bfrag_scale(const double *__restrict__ w, int nb,
+                                                      double *__restrict__ bscale) {
   __shared__ double sh[1024];
   double mx = 0.0;
   for (int64_t e = threadIdx.x; e < 2 * (int64_t)nb * nb; e += 1024) mx = fmax(mx, fabs(w[e]));
@@ -54,58 +55,130 @@ __global__ void __launch_bounds__(1024) k_build_bfrag16(const double *__restrict
     if (threadIdx.x < (unsigned)off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
     __syncthreads();
   }
-  int f = 0;
-  if (sh[0] > 0.0) frexp(sh[0], &f);
-  const double sc = ldexp(1.0, -f);
-  if (threadIdx.x == 0) bscale[0] = ldexp(1.0, 2 * f);
-  const int n_ks = (int)(m / 32), n_cg = (int)(m / 16);
-  const int64_t total = (int64_t)n_ks * n_cg * 64;
-  for (int64_t e = threadIdx.x; e < total; e += 1024) {
-    const int lane = (int)(e % 64);
-    const int64_t frag = e / 64;
-    const int cg = (int)(frag % n_cg), ks = (int)(frag / n_cg);
-    half8_t hi, lo;
-    for (int j = 0; j < 8; j++) {
-      int64_t kr = 32 * ks + 8 * (lane / 16) + j;
-      int64_t col = 16 * cg + (lane % 16);
-      float v = (float)(bmat_entry(w, nb, kr, col) * sc);
-      _Float16 h, l;
-      split_f32(v, h, l);
-      hi[j] = h;
-      lo[j] = l;
-    }
-    bfrag[(frag * 2 + 0) * 64 + lane] = hi;
-    bfrag[(frag * 2 + 1) * 64 + lane] = lo;
+  if (threadIdx.x == 0) {
+    int f = 0;
+    if (sh[0] > 0.0) frexp(sh[0], &f);
+    bscale[0] = ldexp(1.0, 2 * f);
+    bscale[1] = ldexp(1.0, -f);
   }
+}
+
+// One thread per (fragment, lane).
+__global__ void k_bfrag_fill(const double *__restrict__ w, int nb, int64_t m,
+                             const double *__restrict__ bscale, half8_t *__restrict__ bfrag) {
+  const int n_ks = (int)(m / 32), n_cg = (int)(m / 16);
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n_ks * n_cg * 64) return;
+  const double sc = bscale[1];
+  const int lane = (int)(e % 64);
+  const int64_t frag = e / 64;
+  const int cg = (int)(frag % n_cg), ks = (int)(frag / n_cg);
+  half8_t hi, lo;
+  for (int j = 0; j < 8; j++) {
+    int64_t kr = 32 * ks + 8 * (lane / 16) + j;
+    int64_t col = 16 * cg + (lane % 16);
+    float v = (float)(bmat_entry(w, nb, kr, col) * sc);
+    _Float16 h, l;
+    split_f32(v, h, l);
+    hi[j] = h;
+    lo[j] = l;
+  }
+  bfrag[(frag * 2 + 0) * 64 + lane] = hi;
+  bfrag[(frag * 2 + 1) * 64 + lane] = lo;
 }
 
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale) {
-  hipLaunchKernelGGL(k_build_bfrag16, dim3(1), dim3(1024), 0, st, v_cplx, nb, m_cols, bfrag,
-                     b_scale);
+  hipLaunchKernelGGL(k_bfrag_scale, dim3(1), dim3(1024), 0, st, v_cplx, nb, b_scale);
+  const int64_t tot = (m_cols / 32) * (m_cols / 16) * 64;
+  hipLaunchKernelGGL(k_bfrag_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, v_cplx,
+                     nb, m_cols, b_scale, bfrag);
 }
 
 // ---------------------------------------------------------------------------------------------
-// D pass. Workgroup: 128 points, 8 waves. Wave w owns the column groups cglist[w][0..max_cg)
-// (balanced on the host by their triangular k-step counts). Per k-step of 32 columns, the point
-// tile is staged in LDS already split into hi/lo halves, [128][40] each (row pad keeps the
-// 16-byte fragment reads bank-conflict free). Output: dsum[p] = sum_j Y_pj^2 (FP64), for the
-// row-scaled x and the 2^-f scaled W.
+// Static operand planes of the split path, built once per workspace from the (orthonormalised)
+// FP64 sections. Row p is scaled by 2^-e_p (max |x| in [0.5, 1)); rs2[p] = 4^-e_p.
+//  xd: D-pass A fragments of v_mfma_f32_16x16x32_f16. For point tile t (16 points), k-step ks
+//      (32 columns), plane h (hi, lo), lane l: half8 of X[16t + l%16][32ks + 8(l/16) + j].
+//      Index ((t * n_ks + ks) * 2 + h) * 64 + l.
+//  xt: T-pass stage records. For stage s (16 points), column c, plane h: 16 halves
+//      X[16s + j][c], j = 0..15, as two half8. Index ((s * m + c) * 2 + h) * 2 + {0, 1}.
 // ---------------------------------------------------------------------------------------------
-static constexpr int DP_PTS = 128;
-static constexpr int DP_LD = 40;
-static constexpr int DP_MAXCG = 5;
+__global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__ sv, int64_t n,
+                                                      int64_t m, int *__restrict__ ex,
+                                                      float *__restrict__ rs2) {
+  int lane = threadIdx.x & 63;
+  int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n) return;
+  double mx = 0.0;
+  for (int64_t j = lane; j < m; j += 64) mx = fmax(mx, fabs(sv[p * m + j]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  if (lane == 0) {
+    ex[p] = e;
+    rs2[p] = (float)ldexp(1.0, -2 * e);
+  }
+}
 
-__global__ void __launch_bounds__(512) k_dpass_split16(const float *__restrict__ x32, int64_t n,
-                                                       int64_t m, const half8_t *__restrict__ bfrag,
-                                                       const int16_t *__restrict__ cglist,
-                                                       double *__restrict__ dsum) {
-  __shared__ _Float16 sh_hi[DP_PTS * DP_LD];
-  __shared__ _Float16 sh_lo[DP_PTS * DP_LD];
-  __shared__ double red[8][DP_PTS];
+__global__ void k_split_planes(const double *__restrict__ sv, const int *__restrict__ ex,
+                               int64_t n, int64_t n_pad, int64_t m,
+                               _Float16 *__restrict__ xd, _Float16 *__restrict__ xt) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_pad * m) return;
+  const int64_t p = q / m, c = q % m;
+  float x = 0.f;
+  if (p < n) x = (float)(sv[q] * ldexp(1.0, -ex[p]));
+  _Float16 h, l;
+  split_f32(x, h, l);
+  const int64_t n_ks = m / 32;
+  const int64_t t = p / 16, ks = c / 32;
+  const int lane = (int)(p % 16) + 16 * (int)((c % 32) / 8), j = (int)(c % 8);
+  xd[((((t * n_ks + ks) * 2 + 0) * 64 + lane) * 8) + j] = h;
+  xd[((((t * n_ks + ks) * 2 + 1) * 64 + lane) * 8) + j] = l;
+  const int64_t sg = p / 16;
+  const int jj = (int)(p % 16);
+  xt[(((sg * m + c) * 2 + 0) * 16) + jj] = h;
+  xt[(((sg * m + c) * 2 + 1) * 16) + jj] = l;
+}
+
+void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
+                         int *ex, float *rs2, _Float16 *xd, _Float16 *xt) {
+  hipLaunchKernelGGL(k_row_exponent, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, sv, n, m, ex,
+                     rs2);
+  int64_t tot = n_pad * m;
+  hipLaunchKernelGGL(k_split_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, sv, ex,
+                     n, n_pad, m, xd, xt);
+}
+
+// ---------------------------------------------------------------------------------------------
+// D pass. Workgroup: 128 points (8 point tiles), 8 waves. Wave w owns the column groups
+// cglist[w][..] (round-robin, so the groups still active at each k-step are spread evenly). Per
+// k-step, the 8 A fragment pairs (16 KB) are staged in LDS (double-buffered) and read back
+// conflict-free. The next k-step's A data and the wave's B fragments are loaded into registers
+// while the current k-step's MFMAs run. The three split products of one accumulator are spread
+// out so consecutive MFMAs write different accumulators.
+// Output: dsum[p] = sum_j Y_pj^2 (FP64), for the row-scaled x and the 2^-f scaled W.
+// ---------------------------------------------------------------------------------------------
+static constexpr int DP_TILES = 8;
+static constexpr int DP_PTS = 16 * DP_TILES;
+static constexpr int DP_WAVES = 8;
+static constexpr int DP_MAXCG = 4;
+
+int dpass_split16_waves() { return DP_WAVES; }
+int dpass_split16_max_cg() { return DP_MAXCG; }
+
+__global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *__restrict__ xd,
+                                                                 int64_t n, int64_t m,
+                                                                 const half8_t *__restrict__ bfrag,
+                                                                 const int16_t *__restrict__ cglist,
+                                                                 double *__restrict__ dsum) {
+  __shared__ half8_t sa[2][DP_TILES * 2 * 64];  // [buf][(rg * 2 + plane) * 64 + lane]
+  __shared__ double red[DP_WAVES][DP_PTS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t p0 = (int64_t)blockIdx.x * DP_PTS;
+  const int64_t tile0 = (int64_t)blockIdx.x * DP_TILES;
+  const int64_t n_tiles = (n + 15) / 16;
   const int n_cg = (int)(m / 16), n_ks = (int)(m / 32);
   int cgs[DP_MAXCG];
   int ncg = 0, ks_end = 0;
@@ -117,109 +190,121 @@ __global__ void __launch_bounds__(512) k_dpass_split16(const float *__restrict__
       ks_end = max(ks_end, cgs[c] / 2 + 1);
     }
   }
-  float4_t acc[8][DP_MAXCG];
+  float4_t acc[DP_TILES][DP_MAXCG];
 #pragma unroll
-  for (int r = 0; r < 8; r++)
+  for (int r = 0; r < DP_TILES; r++)
 #pragma unroll
     for (int c = 0; c < DP_MAXCG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
 
-  // Every wave takes part in staging; the k-loop bound is the workgroup maximum.
-  const int ks_all = n_ks;
-  for (int ks = 0; ks < ks_all; ks++) {
-    // Stage X[p0 .. p0+128)[32ks .. 32ks+32): 4096 floats, 8 per thread (two float4).
-    for (int e = tid; e < DP_PTS * 8; e += 512) {
-      int r = e >> 3, q = (e & 7) * 4;
-      int64_t p = p0 + r;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (p < n) v = *reinterpret_cast<const float4 *>(x32 + p * m + 32 * ks + q);
-      _Float16 h0, l0, h1, l1, h2, l2, h3, l3;
-      split_f32(v.x, h0, l0);
-      split_f32(v.y, h1, l1);
-      split_f32(v.z, h2, l2);
-      split_f32(v.w, h3, l3);
-      _Float16 *dh = sh_hi + r * DP_LD + q, *dl = sh_lo + r * DP_LD + q;
-      dh[0] = h0;
-      dh[1] = h1;
-      dh[2] = h2;
-      dh[3] = h3;
-      dl[0] = l0;
-      dl[1] = l1;
-      dl[2] = l2;
-      dl[3] = l3;
+  // Staging map: 1024 half8 per k-step, two per thread.
+  half8_t stg[2];
+  auto load_a = [&](int ks) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int q = tid + 64 * DP_WAVES * r;
+      const int rg = q >> 7, pl = (q >> 6) & 1, ln = q & 63;
+      const int64_t t = tile0 + rg;
+      half8_t v = {};
+      if (t < n_tiles) v = xd[((t * n_ks + ks) * 2 + pl) * 64 + ln];
+      stg[r] = v;
     }
-    __syncthreads();
-    if (ks < ks_end) {
-      half8_t bh[DP_MAXCG], bl[DP_MAXCG];
+  };
+  half8_t bh[DP_MAXCG], bl[DP_MAXCG], nbh[DP_MAXCG], nbl[DP_MAXCG];
+  auto load_b = [&](int ks, half8_t *hh, half8_t *ll) {
 #pragma unroll
-      for (int c = 0; c < DP_MAXCG; c++) {
-        if (c < ncg && ks <= cgs[c] / 2) {
-          int64_t frag = (int64_t)ks * n_cg + cgs[c];
-          bh[c] = bfrag[(frag * 2 + 0) * 64 + lane];
-          bl[c] = bfrag[(frag * 2 + 1) * 64 + lane];
-        }
+    for (int c = 0; c < DP_MAXCG; c++) {
+      if (c < ncg && ks <= cgs[c] / 2) {
+        const int64_t frag = (int64_t)ks * n_cg + cgs[c];
+        hh[c] = bfrag[(frag * 2 + 0) * 64 + lane];
+        ll[c] = bfrag[(frag * 2 + 1) * 64 + lane];
       }
+    }
+  };
+  load_a(0);
+  load_b(0, bh, bl);
+  for (int ks = 0; ks < n_ks; ks++) {
+    const int buf = ks & 1;
+    sa[buf][tid] = stg[0];
+    sa[buf][tid + 64 * DP_WAVES] = stg[1];
+    __syncthreads();
+    if (ks + 1 < n_ks) {
+      load_a(ks + 1);
+      load_b(ks + 1, nbh, nbl);
+    }
+    if (ks < ks_end) {
 #pragma unroll
-      for (int r = 0; r < 8; r++) {
-        const int row = r * 16 + (lane & 15);
-        const int kof = 8 * (lane >> 4);
-        half8_t ah = *reinterpret_cast<const half8_t *>(sh_hi + row * DP_LD + kof);
-        half8_t al = *reinterpret_cast<const half8_t *>(sh_lo + row * DP_LD + kof);
+      for (int r = 0; r < DP_TILES; r += 2) {
+        const half8_t ah0 = sa[buf][(r * 2 + 0) * 64 + lane];
+        const half8_t al0 = sa[buf][(r * 2 + 1) * 64 + lane];
+        const half8_t ah1 = sa[buf][(r * 2 + 2) * 64 + lane];
+        const half8_t al1 = sa[buf][(r * 2 + 3) * 64 + lane];
 #pragma unroll
-        for (int c = 0; c < DP_MAXCG; c++) {
-          if (c < ncg && ks <= cgs[c] / 2) {
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[r][c], 0, 0, 0);
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[r][c], 0, 0, 0);
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[c], acc[r][c], 0, 0, 0);
+        for (int pr = 0; pr < 3; pr++) {
+#pragma unroll
+          for (int c = 0; c < DP_MAXCG; c++) {
+            if (c < ncg && ks <= cgs[c] / 2) {
+              const half8_t a0 = pr == 2 ? al0 : ah0, a1 = pr == 2 ? al1 : ah1;
+              const half8_t b = pr == 1 ? bl[c] : bh[c];
+              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b, acc[r][c], 0, 0, 0);
+              acc[r + 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b, acc[r + 1][c], 0, 0, 0);
+            }
           }
         }
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < DP_MAXCG; c++) {
+      bh[c] = nbh[c];
+      bl[c] = nbl[c];
+    }
   }
   // Row sums of squares: element e of lane l is point 16r + 4(l/16) + e, column 16cg + l%16.
 #pragma unroll
-  for (int r = 0; r < 8; r++) {
+  for (int r = 0; r < DP_TILES; r++) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      double s = 0.0;
+      double sacc = 0.0;
 #pragma unroll
       for (int c = 0; c < DP_MAXCG; c++) {
         if (c < ncg) {
           double y = (double)acc[r][c][e];
-          s += y * y;
+          sacc += y * y;
         }
       }
-      for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
-      if ((lane & 15) == 0) red[wave][r * 16 + 4 * (lane >> 4) + e] = s;
+      for (int off = 1; off < 16; off <<= 1) sacc += __shfl_xor(sacc, off);
+      if ((lane & 15) == 0) red[wave][r * 16 + 4 * (lane >> 4) + e] = sacc;
     }
   }
   __syncthreads();
   if (tid < DP_PTS) {
-    double s = 0.0;
-    for (int w = 0; w < 8; w++) s += red[w][tid];
-    int64_t p = p0 + tid;
-    if (p < n) dsum[p] = s;
+    double sacc = 0.0;
+    for (int w = 0; w < DP_WAVES; w++) sacc += red[w][tid];
+    const int64_t p = tile0 * 16 + tid;
+    if (p < n) dsum[p] = sacc;
   }
 }
 
-void launch_dpass_split16(hipStream_t st, const float *x32, int64_t n, int64_t m_cols,
+void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, double *dsum) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_dpass_split16, dim3((unsigned)((n + DP_PTS - 1) / DP_PTS)), dim3(512), 0, st,
-                     x32, n, m_cols, bfrag, cglist, dsum);
+  hipLaunchKernelGGL(k_dpass_split16, dim3((unsigned)((n + DP_PTS - 1) / DP_PTS)),
+                     dim3(64 * DP_WAVES), 0, st, xd, n, m_cols, bfrag, cglist, dsum);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Per-point coefficients of the split path. D_p = dsum_p * bscale / rs2_p restores the unscaled
 // s_p^H G s_p. The weight on the scaled row is c'_p = w_p / (rs2_p * max(D_p, 1e-30)), for valid
 // w (finite, > 0), else 0: the same T as the reference's w_p / max(K_p, 1e-30) on the unscaled
-// row. u_p = sqrt(c'_p) (FP32). The largest u (as float bits, an order-independent max) sets the
-// global 2^g of the T pass.
+// row. u_p = sqrt(c'_p) (FP32). The largest u sets the global 2^g of the T pass (per-block
+// maxima, then one workgroup; no atomics).
 // ---------------------------------------------------------------------------------------------
-__global__ void k_coeffs_split16(const double *__restrict__ w, const double *__restrict__ dsum,
-                                 const float *__restrict__ rs2, const double *__restrict__ bscale,
-                                 int64_t n, float *__restrict__ u,
-                                 unsigned int *__restrict__ umax_bits) {
+__global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict__ w,
+                                                        const double *__restrict__ dsum,
+                                                        const float *__restrict__ rs2,
+                                                        const double *__restrict__ bscale,
+                                                        int64_t n, float *__restrict__ u,
+                                                        float *__restrict__ block_max) {
+  __shared__ float sh[4];
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float uv = 0.f;
   if (p < n) {
@@ -232,52 +317,76 @@ __global__ void k_coeffs_split16(const double *__restrict__ w, const double *__r
     }
     u[p] = uv;
   }
-  // wave max then one atomic per wave
   float mv = uv;
   for (int off = 32; off > 0; off >>= 1) mv = fmaxf(mv, __shfl_xor(mv, off));
-  if ((threadIdx.x & 63) == 0 && mv > 0.f) atomicMax(umax_bits, __float_as_uint(mv));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mv;
+  __syncthreads();
+  if (threadIdx.x == 0) block_max[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
-// zscale[0] = 2^g (float, applied to z), tscale[0] = 4^-g (double, applied to T); resets the max.
-__global__ void k_finalize_scale(unsigned int *__restrict__ umax_bits, float *__restrict__ zscale,
-                                 double *__restrict__ tscale) {
-  float um = __uint_as_float(*umax_bits);
-  int e = 0;
-  if (um > 0.f) frexpf(um, &e);  // um < 2^e
-  int g = 14 - e;                // max |z| < 2^14
-  zscale[0] = ldexpf(1.f, g);
-  tscale[0] = ldexp(1.0, -2 * g);
-  *umax_bits = 0u;
+// Max over the block maxima -> zscale[0] = 2^g (float, applied to z) and tscale[0] = 4^-g
+// (double, applied to T), with g chosen so that max |z| < 2^14.
+__global__ void __launch_bounds__(1024) k_finalize_scale(const float *__restrict__ block_max,
+                                                         int64_t nblk, float *__restrict__ zscale,
+                                                         double *__restrict__ tscale) {
+  __shared__ float sh[1024];
+  float mv = 0.f;
+  for (int64_t i = threadIdx.x; i < nblk; i += 1024) mv = fmaxf(mv, block_max[i]);
+  sh[threadIdx.x] = mv;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int e = 0;
+    if (sh[0] > 0.f) frexpf(sh[0], &e);  // max u < 2^e
+    const int g = 14 - e;
+    zscale[0] = ldexpf(1.f, g);
+    tscale[0] = ldexp(1.0, -2 * g);
+  }
 }
 
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *u, unsigned int *umax_bits,
+                           const double *bscale, int64_t n, float *u, float *block_max,
                            float *zscale, double *tscale) {
+  const int64_t nblk = (n + 255) / 256;
   if (n > 0)
-    hipLaunchKernelGGL(k_coeffs_split16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w,
-                       dsum, rs2, bscale, n, u, umax_bits);
-  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1), 0, st, umax_bits, zscale, tscale);
+    hipLaunchKernelGGL(k_coeffs_split16, dim3((unsigned)nblk), dim3(256), 0, st, w, dsum, rs2,
+                       bscale, n, u, block_max);
+  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1024), 0, st, block_max, nblk, zscale,
+                     tscale);
 }
 
 // ---------------------------------------------------------------------------------------------
 // T pass: lower 32x32 sub-tiles of Z^T Z with z_p = 2^g u_p x_p, over one point chunk per
-// workgroup. A stage of 16 points is staged transposed and split in LDS: zT_hi/zT_lo [M][24]
-// halves (column-major per point, padded), so an MFMA operand fragment (8 consecutive points of
-// one column) is one 16-byte read. Double-buffered. v_mfma_f32_32x32x16_f16 operand layout:
-// A[i][k] at lane i + 32(k/8), element k%8; B[k][j] at lane j + 32(k/8).
-// Output: FP32 chunk partials [chunk][part][wave][t][1024] (MFMA output layout).
+// workgroup. Per stage of 16 points, thread c reads column c's 16-point record of both planes
+// (xt), rebuilds x = hi + lo in FP32, scales by 2^g u_p, re-splits and writes the transposed tile
+// zT_hi/zT_lo [m][24] halves to LDS, so an MFMA operand fragment (8 consecutive points of one
+// column) is one 16-byte read. The next stage's records are loaded into registers while the
+// current stage's MFMAs run (double-buffered LDS).
+// v_mfma_f32_32x32x16_f16 layout: A[i][k] at lane i + 32(k/8), element k%8; B[k][j] at lane
+// j + 32(k/8). Output: FP32 chunk partials [chunk][part][wave][t][1024] (MFMA output layout).
 // ---------------------------------------------------------------------------------------------
 static constexpr int SP_KP = 16;
 static constexpr int SP_LD = 24;
-static constexpr int SP_MAXT = 12;
+static constexpr int SP_NT = 12;     // accumulator tiles per wave
+static constexpr int SP_WAVES = 8;
 
-__global__ void __launch_bounds__(512) k_syrk_split16(const float *__restrict__ x32,
-                                                      const float *__restrict__ u,
-                                                      const float *__restrict__ zscale, int64_t n,
-                                                      int64_t m, int64_t chunk, int n_parts,
-                                                      int max_tiles, int64_t n_chunks,
-                                                      const int16_t *__restrict__ tiles,
-                                                      float *__restrict__ partials) {
+int syrk_split16_tiles_per_wave() { return SP_NT; }
+int syrk_split16_waves() { return SP_WAVES; }
+
+// Every wave reads exactly SP_NT list slots (the host pads each list with repeats of its first
+// tile, whose partial slots the reduction skips), so every loop below is static. One column of
+// the stage per thread: m <= 64 * SP_WAVES.
+__global__ void __launch_bounds__(64 * SP_WAVES) k_syrk_split16(const half8_t *__restrict__ xt,
+                                                                const float *__restrict__ u,
+                                                                const float *__restrict__ zscale,
+                                                                int64_t n, int64_t m,
+                                                                int64_t chunk, int n_parts,
+                                                                int max_tiles, int64_t n_chunks,
+                                                                const int16_t *__restrict__ tiles,
+                                                                float *__restrict__ partials) {
   extern __shared__ _Float16 zt[];  // [2 buffers][2 (hi, lo)][m][SP_LD]
   const int64_t plane = m * SP_LD;
   const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
@@ -287,18 +396,16 @@ __global__ void __launch_bounds__(512) k_syrk_split16(const float *__restrict__ 
   const int tid = threadIdx.x, lane = tid & 63;
   // Wave-uniform index: keeps the tile list in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int16_t *tl = tiles + ((int64_t)(part * 8 + wave) * max_tiles) * 2;
-  int trg[SP_MAXT], tcg[SP_MAXT];
-  int ntile = 0;
+  const int16_t *tl = tiles + ((int64_t)(part * SP_WAVES + wave) * max_tiles) * 2;
+  int roff[SP_NT], coff[SP_NT];
 #pragma unroll
-  for (int t = 0; t < SP_MAXT; t++) {
-    trg[t] = t < max_tiles ? tl[2 * t] : -1;
-    tcg[t] = t < max_tiles ? tl[2 * t + 1] : -1;
-    if (trg[t] >= 0) ntile = t + 1;
+  for (int t = 0; t < SP_NT; t++) {
+    roff[t] = tl[2 * t] * 32 * SP_LD;
+    coff[t] = tl[2 * t + 1] * 32 * SP_LD;
   }
-  float16_t acc[SP_MAXT];
+  float16_t acc[SP_NT];
 #pragma unroll
-  for (int t = 0; t < SP_MAXT; t++)
+  for (int t = 0; t < SP_NT; t++)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[t][e] = 0.f;
 
@@ -306,77 +413,95 @@ __global__ void __launch_bounds__(512) k_syrk_split16(const float *__restrict__ 
   const int64_t p_begin = chunk_id * chunk;
   const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
   const int n_stage = (int)((p_end - p_begin + SP_KP - 1) / SP_KP);
+  const int64_t stage0 = p_begin / SP_KP;
+  const int64_t col = tid;
 
-  auto stage = [&](int s, int buf) {
+  half8_t rec[4];  // hi 0-7, hi 8-15, lo 0-7, lo 8-15 of this thread's column
+  auto fetch = [&](int s) {
+    if (col < m) {
+      const half8_t *src = xt + (((stage0 + s) * m + col) * 2) * 2;
+      rec[0] = src[0];
+      rec[1] = src[1];
+      rec[2] = src[2];
+      rec[3] = src[3];
+    }
+  };
+  // u is read here, not prefetched: the stage's 16 values are wave-uniform scalar loads.
+  auto store = [&](int s, int buf) {
     const int64_t ps = p_begin + (int64_t)s * SP_KP;
-    _Float16 *hi = zt + (int64_t)buf * 2 * plane, *lo = hi + plane;
-    for (int64_t col = tid; col < m; col += 512) {
+    float us[SP_KP];
 #pragma unroll
-      for (int half = 0; half < 2; half++) {
-        half8_t h0, l0;
+    for (int j = 0; j < SP_KP; j++) us[j] = (ps + j < p_end) ? u[ps + j] * zs : 0.f;
+    _Float16 *hi = zt + (int64_t)buf * 2 * plane, *lo = hi + plane;
+    if (col < m) {
+#pragma unroll
+      for (int hf = 0; hf < 2; hf++) {
+        half8_t zh, zl;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          int64_t p = ps + half * 8 + j;
-          float z = 0.f;
-          if (p < p_end) z = x32[p * m + col] * (u[p] * zs);
-          _Float16 hh, ll;
-          split_f32(z, hh, ll);
-          h0[j] = hh;
-          l0[j] = ll;
+          const float x = (float)rec[hf][j] + (float)rec[2 + hf][j];
+          _Float16 a, b;
+          split_f32(x * us[8 * hf + j], a, b);
+          zh[j] = a;
+          zl[j] = b;
         }
-        *reinterpret_cast<half8_t *>(hi + col * SP_LD + 8 * half) = h0;
-        *reinterpret_cast<half8_t *>(lo + col * SP_LD + 8 * half) = l0;
+        *reinterpret_cast<half8_t *>(hi + col * SP_LD + 8 * hf) = zh;
+        *reinterpret_cast<half8_t *>(lo + col * SP_LD + 8 * hf) = zl;
       }
     }
   };
 
-  if (n_stage > 0) stage(0, 0);
+  if (n_stage > 0) {
+    fetch(0);
+    store(0, 0);
+  }
   __syncthreads();
+  const int lofs = (lane & 31) * SP_LD + 8 * (lane >> 5);
   for (int s = 0; s < n_stage; s++) {
     const int buf = s & 1;
-    if (s + 1 < n_stage) stage(s + 1, buf ^ 1);
-    const _Float16 *hi = zt + (int64_t)buf * 2 * plane, *lo = hi + plane;
-    const int kof = 8 * (lane >> 5);
-    int last_rg = -1;
-    half8_t ah = {}, al = {};
+    if (s + 1 < n_stage) fetch(s + 1);
+    const _Float16 *hi = zt + (int64_t)buf * 2 * plane + lofs, *lo = hi + plane;
+    // Tiles in pairs: both tiles' three split products alternate, so consecutive MFMAs write
+    // different accumulators.
 #pragma unroll
-    for (int t = 0; t < SP_MAXT; t++) {
-      if (t < ntile) {
-        if (trg[t] != last_rg) {
-          const int64_t row = (int64_t)trg[t] * 32 + (lane & 31);
-          ah = *reinterpret_cast<const half8_t *>(hi + row * SP_LD + kof);
-          al = *reinterpret_cast<const half8_t *>(lo + row * SP_LD + kof);
-          last_rg = trg[t];
-        }
-        const int64_t colr = (int64_t)tcg[t] * 32 + (lane & 31);
-        half8_t bh = *reinterpret_cast<const half8_t *>(hi + colr * SP_LD + kof);
-        half8_t bl = *reinterpret_cast<const half8_t *>(lo + colr * SP_LD + kof);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[t], 0, 0, 0);
-      }
+    for (int t = 0; t < SP_NT; t += 2) {
+      const int t1 = t + 1;
+      const half8_t ah0 = *reinterpret_cast<const half8_t *>(hi + roff[t]);
+      const half8_t al0 = *reinterpret_cast<const half8_t *>(lo + roff[t]);
+      const half8_t bh0 = *reinterpret_cast<const half8_t *>(hi + coff[t]);
+      const half8_t bl0 = *reinterpret_cast<const half8_t *>(lo + coff[t]);
+      const half8_t ah1 = *reinterpret_cast<const half8_t *>(hi + roff[t1]);
+      const half8_t al1 = *reinterpret_cast<const half8_t *>(lo + roff[t1]);
+      const half8_t bh1 = *reinterpret_cast<const half8_t *>(hi + coff[t1]);
+      const half8_t bl1 = *reinterpret_cast<const half8_t *>(lo + coff[t1]);
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh0, acc[t], 0, 0, 0);
+      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh1, acc[t1], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl0, acc[t], 0, 0, 0);
+      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl1, acc[t1], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh0, acc[t], 0, 0, 0);
+      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh1, acc[t1], 0, 0, 0);
     }
+    if (s + 1 < n_stage) store(s + 1, buf ^ 1);
     __syncthreads();
   }
-  float *out = partials + ((chunk_id * n_parts + part) * 8 + wave) * (int64_t)max_tiles * 1024;
+  float *out =
+      partials + ((chunk_id * n_parts + part) * SP_WAVES + wave) * (int64_t)max_tiles * 1024;
 #pragma unroll
-  for (int t = 0; t < SP_MAXT; t++) {
-    if (t < ntile) {
+  for (int t = 0; t < SP_NT; t++) {
 #pragma unroll
-      for (int e = 0; e < 16; e++) out[t * 1024 + e * 64 + lane] = acc[t][e];
-    }
+    for (int e = 0; e < 16; e++) out[t * 1024 + e * 64 + lane] = acc[t][e];
   }
 }
 
 size_t syrk_split16_lds(int64_t m_cols) { return sizeof(_Float16) * 2 * 2 * m_cols * SP_LD; }
 
-void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const float *x32, const float *u,
+void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const half8_t *xt, const float *u,
                          const float *zscale, int64_t n, float *partials) {
   int64_t n_ch8 = (plan.n_chunks + 7) / 8 * 8;
   int64_t grid = n_ch8 * plan.n_parts;
-  hipLaunchKernelGGL(k_syrk_split16, dim3((unsigned)grid), dim3(512), syrk_split16_lds(plan.m_cols),
-                     st, x32, u, zscale, n, plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles,
-                     plan.n_chunks, plan.d_tiles, partials);
+  hipLaunchKernelGGL(k_syrk_split16, dim3((unsigned)grid), dim3(64 * SP_WAVES),
+                     syrk_split16_lds(plan.m_cols), st, xt, u, zscale, n, plan.m_cols, plan.chunk,
+                     plan.n_parts, plan.max_tiles, plan.n_chunks, plan.d_tiles, partials);
 }
 
 // FP32 chunk partials -> FP64 sums in chunk order, scattered to the full M x M (lower kept on
@@ -393,6 +518,10 @@ __global__ void k_syrk_reduce_f32(const float *__restrict__ partials,
   int el = (int)(idx % 1024);
   int rg = tiles[2 * slot], cg = tiles[2 * slot + 1];
   if (rg < 0) return;
+  // A padded list slot repeats an earlier tile of the same list; its sum is skipped.
+  const int64_t wave_base = (slot / max_tiles) * max_tiles;
+  for (int64_t q = wave_base; q < slot; q++)
+    if (tiles[2 * q] == rg && tiles[2 * q + 1] == cg) return;
   double s = 0.0;
   for (int64_t c = 0; c < n_chunks; c++) s += (double)partials[c * per_chunk + idx];
   int e = el / 64, ln = el % 64;

@@ -101,20 +101,24 @@ struct cy3q_solver {
   uint32_t *d_mono = nullptr;
   double *wvalid = nullptr, *sum_w = nullptr;
   // h state: G = W^H W, W lower triangular
-  double *g = nullptr, *wmat = nullptr, *t = nullptr, *lfac = nullptr, *wnew = nullptr,
-         *gnew = nullptr;
+  double *g = nullptr, *wmat = nullptr, *t = nullptr, *wnew = nullptr, *gnew = nullptr;
   double *bmat = nullptr, *dpart = nullptr, *coef = nullptr, *xx = nullptr, *partials = nullptr;
   double *resid = nullptr, *sig_out = nullptr;
+  // elimination workspace of the N x N solve
+  double *abuf = nullptr, *rbuf = nullptr, *einv = nullptr, *partial = nullptr,
+         *scale_buf = nullptr;
   int32_t *status = nullptr;
   bool bmat_valid = false;
   SyrkPlan plan64;
   int64_t partials_cap = 0;
   // split16 path (allocated on first use)
   bool x32_valid = false, bfrag_valid = false;
-  float *x32 = nullptr, *rs2 = nullptr, *u = nullptr, *zscale = nullptr, *partials32 = nullptr;
+  float *rs2 = nullptr, *u = nullptr, *zscale = nullptr, *partials32 = nullptr,
+        *block_max = nullptr;
+  int *row_ex = nullptr;
+  _Float16 *xd = nullptr, *xt = nullptr;
   double *dsum = nullptr, *bscale = nullptr, *tscale = nullptr;
   half8_t *bfrag = nullptr;
-  unsigned int *umax_bits = nullptr;
   int16_t *cglist = nullptr;
   SyrkPlan plan16;
 
@@ -190,10 +194,13 @@ struct cy3q_solver {
 
 namespace {
 
-// Lower 16x16 sub-tiles of the M x M symmetric product, in row-group-major order, dealt in
-// contiguous runs to (part, wave) slots of at most max_tiles each.
+// Lower sub-tiles of the M x M symmetric product, in row-group-major order, dealt evenly in
+// contiguous runs to (part, wave) slots of at most max_tiles each. With pad_repeat, a short
+// list is padded with repeats of its first tile (for kernels that read a static number of
+// slots; the reduction skips the repeats).
 int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_tiles,
-                    int64_t target_wgs, int64_t min_chunk, int64_t kp) {
+                    int64_t target_wgs, int64_t min_chunk, int64_t kp, bool pad_repeat = false,
+                    int fixed_tiles = 0) {
   pl.m_cols = s->m;
   pl.tile = tile;
   pl.waves = waves;
@@ -204,15 +211,21 @@ int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_t
   const int64_t total = (int64_t)tl.size();
   pl.n_parts = (int)((total + (int64_t)waves * cap_tiles - 1) / ((int64_t)waves * cap_tiles));
   const int64_t slots = (int64_t)pl.n_parts * waves;
-  pl.max_tiles = (int)((total + slots - 1) / slots);
+  pl.max_tiles = fixed_tiles > 0 ? fixed_tiles : (int)((total + slots - 1) / slots);
   std::vector<int16_t> h((size_t)slots * pl.max_tiles * 2, (int16_t)-1);
+  const int64_t base = total / slots, extra = total % slots;
   int64_t idx = 0;
   for (int64_t sl = 0; sl < slots; sl++) {
-    int64_t take = std::min<int64_t>(pl.max_tiles, total - idx);
+    const int64_t take = base + (sl < extra ? 1 : 0);
     for (int64_t t = 0; t < take; t++, idx++) {
       h[(sl * pl.max_tiles + t) * 2] = (int16_t)tl[idx].first;
       h[(sl * pl.max_tiles + t) * 2 + 1] = (int16_t)tl[idx].second;
     }
+    if (pad_repeat && take > 0)
+      for (int64_t t = take; t < pl.max_tiles; t++) {
+        h[(sl * pl.max_tiles + t) * 2] = h[(sl * pl.max_tiles) * 2];
+        h[(sl * pl.max_tiles + t) * 2 + 1] = h[(sl * pl.max_tiles) * 2 + 1];
+      }
   }
   int64_t want = std::max<int64_t>(1, target_wgs / pl.n_parts);
   int64_t by_size = std::max<int64_t>(1, (s->n + min_chunk - 1) / min_chunk);
@@ -258,7 +271,6 @@ int finish_workspace(cy3q_solver *s) {
   ST_TRY(s->alloc(&s->g, 2 * nn));
   ST_TRY(s->alloc(&s->wmat, 2 * nn));
   ST_TRY(s->alloc(&s->t, 2 * nn));
-  ST_TRY(s->alloc(&s->lfac, 2 * nn));
   ST_TRY(s->alloc(&s->wnew, 2 * nn));
   ST_TRY(s->alloc(&s->gnew, 2 * nn));
   ST_TRY(s->alloc(&s->bmat, m * m));
@@ -268,11 +280,18 @@ int finish_workspace(cy3q_solver *s) {
   ST_TRY(s->alloc(&s->resid, 1));
   ST_TRY(s->alloc(&s->sig_out, 8));
   ST_TRY(s->alloc(&s->status, 4));
+  const int nblk = (nb + 31) / 32;
+  ST_TRY(s->alloc(&s->abuf, 2 * nn));
+  ST_TRY(s->alloc(&s->rbuf, 2 * nn));
+  ST_TRY(s->alloc(&s->einv, (size_t)nblk * 32 * 32 * 2));
+  ST_TRY(s->alloc(&s->partial, (size_t)nblk * nblk));
+  ST_TRY(s->alloc(&s->scale_buf, 2));
   launch_identity_cplx(s->st, nb, s->g);
   launch_identity_cplx(s->st, nb, s->wmat);
   launch_identity_cplx(s->st, nb, s->t);
   s->bmat_valid = false;
-  ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, 24, 1024, 256, 16));
+  if (s->m > 1024) return fail(CY3Q_ERR_ARG, "2N > 1024 (k > 9) is not supported");
+  ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, syrk_f64_max_tiles(s->m), 1024, 256, 16));
   s->partials_cap = syrk_partials_count(s->plan64);
   ST_TRY(s->alloc(&s->partials, s->partials_cap));
   HIP_TRY(hipStreamSynchronize(s->st));
@@ -379,17 +398,14 @@ int t_step_fp64(cy3q_solver *s) {
   return CY3Q_OK;
 }
 
-// T from xx, then Cholesky, W = L^-1, G = W^H W, residual, swap.
+// T from xx, then W = L^-1 (T = L L^H), G = W^H W and the residual; W becomes the state.
 int finish_step(cy3q_solver *s, double scale, const double *dscale) {
   const int nb = s->nb;
-  const size_t nn2 = sizeof(double) * 2 * (size_t)nb * nb;
   s->stage_begin(STG_NXN);
-  launch_finish_t(s->st, s->xx, s->m, nb, s->sum_w, 1, scale, dscale, s->t);
-  HIP_TRY(hipMemcpyAsync(s->lfac, s->t, nn2, hipMemcpyDeviceToDevice, s->st));
-  launch_cholesky(s->st, s->lfac, nb, s->status);
-  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
-  launch_gram_from_w(s->st, s->wnew, nb, s->gnew);
-  launch_residual_and_swap(s->st, s->gnew, s->wnew, s->g, nb, s->status, s->resid);
+  launch_t_from_xx(s->st, s->xx, s->m, nb, s->sum_w, scale, dscale, 1, s->scale_buf, s->t,
+                   s->abuf, s->rbuf);
+  launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wnew, nb, s->status);
+  launch_gram_residual(s->st, s->wnew, nb, s->g, s->status, s->partial, s->resid);
   std::swap(s->wmat, s->wnew);
   s->bmat_valid = false;
   s->stage_end(STG_NXN);
@@ -397,46 +413,47 @@ int finish_step(cy3q_solver *s, double scale, const double *dscale) {
   return CY3Q_OK;
 }
 
-// Split path set-up: the row-scaled FP32 copy of the (orthonormalised) sections, the D-pass
-// column-group lists and the T-pass plan.
+// Split path set-up: the static fp16 hi/lo operand planes of the (orthonormalised) sections,
+// the D-pass column-group lists and the T-pass plan.
 int ensure_split16(cy3q_solver *s) {
   const int64_t n = s->n, m = s->m;
   const int n_cg = (int)(m / 16);
-  if (n_cg > 8 * kDpassMaxCg)
-    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= " +
-                                  std::to_string(8 * kDpassMaxCg * 16) + " (use FP64)");
-  if (!s->x32) {
-    ST_TRY(s->alloc(&s->x32, n * m));
+  const int dw = dpass_split16_waves(), dmc = dpass_split16_max_cg();
+  if (n_cg > dw * dmc || m > 64 * syrk_split16_waves())
+    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 512 (N <= 256); use FP64");
+  if (!s->xd) {
+    const int64_t n_pad = (n + 127) / 128 * 128;
+    ST_TRY(s->alloc(&s->xd, n_pad * m * 2));
+    ST_TRY(s->alloc(&s->xt, n_pad * m * 2));
+    ST_TRY(s->alloc(&s->row_ex, n));
     ST_TRY(s->alloc(&s->rs2, n));
     ST_TRY(s->alloc(&s->u, n));
     ST_TRY(s->alloc(&s->dsum, n));
-    ST_TRY(s->alloc(&s->bscale, 1));
+    ST_TRY(s->alloc(&s->block_max, (n + 255) / 256 + 1));
+    ST_TRY(s->alloc(&s->bscale, 2));
     ST_TRY(s->alloc(&s->tscale, 1));
     ST_TRY(s->alloc(&s->zscale, 1));
-    ST_TRY(s->alloc(&s->umax_bits, 1));
-    HIP_TRY(hipMemsetAsync(s->umax_bits, 0, sizeof(unsigned int), s->st));
     ST_TRY(s->alloc(&s->bfrag, (size_t)(m / 32) * n_cg * 2 * 64));
-    // Column groups to waves: longest-processing-time first on the triangular k-step counts.
-    std::vector<int16_t> cl(8 * kDpassMaxCg, (int16_t)-1);
-    std::vector<int> load(8, 0), cnt(8, 0);
-    for (int cg = n_cg - 1; cg >= 0; cg--) {
-      int best = -1;
-      for (int w = 0; w < 8; w++)
-        if (cnt[w] < kDpassMaxCg && (best < 0 || load[w] < load[best])) best = w;
-      cl[best * kDpassMaxCg + cnt[best]] = (int16_t)cg;
-      cnt[best]++;
-      load[best] += cg / 2 + 1;
-    }
+    // Column groups to waves round-robin: at k-step ks the active groups are cg >= 2 ks, a
+    // contiguous range, which round-robin spreads evenly over the waves (each k-step ends at a
+    // barrier, so the per-step balance is what counts).
+    std::vector<int16_t> cl((size_t)dw * dmc, (int16_t)-1);
+    for (int cg = 0; cg < n_cg; cg++) cl[(cg % dw) * dmc + cg / dw] = (int16_t)cg;
     ST_TRY(s->alloc(&s->cglist, cl.size()));
     HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
                            hipMemcpyHostToDevice, s->st));
-    int64_t target = 512;
-    ST_TRY(build_syrk_plan(s, s->plan16, 32, 8, 12, target, 1024, 16));
+    // One workgroup per CU fills the machine; chunks of >= 1024 points. The kernel reads
+    // exactly syrk_split16_tiles_per_wave() slots per wave.
+    const int nt = syrk_split16_tiles_per_wave(), wv = syrk_split16_waves();
+    ST_TRY(build_syrk_plan(s, s->plan16, 32, wv, nt, 256, 1024, 16, true, nt));
+    if (s->plan16.n_parts != 1)
+      return fail(CY3Q_ERR_ARG, "split16 T-pass tiles do not fit one workgroup per chunk");
     ST_TRY(s->alloc(&s->partials32, syrk_partials_count(s->plan16)));
     HIP_TRY(hipStreamSynchronize(s->st));
   }
   if (!s->x32_valid) {
-    launch_row_scale_f32(s->st, s->sv, n, m, s->x32, s->rs2);
+    const int64_t n_pad = (n + 127) / 128 * 128;
+    launch_split_planes(s->st, s->sv, n, n_pad, m, s->row_ex, s->rs2, s->xd, s->xt);
     s->x32_valid = true;
   }
   return CY3Q_OK;
@@ -450,14 +467,15 @@ int t_step_split16(cy3q_solver *s) {
     s->bfrag_valid = true;
   }
   s->stage_begin(STG_DPASS);
-  launch_dpass_split16(s->st, s->x32, n, m, s->bfrag, s->cglist, s->dsum);
+  launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->dsum);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
-  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->umax_bits,
+  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->block_max,
                         s->zscale, s->tscale);
   s->stage_end(STG_COEFF);
   s->stage_begin(STG_TPASS);
-  launch_syrk_split16(s->st, s->plan16, s->x32, s->u, s->zscale, n, s->partials32);
+  launch_syrk_split16(s->st, s->plan16, (const half8_t *)s->xt, s->u, s->zscale, n,
+                      s->partials32);
   s->stage_end(STG_TPASS);
   s->stage_begin(STG_TREDUCE);
   launch_syrk_reduce_f32(s->st, s->plan16, s->partials32, s->xx);
@@ -477,16 +495,15 @@ int one_step(cy3q_solver *s) {
   return CY3Q_OK;
 }
 
-// W from an arbitrary Hermitian PD G: R = chol(G), G^-1 = R^-H R^-1, L = chol(G^-1), W = L^-1.
+// W from an arbitrary Hermitian PD G: W1 = chol(G)^-1, G^-1 = W1^H W1, W = chol(G^-1)^-1,
+// so that W^H W = G.
 int w_from_g(cy3q_solver *s) {
   const int nb = s->nb;
-  const size_t nn2 = sizeof(double) * 2 * (size_t)nb * nb;
-  HIP_TRY(hipMemcpyAsync(s->lfac, s->g, nn2, hipMemcpyDeviceToDevice, s->st));
-  launch_cholesky(s->st, s->lfac, nb, s->status);
-  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
-  launch_gram_from_w(s->st, s->wnew, nb, s->gnew);  // G^-1
-  launch_cholesky(s->st, s->gnew, nb, s->status + 1);
-  launch_tri_inverse(s->st, s->gnew, nb, s->wmat);
+  launch_elim_init(s->st, s->g, nb, s->abuf, s->rbuf);
+  launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wnew, nb, s->status);
+  launch_gram_residual(s->st, s->wnew, nb, s->gnew, s->status, nullptr, nullptr);  // G^-1
+  launch_elim_init(s->st, s->gnew, nb, s->abuf, s->rbuf);
+  launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wmat, nb, s->status + 1);
   s->bmat_valid = false;
   s->bfrag_valid = false;
   int32_t st2[2] = {0, 0};
@@ -634,15 +651,16 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s) {
   // normalisation (compute_fs_gram_matrix, quintic.rs:2957-2995).
   launch_syrk_f64(s->st, s->plan64, s->sv, s->wvalid, n, s->partials);
   launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
-  launch_finish_t(s->st, s->xx, m, nb, s->sum_w, 0, 1.0, nullptr, s->lfac);
-  launch_cholesky(s->st, s->lfac, nb, s->status);
+  launch_t_from_xx(s->st, s->xx, m, nb, s->sum_w, 1.0, nullptr, 0, s->scale_buf, nullptr, s->abuf,
+                   s->rbuf);
+  // Gram = L L^H; W = L^-1 comes out of the same elimination.
+  launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wnew, nb, s->status);
   int32_t stt = 0;
   HIP_TRY(hipMemcpyAsync(&stt, s->status, sizeof(int32_t), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
   if (stt != 0) return fail(CY3Q_ERR_NOT_PD, "FS Gram matrix not positive-definite");
-  // s' = conj(L)^-1 s for the section rows and all derivative rows
+  // s' = conj(L)^-1 s = conj(W) s for the section rows and all derivative rows
   // (apply_inverse_l_to_sections/derivs, quintic.rs:3095-3133), as triangular contractions.
-  launch_tri_inverse(s->st, s->lfac, nb, s->wnew);
   launch_build_bmat(s->st, s->wnew, nb, m, 1, s->bmat);
   double *sv2 = nullptr, *sd2 = nullptr;
   ST_TRY(s->alloc(&sv2, n * m));
