@@ -11,10 +11,11 @@
 // W comes from one blocked elimination of the augmented matrix [T | I] in 32 x 32 complex tiles.
 // For each block column J:
 //   k_elim_panel  (1 workgroup): factors the diagonal tile and inverts it in the same 32-step
-//                 sweep (one barrier per column). It then forms row block J of W,
-//                 W_JK = L_JJ^-1 R_JK for K <= J, where R is the updated right half.
+//                 sweep (one barrier per column): W_JJ = L_JJ^-1.
 //   k_elim_update (one workgroup per tile): L_IJ = A_IJ L_JJ^-H, then A_IK -= L_IJ L_KJ^H for
-//                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J.
+//                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J, where R is the updated right
+//                 half and W_JK = L_JJ^-1 R_JK (K < J) is formed in the same kernel.
+//   k_elim_lastrow: the last block row of W.
 // A non-positive or non-finite pivot sets status = ST_ERR_NOT_PD. Later kernels then return
 // early, and the gram kernel installs the reference's identity fallback (G = W = I).
 // Complex matrices are row-major [n][n] of (re, im).
@@ -33,9 +34,26 @@ __device__ inline Cz cmul(Cz a, Cz b) { return Cz{a.r * b.r - a.i * b.i, a.r * b
 __device__ inline Cz cmulc(Cz a, Cz b) { return Cz{a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }
 
 // Loads tile (r0, c0) of an n x n complex matrix into LDS, zero outside the matrix (or the
-// identity on the diagonal when pad_identity).
+// identity on the diagonal when pad_identity). With 256 threads, all four global loads of a
+// thread are issued before any LDS store, so the tile costs one memory latency.
 __device__ inline void load_tile(Cz (*dst)[TB + 1], const double *__restrict__ src, int n, int r0,
                                  int c0, bool pad_identity, int tid, int nthr) {
+  if (nthr == 256) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u, r = q / TB, c = q % TB;
+      v[u] = double2{(pad_identity && r == c) ? 1.0 : 0.0, 0.0};
+      if (r0 + r < n && c0 + c < n)
+        v[u] = *reinterpret_cast<const double2 *>(src + 2 * ((int64_t)(r0 + r) * n + c0 + c));
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u;
+      dst[q / TB][q % TB] = Cz{v[u].x, v[u].y};
+    }
+    return;
+  }
   for (int q = tid; q < TB * TB; q += nthr) {
     const int r = q / TB, c = q % TB;
     Cz z{(pad_identity && r == c) ? 1.0 : 0.0, 0.0};
@@ -150,7 +168,6 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
   __shared__ Cz d[TB][TB + 1];   // A_JJ, factored in place (lower part)
   __shared__ Cz rr[TB][TB + 1];  // right half of the tile, starts as I
   __shared__ Cz e[TB][TB + 1];   // L_JJ^-1
-  __shared__ Cz t2[TB][TB + 1];  // R_JK tiles for W_JK
   __shared__ int bad;
   const int tid = threadIdx.x;
   const int j0 = jblk * TB;
@@ -169,7 +186,9 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
   __syncthreads();
   // Step j: l_jj = sqrt(d_jj); row j of L^-1 is rr_j / l_jj; for i > j the trailing left part
   // loses d_ij conj(d_kj) / d_jj and the right part loses d_ij rr_j / d_jj. Column j of d and
-  // row j of rr are only read in step j, so one barrier per step suffices.
+  // row j of rr are only read in step j, so one barrier per step suffices. Each thread owns
+  // fixed elements (at most 4 of the 32 x 32 left tile and 4 of the right): it gathers all its
+  // operands first, then writes, so the LDS reads of a step overlap.
   for (int j = 0; j < TB; j++) {
     const double piv = d[j][j].r;
     if (!(piv > 0.0) || !isfinite(piv)) {
@@ -178,21 +197,29 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
     }
     const double inv_l = 1.0 / sqrt(piv), ipiv = 1.0 / piv;
     if (tid <= j) e[j][tid] = Cz{rr[j][tid].r * inv_l, rr[j][tid].i * inv_l};
-    const int m = TB - j - 1;
-    for (int q = tid; q < m * m; q += 256) {
-      const int ii = q / m, kk = q % m;
-      if (kk > ii) continue;
-      const int i = j + 1 + ii, k = j + 1 + kk;
-      const Cz p = cmulc(d[i][j], d[k][j]);
-      d[i][k].r -= p.r * ipiv;
-      d[i][k].i -= p.i * ipiv;
+    Cz dl[4], dj[4], rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u, i = q / TB, k = q % TB;
+      // left: rows i > j, columns j < k <= i; right: rows i > j, columns k <= j
+      dl[u] = d[i][j];
+      dj[u] = d[k][j];
+      rj[u] = rr[j][k];
     }
-    for (int q = tid; q < m * (j + 1); q += 256) {
-      const int ii = q / (j + 1), k = q % (j + 1);
-      const int i = j + 1 + ii;
-      const Cz p = cmul(d[i][j], rr[j][k]);
-      rr[i][k].r -= p.r * ipiv;
-      rr[i][k].i -= p.i * ipiv;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u, i = q / TB, k = q % TB;
+      if (i <= j) continue;
+      if (k > j) {
+        if (k > i) continue;
+        const Cz p = cmulc(dl[u], dj[u]);
+        d[i][k].r -= p.r * ipiv;
+        d[i][k].i -= p.i * ipiv;
+      } else {
+        const Cz p = cmul(dl[u], rj[u]);
+        rr[i][k].r -= p.r * ipiv;
+        rr[i][k].i -= p.i * ipiv;
+      }
     }
     __syncthreads();
   }
@@ -210,26 +237,39 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
     *reinterpret_cast<double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q)) = v;
   }
   store_tile(w, n, j0, j0, e, tid, 256);
-  // W_JK = L_JJ^-1 R_JK for K < J.
-  for (int kb = 0; kb < jblk; kb++) {
-    __syncthreads();
-    load_tile(t2, r, n, j0, kb * TB, false, tid, 256);
-    __syncthreads();
-    for (int q = tid; q < TB * TB; q += 256) {
-      const int i = q / TB, k = q % TB;
-      double sr = 0.0, si = 0.0;
-      for (int mm = 0; mm <= i; mm++) {
-        const Cz p = cmul(e[i][mm], t2[mm][k]);
-        sr += p.r;
-        si += p.i;
+}
+
+// 32 x 32 x 32 tile products with four outputs per thread accumulated together, so the LDS reads
+// of one step serve four independent FMA chains. out[q] for q = tid + 256u, u = 0..3.
+// prod_nn: C = A B over m <= mmax(i, k); prod_nc: C = A conj(B)^T.
+__device__ inline void tile_prod(const Cz (*a)[TB + 1], const Cz (*b)[TB + 1], int tid, int mode,
+                                 double acc[4][2]) {
+  // mode 0: sum_m a[i][m] b[m][k]          (A B)
+  // mode 1: sum_m a[i][m] conj(b[k][m])    (A B^H)
+  // mode 2: sum_{m<=i} a[i][m] b[m][k]     (lower A times B)
+  // mode 3: sum_{m<=k} a[i][m] conj(b[k][m])  (A times upper B^H, B lower)
+  int iu[4], ku[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int q = tid + 256 * u;
+    iu[u] = q / TB;
+    ku[u] = q % TB;
+    acc[u][0] = acc[u][1] = 0.0;
+  }
+  for (int mm = 0; mm < TB; mm++) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = iu[u], k = ku[u];
+      Cz p;
+      if (mode == 0 || mode == 2) {
+        if (mode == 2 && mm > i) continue;
+        p = cmul(a[i][mm], b[mm][k]);
+      } else {
+        if (mode == 3 && mm > k) continue;
+        p = cmulc(a[i][mm], b[k][mm]);
       }
-      const int gi = j0 + i, gk = kb * TB + k;
-      if (gi < n && gk < n) {
-        double2 v;
-        v.x = sr;
-        v.y = si;
-        *reinterpret_cast<double2 *>(w + 2 * ((int64_t)gi * n + gk)) = v;
-      }
+      acc[u][0] += p.r;
+      acc[u][1] += p.i;
     }
   }
 }
@@ -237,22 +277,19 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
 // L_IJ = A_IJ L_JJ^-H into dst (x_ik = sum_{m <= k} a_im conj(e_km)).
 __device__ inline void panel_row(Cz (*dst)[TB + 1], const Cz (*aij)[TB + 1],
                                  const Cz (*e)[TB + 1], int tid) {
-  for (int q = tid; q < TB * TB; q += 256) {
-    const int i = q / TB, k = q % TB;
-    double sr = 0.0, si = 0.0;
-    for (int mm = 0; mm <= k; mm++) {
-      const Cz p = cmulc(aij[i][mm], e[k][mm]);
-      sr += p.r;
-      si += p.i;
-    }
-    dst[i][k] = Cz{sr, si};
+  double acc[4][2];
+  tile_prod(aij, e, tid, 3, acc);
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int q = tid + 256 * u;
+    dst[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
   }
 }
 
 __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
                                                      double *__restrict__ r,
                                                      const double *__restrict__ einv,
-                                                     const double *__restrict__ w, int n, int jblk,
+                                                     double *__restrict__ w, int n, int jblk,
                                                      const int32_t *__restrict__ status) {
   __shared__ Cz e[TB][TB + 1], x[TB][TB + 1], li[TB][TB + 1], y[TB][TB + 1];
   if (*status != ST_OK) return;
@@ -299,39 +336,74 @@ __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
     __syncthreads();
     load_tile(y, a, n, I * TB, K * TB, false, tid, 256);
     __syncthreads();
-    for (int q = tid; q < TB * TB; q += 256) {
-      const int i = q / TB, k = q % TB;
-      if (I == K && k > i) continue;
-      double sr = 0.0, si = 0.0;
-      for (int mm = 0; mm < TB; mm++) {
-        const Cz p = cmulc(li[i][mm], x[k][mm]);
-        sr += p.r;
-        si += p.i;
-      }
-      y[i][k].r -= sr;
-      y[i][k].i -= si;
+    double acc[4][2];
+    tile_prod(li, x, tid, 1, acc);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u;
+      y[q / TB][q % TB].r -= acc[u][0];
+      y[q / TB][q % TB].i -= acc[u][1];
     }
     __syncthreads();
     store_tile(a, n, I * TB, K * TB, y, tid, 256);
   } else {
-    // R_IK -= L_IJ W_JK.
-    load_tile(x, w, n, j0, K * TB, false, tid, 256);
+    // R_IK -= L_IJ W_JK. For K < J, W_JK = L_JJ^-1 R_JK is formed here (R_JK is final after
+    // step J-1); the workgroup with I = J + 1 also stores it. W_JJ came from the panel.
+    if (K < jblk) {
+      load_tile(y, r, n, j0, K * TB, false, tid, 256);
+      __syncthreads();
+      double acc[4][2];
+      tile_prod(e, y, tid, 2, acc);
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = tid + 256 * u;
+        x[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
+      }
+      __syncthreads();
+      if (I == jblk + 1) store_tile(w, n, j0, K * TB, x, tid, 256);
+    } else {
+      load_tile(x, w, n, j0, K * TB, false, tid, 256);
+    }
     load_tile(y, r, n, I * TB, K * TB, false, tid, 256);
     __syncthreads();
-    for (int q = tid; q < TB * TB; q += 256) {
-      const int i = q / TB, k = q % TB;
-      double sr = 0.0, si = 0.0;
-      for (int mm = 0; mm < TB; mm++) {
-        const Cz p = cmul(li[i][mm], x[mm][k]);
-        sr += p.r;
-        si += p.i;
-      }
-      y[i][k].r -= sr;
-      y[i][k].i -= si;
+    double acc[4][2];
+    tile_prod(li, x, tid, 0, acc);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = tid + 256 * u;
+      y[q / TB][q % TB].r -= acc[u][0];
+      y[q / TB][q % TB].i -= acc[u][1];
     }
     __syncthreads();
     store_tile(r, n, I * TB, K * TB, y, tid, 256);
   }
+}
+
+// Last block row of W: W_JK = L_JJ^-1 R_JK for K < J = nblk - 1 (no update kernel follows it).
+__global__ void __launch_bounds__(256) k_elim_lastrow(const double *__restrict__ r,
+                                                      const double *__restrict__ einv,
+                                                      double *__restrict__ w, int n,
+                                                      const int32_t *__restrict__ status) {
+  __shared__ Cz e[TB][TB + 1], y[TB][TB + 1], x[TB][TB + 1];
+  if (*status != ST_OK) return;
+  const int tid = threadIdx.x;
+  const int jblk = (n + TB - 1) / TB - 1, j0 = jblk * TB, K = blockIdx.x;
+  for (int q = tid; q < TB * TB; q += 256) {
+    const double2 v = *reinterpret_cast<const double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q));
+    e[q / TB][q % TB] = Cz{v.x, v.y};
+  }
+  load_tile(y, r, n, j0, K * TB, false, tid, 256);
+  __syncthreads();
+  double acc[4][2];
+  tile_prod(e, y, tid, 2, acc);
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int q = tid + 256 * u;
+    x[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
+  }
+  __syncthreads();
+  store_tile(w, n, j0, K * TB, x, tid, 256);
 }
 
 // Runs the elimination on (a, r) prepared by launch_t_from_xx or launch_elim_init; W -> w.
@@ -346,6 +418,8 @@ void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv,
       hipLaunchKernelGGL(k_elim_update, dim3(tiles), dim3(256), 0, st, a, r, einv, w, nb, j,
                          status);
   }
+  if (nblk > 1)
+    hipLaunchKernelGGL(k_elim_lastrow, dim3(nblk - 1), dim3(256), 0, st, r, einv, w, nb, status);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -372,9 +446,10 @@ __global__ void __launch_bounds__(256) k_gram_tiles(double *__restrict__ w, int 
       load_tile(wi, w, n, J * TB, I * TB, false, tid, 256);
       load_tile(wk, w, n, J * TB, K * TB, false, tid, 256);
       __syncthreads();
-      for (int u = 0; u < 4; u++) {
-        const int q = tid + 256 * u, i = q / TB, k = q % TB;
-        for (int mm = 0; mm < TB; mm++) {
+      for (int mm = 0; mm < TB; mm++) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int q = tid + 256 * u, i = q / TB, k = q % TB;
           const Cz p = cmulc(wk[mm][k], wi[mm][i]);  // conj(W_mi) W_mk
           acc[u][0] += p.r;
           acc[u][1] += p.i;
