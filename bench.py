@@ -92,12 +92,28 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy kernel of this many MiB (3 "
+                         "launches): the known-size reference for scaling rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(the point-sharded multi-GPU design is documented, not built)"}))
         return 2
+
+    if args.copy_probe_mib > 0:
+        import torch
+
+        n_f = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n_f, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n_f, "launches": 3}))
+        return 0
 
     import numpy as np
     import papers_amd as P
@@ -151,13 +167,16 @@ def main() -> int:
 
     flops_contraction = 8.0 * n * N * N  # one complex N_p x N x N contraction, no symmetry credit
     m_cols = (2 * N + 31) // 32 * 32
-    bytes_pass = 4.0 * n * m_cols  # FP32 row-scaled sections read once per pass
+    # Algorithmic HBM bytes per launch: both passes stream the fp16 hi and lo planes of the
+    # row-scaled sections once (2 x 2 bytes per point and real column); the D pass writes one
+    # FP64 D_p per point, the T pass reads one FP32 coefficient per point.
+    bytes_pass = {"d_pass": 4.0 * n * m_cols + 8.0 * n, "t_pass": 4.0 * n * m_cols + 4.0 * n}
     stages = {}
     for name in ("d_pass", "t_pass"):
         ms, cnt = st[name]
         avg = ms / cnt
         tf = flops_contraction / (avg * 1e-3) / 1e12
-        gbs = bytes_pass / (avg * 1e-3) / 1e9
+        gbs = bytes_pass[name] / (avg * 1e-3) / 1e9
         stages[name] = {"avg_ms": avg, "tflops": tf, "gbs": gbs}
     dom = max(stages, key=lambda x: stages[x]["avg_ms"])
     kern = {"d_pass": "k_dpass_split16", "t_pass": "k_syrk_split16"}[dom]
