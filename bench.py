@@ -5,12 +5,13 @@ Workload: Fermat quintic, degree k = 6 with the quotient basis (N = 205), 1,000,
 Shiffman–Zelditch points (seed 42), FS-Gram orthonormalised in FP64, h_0 = I. Both T-operator
 paths run on the same points and the same h_0:
   (1) plain FP64 (FP64 MFMA contractions);
-  (2) split precision: fp16 hi/lo MFMA with FP32 accumulation, chunk sums in FP64.
+  (2) split precision: fp16 hi/lo MFMA with FP32 accumulation, chunk sums in FP64, and one
+      FP64 refinement sweep: the last of the --steps iterations is an FP64 T-operator step.
 Both use the FP64 Cholesky/inversion of T on the GPU.
 A step is one T-operator iteration (D pass, T pass, FP64 N x N solve). `value` is the rate of path (2)
-over exactly --steps iterations with all inputs resident in HBM. The report also gives path (1)'s
-rate, the speed-up, and the relative error of G and of sigma (FP64 sigma on both G) after those
-iterations against path (1).
+over exactly --steps iterations (refinement sweep included) with all inputs resident in HBM. The
+report also gives path (1)'s rate, the speed-up, the split path without the sweep, and the
+relative error of G and of sigma (FP64 sigma on both G) after those iterations against path (1).
 
 The `roofline` object covers the dominant kernel of path (2), timed with HIP events on the
 solver's stream. The `cpu_baseline` object times the oracle (the FP64 restatement of the
@@ -126,8 +127,9 @@ def main() -> int:
     N, n = s.n_basis, s.n_points
     K, W = args.steps, args.warmup
 
-    def run(prec):
+    def run(prec, refine=1):
         s.set_precision(prec)
+        s.set_refine(refine)
         s.h_block = h0
         if W:
             s.donaldson_solve(W, 0.0)
@@ -144,20 +146,27 @@ def main() -> int:
     _log("FP64 path")
     t64, G64, sig64 = run(P.PREC_FP64)
     _log(f"FP64: {t64 / K * 1e3:.3f} ms/iter, sigma {sig64!r}")
-    _log("split16 path")
-    t16, G16, sig16 = run(P.PREC_SPLIT16)
-    _log(f"split16: {t16 / K * 1e3:.3f} ms/iter, sigma {sig16!r}")
+    _log("split16 path + FP64 refinement sweep")
+    t16, G16, sig16 = run(P.PREC_SPLIT16, refine=1)
+    _log(f"split16+refine: {t16 / K * 1e3:.3f} ms/iter, sigma {sig16!r}")
     rel_G = float(np.linalg.norm(G16 - G64) / np.linalg.norm(G64))
     rel_sig = float(abs(sig16 / sig64 - 1.0))
+    _log("split16 path, no refinement")
+    t16n, G16n, sig16n = run(P.PREC_SPLIT16, refine=0)
+    _log(f"split16 only: {t16n / K * 1e3:.3f} ms/iter, sigma {sig16n!r}")
 
     # Kernel-level timing of the split path (HIP events around each stage on the solver's
-    # stream), in a separate run so the events do not perturb the timed region above.
+    # stream), in a separate run so the events do not perturb the timed region above. The
+    # refinement sweep is left out here so that the stage averages are those of the split
+    # kernels alone.
     s.set_precision(P.PREC_SPLIT16)
+    s.set_refine(0)
     s.h_block = h0
     s.set_profiling(True)
     s.donaldson_solve(K, 0.0)
     st = s.stage_times()
     s.set_profiling(False)
+    s.set_refine(1)
     s.set_precision(P.PREC_FP64)
     s.h_block = h0
     s.set_profiling(True)
@@ -209,13 +218,17 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16-split/fp32-acc (fp64 N x N)",
+        "dtype": "fp16-split/fp32-acc (fp64 N x N and refinement sweep)",
         "data": "synthetic: SZ line-intersection sample of the Fermat quintic, LCG seed 42",
         "config": {"workload": f"quintic Donaldson T-operator, k={k} quotient basis N={N}, "
                                f"N_p={n} points, FS-Gram orthonormalised, h0=I",
                    "k": k, "n_basis": N, "n_points": n},
         "fp64_path": {"value": K / t64, "ms_per_step": t64 / K * 1e3},
         "speedup_vs_fp64": t64 / t16,
+        "split16_no_refine": {"value": K / t16n, "ms_per_step": t16n / K * 1e3,
+                              "speedup_vs_fp64": t64 / t16n,
+                              "sigma_rel_err": float(abs(sig16n / sig64 - 1.0)),
+                              "G_rel_err": float(np.linalg.norm(G16n - G64) / np.linalg.norm(G64))},
         "sigma_fp64": sig64,
         "sigma_split16": sig16,
         "sigma_rel_err": rel_sig,

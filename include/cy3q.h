@@ -87,9 +87,17 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s);
 int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual);
 /* QuinticSolver::donaldson_solve (quintic.rs:433-443). With tol <= 0 no early stop is taken
  * and the iterations run back to back with no host synchronisation. residuals: max_iter
- * entries or NULL. */
+ * entries or NULL. In CY3Q_PREC_SPLIT16 the last min(refine, max_iter) iterations are FP64
+ * refinement sweeps (see cy3q_solver_set_refine); with tol > 0 a fast iteration below tol
+ * ends the fast phase early, the refinement sweeps follow, and the solve stops at the first
+ * sweep below tol or after `refine` sweeps. */
 int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
                                 double *residuals);
+/* Number of FP64 T-operator refinement sweeps that end every split-precision donaldson_solve
+ * (default 1; 0 turns refinement off). No counterpart in the reference, which has only the
+ * FP64 path: the sweep re-applies the exact FP64 T-operator to the fast path's metric, so the
+ * returned h and residual are those of one FP64 iteration. */
+int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps);
 /* QuinticSolver::sigma (quintic.rs:423-425, 1301-1499). */
 int cy3q_solver_sigma(cy3q_solver *s, double *sigma);
 /* sigma with its side outputs: kappa, sigma_L2^2 = sum w (eta/kappa-1)^2 / sum w, and the

@@ -92,6 +92,7 @@ struct cy3q_solver {
   int64_t m = 0;  // padded real columns
   double psi = 0.0;
   int precision = CY3Q_PREC_FP64;
+  int64_t refine = 1;  // FP64 sweeps that end a split-precision donaldson_solve
   std::vector<uint32_t> mono;
   std::vector<void *> allocs;
 
@@ -483,8 +484,8 @@ int t_step_split16(cy3q_solver *s) {
   return CY3Q_OK;
 }
 
-int one_step(cy3q_solver *s) {
-  if (s->precision == CY3Q_PREC_FP64) {
+int step_in(cy3q_solver *s, int precision) {
+  if (precision == CY3Q_PREC_FP64) {
     ST_TRY(t_step_fp64(s));
     ST_TRY(finish_step(s, 1.0, nullptr));
   } else {
@@ -494,6 +495,8 @@ int one_step(cy3q_solver *s) {
   s->bfrag_valid = false;
   return CY3Q_OK;
 }
+
+int one_step(cy3q_solver *s) { return step_in(s, s->precision); }
 
 // W from an arbitrary Hermitian PD G: W1 = chol(G)^-1, G^-1 = W1^H W1, W = chol(G^-1)^-1,
 // so that W^H W = G.
@@ -699,26 +702,55 @@ int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual) {
   return CY3Q_OK;
 }
 
+int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (sweeps < 0) return fail(CY3Q_ERR_ARG, "refinement sweeps must be >= 0");
+  s->refine = sweeps;
+  return CY3Q_OK;
+}
+
 int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
                                 double *residuals) {
   if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (max_iter < 0) return fail(CY3Q_ERR_ARG, "max_iter must be >= 0");
   HIP_TRY(hipSetDevice(s->device));
+  // Split precision: the fast iterations run first; the last `refine` iterations (at most
+  // max_iter) are FP64 T-operator sweeps from the fast path's metric.
+  const int64_t n_ref = s->precision == CY3Q_PREC_FP64 ? 0 : std::min(s->refine, max_iter);
+  const int64_t n_fast = max_iter - n_ref;
+  auto prec_of = [&](int64_t it, bool fast_done) {
+    return (it >= n_fast || fast_done) ? CY3Q_PREC_FP64 : s->precision;
+  };
   int64_t done = 0;
   if (tol > 0.0) {
+    // Early stop: a fast iteration below tol ends the fast phase; the refinement sweeps follow
+    // and the solve stops at the first of them below tol.
+    bool fast_done = false;
+    int64_t ref_done = 0;
     for (int64_t it = 0; it < max_iter; it++) {
+      const int prec = prec_of(it, fast_done);
+      ST_TRY(step_in(s, prec));
       double r = 0.0;
-      ST_TRY(cy3q_solver_donaldson_step(s, &r));
+      HIP_TRY(hipMemcpyAsync(&r, s->resid, sizeof(double), hipMemcpyDeviceToHost, s->st));
+      HIP_TRY(hipStreamSynchronize(s->st));
       if (residuals) residuals[it] = r;
       done = it + 1;
-      if (r < tol) break;
+      if (prec != s->precision || s->precision == CY3Q_PREC_FP64) {
+        ref_done++;
+        if (r < tol || (s->precision != CY3Q_PREC_FP64 && ref_done >= n_ref)) break;
+      } else if (r < tol) {
+        fast_done = true;
+        if (n_ref == 0) break;
+      }
     }
+    s->collect();
   } else {
     double *dres = nullptr;
     ST_TRY(s->alloc(&dres, std::max<int64_t>(max_iter, 1)));
     for (int64_t it = 0; it < max_iter; it++) {
       double *keep = s->resid;
       s->resid = dres + it;
-      int rc = one_step(s);
+      int rc = step_in(s, prec_of(it, false));
       s->resid = keep;
       if (rc != CY3Q_OK) return rc;
       done = it + 1;

@@ -65,6 +65,7 @@ def _load():
     L.cy3q_solver_free.restype = None
     L.cy3q_solver_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(u32), C.POINTER(dbl)]
     L.cy3q_solver_set_precision.argtypes = [vp, ip]
+    L.cy3q_solver_set_refine.argtypes = [vp, i64]
     L.cy3q_solver_orthonormalise_fs_gram.argtypes = [vp]
     L.cy3q_solver_donaldson_step.argtypes = [vp, C.POINTER(dbl)]
     L.cy3q_solver_donaldson_solve.argtypes = [vp, i64, dbl, C.POINTER(i64), vp]
@@ -176,6 +177,10 @@ class QuinticSolver:
     # ---- reference methods --------------------------------------------------------------
     def set_precision(self, precision: int):
         _check(_load().cy3q_solver_set_precision(self._h, precision))
+
+    def set_refine(self, sweeps: int):
+        """FP64 T-operator sweeps that end a split-precision donaldson_solve (default 1)."""
+        _check(_load().cy3q_solver_set_refine(self._h, sweeps))
 
     def orthonormalise_basis_fs_gram(self):
         rc = _load().cy3q_solver_orthonormalise_fs_gram(self._h)

@@ -47,6 +47,7 @@ def test_split16_switch_precision_midway(cy3q):
     a.orthonormalise_basis_fs_gram()
     a.donaldson_solve(3, 0.0)
     a.set_precision(cy3q.PREC_SPLIT16)
+    a.set_refine(0)
     a.donaldson_solve(3, 0.0)
     a.set_precision(cy3q.PREC_FP64)
     a.donaldson_solve(2, 0.0)
@@ -54,3 +55,48 @@ def test_split16_switch_precision_midway(cy3q):
     b.orthonormalise_basis_fs_gram()
     b.donaldson_solve(8, 0.0)
     assert _rel(a.h_block, b.h_block) < 1e-6
+
+
+def test_refinement_sweep_is_one_fp64_step(cy3q):
+    # A split-precision solve of K iterations with one refinement sweep equals K - 1 split
+    # iterations followed by one FP64 T-operator step, bit for bit (deterministic reductions).
+    mk = lambda: cy3q.QuinticSolver.new_with_sampler(4, 30000, 5, precision=cy3q.PREC_SPLIT16)
+    a, b = mk(), mk()
+    for s in (a, b):
+        s.orthonormalise_basis_fs_gram()
+    a.set_refine(1)
+    assert a.donaldson_solve(4, 0.0) == 4
+    b.set_refine(0)
+    b.donaldson_solve(3, 0.0)
+    b.set_precision(cy3q.PREC_FP64)
+    r = b.donaldson_step()
+    assert np.array_equal(a.h_block, b.h_block)
+    assert a.donaldson_residuals[-1] == r
+    # With an early-stop tolerance the sweep still ends the solve.
+    c = mk()
+    c.orthonormalise_basis_fs_gram()
+    c.set_refine(1)
+    it = c.donaldson_solve(30, 1e-4)
+    assert 2 <= it < 30
+    assert c.donaldson_residuals[-2] < 1e-4
+
+
+def test_refinement_sweep_improves_fixed_point(cy3q):
+    # After the split iterations have settled, two FP64 sweeps bring G closer to the FP64
+    # iterate of the same index than two more split iterations do.
+    a = cy3q.QuinticSolver.new_with_sampler(5, 60000, 11, quotient=True)
+    a.orthonormalise_basis_fs_gram()
+    a.donaldson_solve(42, 0.0)
+    g42 = a.h_block
+    b = cy3q.QuinticSolver.new_with_sampler(5, 60000, 11, quotient=True, precision=cy3q.PREC_SPLIT16)
+    b.orthonormalise_basis_fs_gram()
+    b.set_refine(0)
+    b.donaldson_solve(40, 0.0)
+    h40 = b.h_block
+    b.donaldson_solve(2, 0.0)
+    e_split = _rel(b.h_block, g42)
+    b.h_block = h40
+    b.set_refine(2)
+    b.donaldson_solve(2, 0.0)
+    e_ref = _rel(b.h_block, g42)
+    assert e_ref < e_split
