@@ -29,7 +29,8 @@ __device__ inline void split_f32(float x, _Float16 &h, _Float16 &l) {
 // (see k_build_bmat): B[2a][2b] = Re W_ba, B[2a+1][2b] = -Im W_ba, B[2a][2b+1] = Im W_ba,
 // B[2a+1][2b+1] = Re W_ba for a <= b. Fragment (ks, cg) holds B[32ks + 8(l/16) + j][16cg + l%16],
 // j = 0..7, for lane l: the B operand layout of v_mfma_f32_16x16x32_f16.
-// Layout: bfrag[((ks * n_cg + cg) * 2 + {0: hi, 1: lo}) * 64 + lane]. bscale[0] = 4^f restores D
+// Layout: bfrag[((ks * n_cg + cg) * 2 + {0: hi, 1: lo}) * 64 + lane], followed by one all-zero
+// fragment pair (index n_ks * n_cg) for the D pass's idle slots. bscale[0] = 4^f restores D
 // from the scaled contraction; bscale[1] = 2^-f is the scale applied to W.
 // ---------------------------------------------------------------------------------------------
 __device__ inline double bmat_entry(const double *__restrict__ w, int nb, int64_t i, int64_t j) {
@@ -68,10 +69,15 @@ __global__ void k_bfrag_fill(const double *__restrict__ w, int nb, int64_t m,
                              const double *__restrict__ bscale, half8_t *__restrict__ bfrag) {
   const int n_ks = (int)(m / 32), n_cg = (int)(m / 16);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n_ks * n_cg * 64) return;
+  if (e >= ((int64_t)n_ks * n_cg + 1) * 64) return;
   const double sc = bscale[1];
   const int lane = (int)(e % 64);
   const int64_t frag = e / 64;
+  if (frag == (int64_t)n_ks * n_cg) {  // the all-zero fragment pair
+    bfrag[(frag * 2 + 0) * 64 + lane] = half8_t{};
+    bfrag[(frag * 2 + 1) * 64 + lane] = half8_t{};
+    return;
+  }
   const int cg = (int)(frag % n_cg), ks = (int)(frag / n_cg);
   half8_t hi, lo;
   for (int j = 0; j < 8; j++) {
@@ -90,7 +96,7 @@ __global__ void k_bfrag_fill(const double *__restrict__ w, int nb, int64_t m,
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale) {
   hipLaunchKernelGGL(k_bfrag_scale, dim3(1), dim3(1024), 0, st, v_cplx, nb, b_scale);
-  const int64_t tot = (m_cols / 32) * (m_cols / 16) * 64;
+  const int64_t tot = ((m_cols / 32) * (m_cols / 16) + 1) * 64;
   hipLaunchKernelGGL(k_bfrag_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, v_cplx,
                      nb, m_cols, b_scale, bfrag);
 }
@@ -152,133 +158,200 @@ void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_
 }
 
 // ---------------------------------------------------------------------------------------------
-// D pass. Workgroup: 128 points (8 point tiles), 8 waves. Wave w owns the column groups
-// cglist[w][..] (round-robin, so the groups still active at each k-step are spread evenly). Per
-// k-step, the 8 A fragment pairs (16 KB) are staged in LDS (double-buffered) and read back
-// conflict-free. The next k-step's A data and the wave's B fragments are loaded into registers
-// while the current k-step's MFMAs run. The three split products of one accumulator are spread
-// out so consecutive MFMAs write different accumulators.
+// D pass. Workgroup: 128 points (8 point tiles), 8 waves. Waves 0..6 compute: wave w owns the
+// column groups cglist[w][..] (round-robin, so the groups still active at each k-step are spread
+// evenly) and keeps their accumulators for all 8 tiles. Wave 7 only moves data: it streams the
+// A fragments (16 KB per k-step) straight from HBM into a DP_RING-slot LDS ring with
+// global_load_lds_dwordx4, DP_RING - 1 k-steps ahead. Its vector-memory counter tracks these
+// copies alone, so it can wait for exactly the slot needed next, while the compute waves' counters
+// track only their B-fragment loads (register prefetch one k-step ahead). One barrier per k-step.
+// The three split products of one accumulator are spread out so consecutive MFMAs write
+// different accumulators.
 // Output: dsum[p] = sum_j Y_pj^2 (FP64), for the row-scaled x and the 2^-f scaled W.
+// Requires xd padded with zeros to a multiple of 128 points.
 // ---------------------------------------------------------------------------------------------
 static constexpr int DP_TILES = 8;
 static constexpr int DP_PTS = 16 * DP_TILES;
 static constexpr int DP_WAVES = 8;
+static constexpr int DP_CW = DP_WAVES - 1;  // compute waves
 static constexpr int DP_MAXCG = 4;
+static constexpr int DP_RING = 4;
+static constexpr int DP_SLOT = DP_TILES * 2 * 64;  // half8 per ring slot (16 KB)
 
-int dpass_split16_waves() { return DP_WAVES; }
+int dpass_split16_waves() { return DP_CW; }
 int dpass_split16_max_cg() { return DP_MAXCG; }
 
+// s_waitcnt with vmcnt = v (0..63) and the other counters at their maxima (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
+#define DP_WAIT_VM(v) \
+  __builtin_amdgcn_s_waitcnt(((v)&15) | (((v) >> 4) << 14) | (7 << 4) | (15 << 8))
+
+
+// Largest number of active column groups (cg >= 2 ks) held by any compute wave at k-step ks,
+// for the round-robin assignment cg -> wave cg % DP_CW.
+__host__ __device__ constexpr int dp_slots(int n_cg, int ks) {
+  int best = 0;
+  for (int w = 0; w < DP_CW; w++) {
+    int cnt = 0;
+    for (int cg = w; cg < n_cg; cg += DP_CW) cnt += cg >= 2 * ks ? 1 : 0;
+    best = cnt > best ? cnt : best;
+  }
+  return best;
+}
+
+// NKS = M / 32 k-steps, a compile-time count: the k-step loops unroll completely, so the
+// compiler's memory-counter waits are exact (no loop-carried merging of pending loads).
+template <int NKS>
 __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *__restrict__ xd,
-                                                                 int64_t n, int64_t m,
+                                                                 int64_t n,
                                                                  const half8_t *__restrict__ bfrag,
                                                                  const int16_t *__restrict__ cglist,
                                                                  double *__restrict__ dsum) {
-  __shared__ half8_t sa[2][DP_TILES * 2 * 64];  // [buf][(rg * 2 + plane) * 64 + lane]
-  __shared__ double red[DP_WAVES][DP_PTS];
+  __shared__ half8_t sa[DP_RING][DP_SLOT];  // slot: [(tile * 2 + plane) * 64 + lane]
+  __shared__ double red[DP_CW][DP_PTS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t tile0 = (int64_t)blockIdx.x * DP_TILES;
-  const int64_t n_tiles = (n + 15) / 16;
-  const int n_cg = (int)(m / 16), n_ks = (int)(m / 32);
-  int cgs[DP_MAXCG];
-  int ncg = 0, ks_end = 0;
+  constexpr int n_ks = NKS, n_cg = 2 * NKS;
+
+  if (wave == DP_CW) {
+    // Data mover. Slot (ks + DP_RING - 1) % DP_RING was last read in k-step ks - 1, which every
+    // compute wave finished before the barrier of k-step ks.
+    const half8_t *src = xd + (tile0 * n_ks * 2) * 64 + lane;
+    auto issue = [&](int ks) {
+      half8_t *slot = sa[ks % DP_RING];
 #pragma unroll
-  for (int c = 0; c < DP_MAXCG; c++) {
-    cgs[c] = cglist[wave * DP_MAXCG + c];
-    if (cgs[c] >= 0) {
-      ncg = c + 1;
-      ks_end = max(ks_end, cgs[c] / 2 + 1);
+      for (int b = 0; b < DP_TILES * 2; b++) {
+        const int rg = b >> 1, pl = b & 1;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(src + (((int64_t)rg * n_ks + ks) * 2 + pl) * 64),
+            (__attribute__((address_space(3))) void *)(slot + b * 64), 16, 0, 0);
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < DP_RING - 1 && u < n_ks; u++) issue(u);
+#pragma unroll
+    for (int ks = 0; ks < n_ks; ks++) {
+      // k-steps already issued after ks: min(ks + DP_RING - 2, n_ks - 1) - ks.
+      const int ahead = min(DP_RING - 2, n_ks - 1 - ks);
+      if (ahead >= 2)
+        DP_WAIT_VM(2 * DP_TILES * 2);
+      else if (ahead == 1)
+        DP_WAIT_VM(DP_TILES * 2);
+      else
+        DP_WAIT_VM(0);
+      // A bare barrier: a workgroup fence here would wait for every copy in flight.
+      __builtin_amdgcn_s_barrier();
+      if (ks + DP_RING - 1 < n_ks) issue(ks + DP_RING - 1);
     }
+    __syncthreads();
+    return;
   }
+
+  // The wave's column groups: round-robin (cg = wave + DP_CW j), ascending and right-aligned in
+  // the DP_MAXCG slots (unused slots first, -1). At k-step ks the groups cg >= 2 ks are active,
+  // always the wave's last slots. Every wave runs the same dp_slots(n_cg, ks) slots (the largest
+  // active count of any wave, a compile-time number), so the k-step body has no branches; a slot
+  // that is idle for this wave multiplies by the all-zero fragment and leaves its sums unchanged.
+  // The list is also passed in (cglist) by the host, which builds the same table.
+  int cgs[DP_MAXCG];
+#pragma unroll
+  for (int c = 0; c < DP_MAXCG; c++) cgs[c] = cglist[wave * DP_MAXCG + c];
   float4_t acc[DP_TILES][DP_MAXCG];
 #pragma unroll
   for (int r = 0; r < DP_TILES; r++)
 #pragma unroll
     for (int c = 0; c < DP_MAXCG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
 
-  // Staging map: 1024 half8 per k-step, two per thread.
-  half8_t stg[2];
-  auto load_a = [&](int ks) {
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-      const int q = tid + 64 * DP_WAVES * r;
-      const int rg = q >> 7, pl = (q >> 6) & 1, ln = q & 63;
-      const int64_t t = tile0 + rg;
-      half8_t v = {};
-      if (t < n_tiles) v = xd[((t * n_ks + ks) * 2 + pl) * 64 + ln];
-      stg[r] = v;
-    }
-  };
-  half8_t bh[DP_MAXCG], bl[DP_MAXCG], nbh[DP_MAXCG], nbl[DP_MAXCG];
+  // Two B register sets, alternating by k-step parity (no register copies, so the compiler's
+  // wait for the set in use leaves the other set's loads in flight).
+  half8_t b0h[DP_MAXCG], b0l[DP_MAXCG], b1h[DP_MAXCG], b1l[DP_MAXCG];
+  constexpr int64_t zero_frag = (int64_t)n_ks * n_cg;
   auto load_b = [&](int ks, half8_t *hh, half8_t *ll) {
 #pragma unroll
     for (int c = 0; c < DP_MAXCG; c++) {
-      if (c < ncg && ks <= cgs[c] / 2) {
-        const int64_t frag = (int64_t)ks * n_cg + cgs[c];
-        hh[c] = bfrag[(frag * 2 + 0) * 64 + lane];
-        ll[c] = bfrag[(frag * 2 + 1) * 64 + lane];
+      if (c < DP_MAXCG - dp_slots(n_cg, ks)) continue;
+      const int64_t frag = cgs[c] >= 2 * ks ? (int64_t)ks * n_cg + cgs[c] : zero_frag;
+      hh[c] = bfrag[(frag * 2 + 0) * 64 + lane];
+      ll[c] = bfrag[(frag * 2 + 1) * 64 + lane];
+    }
+  };
+  // MFMAs with swapped operands: B as the row operand and X as the column operand give the
+  // transpose Y^T, so that element e of lane l is column 16 cg + 4 (l / 16) + e of point
+  // 16 r + l % 16 (the fragment layouts of both operands serve either role).
+  auto step = [&](int ks, const half8_t *bh, const half8_t *bl) {
+    const half8_t *sl = sa[ks % DP_RING];
+    const int c0 = DP_MAXCG - dp_slots(n_cg, ks);
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r += 2) {
+      if (c0 >= DP_MAXCG) break;
+      const half8_t ah0 = sl[(r * 2 + 0) * 64 + lane];
+      const half8_t al0 = sl[(r * 2 + 1) * 64 + lane];
+      const half8_t ah1 = sl[(r * 2 + 2) * 64 + lane];
+      const half8_t al1 = sl[(r * 2 + 3) * 64 + lane];
+#pragma unroll
+      for (int pr = 0; pr < 3; pr++) {
+#pragma unroll
+        for (int c = 0; c < DP_MAXCG; c++) {
+          if (c < c0) continue;
+          const half8_t a0 = pr == 2 ? al0 : ah0, a1 = pr == 2 ? al1 : ah1;
+          const half8_t b = pr == 1 ? bl[c] : bh[c];
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a0, acc[r][c], 0, 0, 0);
+          acc[r + 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a1, acc[r + 1][c], 0, 0, 0);
+        }
       }
     }
   };
-  load_a(0);
-  load_b(0, bh, bl);
-  for (int ks = 0; ks < n_ks; ks++) {
-    const int buf = ks & 1;
-    sa[buf][tid] = stg[0];
-    sa[buf][tid + 64 * DP_WAVES] = stg[1];
+  load_b(0, b0h, b0l);
+#pragma unroll
+  for (int ks = 0; ks < n_ks; ks += 2) {
     __syncthreads();
-    if (ks + 1 < n_ks) {
-      load_a(ks + 1);
-      load_b(ks + 1, nbh, nbl);
-    }
-    if (ks < ks_end) {
-#pragma unroll
-      for (int r = 0; r < DP_TILES; r += 2) {
-        const half8_t ah0 = sa[buf][(r * 2 + 0) * 64 + lane];
-        const half8_t al0 = sa[buf][(r * 2 + 1) * 64 + lane];
-        const half8_t ah1 = sa[buf][(r * 2 + 2) * 64 + lane];
-        const half8_t al1 = sa[buf][(r * 2 + 3) * 64 + lane];
-#pragma unroll
-        for (int pr = 0; pr < 3; pr++) {
-#pragma unroll
-          for (int c = 0; c < DP_MAXCG; c++) {
-            if (c < ncg && ks <= cgs[c] / 2) {
-              const half8_t a0 = pr == 2 ? al0 : ah0, a1 = pr == 2 ? al1 : ah1;
-              const half8_t b = pr == 1 ? bl[c] : bh[c];
-              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b, acc[r][c], 0, 0, 0);
-              acc[r + 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b, acc[r + 1][c], 0, 0, 0);
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < DP_MAXCG; c++) {
-      bh[c] = nbh[c];
-      bl[c] = nbl[c];
-    }
+    if (ks + 1 < n_ks) load_b(ks + 1, b1h, b1l);
+    step(ks, b0h, b0l);
+    if (ks + 1 >= n_ks) break;
+    __syncthreads();
+    if (ks + 2 < n_ks) load_b(ks + 2, b0h, b0l);
+    step(ks + 1, b1h, b1l);
   }
-  // Row sums of squares: element e of lane l is point 16r + 4(l/16) + e, column 16cg + l%16.
+  // Row sums of squares (FP64). Lane l holds, for tile r, point 16 r + l % 16 and columns
+  // 4 (l / 16) + e of every slot (inactive slots are still zero): sum them in the lane, then over
+  // the four 16-lane rows in two halving exchanges (lanes l ^ 32, then l ^ 16), after which lane
+  // l owns the complete sums of tiles 4 (l / 32) + 2 ((l / 16) % 2) + {0, 1}.
+  double part[DP_TILES];
 #pragma unroll
   for (int r = 0; r < DP_TILES; r++) {
+    double sacc = 0.0;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      double sacc = 0.0;
+    for (int c = 0; c < DP_MAXCG; c++)
 #pragma unroll
-      for (int c = 0; c < DP_MAXCG; c++) {
-        if (c < ncg) {
-          double y = (double)acc[r][c][e];
-          sacc += y * y;
-        }
+      for (int e = 0; e < 4; e++) {
+        const double y = (double)acc[r][c][e];
+        sacc += y * y;
       }
-      for (int off = 1; off < 16; off <<= 1) sacc += __shfl_xor(sacc, off);
-      if ((lane & 15) == 0) red[wave][r * 16 + 4 * (lane >> 4) + e] = sacc;
-    }
+    part[r] = sacc;
   }
+  const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
+  double h4[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double send = up32 ? part[i] : part[4 + i];
+    const double keep = up32 ? part[4 + i] : part[i];
+    h4[i] = keep + __shfl_xor(send, 32);
+  }
+  double h2[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const double send = up16 ? h4[i] : h4[2 + i];
+    const double keep = up16 ? h4[2 + i] : h4[i];
+    h2[i] = keep + __shfl_xor(send, 16);
+  }
+  const int rbase = 4 * (lane >> 5) + 2 * ((lane >> 4) & 1);
+  red[wave][(rbase + 0) * 16 + (lane & 15)] = h2[0];
+  red[wave][(rbase + 1) * 16 + (lane & 15)] = h2[1];
   __syncthreads();
   if (tid < DP_PTS) {
     double sacc = 0.0;
-    for (int w = 0; w < DP_WAVES; w++) sacc += red[w][tid];
+    for (int w = 0; w < DP_CW; w++) sacc += red[w][tid];
     const int64_t p = tile0 * 16 + tid;
     if (p < n) dsum[p] = sacc;
   }
@@ -287,8 +360,17 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, double *dsum) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_dpass_split16, dim3((unsigned)((n + DP_PTS - 1) / DP_PTS)),
-                     dim3(64 * DP_WAVES), 0, st, xd, n, m_cols, bfrag, cglist, dsum);
+  const dim3 grid((unsigned)((n + DP_PTS - 1) / DP_PTS)), block(64 * DP_WAVES);
+  switch (m_cols / 32) {
+#define DP_CASE(k)                                                                       \
+  case k:                                                                                \
+    hipLaunchKernelGGL(k_dpass_split16<k>, grid, block, 0, st, xd, n, bfrag, cglist, dsum); \
+    break;
+    DP_CASE(1) DP_CASE(2) DP_CASE(3) DP_CASE(4) DP_CASE(5) DP_CASE(6) DP_CASE(7)
+    DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(13) DP_CASE(14)
+#undef DP_CASE
+    default: break;  // rejected by the host (M <= 448)
+  }
 }
 
 // ---------------------------------------------------------------------------------------------

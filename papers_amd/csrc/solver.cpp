@@ -421,7 +421,7 @@ int ensure_split16(cy3q_solver *s) {
   const int n_cg = (int)(m / 16);
   const int dw = dpass_split16_waves(), dmc = dpass_split16_max_cg();
   if (n_cg > dw * dmc || m > 64 * syrk_split16_waves())
-    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 512 (N <= 256); use FP64");
+    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 448 (N <= 224); use FP64");
   if (!s->xd) {
     const int64_t n_pad = (n + 127) / 128 * 128;
     ST_TRY(s->alloc(&s->xd, n_pad * m * 2));
@@ -434,12 +434,16 @@ int ensure_split16(cy3q_solver *s) {
     ST_TRY(s->alloc(&s->bscale, 2));
     ST_TRY(s->alloc(&s->tscale, 1));
     ST_TRY(s->alloc(&s->zscale, 1));
-    ST_TRY(s->alloc(&s->bfrag, (size_t)(m / 32) * n_cg * 2 * 64));
+    ST_TRY(s->alloc(&s->bfrag, ((size_t)(m / 32) * n_cg + 1) * 2 * 64));
     // Column groups to waves round-robin: at k-step ks the active groups are cg >= 2 ks, a
     // contiguous range, which round-robin spreads evenly over the waves (each k-step ends at a
-    // barrier, so the per-step balance is what counts).
+    // barrier, so the per-step balance is what counts). Each wave's list is ascending and
+    // right-aligned, unused slots first (-1), so its active groups are always its last slots.
     std::vector<int16_t> cl((size_t)dw * dmc, (int16_t)-1);
-    for (int cg = 0; cg < n_cg; cg++) cl[(cg % dw) * dmc + cg / dw] = (int16_t)cg;
+    for (int w = 0; w < dw; w++) {
+      const int cnt = (n_cg - w + dw - 1) / dw;
+      for (int j = 0; j < cnt; j++) cl[(size_t)w * dmc + (dmc - cnt) + j] = (int16_t)(w + j * dw);
+    }
     ST_TRY(s->alloc(&s->cglist, cl.size()));
     HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
                            hipMemcpyHostToDevice, s->st));
