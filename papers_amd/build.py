@@ -30,9 +30,10 @@ SOURCES = {
     "k_dense.hip": [],
     "k_hsolve.hip": [],
     "k_sigma.hip": ["-ffp-contract=off"],
+    "k_split16.hip": [],
+    "k_tpass16.hip": [],
     "solver.cpp": ["-x", "hip"],
 }
-OPTIONAL = ["k_split16.hip", "k_adjoint.hip", "k_galerkin.hip"]
 
 
 def _compile(src: str, extra: list[str]) -> tuple[str, str]:
@@ -47,9 +48,6 @@ def _compile(src: str, extra: list[str]) -> tuple[str, str]:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = dict(SOURCES)
-    for s in OPTIONAL:
-        if os.path.exists(os.path.join(CSRC, s)):
-            srcs[s] = []
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         futs = [ex.submit(_compile, s, e) for s, e in srcs.items()]
         objs = []

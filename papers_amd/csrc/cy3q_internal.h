@@ -78,14 +78,15 @@ int dpass_split16_max_cg();
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, double *dsum);
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *u, float *block_max,
-                           float *zscale, double *tscale);
-int syrk_split16_tiles_per_wave();
-int syrk_split16_waves();
-void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const half8_t *xt, const float *u,
-                         const float *zscale, int64_t n, float *partials);
-void launch_syrk_reduce_f32(hipStream_t st, const SyrkPlan &plan, const float *partials,
-                            double *xx_full);
+                           const double *bscale, int64_t n, float *c, float *block_max,
+                           float *cscale, double *tscale);
+int tpass16_max_tiles();
+size_t tpass16_partials(int64_t m_cols, int64_t n_chunks);
+size_t tpass16_seg_count(int64_t m_cols, int64_t n_chunks);
+void launch_tpass16(hipStream_t st, const _Float16 *xt, const float *cvec, int64_t n,
+                    int64_t m_cols, int64_t chunk, int64_t n_chunks, float *partials);
+void launch_tpass16_reduce(hipStream_t st, const float *partials, int64_t m_cols,
+                           int64_t n_chunks, double *seg, double *xx);
 
 // Per-point coefficients, reductions, B operand, layout conversions (k_dense.hip).
 void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpart, int n_parts,

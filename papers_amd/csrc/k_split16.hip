@@ -7,14 +7,13 @@
 //   * every section row is stored scaled by 2^-e_p (max |x| in [0.5, 1)) and split into fp16
 //     hi/lo planes once per workspace;
 //   * W (the D-pass operand) is scaled globally by 2^-f;
-//   * z_p = sqrt(c_p) x_p (the T-pass operand) is scaled globally by 2^g.
+//   * c_p (the T-pass coefficient) is scaled globally by 2^g.
 // The host-side bookkeeping undoes each scaling exactly. D_p and the chunk sums are accumulated in
 // FP64. T is summed across point chunks in FP64, in a fixed order.
 //
 // D pass  (k_dpass_split16): v_mfma_f32_16x16x32_f16, 128 points x all M columns per workgroup,
 //                            so D_p = sum_j Y_pj^2 completes inside the workgroup.
-// T pass  (k_syrk_split16):  v_mfma_f32_32x32x16_f16 on the lower 32x32 sub-tiles of Z^T Z,
-//                            one workgroup per point chunk.
+// T pass: k_tpass16.hip.
 #include "cy3q_internal.h"
 
 namespace cy3q {
@@ -44,12 +43,16 @@ __device__ inline double bmat_entry(const double *__restrict__ w, int nb, int64_
   return vr;
 }
 
-// Scale: bscale[0] = 4^f and bscale[1] = 2^-f with 2^f > max |W| (one workgroup).
+// Scale: bscale[0] = 4^f and bscale[1] = 2^-f with 2^f > max |W| (one workgroup). W is lower
+// triangular and only its lower triangle (row >= column) is defined; the rest is not read.
 __global__ void __launch_bounds__(1024) k_bfrag_scale(const double *__restrict__ w, int nb,
                                                       double *__restrict__ bscale) {
   __shared__ double sh[1024];
   double mx = 0.0;
-  for (int64_t e = threadIdx.x; e < 2 * (int64_t)nb * nb; e += 1024) mx = fmax(mx, fabs(w[e]));
+  for (int64_t e = threadIdx.x; e < (int64_t)nb * nb; e += 1024) {
+    const int64_t row = e / nb, col = e % nb;
+    if (col <= row) mx = fmax(mx, fmax(fabs(w[2 * e]), fabs(w[2 * e + 1])));
+  }
   sh[threadIdx.x] = mx;
   __syncthreads();
   for (int off = 512; off > 0; off >>= 1) {
@@ -377,8 +380,8 @@ void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t 
 // Per-point coefficients of the split path. D_p = dsum_p * bscale / rs2_p restores the unscaled
 // s_p^H G s_p. The weight on the scaled row is c'_p = w_p / (rs2_p * max(D_p, 1e-30)), for valid
 // w (finite, > 0), else 0: the same T as the reference's w_p / max(K_p, 1e-30) on the unscaled
-// row. u_p = sqrt(c'_p) (FP32). The largest u sets the global 2^g of the T pass (per-block
-// maxima, then one workgroup; no atomics).
+// row (quintic.rs:1761-1775). c'_p is stored in FP32; the largest one sets the global 2^g of the
+// T pass (per-block maxima, then one workgroup; no atomics).
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict__ w,
                                                         const double *__restrict__ dsum,
@@ -395,7 +398,7 @@ __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict
       double r = (double)rs2[p];
       double d = dsum[p] * bscale[0] / r;
       double c = wp / (r * fmax(d, 1e-30));
-      uv = (float)sqrt(c);
+      uv = (float)c;
     }
     u[p] = uv;
   }
@@ -406,10 +409,11 @@ __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict
   if (threadIdx.x == 0) block_max[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
-// Max over the block maxima -> zscale[0] = 2^g (float, applied to z) and tscale[0] = 4^-g
-// (double, applied to T), with g chosen so that max |z| < 2^14.
+// Max over the block maxima -> cscale[0] = 2^g (float, applied to c'), tscale[0] = 2^-g
+// (double, applied to T), with g chosen so that max c' 2^g < 2^14; the row-scaled |x| < 1, so
+// every B operand of the T pass stays below 2^14.
 __global__ void __launch_bounds__(1024) k_finalize_scale(const float *__restrict__ block_max,
-                                                         int64_t nblk, float *__restrict__ zscale,
+                                                         int64_t nblk, float *__restrict__ cscale,
                                                          double *__restrict__ tscale) {
   __shared__ float sh[1024];
   float mv = 0.f;
@@ -422,204 +426,32 @@ __global__ void __launch_bounds__(1024) k_finalize_scale(const float *__restrict
   }
   if (threadIdx.x == 0) {
     int e = 0;
-    if (sh[0] > 0.f) frexpf(sh[0], &e);  // max u < 2^e
+    if (sh[0] > 0.f) frexpf(sh[0], &e);  // max c' < 2^e
     const int g = 14 - e;
-    zscale[0] = ldexpf(1.f, g);
-    tscale[0] = ldexp(1.0, -2 * g);
+    cscale[0] = ldexpf(1.f, g);
+    tscale[0] = ldexp(1.0, -g);
   }
+}
+
+// c'' = c' 2^g, in place.
+__global__ void __launch_bounds__(256) k_apply_cscale(float *__restrict__ c,
+                                                      const float *__restrict__ cscale,
+                                                      int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) c[p] *= cscale[0];
 }
 
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *u, float *block_max,
-                           float *zscale, double *tscale) {
+                           const double *bscale, int64_t n, float *c, float *block_max,
+                           float *cscale, double *tscale) {
   const int64_t nblk = (n + 255) / 256;
   if (n > 0)
     hipLaunchKernelGGL(k_coeffs_split16, dim3((unsigned)nblk), dim3(256), 0, st, w, dsum, rs2,
-                       bscale, n, u, block_max);
-  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1024), 0, st, block_max, nblk, zscale,
+                       bscale, n, c, block_max);
+  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1024), 0, st, block_max, nblk, cscale,
                      tscale);
-}
-
-// ---------------------------------------------------------------------------------------------
-// T pass: lower 32x32 sub-tiles of Z^T Z with z_p = 2^g u_p x_p, over one point chunk per
-// workgroup. Per stage of 16 points, thread c reads column c's 16-point record of both planes
-// (xt), rebuilds x = hi + lo in FP32, scales by 2^g u_p, re-splits and writes the transposed tile
-// zT_hi/zT_lo [m][24] halves to LDS, so an MFMA operand fragment (8 consecutive points of one
-// column) is one 16-byte read. The next stage's records are loaded into registers while the
-// current stage's MFMAs run (double-buffered LDS).
-// v_mfma_f32_32x32x16_f16 layout: A[i][k] at lane i + 32(k/8), element k%8; B[k][j] at lane
-// j + 32(k/8). Output: FP32 chunk partials [chunk][part][wave][t][1024] (MFMA output layout).
-// ---------------------------------------------------------------------------------------------
-static constexpr int SP_KP = 16;
-static constexpr int SP_LD = 24;
-static constexpr int SP_NT = 12;     // accumulator tiles per wave
-static constexpr int SP_WAVES = 8;
-
-int syrk_split16_tiles_per_wave() { return SP_NT; }
-int syrk_split16_waves() { return SP_WAVES; }
-
-// Every wave reads exactly SP_NT list slots (the host pads each list with repeats of its first
-// tile, whose partial slots the reduction skips), so every loop below is static. One column of
-// the stage per thread: m <= 64 * SP_WAVES.
-__global__ void __launch_bounds__(64 * SP_WAVES) k_syrk_split16(const half8_t *__restrict__ xt,
-                                                                const float *__restrict__ u,
-                                                                const float *__restrict__ zscale,
-                                                                int64_t n, int64_t m,
-                                                                int64_t chunk, int n_parts,
-                                                                int max_tiles, int64_t n_chunks,
-                                                                const int16_t *__restrict__ tiles,
-                                                                float *__restrict__ partials) {
-  extern __shared__ _Float16 zt[];  // [2 buffers][2 (hi, lo)][m][SP_LD]
-  const int64_t plane = m * SP_LD;
-  const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
-  const int part = (int)(qq % n_parts);
-  const int64_t chunk_id = (qq / n_parts) * 8 + xcd;
-  if (chunk_id >= n_chunks) return;
-  const int tid = threadIdx.x, lane = tid & 63;
-  // Wave-uniform index: keeps the tile list in scalar registers.
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int16_t *tl = tiles + ((int64_t)(part * SP_WAVES + wave) * max_tiles) * 2;
-  int roff[SP_NT], coff[SP_NT];
-#pragma unroll
-  for (int t = 0; t < SP_NT; t++) {
-    roff[t] = tl[2 * t] * 32 * SP_LD;
-    coff[t] = tl[2 * t + 1] * 32 * SP_LD;
-  }
-  float16_t acc[SP_NT];
-#pragma unroll
-  for (int t = 0; t < SP_NT; t++)
-#pragma unroll
-    for (int e = 0; e < 16; e++) acc[t][e] = 0.f;
-
-  const float zs = zscale[0];
-  const int64_t p_begin = chunk_id * chunk;
-  const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
-  const int n_stage = (int)((p_end - p_begin + SP_KP - 1) / SP_KP);
-  const int64_t stage0 = p_begin / SP_KP;
-  const int64_t col = tid;
-
-  half8_t rec[4];  // hi 0-7, hi 8-15, lo 0-7, lo 8-15 of this thread's column
-  auto fetch = [&](int s) {
-    if (col < m) {
-      const half8_t *src = xt + (((stage0 + s) * m + col) * 2) * 2;
-      rec[0] = src[0];
-      rec[1] = src[1];
-      rec[2] = src[2];
-      rec[3] = src[3];
-    }
-  };
-  // u is read here, not prefetched: the stage's 16 values are wave-uniform scalar loads.
-  auto store = [&](int s, int buf) {
-    const int64_t ps = p_begin + (int64_t)s * SP_KP;
-    float us[SP_KP];
-#pragma unroll
-    for (int j = 0; j < SP_KP; j++) us[j] = (ps + j < p_end) ? u[ps + j] * zs : 0.f;
-    _Float16 *hi = zt + (int64_t)buf * 2 * plane, *lo = hi + plane;
-    if (col < m) {
-#pragma unroll
-      for (int hf = 0; hf < 2; hf++) {
-        half8_t zh, zl;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          const float x = (float)rec[hf][j] + (float)rec[2 + hf][j];
-          _Float16 a, b;
-          split_f32(x * us[8 * hf + j], a, b);
-          zh[j] = a;
-          zl[j] = b;
-        }
-        *reinterpret_cast<half8_t *>(hi + col * SP_LD + 8 * hf) = zh;
-        *reinterpret_cast<half8_t *>(lo + col * SP_LD + 8 * hf) = zl;
-      }
-    }
-  };
-
-  if (n_stage > 0) {
-    fetch(0);
-    store(0, 0);
-  }
-  __syncthreads();
-  const int lofs = (lane & 31) * SP_LD + 8 * (lane >> 5);
-  for (int s = 0; s < n_stage; s++) {
-    const int buf = s & 1;
-    if (s + 1 < n_stage) fetch(s + 1);
-    const _Float16 *hi = zt + (int64_t)buf * 2 * plane + lofs, *lo = hi + plane;
-    // Tiles in pairs: both tiles' three split products alternate, so consecutive MFMAs write
-    // different accumulators.
-#pragma unroll
-    for (int t = 0; t < SP_NT; t += 2) {
-      const int t1 = t + 1;
-      const half8_t ah0 = *reinterpret_cast<const half8_t *>(hi + roff[t]);
-      const half8_t al0 = *reinterpret_cast<const half8_t *>(lo + roff[t]);
-      const half8_t bh0 = *reinterpret_cast<const half8_t *>(hi + coff[t]);
-      const half8_t bl0 = *reinterpret_cast<const half8_t *>(lo + coff[t]);
-      const half8_t ah1 = *reinterpret_cast<const half8_t *>(hi + roff[t1]);
-      const half8_t al1 = *reinterpret_cast<const half8_t *>(lo + roff[t1]);
-      const half8_t bh1 = *reinterpret_cast<const half8_t *>(hi + coff[t1]);
-      const half8_t bl1 = *reinterpret_cast<const half8_t *>(lo + coff[t1]);
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh0, acc[t], 0, 0, 0);
-      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh1, acc[t1], 0, 0, 0);
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl0, acc[t], 0, 0, 0);
-      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl1, acc[t1], 0, 0, 0);
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh0, acc[t], 0, 0, 0);
-      acc[t1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh1, acc[t1], 0, 0, 0);
-    }
-    if (s + 1 < n_stage) store(s + 1, buf ^ 1);
-    __syncthreads();
-  }
-  float *out =
-      partials + ((chunk_id * n_parts + part) * SP_WAVES + wave) * (int64_t)max_tiles * 1024;
-#pragma unroll
-  for (int t = 0; t < SP_NT; t++) {
-#pragma unroll
-    for (int e = 0; e < 16; e++) out[t * 1024 + e * 64 + lane] = acc[t][e];
-  }
-}
-
-size_t syrk_split16_lds(int64_t m_cols) { return sizeof(_Float16) * 2 * 2 * m_cols * SP_LD; }
-
-void launch_syrk_split16(hipStream_t st, const SyrkPlan &plan, const half8_t *xt, const float *u,
-                         const float *zscale, int64_t n, float *partials) {
-  int64_t n_ch8 = (plan.n_chunks + 7) / 8 * 8;
-  int64_t grid = n_ch8 * plan.n_parts;
-  hipLaunchKernelGGL(k_syrk_split16, dim3((unsigned)grid), dim3(64 * SP_WAVES),
-                     syrk_split16_lds(plan.m_cols), st, xt, u, zscale, n, plan.m_cols, plan.chunk,
-                     plan.n_parts, plan.max_tiles, plan.n_chunks, plan.d_tiles, partials);
-}
-
-// FP32 chunk partials -> FP64 sums in chunk order, scattered to the full M x M (lower kept on
-// diagonal sub-tiles). 32x32 MFMA output layout: element e of lane l is
-// (row 8(e/4) + 4(l/32) + e%4, col l%32).
-__global__ void k_syrk_reduce_f32(const float *__restrict__ partials,
-                                  const int16_t *__restrict__ tiles, int n_parts, int waves,
-                                  int max_tiles, int64_t n_chunks, int64_t m,
-                                  double *__restrict__ xx) {
-  const int64_t per_chunk = (int64_t)n_parts * waves * max_tiles * 1024;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= per_chunk) return;
-  int64_t slot = idx / 1024;
-  int el = (int)(idx % 1024);
-  int rg = tiles[2 * slot], cg = tiles[2 * slot + 1];
-  if (rg < 0) return;
-  // A padded list slot repeats an earlier tile of the same list; its sum is skipped.
-  const int64_t wave_base = (slot / max_tiles) * max_tiles;
-  for (int64_t q = wave_base; q < slot; q++)
-    if (tiles[2 * q] == rg && tiles[2 * q + 1] == cg) return;
-  double s = 0.0;
-  for (int64_t c = 0; c < n_chunks; c++) s += (double)partials[c * per_chunk + idx];
-  int e = el / 64, ln = el % 64;
-  int r = 8 * (e / 4) + 4 * (ln / 32) + (e % 4), col = ln % 32;
-  if (rg == cg && r < col) return;
-  int64_t gi = (int64_t)rg * 32 + r, gj = (int64_t)cg * 32 + col;
-  xx[gi * m + gj] = s;
-  xx[gj * m + gi] = s;
-}
-
-void launch_syrk_reduce_f32(hipStream_t st, const SyrkPlan &plan, const float *partials,
-                            double *xx_full) {
-  int64_t per_chunk = (int64_t)plan.n_parts * plan.waves * plan.max_tiles * 1024;
-  hipLaunchKernelGGL(k_syrk_reduce_f32, dim3((unsigned)((per_chunk + 255) / 256)), dim3(256), 0,
-                     st, partials, plan.d_tiles, plan.n_parts, plan.waves, plan.max_tiles,
-                     plan.n_chunks, plan.m_cols, xx_full);
+  if (n > 0)
+    hipLaunchKernelGGL(k_apply_cscale, dim3((unsigned)nblk), dim3(256), 0, st, c, cscale, n);
 }
 
 }  // namespace cy3q

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +23,14 @@ using namespace cy3q;
 namespace {
 
 thread_local std::string g_last_error;
+
+int poison_allocs() {
+  static const int mode = [] {
+    const char *v = std::getenv("CY3Q_POISON");
+    return (v && (v[0] == '1' || v[0] == '2')) ? v[0] - '0' : 0;
+  }();
+  return mode;
+}
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -114,14 +123,15 @@ struct cy3q_solver {
   int64_t partials_cap = 0;
   // split16 path (allocated on first use)
   bool x32_valid = false, bfrag_valid = false;
-  float *rs2 = nullptr, *u = nullptr, *zscale = nullptr, *partials32 = nullptr,
+  float *rs2 = nullptr, *cvec = nullptr, *cscale = nullptr, *partials32 = nullptr,
         *block_max = nullptr;
+  double *tseg = nullptr;            // T-pass segment sums
+  int64_t tp_chunk = 0, tp_chunks = 0;
   int *row_ex = nullptr;
   _Float16 *xd = nullptr, *xt = nullptr;
   double *dsum = nullptr, *bscale = nullptr, *tscale = nullptr;
   half8_t *bfrag = nullptr;
   int16_t *cglist = nullptr;
-  SyrkPlan plan16;
 
   // profiling
   bool profiling = false;
@@ -139,7 +149,6 @@ struct cy3q_solver {
     }
     for (void *p : allocs) hipFree(p);
     if (plan64.d_tiles) hipFree(plan64.d_tiles);
-    if (plan16.d_tiles) hipFree(plan16.d_tiles);
     if (st) hipStreamDestroy(st);
   }
 
@@ -150,6 +159,13 @@ struct cy3q_solver {
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess)
       return fail(CY3Q_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    // CY3Q_POISON=1 fills every new buffer with 0xFF bytes (NaN in f64, f32 and f16), =2 with
+    // 0x7F bytes (huge finite f64 and f32), so that a read before the first write shows up in
+    // the results, also through max-reductions that skip NaN (a checking aid).
+    if (const int pv = poison_allocs()) {
+      e = hipMemsetAsync(q, pv == 2 ? 0x7F : 0xFF, bytes, st);
+      if (e != hipSuccess) return fail(CY3Q_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    }
     allocs.push_back(q);
     *p = (T *)q;
     return CY3Q_OK;
@@ -420,20 +436,22 @@ int ensure_split16(cy3q_solver *s) {
   const int64_t n = s->n, m = s->m;
   const int n_cg = (int)(m / 16);
   const int dw = dpass_split16_waves(), dmc = dpass_split16_max_cg();
-  if (n_cg > dw * dmc || m > 64 * syrk_split16_waves())
-    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 448 (N <= 224); use FP64");
+  if (n_cg > dw * dmc || m / 16 > tpass16_max_tiles())
+    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 416 (N <= 208); use FP64");
   if (!s->xd) {
     const int64_t n_pad = (n + 127) / 128 * 128;
     ST_TRY(s->alloc(&s->xd, n_pad * m * 2));
     ST_TRY(s->alloc(&s->xt, n_pad * m * 2));
     ST_TRY(s->alloc(&s->row_ex, n));
     ST_TRY(s->alloc(&s->rs2, n));
-    ST_TRY(s->alloc(&s->u, n));
+    // T-pass coefficients, zero past n: the last stage of the last chunk reads up to n_pad.
+    ST_TRY(s->alloc(&s->cvec, n_pad));
+    HIP_TRY(hipMemsetAsync(s->cvec, 0, n_pad * sizeof(float), s->st));
     ST_TRY(s->alloc(&s->dsum, n));
     ST_TRY(s->alloc(&s->block_max, (n + 255) / 256 + 1));
     ST_TRY(s->alloc(&s->bscale, 2));
     ST_TRY(s->alloc(&s->tscale, 1));
-    ST_TRY(s->alloc(&s->zscale, 1));
+    ST_TRY(s->alloc(&s->cscale, 1));
     ST_TRY(s->alloc(&s->bfrag, ((size_t)(m / 32) * n_cg + 1) * 2 * 64));
     // Column groups to waves round-robin: at k-step ks the active groups are cg >= 2 ks, a
     // contiguous range, which round-robin spreads evenly over the waves (each k-step ends at a
@@ -447,13 +465,14 @@ int ensure_split16(cy3q_solver *s) {
     ST_TRY(s->alloc(&s->cglist, cl.size()));
     HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
                            hipMemcpyHostToDevice, s->st));
-    // One workgroup per CU fills the machine; chunks of >= 1024 points. The kernel reads
-    // exactly syrk_split16_tiles_per_wave() slots per wave.
-    const int nt = syrk_split16_tiles_per_wave(), wv = syrk_split16_waves();
-    ST_TRY(build_syrk_plan(s, s->plan16, 32, wv, nt, 256, 1024, 16, true, nt));
-    if (s->plan16.n_parts != 1)
-      return fail(CY3Q_ERR_ARG, "split16 T-pass tiles do not fit one workgroup per chunk");
-    ST_TRY(s->alloc(&s->partials32, syrk_partials_count(s->plan16)));
+    // T pass: one workgroup per CU, each streaming one chunk of a multiple of 32 points.
+    int n_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+    n_cu = std::max(n_cu, 1);
+    s->tp_chunk = std::max<int64_t>((n + (int64_t)n_cu * 32 - 1) / ((int64_t)n_cu * 32), 1) * 32;
+    s->tp_chunks = (n + s->tp_chunk - 1) / s->tp_chunk;
+    ST_TRY(s->alloc(&s->partials32, tpass16_partials(m, s->tp_chunks)));
+    ST_TRY(s->alloc(&s->tseg, tpass16_seg_count(m, s->tp_chunks)));
     HIP_TRY(hipStreamSynchronize(s->st));
   }
   if (!s->x32_valid) {
@@ -475,15 +494,14 @@ int t_step_split16(cy3q_solver *s) {
   launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->dsum);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
-  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->block_max,
-                        s->zscale, s->tscale);
+  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->cvec, s->block_max,
+                        s->cscale, s->tscale);
   s->stage_end(STG_COEFF);
   s->stage_begin(STG_TPASS);
-  launch_syrk_split16(s->st, s->plan16, (const half8_t *)s->xt, s->u, s->zscale, n,
-                      s->partials32);
+  launch_tpass16(s->st, s->xt, s->cvec, n, m, s->tp_chunk, s->tp_chunks, s->partials32);
   s->stage_end(STG_TPASS);
   s->stage_begin(STG_TREDUCE);
-  launch_syrk_reduce_f32(s->st, s->plan16, s->partials32, s->xx);
+  launch_tpass16_reduce(s->st, s->partials32, m, s->tp_chunks, s->tseg, s->xx);
   s->stage_end(STG_TREDUCE);
   return CY3Q_OK;
 }
