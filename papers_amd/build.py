@@ -31,7 +31,7 @@ SOURCES = {
     "k_hsolve.hip": [],
     "k_sigma.hip": ["-ffp-contract=off"],
     "k_split16.hip": [],
-    "k_tpass16.hip": [],
+    "k_tpass.hip": [],
     "solver.cpp": ["-x", "hip"],
 }
 

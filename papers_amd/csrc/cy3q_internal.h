@@ -70,7 +70,7 @@ void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *part
 
 // Split-fp16 MFMA contractions (k_split16.hip).
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
-                         int *ex, float *rs2, _Float16 *xd, _Float16 *xt);
+                         int *ex, float *rs2, _Float16 *xd, float *xt);
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale);
 int dpass_split16_waves();
@@ -78,15 +78,16 @@ int dpass_split16_max_cg();
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, double *dsum);
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *c, float *block_max,
-                           float *cscale, double *tscale);
-int tpass16_max_tiles();
-size_t tpass16_partials(int64_t m_cols, int64_t n_chunks);
-size_t tpass16_seg_count(int64_t m_cols, int64_t n_chunks);
-void launch_tpass16(hipStream_t st, const _Float16 *xt, const float *cvec, int64_t n,
-                    int64_t m_cols, int64_t chunk, int64_t n_chunks, float *partials);
-void launch_tpass16_reduce(hipStream_t st, const float *partials, int64_t m_cols,
-                           int64_t n_chunks, double *seg, double *xx);
+                           const double *bscale, int64_t n, float *u, float *block_max,
+                           float *zscale, double *tscale);
+// Split-fp16 T pass (k_tpass.hip).
+int tpass_max_cols();
+size_t tpass_partials(int64_t m_cols, int64_t n_chunks);
+size_t tpass_seg_count(int64_t m_cols, int64_t n_chunks);
+void launch_tpass(hipStream_t st, const float *xt, const float *u, int64_t n, int64_t m_cols,
+                  int64_t chunk, int64_t n_chunks, float *partials);
+void launch_tpass_reduce(hipStream_t st, const float *partials, int64_t m_cols, int64_t n_chunks,
+                         double *seg, double *xx);
 
 // Per-point coefficients, reductions, B operand, layout conversions (k_dense.hip).
 void launch_point_coeffs_f64(hipStream_t st, const double *w, const double *dpart, int n_parts,

@@ -7,13 +7,13 @@
 //   * every section row is stored scaled by 2^-e_p (max |x| in [0.5, 1)) and split into fp16
 //     hi/lo planes once per workspace;
 //   * W (the D-pass operand) is scaled globally by 2^-f;
-//   * c_p (the T-pass coefficient) is scaled globally by 2^g.
+//   * z_p = sqrt(c_p) x_p (the T-pass operand) is scaled globally by 2^g.
 // The host-side bookkeeping undoes each scaling exactly. D_p and the chunk sums are accumulated in
 // FP64. T is summed across point chunks in FP64, in a fixed order.
 //
 // D pass  (k_dpass_split16): v_mfma_f32_16x16x32_f16, 128 points x all M columns per workgroup,
 //                            so D_p = sum_j Y_pj^2 completes inside the workgroup.
-// T pass: k_tpass16.hip.
+// T pass: k_tpass.hip.
 #include "cy3q_internal.h"
 
 namespace cy3q {
@@ -110,8 +110,8 @@ void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t 
 //  xd: D-pass A fragments of v_mfma_f32_16x16x32_f16. For point tile t (16 points), k-step ks
 //      (32 columns), plane h (hi, lo), lane l: half8 of X[16t + l%16][32ks + 8(l/16) + j].
 //      Index ((t * n_ks + ks) * 2 + h) * 64 + l.
-//  xt: T-pass stage records. For stage s (16 points), column c, plane h: 16 halves
-//      X[16s + j][c], j = 0..15, as two half8. Index ((s * m + c) * 2 + h) * 2 + {0, 1}.
+//  xt: T-pass stage records (FP32). For stage s (16 points), column c: the 16 floats
+//      X[16s + j][c], j = 0..15. Index (s * m + c) * 16 + j.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__ sv, int64_t n,
                                                       int64_t m, int *__restrict__ ex,
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__
 
 __global__ void k_split_planes(const double *__restrict__ sv, const int *__restrict__ ex,
                                int64_t n, int64_t n_pad, int64_t m,
-                               _Float16 *__restrict__ xd, _Float16 *__restrict__ xt) {
+                               _Float16 *__restrict__ xd, float *__restrict__ xt) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_pad * m) return;
   const int64_t p = q / m, c = q % m;
@@ -145,14 +145,11 @@ __global__ void k_split_planes(const double *__restrict__ sv, const int *__restr
   const int lane = (int)(p % 16) + 16 * (int)((c % 32) / 8), j = (int)(c % 8);
   xd[((((t * n_ks + ks) * 2 + 0) * 64 + lane) * 8) + j] = h;
   xd[((((t * n_ks + ks) * 2 + 1) * 64 + lane) * 8) + j] = l;
-  const int64_t sg = p / 16;
-  const int jj = (int)(p % 16);
-  xt[(((sg * m + c) * 2 + 0) * 16) + jj] = h;
-  xt[(((sg * m + c) * 2 + 1) * 16) + jj] = l;
+  xt[((p / 16) * m + c) * 16 + p % 16] = x;
 }
 
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
-                         int *ex, float *rs2, _Float16 *xd, _Float16 *xt) {
+                         int *ex, float *rs2, _Float16 *xd, float *xt) {
   hipLaunchKernelGGL(k_row_exponent, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, sv, n, m, ex,
                      rs2);
   int64_t tot = n_pad * m;
@@ -380,8 +377,8 @@ void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t 
 // Per-point coefficients of the split path. D_p = dsum_p * bscale / rs2_p restores the unscaled
 // s_p^H G s_p. The weight on the scaled row is c'_p = w_p / (rs2_p * max(D_p, 1e-30)), for valid
 // w (finite, > 0), else 0: the same T as the reference's w_p / max(K_p, 1e-30) on the unscaled
-// row (quintic.rs:1761-1775). c'_p is stored in FP32; the largest one sets the global 2^g of the
-// T pass (per-block maxima, then one workgroup; no atomics).
+// row (quintic.rs:1761-1775). u_p = sqrt(c'_p) (FP32) scales the T-pass operand z_p = 2^g u_p x_p;
+// the largest u sets the global 2^g (per-block maxima, then one workgroup; no atomics).
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict__ w,
                                                         const double *__restrict__ dsum,
@@ -398,7 +395,7 @@ __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict
       double r = (double)rs2[p];
       double d = dsum[p] * bscale[0] / r;
       double c = wp / (r * fmax(d, 1e-30));
-      uv = (float)c;
+      uv = (float)sqrt(c);
     }
     u[p] = uv;
   }
@@ -409,11 +406,11 @@ __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict
   if (threadIdx.x == 0) block_max[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
-// Max over the block maxima -> cscale[0] = 2^g (float, applied to c'), tscale[0] = 2^-g
-// (double, applied to T), with g chosen so that max c' 2^g < 2^14; the row-scaled |x| < 1, so
-// every B operand of the T pass stays below 2^14.
+// Max over the block maxima -> zscale[0] = 2^g (float, applied to z) and tscale[0] = 4^-g
+// (double, applied to T), with g chosen so that max u 2^g < 2^14; the row-scaled |x| < 1, so
+// every z stays below 2^14.
 __global__ void __launch_bounds__(1024) k_finalize_scale(const float *__restrict__ block_max,
-                                                         int64_t nblk, float *__restrict__ cscale,
+                                                         int64_t nblk, float *__restrict__ zscale,
                                                          double *__restrict__ tscale) {
   __shared__ float sh[1024];
   float mv = 0.f;
@@ -426,32 +423,32 @@ __global__ void __launch_bounds__(1024) k_finalize_scale(const float *__restrict
   }
   if (threadIdx.x == 0) {
     int e = 0;
-    if (sh[0] > 0.f) frexpf(sh[0], &e);  // max c' < 2^e
+    if (sh[0] > 0.f) frexpf(sh[0], &e);  // max u < 2^e
     const int g = 14 - e;
-    cscale[0] = ldexpf(1.f, g);
-    tscale[0] = ldexp(1.0, -g);
+    zscale[0] = ldexpf(1.f, g);
+    tscale[0] = ldexp(1.0, -2 * g);
   }
 }
 
-// c'' = c' 2^g, in place.
-__global__ void __launch_bounds__(256) k_apply_cscale(float *__restrict__ c,
-                                                      const float *__restrict__ cscale,
+// u <- 2^g u, in place (exact: a power of two).
+__global__ void __launch_bounds__(256) k_apply_zscale(float *__restrict__ u,
+                                                      const float *__restrict__ zscale,
                                                       int64_t n) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) c[p] *= cscale[0];
+  if (p < n) u[p] *= zscale[0];
 }
 
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
-                           const double *bscale, int64_t n, float *c, float *block_max,
-                           float *cscale, double *tscale) {
+                           const double *bscale, int64_t n, float *u, float *block_max,
+                           float *zscale, double *tscale) {
   const int64_t nblk = (n + 255) / 256;
   if (n > 0)
     hipLaunchKernelGGL(k_coeffs_split16, dim3((unsigned)nblk), dim3(256), 0, st, w, dsum, rs2,
-                       bscale, n, c, block_max);
-  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1024), 0, st, block_max, nblk, cscale,
+                       bscale, n, u, block_max);
+  hipLaunchKernelGGL(k_finalize_scale, dim3(1), dim3(1024), 0, st, block_max, nblk, zscale,
                      tscale);
   if (n > 0)
-    hipLaunchKernelGGL(k_apply_cscale, dim3((unsigned)nblk), dim3(256), 0, st, c, cscale, n);
+    hipLaunchKernelGGL(k_apply_zscale, dim3((unsigned)nblk), dim3(256), 0, st, u, zscale, n);
 }
 
 }  // namespace cy3q

@@ -123,12 +123,13 @@ struct cy3q_solver {
   int64_t partials_cap = 0;
   // split16 path (allocated on first use)
   bool x32_valid = false, bfrag_valid = false;
-  float *rs2 = nullptr, *cvec = nullptr, *cscale = nullptr, *partials32 = nullptr,
+  float *rs2 = nullptr, *u = nullptr, *zscale = nullptr, *partials32 = nullptr,
         *block_max = nullptr;
   double *tseg = nullptr;            // T-pass segment sums
   int64_t tp_chunk = 0, tp_chunks = 0;
   int *row_ex = nullptr;
-  _Float16 *xd = nullptr, *xt = nullptr;
+  _Float16 *xd = nullptr;
+  float *xt = nullptr;
   double *dsum = nullptr, *bscale = nullptr, *tscale = nullptr;
   half8_t *bfrag = nullptr;
   int16_t *cglist = nullptr;
@@ -436,22 +437,23 @@ int ensure_split16(cy3q_solver *s) {
   const int64_t n = s->n, m = s->m;
   const int n_cg = (int)(m / 16);
   const int dw = dpass_split16_waves(), dmc = dpass_split16_max_cg();
-  if (n_cg > dw * dmc || m / 16 > tpass16_max_tiles())
+  if (n_cg > dw * dmc || m > tpass_max_cols())
     return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 416 (N <= 208); use FP64");
   if (!s->xd) {
     const int64_t n_pad = (n + 127) / 128 * 128;
     ST_TRY(s->alloc(&s->xd, n_pad * m * 2));
-    ST_TRY(s->alloc(&s->xt, n_pad * m * 2));
+    ST_TRY(s->alloc(&s->xt, n_pad * m));
     ST_TRY(s->alloc(&s->row_ex, n));
     ST_TRY(s->alloc(&s->rs2, n));
-    // T-pass coefficients, zero past n: the last stage of the last chunk reads up to n_pad.
-    ST_TRY(s->alloc(&s->cvec, n_pad));
-    HIP_TRY(hipMemsetAsync(s->cvec, 0, n_pad * sizeof(float), s->st));
+    // T-pass coefficients, zero past n: the last stage of the last chunk reads up to a
+    // multiple of 16 points (<= n_pad).
+    ST_TRY(s->alloc(&s->u, n_pad));
+    HIP_TRY(hipMemsetAsync(s->u, 0, n_pad * sizeof(float), s->st));
     ST_TRY(s->alloc(&s->dsum, n));
     ST_TRY(s->alloc(&s->block_max, (n + 255) / 256 + 1));
     ST_TRY(s->alloc(&s->bscale, 2));
     ST_TRY(s->alloc(&s->tscale, 1));
-    ST_TRY(s->alloc(&s->cscale, 1));
+    ST_TRY(s->alloc(&s->zscale, 1));
     ST_TRY(s->alloc(&s->bfrag, ((size_t)(m / 32) * n_cg + 1) * 2 * 64));
     // Column groups to waves round-robin: at k-step ks the active groups are cg >= 2 ks, a
     // contiguous range, which round-robin spreads evenly over the waves (each k-step ends at a
@@ -465,14 +467,14 @@ int ensure_split16(cy3q_solver *s) {
     ST_TRY(s->alloc(&s->cglist, cl.size()));
     HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
                            hipMemcpyHostToDevice, s->st));
-    // T pass: one workgroup per CU, each streaming one chunk of a multiple of 32 points.
+    // T pass: one workgroup per CU, each streaming one chunk of a multiple of 16 points.
     int n_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
     n_cu = std::max(n_cu, 1);
-    s->tp_chunk = std::max<int64_t>((n + (int64_t)n_cu * 32 - 1) / ((int64_t)n_cu * 32), 1) * 32;
+    s->tp_chunk = std::max<int64_t>((n + (int64_t)n_cu * 16 - 1) / ((int64_t)n_cu * 16), 1) * 16;
     s->tp_chunks = (n + s->tp_chunk - 1) / s->tp_chunk;
-    ST_TRY(s->alloc(&s->partials32, tpass16_partials(m, s->tp_chunks)));
-    ST_TRY(s->alloc(&s->tseg, tpass16_seg_count(m, s->tp_chunks)));
+    ST_TRY(s->alloc(&s->partials32, tpass_partials(m, s->tp_chunks)));
+    ST_TRY(s->alloc(&s->tseg, tpass_seg_count(m, s->tp_chunks)));
     HIP_TRY(hipStreamSynchronize(s->st));
   }
   if (!s->x32_valid) {
@@ -494,14 +496,14 @@ int t_step_split16(cy3q_solver *s) {
   launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->dsum);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
-  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->cvec, s->block_max,
-                        s->cscale, s->tscale);
+  launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->block_max,
+                        s->zscale, s->tscale);
   s->stage_end(STG_COEFF);
   s->stage_begin(STG_TPASS);
-  launch_tpass16(s->st, s->xt, s->cvec, n, m, s->tp_chunk, s->tp_chunks, s->partials32);
+  launch_tpass(s->st, s->xt, s->u, n, m, s->tp_chunk, s->tp_chunks, s->partials32);
   s->stage_end(STG_TPASS);
   s->stage_begin(STG_TREDUCE);
-  launch_tpass16_reduce(s->st, s->partials32, m, s->tp_chunks, s->tseg, s->xx);
+  launch_tpass_reduce(s->st, s->partials32, m, s->tp_chunks, s->tseg, s->xx);
   s->stage_end(STG_TREDUCE);
   return CY3Q_OK;
 }
