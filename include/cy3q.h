@@ -105,6 +105,22 @@ int cy3q_solver_sigma(cy3q_solver *s, double *sigma);
 int cy3q_solver_sigma_ext(cy3q_solver *s, double *sigma_l1, double *kappa, double *sigma_l2sq,
                           double *eta);
 
+/* sigma_squared_and_gradient (quintic.rs:2714-2860): sigma_L2^2 = sum w (eta/kappa-1)^2 / sum w,
+ * its gradient with respect to h and sigma_L1. grad: 2 N^2 doubles or NULL; grad[a N + b] =
+ * d/dh_re(a, b) (the diagonal halved, as the reference), grad[N^2 + a N + b] = d/dh_im(a, b) for
+ * a < b (0 otherwise), where h_re = Re G and h_im = Im G. The per-point metric is recomputed on
+ * the GPU (no per-point gradient is stored). */
+int cy3q_solver_sigma_sq_grad(cy3q_solver *s, double *sigma_l2sq, double *grad,
+                              double *sigma_l1);
+/* QuinticSolver::sigma_refine_analytic (quintic.rs:453-514): n_iter Adam steps on (h_re, h_im)
+ * driven by the analytic gradient, each followed by the Hermitian projection and the trace
+ * renormalisation; the moments start at zero. history: n_iter doubles (sigma_L1 before each
+ * step) or NULL; *n_done: steps taken (the loop stops at a non-finite sigma). Default Adam
+ * parameters are the reference's (quintic.rs:386-388); cy3q_solver_set_adam changes them. */
+int cy3q_solver_sigma_refine_analytic(cy3q_solver *s, int64_t n_iter, double lr,
+                                      double *history, int64_t *n_done);
+int cy3q_solver_set_adam(cy3q_solver *s, double beta1, double beta2, double eps);
+
 /* h_block access (4 N^2 doubles). set_h_block takes the Hermitian part. */
 int cy3q_solver_get_h_block(cy3q_solver *s, double *h_block);
 int cy3q_solver_set_h_block(cy3q_solver *s, const double *h_block);

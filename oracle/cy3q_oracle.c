@@ -967,6 +967,381 @@ ORC_API double orc_sigma(const double *pts, const double *sv, const double *sd,
     return s;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* sigma^2 and its analytic gradient (quintic.rs:2194-2860)                                    */
+/* ------------------------------------------------------------------------------------------ */
+/* invert_3x3_complex_hermitian (quintic.rs:2194-2245): cofactor transpose / det, then the
+ * Hermitian average of each off-diagonal pair. Returns 0 when det is not usable. */
+static int inv3_herm(const double g[3][3][2], double inv[3][3][2]) {
+#define CM(x, y, o)                                   \
+    do {                                              \
+        (o)[0] = (x)[0] * (y)[0] - (x)[1] * (y)[1];   \
+        (o)[1] = (x)[0] * (y)[1] + (x)[1] * (y)[0];   \
+    } while (0)
+    double det = det3(g);
+    if (!isfinite(det) || fabs(det) < 1e-30) return 0;
+    double c[3][3][2], t1[2], t2[2];
+    static const int id[9][6] = {
+        /* cij = g[a][b]*g[c][d] - g[e][f]*g[g][h]: (a,b,c,d) then (e,f,g,h) */
+        {1, 1, 2, 2, 1, 2}, {1, 2, 2, 0, 1, 0}, {1, 0, 2, 1, 1, 1},
+        {0, 2, 2, 1, 0, 1}, {0, 0, 2, 2, 0, 2}, {0, 1, 2, 0, 0, 0},
+        {0, 1, 1, 2, 0, 2}, {0, 2, 1, 0, 0, 0}, {0, 0, 1, 1, 0, 1}};
+    static const int id2[9][2] = {{2, 1}, {2, 2}, {2, 0}, {2, 2}, {2, 0}, {2, 1},
+                                  {1, 1}, {1, 2}, {1, 0}};
+    for (int q = 0; q < 9; q++) {
+        CM(g[id[q][0]][id[q][1]], g[id[q][2]][id[q][3]], t1);
+        CM(g[id[q][4]][id[q][5]], g[id2[q][0]][id2[q][1]], t2);
+        c[q / 3][q % 3][0] = t1[0] - t2[0];
+        c[q / 3][q % 3][1] = t1[1] - t2[1];
+    }
+    double idet = 1.0 / det;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            inv[i][j][0] = c[j][i][0] * idet;
+            inv[i][j][1] = c[j][i][1] * idet;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++) {
+            double ar = 0.5 * (inv[j][i][0] + inv[i][j][0]);
+            double ai = 0.5 * (inv[j][i][1] + (-inv[i][j][1]));
+            inv[j][i][0] = ar;
+            inv[j][i][1] = ai;
+            inv[i][j][0] = ar;
+            inv[i][j][1] = -ai;
+        }
+    return 1;
+#undef CM
+}
+
+/* per_point_log_det_gradient (quintic.rs:2311-2695), in the reference's operation order.
+ * grad: [2 nb^2] (d/dh_re_ab with the diagonal halved, then d/dh_im_ab for a < b). Returns
+ * eta_p (exp(log R)) or NaN for a skipped point; *ok = 0 when the point is skipped. */
+static double grad_point(const double *s, const double *df, const double *h, const double *z,
+                         double log_om, int64_t nb, double psi, double *grad, double *scr,
+                         int *ok) {
+    const int64_t two_n = 2 * nb;
+    double *h_s = scr, *h_dfi = scr + two_n; /* [5][two_n] */
+    double *q = h_dfi + 5 * two_n, *qp = q + two_n, *phi = qp + two_n; /* phi [5][nb][2] */
+    double *amr = phi + 10 * nb, *ami = amr + nb * nb; /* scratch: 26 nb + 2 nb^2 doubles */
+    *ok = 0;
+    for (int64_t i = 0; i < two_n; i++) {
+        double rs = 0.0;
+        for (int64_t j = 0; j < two_n; j++) rs += h[i * two_n + j] * s[j];
+        h_s[i] = rs;
+    }
+    double k_val = 0.0;
+    for (int64_t i = 0; i < two_n; i++) k_val += s[i] * h_s[i];
+    double k_safe = fmax(k_val, 1e-30);
+    double inv_k = 1.0 / k_safe, inv_k2 = inv_k * inv_k, inv_k3 = inv_k * inv_k2;
+    for (int i = 0; i < 5; i++) {
+        const double *dfi = df + i * two_n;
+        for (int64_t kk = 0; kk < two_n; kk++) {
+            double rs = 0.0;
+            for (int64_t l = 0; l < two_n; l++) rs += h[kk * two_n + l] * dfi[l];
+            h_dfi[i * two_n + kk] = rs;
+        }
+    }
+    double dk[5][2];
+    for (int i = 0; i < 5; i++) {
+        const double *dfi = df + i * two_n;
+        double sr = 0.0, si = 0.0;
+        for (int64_t a = 0; a < nb; a++) {
+            double hdr = h_s[2 * a], hdi = h_s[2 * a + 1], dr = dfi[2 * a], di = dfi[2 * a + 1];
+            sr += hdr * dr + hdi * di;
+            si += hdr * di - hdi * dr;
+        }
+        dk[i][0] = sr;
+        dk[i][1] = si;
+    }
+    double m[5][5][2];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+            const double *dfj = df + j * two_n;
+            double sr = 0.0, si = 0.0;
+            for (int64_t a = 0; a < nb; a++) {
+                double jr = dfj[2 * a], ji = dfj[2 * a + 1];
+                double hr = h_dfi[i * two_n + 2 * a], hi = h_dfi[i * two_n + 2 * a + 1];
+                sr += jr * hr + ji * hi;
+                si += jr * hi - ji * hr;
+            }
+            m[i][j][0] = sr;
+            m[i][j][1] = si;
+        }
+    double ga[5][5][2];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+            double pr = dk[i][0] * dk[j][0] + dk[i][1] * dk[j][1];
+            double pi = dk[i][1] * dk[j][0] - dk[i][0] * dk[j][1];
+            ga[i][j][0] = m[i][j][0] * inv_k - pr * inv_k2;
+            ga[i][j][1] = m[i][j][1] * inv_k - pi * inv_k2;
+        }
+    double gf[10];
+    deformed_gradient(z, psi, gf);
+    int chart, elim;
+    chart_and_elim(z, gf, &chart, &elim, NULL);
+    double fr[3][10];
+    affine_frame(gf, chart, elim, fr);
+    double gt[3][3][2];
+    project_tangent(ga, fr, gt);
+    double det = det3(gt);
+    for (int64_t t = 0; t < 2 * nb * nb; t++) grad[t] = 0.0;
+    if (!isfinite(det) || fabs(det) < 1e-30 || !isfinite(log_om)) return NAN;
+    double log_r = log(fabs(det)) - log_om;
+    double r_val = exp(log_r);
+    double gi[3][3][2];
+    if (!inv3_herm(gt, gi)) return r_val; /* zero gradient, as the reference */
+    /* compute_y_5x5 (quintic.rs:2260-2301): y[i][j] = sum_ab conj(T_a[i]) ginv[a][b] T_b[j] */
+    double y[5][5][2];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+            double sr = 0.0, si = 0.0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    double gar = gi[a][b][0], gai = gi[a][b][1];
+                    double tiar = fr[a][2 * i], tiai = -fr[a][2 * i + 1];
+                    double tjbr = fr[b][2 * j], tjbi = fr[b][2 * j + 1];
+                    double p1r = tiar * gar - tiai * gai, p1i = tiar * gai + tiai * gar;
+                    sr += p1r * tjbr - p1i * tjbi;
+                    si += p1r * tjbi + p1i * tjbr;
+                }
+            y[i][j][0] = sr;
+            y[i][j][1] = si;
+        }
+    double tr_ym = 0.0;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) tr_ym += y[j][i][0] * m[i][j][0] - y[j][i][1] * m[i][j][1];
+    double tr_yd = 0.0;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) {
+            double pr = dk[i][0] * dk[j][0] + dk[i][1] * dk[j][1];
+            double pi = dk[i][1] * dk[j][0] - dk[i][0] * dk[j][1];
+            tr_yd += y[j][i][0] * pr - y[j][i][1] * pi;
+        }
+    double vi[5][2], wj[5][2];
+    for (int i = 0; i < 5; i++) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < 5; j++) {
+            sr += y[j][i][0] * dk[j][0] + y[j][i][1] * dk[j][1];
+            si += y[j][i][1] * dk[j][0] - y[j][i][0] * dk[j][1];
+        }
+        vi[i][0] = sr;
+        vi[i][1] = si;
+    }
+    for (int64_t b = 0; b < nb; b++) {
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < 5; i++) {
+            double dr = df[i * two_n + 2 * b], di = df[i * two_n + 2 * b + 1];
+            sr += dr * vi[i][0] - di * vi[i][1];
+            si += dr * vi[i][1] + di * vi[i][0];
+        }
+        q[2 * b] = sr;
+        q[2 * b + 1] = si;
+    }
+    for (int j = 0; j < 5; j++) {
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < 5; i++) {
+            sr += y[j][i][0] * dk[i][0] - y[j][i][1] * dk[i][1];
+            si += y[j][i][0] * dk[i][1] + y[j][i][1] * dk[i][0];
+        }
+        wj[j][0] = sr;
+        wj[j][1] = si;
+    }
+    for (int64_t b = 0; b < nb; b++) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < 5; j++) {
+            double dr = df[j * two_n + 2 * b], di = df[j * two_n + 2 * b + 1];
+            sr += dr * wj[j][0] + di * wj[j][1];
+            si += dr * wj[j][1] - di * wj[j][0];
+        }
+        qp[2 * b] = sr;
+        qp[2 * b + 1] = si;
+    }
+    for (int j = 0; j < 5; j++)
+        for (int64_t b = 0; b < nb; b++) {
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < 5; i++) {
+                double dr = df[i * two_n + 2 * b], di = df[i * two_n + 2 * b + 1];
+                sr += y[j][i][0] * dr - y[j][i][1] * di;
+                si += y[j][i][0] * di + y[j][i][1] * dr;
+            }
+            phi[2 * (j * nb + b)] = sr;
+            phi[2 * (j * nb + b) + 1] = si;
+        }
+    for (int64_t a = 0; a < nb; a++)
+        for (int64_t b = 0; b < nb; b++) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < 5; j++) {
+                double dr = df[j * two_n + 2 * a], di = df[j * two_n + 2 * a + 1];
+                double pr = phi[2 * (j * nb + b)], pi = phi[2 * (j * nb + b) + 1];
+                sr += dr * pr + di * pi;
+                si += dr * pi - di * pr;
+            }
+            amr[a * nb + b] = sr;
+            ami[a * nb + b] = si;
+        }
+    for (int64_t a = 0; a < nb; a++) {
+        double sar = s[2 * a], sai = s[2 * a + 1], qar = q[2 * a], qai = q[2 * a + 1];
+        double qpar = qp[2 * a], qpai = qp[2 * a + 1];
+        for (int64_t b = 0; b < nb; b++) {
+            double sbr = s[2 * b], sbi = s[2 * b + 1], qbr = q[2 * b], qbi = q[2 * b + 1];
+            double qpbr = qp[2 * b], qpbi = qp[2 * b + 1];
+            double re_ss = sar * sbr + sai * sbi, im_ss = sar * sbi - sai * sbr;
+            double a_ab_re = amr[a * nb + b], a_ba_re = amr[b * nb + a];
+            double a_ab_im = ami[a * nb + b], a_ba_im = ami[b * nb + a];
+            double mixed = (sar * qbr + sai * qbi) + (sbr * qar + sbi * qai) +
+                           (sar * qpbr - sai * qpbi) + (sbr * qpar - sbi * qpai);
+            double im_sqab = sar * qbi - sai * qbr, im_sqba = sbr * qai - sbi * qar;
+            double im_sqpab = sar * qpbi + sai * qpbr, im_sqpba = sbr * qpai + sbi * qpar;
+            double pair = (a_ab_re + a_ba_re) * inv_k - 2.0 * re_ss * tr_ym * inv_k2 -
+                          mixed * inv_k2 + 4.0 * re_ss * tr_yd * inv_k3;
+            double dl_re = (a == b) ? 0.5 * pair : pair;
+            double dl_im = 0.0;
+            if (a < b)
+                dl_im = -(a_ab_im - a_ba_im) * inv_k + 2.0 * im_ss * tr_ym * inv_k2 +
+                        (im_sqab - im_sqba - im_sqpab + im_sqpba) * inv_k2 -
+                        4.0 * im_ss * tr_yd * inv_k3;
+            grad[a * nb + b] = dl_re;
+            grad[nb * nb + a * nb + b] = dl_im;
+        }
+    }
+    *ok = 1;
+    return r_val;
+}
+
+/* sigma_squared_and_gradient (quintic.rs:2714-2860). Returns sigma_L2^2; grad [2 nb^2];
+ * *sigma_l1 = sigma_L1. Per-point buffer n * (2 + 2 nb^2), as the reference. */
+ORC_API double orc_sigma_sq_grad(const double *pts, const double *sv, const double *sd,
+                                 const double *log_om, const double *w, const double *h,
+                                 int64_t n, int64_t nb, double psi, double *grad,
+                                 double *sigma_l1) {
+    const int64_t n_dof = 2 * nb * nb, stride = 2 + n_dof;
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * (size_t)stride);
+#pragma omp parallel
+    {
+        double *scr = (double *)malloc(sizeof(double) * (size_t)(26 * nb + 2 * nb * nb + 16));
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; p++) {
+            double *slot = buf + p * stride;
+            double wp = w[p];
+            if (!isfinite(wp) || wp <= 0.0) {
+                slot[0] = NAN;
+                slot[1] = 0.0;
+                continue;
+            }
+            int ok = 0;
+            double eta = grad_point(sv + p * 2 * nb, sd + p * 10 * nb, h, pts + 10 * p, log_om[p],
+                                    nb, psi, slot + 2, scr, &ok);
+            if (!isfinite(eta) || eta <= 0.0) {
+                slot[0] = NAN;
+                slot[1] = 0.0;
+                continue;
+            }
+            slot[0] = eta;
+            slot[1] = wp;
+        }
+        free(scr);
+    }
+    double tw = 0.0, swe = 0.0;
+    double *swg = (double *)calloc((size_t)n_dof, sizeof(double));
+    for (int64_t p = 0; p < n; p++) {
+        const double *slot = buf + p * stride;
+        double e = slot[0], wp = slot[1];
+        if (!isfinite(e) || e <= 0.0 || wp == 0.0) continue;
+        tw += wp;
+        swe += wp * e;
+        for (int64_t k = 0; k < n_dof; k++) swg[k] += wp * e * slot[2 + k];
+    }
+    for (int64_t k = 0; k < n_dof; k++) grad[k] = 0.0;
+    if (sigma_l1) *sigma_l1 = NAN;
+    if (tw < 1e-12) {
+        free(swg);
+        free(buf);
+        return NAN;
+    }
+    double kappa = swe / tw;
+    if (fabs(kappa) < 1e-30) {
+        free(swg);
+        free(buf);
+        return NAN;
+    }
+    for (int64_t k = 0; k < n_dof; k++) swg[k] /= tw; /* dkappa */
+    double l1 = 0.0, l2 = 0.0;
+    for (int64_t p = 0; p < n; p++) {
+        const double *slot = buf + p * stride;
+        double e = slot[0], wp = slot[1];
+        if (!isfinite(e) || e <= 0.0 || wp == 0.0) continue;
+        double en = e / kappa, dev = en - 1.0;
+        l1 += wp * fabs(dev);
+        l2 += wp * dev * dev;
+        for (int64_t k = 0; k < n_dof; k++) {
+            double d_en = en * (slot[2 + k] - swg[k] / kappa);
+            grad[k] += wp * 2.0 * dev * d_en;
+        }
+    }
+    for (int64_t k = 0; k < n_dof; k++) grad[k] /= tw;
+    if (sigma_l1) *sigma_l1 = l1 / tw;
+    free(swg);
+    free(buf);
+    return l2 / tw;
+}
+
+/* One step of sigma_refine_analytic (quintic.rs:453-514) after the gradient: Adam on
+ * (h_re, h_im) = (Re G, Im G) read from the h block, with the update rule written inline at
+ * quintic.rs:5421-5437 (scale = lr sqrt(1-b2^t)/(1-b1^t), upd = scale m/(sqrt(v)+eps)); both
+ * halves use the same t (adam_update_split, quintic.rs:2071-2117). Then the Hermitian
+ * projection (:2122-2146), pack (:2148-2161) and trace renormalisation (:3169-3182).
+ * h is updated in place; m, v: [2 nb^2]; returns the new t. */
+ORC_API int64_t orc_adam_step(double *h, int64_t nb, const double *grad, double *m, double *v,
+                              int64_t t, double lr, double beta1, double beta2, double eps) {
+    const int64_t n2 = nb * nb, tn = 2 * nb;
+    double *hre = (double *)malloc(sizeof(double) * (size_t)n2);
+    double *him = (double *)malloc(sizeof(double) * (size_t)n2);
+    for (int64_t a = 0; a < nb; a++)
+        for (int64_t b = 0; b < nb; b++) {
+            hre[a * nb + b] = h[(2 * a) * tn + 2 * b];
+            him[a * nb + b] = -h[(2 * a) * tn + 2 * b + 1];
+        }
+    int64_t t1 = t + 1;
+    double bc1 = 1.0 - pow(beta1, (double)t1), bc2 = 1.0 - pow(beta2, (double)t1);
+    double scale = lr * (sqrt(bc2) / bc1);
+    for (int half = 0; half < 2; half++) {
+        double *x = half == 0 ? hre : him;
+        for (int64_t k = 0; k < n2; k++) {
+            int64_t g = half * n2 + k;
+            m[g] = beta1 * m[g] + (1.0 - beta1) * grad[g];
+            v[g] = beta2 * v[g] + (1.0 - beta2) * grad[g] * grad[g];
+            x[k] -= scale * m[g] / (sqrt(v[g]) + eps);
+        }
+    }
+    for (int64_t a = 0; a < nb; a++)
+        for (int64_t b = a + 1; b < nb; b++) {
+            double avg = 0.5 * (hre[a * nb + b] + hre[b * nb + a]);
+            hre[a * nb + b] = avg;
+            hre[b * nb + a] = avg;
+            double avi = 0.5 * (him[a * nb + b] - him[b * nb + a]);
+            him[a * nb + b] = avi;
+            him[b * nb + a] = -avi;
+        }
+    for (int64_t a = 0; a < nb; a++) him[a * nb + a] = 0.0;
+    for (int64_t a = 0; a < nb; a++)
+        for (int64_t b = 0; b < nb; b++) {
+            double hr = hre[a * nb + b], hi = him[a * nb + b];
+            h[(2 * a) * tn + 2 * b] = hr;
+            h[(2 * a) * tn + 2 * b + 1] = -hi;
+            h[(2 * a + 1) * tn + 2 * b] = hi;
+            h[(2 * a + 1) * tn + 2 * b + 1] = hr;
+        }
+    double tr = 0.0;
+    for (int64_t a = 0; a < nb; a++) tr += h[(2 * a) * tn + 2 * a];
+    if (tr > 1e-10) {
+        double sc = (double)nb / tr;
+        for (int64_t k = 0; k < tn * tn; k++) h[k] *= sc;
+    }
+    free(hre);
+    free(him);
+    return t1;
+}
+
 ORC_API int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -63,9 +63,10 @@ def lib():
     L.orc_sigma_reduce.argtypes = [_dp, _dp, i64, vp, vp]
     L.orc_sigma_reduce.restype = dbl
     L.orc_num_threads.restype = C.c_int
-    for name in ("orc_sigma_grad", "orc_adam_split", "orc_laplacian", "orc_yukawa"):
-        if hasattr(L, name):
-            pass
+    L.orc_sigma_sq_grad.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i64, i64, dbl, _dp, vp]
+    L.orc_sigma_sq_grad.restype = dbl
+    L.orc_adam_step.argtypes = [_dp, i64, _dp, _dp, _dp, i64, dbl, dbl, dbl, dbl]
+    L.orc_adam_step.restype = i64
     _lib = L
     return L
 
@@ -230,6 +231,48 @@ def sigma(pts, sv, sd, log_om, w, h_block, psi: float = 0.0) -> float:
     return sigma_reduce(eta(pts, sv, sd, log_om, h_block, psi), w)[0]
 
 
+def sigma_sq_grad(pts, sv, sd, log_om, w, h_block, psi: float = 0.0):
+    """sigma_squared_and_gradient (quintic.rs:2714-2860): (sigma_L2^2, grad [2 N^2], sigma_L1).
+    grad[:N^2] = d/dh_re (a, b) with the diagonal halved; grad[N^2:] = d/dh_im (a, b), a < b."""
+    n, two_n = sv.shape
+    nb = two_n // 2
+    g = np.zeros(2 * nb * nb)
+    l1 = C.c_double(0)
+    s2 = lib().orc_sigma_sq_grad(np.ascontiguousarray(pts), np.ascontiguousarray(sv),
+                                 np.ascontiguousarray(sd), np.ascontiguousarray(log_om),
+                                 np.ascontiguousarray(w), np.ascontiguousarray(h_block), n, nb, psi,
+                                 g, C.byref(l1))
+    return s2, g, l1.value
+
+
+def adam_step(h_block, grad, m, v, t: int, lr: float, beta1: float = 0.9, beta2: float = 0.999,
+              eps: float = 1e-8):
+    """One Adam step of sigma_refine_analytic (quintic.rs:453-514, 2071-2161, 3169-3182) on a
+    copy of h_block; m, v are updated in place. Returns (h_new, t_new)."""
+    h = np.ascontiguousarray(h_block, dtype=np.float64).copy()
+    nb = h.shape[0] // 2
+    t2 = lib().orc_adam_step(h, nb, np.ascontiguousarray(grad), m, v, t, lr, beta1, beta2, eps)
+    return h, t2
+
+
+def sigma_refine_analytic(prob, h_block, n_iter: int, lr: float):
+    """QuinticSolver::sigma_refine_analytic (quintic.rs:453-514): Adam with zeroed moments.
+    Returns (history of sigma_L1, final h_block)."""
+    nb = h_block.shape[0] // 2
+    m = np.zeros(2 * nb * nb)
+    v = np.zeros(2 * nb * nb)
+    t = 0
+    h = np.array(h_block, dtype=np.float64)
+    hist = []
+    for _ in range(n_iter):
+        _, g, s1 = sigma_sq_grad(prob.pts, prob.sv, prob.sd, prob.log_om, prob.w, h, prob.psi)
+        if not np.isfinite(s1):
+            break
+        hist.append(s1)
+        h, t = adam_step(h, g, m, v, t, lr)
+    return hist, h
+
+
 class Problem:
     """The static inputs of QuinticSolver::new_with_sampler(.., ShiffmanZelditch) or
     ::new_with_psi_sz (quintic.rs:241-322), computed by the oracle."""
@@ -252,3 +295,11 @@ class Problem:
     def sigma(self, h=None) -> float:
         return sigma(self.pts, self.sv, self.sd, self.log_om, self.w, self.h if h is None else h,
                      self.psi)
+
+    def sigma_l2sq(self, h=None) -> float:
+        e = eta(self.pts, self.sv, self.sd, self.log_om, self.h if h is None else h, self.psi)
+        return sigma_reduce(e, self.w)[2]
+
+    def sigma_sq_grad(self, h=None):
+        return sigma_sq_grad(self.pts, self.sv, self.sd, self.log_om, self.w,
+                             self.h if h is None else h, self.psi)

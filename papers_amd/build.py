@@ -32,6 +32,7 @@ SOURCES = {
     "k_sigma.hip": ["-ffp-contract=off"],
     "k_split16.hip": [],
     "k_tpass.hip": [],
+    "k_adjoint.hip": [],
     "solver.cpp": ["-x", "hip"],
 }
 

@@ -117,6 +117,18 @@ void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const
                       double *eta);
 void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
                          double *out5);
+// Adjoint of sigma_L2^2 and Adam (k_adjoint.hip).
+void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const double *yd,
+                      const double *log_om, const double *w, const double *sv, const double *sd,
+                      int64_t n, int64_t m_cols, double psi, const double *sig, double *phi,
+                      double *vs);
+int64_t gemm_tn_groups(int64_t rows, int64_t m_cols);
+void launch_gemm_tn_acc(hipStream_t st, const double *u, const double *v, int64_t rows,
+                        int64_t m_cols, int64_t n_groups, double *part, double *acc);
+void launch_grad_assemble(hipStream_t st, const double *cm, int nb, int64_t m_cols,
+                          const double *sig, double *grad);
+void launch_adam_step(hipStream_t st, double *g_cplx, int nb, const double *grad, double *m,
+                      double *v, double scale, double beta1, double beta2, double eps);
 
 // ---------------------------------------------------------------------------------------
 // SYRK work partition: lower 16x16 (FP64) or 32x32 (split16) output sub-tiles of the

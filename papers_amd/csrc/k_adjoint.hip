@@ -1,0 +1,348 @@
+// Adjoint of the sigma_L2^2 functional with respect to h, and the Adam step on h (FP64).
+//
+// Reference: per_point_log_det_gradient (cy3/src/quintic.rs:2311-2695),
+// sigma_squared_and_gradient (:2714-2860), sigma_refine_analytic (:453-514),
+// adam_update_split (:2071-2117), the Hermitian projection (:2122-2146), pack_h_block
+// (:2148-2161) and renormalise_h_trace (:3169-3182).
+//
+// The reference stores the full per-point gradient (2 N^2 numbers per point). Here the per-point
+// metric is recomputed and the gradient is formed by contractions over the points instead:
+//   d sigma_L2^2 / dh = (1 / sum w) sum_p gamma_p grad L_p,
+//   gamma_p = w_p 2 (eta_p/kappa - 1)(eta_p/kappa) - (sum_q w_q 2 (..)(..)) / (kappa sum w) w_p eta_p,
+// which folds the d kappa / dh term of the reference into one weight per point. With Y_p, dK_p,
+// q_p, q'_p, tr(Y M) and tr(Y dK dK^H) as in the reference, every term of grad L_p is a complex
+// outer product, and all of them collapse into one contraction
+//   A = sum_p [ sum_j conj(ds_pj) (x) phi'_pj + conj(s_p) (x) v_p ],
+//   phi'_pj = (gamma/K) sum_i Y_ji ds_pi,   v_p = (beta_p/2) s_p - (gamma/K^2) (q_p + conj(q'_p)),
+//   beta_p = gamma (-2 tr(YM)/K^2 + 4 tr(Y dK dK^H)/K^3),
+// whose Hermitian part H = A + A^H gives d/dh_re = Re H (halved on the diagonal) and
+// d/dh_im(a < b) = -Im H, over sum w. The contraction is a real GEMM U^T V over the real-ified
+// rows (U = the section and derivative rows, V = the rows built here), on the FP64 MFMA.
+#include "cy3q_point.h"
+
+namespace cy3q {
+
+// One wavefront per point. pts/log_om/w/sv/sd: the solver's point data from point p0; y0/yd: the
+// W-contracted rows of the same points; sig: the sigma reduction (kappa, sum w, sum of the
+// gamma numerators). Writes phi [n][5][m] and vs [n][m] (zero rows for points that do not
+// contribute).
+__global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pts,
+                                                   const double *__restrict__ y0,
+                                                   const double *__restrict__ yd,
+                                                   const double *__restrict__ log_om,
+                                                   const double *__restrict__ w,
+                                                   const double *__restrict__ sv,
+                                                   const double *__restrict__ sd, int64_t n,
+                                                   int64_t m, double psi,
+                                                   const double *__restrict__ sig,
+                                                   double *__restrict__ phi,
+                                                   double *__restrict__ vs) {
+  // Per wave: [0] gamma/K, [1] gamma/K^2, [2] beta/2, [3..52] Y (Y[j][i] at 3 + 2 (5 j + i)),
+  // [53..62] v_i, [63..72] w_j.
+  __shared__ double cf[4][73];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 4 + wave;
+  const bool in = p < n;
+  double acc[61];
+  if (in) point_dots(y0 + p * m, yd + p * 5 * m, m, lane, acc);
+  if (lane == 0) {
+    double *c = cf[wave];
+    for (int q = 0; q < 73; q++) c[q] = 0.0;
+    const double kappa = sig[1], tw = sig[3], sa2 = sig[5];
+    const double wp = in ? w[p] : 0.0, lo = in ? log_om[p] : NAN;
+    if (in && isfinite(wp) && wp > 0.0 && isfinite(lo) && isfinite(kappa) && isfinite(sa2) &&
+        tw >= 1e-12) {
+      double dk[5][2], ddk[5][5][2];
+      unpack_dots(acc, dk, ddk);
+      PointMetric pm;
+      ambient_from_dots(acc[0], dk, ddk, pm);
+      double z[10];
+      for (int q = 0; q < 10; q++) z[q] = pts[10 * p + q];
+      frame_and_det(z, psi, pm);
+      double gi[3][3][2];
+      if (isfinite(pm.det) && fabs(pm.det) >= 1e-30) {
+        const double eta = exp(log(fabs(pm.det)) - lo);
+        if (isfinite(eta) && eta > 0.0 && inv3_herm(pm.gt, pm.det, gi)) {
+          const double en = eta / kappa;
+          const double gamma = wp * 2.0 * (en - 1.0) * en - sa2 / (kappa * tw) * (wp * eta);
+          double y[5][5][2];
+          y5_from_frame(pm.fr, gi, y);
+          double tr_ym = 0.0, tr_yd = 0.0;
+          for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 5; j++) {
+              tr_ym += y[j][i][0] * ddk[i][j][0] - y[j][i][1] * ddk[i][j][1];
+              const double pr = dk[i][0] * dk[j][0] + dk[i][1] * dk[j][1];
+              const double pi = dk[i][1] * dk[j][0] - dk[i][0] * dk[j][1];
+              tr_yd += y[j][i][0] * pr - y[j][i][1] * pi;
+            }
+          const double ik = 1.0 / pm.k, ik2 = ik * ik, ik3 = ik2 * ik;
+          c[0] = gamma * ik;
+          c[1] = gamma * ik2;
+          c[2] = 0.5 * gamma * (-2.0 * tr_ym * ik2 + 4.0 * tr_yd * ik3);
+          for (int j = 0; j < 5; j++)
+            for (int i = 0; i < 5; i++) {
+              c[3 + 2 * (5 * j + i)] = y[j][i][0];
+              c[4 + 2 * (5 * j + i)] = y[j][i][1];
+            }
+          for (int i = 0; i < 5; i++) {  // v_i = sum_j Y_ji conj(dK_j)
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < 5; j++) {
+              sr += y[j][i][0] * dk[j][0] + y[j][i][1] * dk[j][1];
+              si += y[j][i][1] * dk[j][0] - y[j][i][0] * dk[j][1];
+            }
+            c[53 + 2 * i] = sr;
+            c[54 + 2 * i] = si;
+          }
+          for (int j = 0; j < 5; j++) {  // w_j = sum_i Y_ji dK_i
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < 5; i++) {
+              sr += y[j][i][0] * dk[i][0] - y[j][i][1] * dk[i][1];
+              si += y[j][i][0] * dk[i][1] + y[j][i][1] * dk[i][0];
+            }
+            c[63 + 2 * j] = sr;
+            c[64 + 2 * j] = si;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (!in) return;
+  const double *c = cf[wave];
+  const double gk = c[0], gk2 = c[1], hb = c[2];
+  const double *sp = sv + p * m, *dp = sd + p * 5 * m;
+  double *ph = phi + p * 5 * m, *vp = vs + p * m;
+  for (int64_t col = 2 * lane; col < m; col += 128) {
+    double dr[5], di[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      dr[i] = dp[i * m + col];
+      di[i] = dp[i * m + col + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) {  // phi'_j = (gamma/K) sum_i Y_ji ds_i
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        const double yr = c[3 + 2 * (5 * j + i)], yi = c[4 + 2 * (5 * j + i)];
+        sr += yr * dr[i] - yi * di[i];
+        si += yr * di[i] + yi * dr[i];
+      }
+      ph[j * m + col] = gk * sr;
+      ph[j * m + col + 1] = gk * si;
+    }
+    double qr = 0.0, qi = 0.0, pr = 0.0, pi = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const double vr = c[53 + 2 * i], vi = c[54 + 2 * i];
+      qr += dr[i] * vr - di[i] * vi;  // q = sum_i ds_i v_i
+      qi += dr[i] * vi + di[i] * vr;
+      const double wr = c[63 + 2 * i], wi = c[64 + 2 * i];
+      pr += dr[i] * wr + di[i] * wi;  // q' = sum_j conj(ds_j) w_j
+      pi += dr[i] * wi - di[i] * wr;
+    }
+    // v = (beta/2) s - (gamma/K^2) (q + conj(q'))
+    vp[col] = hb * sp[col] - gk2 * (qr + pr);
+    vp[col + 1] = hb * sp[col + 1] - gk2 * (qi - pi);
+  }
+}
+
+void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const double *yd,
+                      const double *log_om, const double *w, const double *sv, const double *sd,
+                      int64_t n, int64_t m_cols, double psi, const double *sig, double *phi,
+                      double *vs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_grad_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, pts, y0, yd,
+                     log_om, w, sv, sd, n, m_cols, psi, sig, phi, vs);
+}
+
+// ---------------------------------------------------------------------------------------------
+// C_g = sum over the rows r of group g of U[r]^T V[r] (U, V: [rows][m] row-major, FP64). A
+// workgroup computes a 64 x 64 tile of C (4 waves, 32 x 32 each, v_mfma_f64_16x16x4f64) over its
+// rows, staged through LDS in blocks of 16 rows. Output: part[g][m][m].
+// v_mfma_f64_16x16x4f64 layout: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k,
+// D[i][j] at lane j + 16 (i % 4), element i / 4.
+// ---------------------------------------------------------------------------------------------
+static constexpr int GT = 64;   // output tile edge
+static constexpr int GKB = 16;  // rows per LDS block
+
+__global__ void __launch_bounds__(256) k_gemm_tn_f64(const double *__restrict__ u,
+                                                     const double *__restrict__ v, int64_t rows,
+                                                     int64_t m, int64_t rows_per_group,
+                                                     double *__restrict__ part) {
+  __shared__ double su[GKB][GT], sv_[GKB][GT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * GT, d0 = (int64_t)blockIdx.y * GT;
+  const int64_t g = blockIdx.z;
+  const int64_t r0 = g * rows_per_group;
+  const int64_t r1 = r0 + rows_per_group < rows ? r0 + rows_per_group : rows;
+  double4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
+  const int lrow = tid / 16, lcol = (tid % 16) * 4;
+  for (int64_t rb = r0; rb < r1; rb += GKB) {
+    const int64_t r = rb + lrow;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t cu = c0 + lcol + q, cv = d0 + lcol + q;
+      su[lrow][lcol + q] = (r < r1 && cu < m) ? u[r * m + cu] : 0.0;
+      sv_[lrow][lcol + q] = (r < r1 && cv < m) ? v[r * m + cv] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GKB / 4; kk++) {
+      const int kr = 4 * kk + lane / 16;
+      double a[2], b[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a[t] = su[kr][wr + 16 * t + lane % 16];
+        b[t] = sv_[kr][wc + 16 * t + lane % 16];
+      }
+#pragma unroll
+      for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+        for (int tb = 0; tb < 2; tb++)
+          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double *out = part + g * m * m;
+#pragma unroll
+  for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+    for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int64_t i = c0 + wr + 16 * ta + 4 * e + lane / 16;
+        const int64_t j = d0 + wc + 16 * tb + lane % 16;
+        if (i < m && j < m) out[i * m + j] = acc[ta][tb][e];
+      }
+}
+
+// acc[i] += sum_g part[g][i], in group order.
+__global__ void k_gemm_accumulate(const double *__restrict__ part, int64_t n_groups, int64_t mm,
+                                  double *__restrict__ acc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mm) return;
+  double s = acc[i];
+  for (int64_t g = 0; g < n_groups; g++) s += part[g * mm + i];
+  acc[i] = s;
+}
+
+int64_t gemm_tn_groups(int64_t rows, int64_t m_cols) {
+  const int64_t nt = (m_cols + GT - 1) / GT;
+  int64_t g = std::max<int64_t>(1024 / (nt * nt), 1);
+  g = std::min<int64_t>(g, std::max<int64_t>(rows / 256, 1));
+  return g;
+}
+
+void launch_gemm_tn_acc(hipStream_t st, const double *u, const double *v, int64_t rows,
+                        int64_t m_cols, int64_t n_groups, double *part, double *acc) {
+  if (rows <= 0) return;
+  const int64_t nt = (m_cols + GT - 1) / GT;
+  const int64_t rpg = ((rows + n_groups - 1) / n_groups + GKB - 1) / GKB * GKB;
+  hipLaunchKernelGGL(k_gemm_tn_f64, dim3((unsigned)nt, (unsigned)nt, (unsigned)n_groups),
+                     dim3(256), 0, st, u, v, rows, m_cols, rpg, part);
+  const int64_t mm = m_cols * m_cols;
+  hipLaunchKernelGGL(k_gemm_accumulate, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, st, part,
+                     n_groups, mm, acc);
+}
+
+// grad[a nb + b] = d sigma^2 / d h_re(a, b), grad[nb^2 + a nb + b] = d sigma^2 / d h_im(a, b)
+// from the real-ified contraction C (m x m): A_ab = (C[2a][2b] + C[2a+1][2b+1])
+// + i (C[2a][2b+1] - C[2a+1][2b]), H = A + A^H.
+__global__ void k_grad_assemble(const double *__restrict__ cm, int nb, int64_t m,
+                                const double *__restrict__ sig, double *__restrict__ grad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nb * nb) return;
+  const int a = (int)(e / nb), b = (int)(e % nb);
+  const double tw = sig[3];
+  const bool ok = isfinite(sig[1]) && isfinite(sig[5]) && tw >= 1e-12;
+  auto ar = [&](int x, int y) {
+    return cm[(2 * (int64_t)x) * m + 2 * y] + cm[(2 * (int64_t)x + 1) * m + 2 * y + 1];
+  };
+  auto ai = [&](int x, int y) {
+    return cm[(2 * (int64_t)x) * m + 2 * y + 1] - cm[(2 * (int64_t)x + 1) * m + 2 * y];
+  };
+  const double hr = ar(a, b) + ar(b, a), hi = ai(a, b) - ai(b, a);
+  double gre = ok ? hr / tw : 0.0, gim = ok ? -hi / tw : 0.0;
+  if (a == b) gre *= 0.5;
+  if (a >= b) gim = 0.0;
+  grad[e] = gre;
+  grad[(int64_t)nb * nb + e] = gim;
+}
+
+void launch_grad_assemble(hipStream_t st, const double *cm, int nb, int64_t m_cols,
+                          const double *sig, double *grad) {
+  const int64_t nn = (int64_t)nb * nb;
+  hipLaunchKernelGGL(k_grad_assemble, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, cm, nb,
+                     m_cols, sig, grad);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Adam on (h_re, h_im) = (Re G, Im G): the update rule written inline at quintic.rs:5421-5437
+// (t <- t + 1, scale = lr sqrt(1 - b2^t) / (1 - b1^t), m, v moments, x -= scale m / (sqrt v + eps)),
+// both halves with the same t, then the Hermitian projection (h_re symmetrised, h_im
+// antisymmetrised with a zero diagonal), written back into G (complex [nb][nb]). One thread per
+// pair a <= b.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_adam_herm(double *__restrict__ g, int nb, const double *__restrict__ grad,
+                            double *__restrict__ mo, double *__restrict__ ve, double scale,
+                            double beta1, double beta2, double eps) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nb * nb) return;
+  const int a = (int)(e / nb), b = (int)(e % nb);
+  if (a > b) return;
+  const int64_t n2 = (int64_t)nb * nb;
+  const int64_t ab = (int64_t)a * nb + b, ba = (int64_t)b * nb + a;
+  auto step = [&](int64_t k, double x) {
+    const double gk = grad[k];
+    mo[k] = beta1 * mo[k] + (1.0 - beta1) * gk;
+    ve[k] = beta2 * ve[k] + (1.0 - beta2) * gk * gk;
+    return x - scale * mo[k] / (sqrt(ve[k]) + eps);
+  };
+  if (a == b) {
+    const double re = step(ab, g[2 * ab]);
+    step(n2 + ab, g[2 * ab + 1]);  // moments advance; the diagonal of h_im is then zeroed
+    g[2 * ab] = re;
+    g[2 * ab + 1] = 0.0;
+    return;
+  }
+  const double re_ab = step(ab, g[2 * ab]), re_ba = step(ba, g[2 * ba]);
+  const double im_ab = step(n2 + ab, g[2 * ab + 1]), im_ba = step(n2 + ba, g[2 * ba + 1]);
+  const double re = 0.5 * (re_ab + re_ba), im = 0.5 * (im_ab - im_ba);
+  g[2 * ab] = re;
+  g[2 * ab + 1] = im;
+  g[2 * ba] = re;
+  g[2 * ba + 1] = -im;
+}
+
+// Trace renormalisation to tr Re G = nb (when the trace exceeds 1e-10). One workgroup.
+__global__ void __launch_bounds__(256) k_trace_renorm(double *__restrict__ g, int nb) {
+  __shared__ double sh[256];
+  double t = 0.0;
+  for (int a = threadIdx.x; a < nb; a += 256) t += g[2 * ((int64_t)a * nb + a)];
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double tr = sh[0];
+  if (!(tr > 1e-10)) return;
+  const double sc = (double)nb / tr;
+  for (int64_t e = threadIdx.x; e < 2 * (int64_t)nb * nb; e += 256) g[e] *= sc;
+}
+
+void launch_adam_step(hipStream_t st, double *g_cplx, int nb, const double *grad, double *m,
+                      double *v, double scale, double beta1, double beta2, double eps) {
+  const int64_t nn = (int64_t)nb * nb;
+  hipLaunchKernelGGL(k_adam_herm, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, g_cplx, nb,
+                     grad, m, v, scale, beta1, beta2, eps);
+  hipLaunchKernelGGL(k_trace_renorm, dim3(1), dim3(256), 0, st, g_cplx, nb);
+}
+
+}  // namespace cy3q

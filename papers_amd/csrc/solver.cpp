@@ -134,6 +134,10 @@ struct cy3q_solver {
   half8_t *bfrag = nullptr;
   int16_t *cglist = nullptr;
 
+  // sigma^2 gradient and Adam (allocated on first use)
+  double *grad_dev = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  double adam_beta1 = 0.9, adam_beta2 = 0.999, adam_eps = 1e-8;
+
   // profiling
   bool profiling = false;
   double stage_ms[STG_COUNT] = {0};
@@ -542,15 +546,14 @@ int w_from_g(cy3q_solver *s) {
 
 // Contract W with the section rows and the derivative rows in point chunks, then eta per
 // point and the two-pass reduction.
-int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
+// eta for every point (device, [n]) and the two-pass reduction into s->sig_out.
+int sigma_device(cy3q_solver *s, double *eta) {
   const int64_t n = s->n, m = s->m;
   ensure_bmat(s);
-  s->stage_begin(STG_SIGMA);
   const int64_t chunk = std::min<int64_t>(n, 65536);
-  double *y0 = nullptr, *yd = nullptr, *eta = nullptr;
+  double *y0 = nullptr, *yd = nullptr;
   ST_TRY(s->alloc(&y0, chunk * m));
   ST_TRY(s->alloc(&yd, chunk * 5 * m));
-  ST_TRY(s->alloc(&eta, n));
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     int64_t cn = std::min(chunk, n - p0);
     launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, y0, 0, 0);
@@ -558,15 +561,79 @@ int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
     launch_sigma_eta(s->st, s->pts + 10 * p0, y0, yd, s->log_om + p0, cn, m, s->psi, eta + p0);
   }
   launch_sigma_reduce(s->st, eta, s->w, n, s->sig_out);
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(y0);
+  s->release(yd);
+  return CY3Q_OK;
+}
+
+int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
+  const int64_t n = s->n;
+  s->stage_begin(STG_SIGMA);
+  double *eta = nullptr;
+  ST_TRY(s->alloc(&eta, n));
+  ST_TRY(sigma_device(s, eta));
   s->stage_end(STG_SIGMA);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out5, s->sig_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   if (eta_host)
     HIP_TRY(hipMemcpyAsync(eta_host, eta, n * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(eta);
+  s->collect();
+  return CY3Q_OK;
+}
+
+// sigma_L2^2 and its gradient (sigma_squared_and_gradient, quintic.rs:2714-2860) into
+// s->grad_dev [2 nb^2]; out6 = the sigma reduction (sigma_L1, kappa, sigma_L2^2, sum w, count,
+// gamma numerator sum). Pass 1 is the sigma pass; pass 2 recomputes the contracted rows per
+// chunk, builds the adjoint rows and contracts them with the section and derivative rows.
+int sigma_grad_impl(cy3q_solver *s, double *out6) {
+  const int64_t n = s->n, m = s->m;
+  const int nb = s->nb;
+  if (!s->grad_dev) {
+    ST_TRY(s->alloc(&s->grad_dev, 2 * (int64_t)nb * nb));
+    ST_TRY(s->alloc(&s->adam_m, 2 * (int64_t)nb * nb));
+    ST_TRY(s->alloc(&s->adam_v, 2 * (int64_t)nb * nb));
+  }
+  s->stage_begin(STG_SIGMA);
+  double *eta = nullptr;
+  ST_TRY(s->alloc(&eta, n));
+  ST_TRY(sigma_device(s, eta));
+  s->release(eta);
+  const int64_t chunk = std::min<int64_t>(n, 65536);
+  double *y0 = nullptr, *yd = nullptr, *phi = nullptr, *vs = nullptr, *part = nullptr,
+         *cm = nullptr;
+  const int64_t groups = gemm_tn_groups(5 * chunk, m);
+  ST_TRY(s->alloc(&y0, chunk * m));
+  ST_TRY(s->alloc(&yd, chunk * 5 * m));
+  ST_TRY(s->alloc(&phi, chunk * 5 * m));
+  ST_TRY(s->alloc(&vs, chunk * m));
+  ST_TRY(s->alloc(&part, groups * m * m));
+  ST_TRY(s->alloc(&cm, m * m));
+  HIP_TRY(hipMemsetAsync(cm, 0, sizeof(double) * m * m, s->st));
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    const int64_t cn = std::min(chunk, n - p0);
+    launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, y0, 0, 0);
+    launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, yd, 0, 0);
+    launch_grad_rows(s->st, s->pts + 10 * p0, y0, yd, s->log_om + p0, s->w + p0, s->sv + p0 * m,
+                     s->sd + p0 * 5 * m, cn, m, s->psi, s->sig_out, phi, vs);
+    launch_gemm_tn_acc(s->st, s->sd + p0 * 5 * m, phi, cn * 5, m,
+                       std::min(groups, gemm_tn_groups(5 * cn, m)), part, cm);
+    launch_gemm_tn_acc(s->st, s->sv + p0 * m, vs, cn, m, std::min(groups, gemm_tn_groups(cn, m)),
+                       part, cm);
+  }
+  launch_grad_assemble(s->st, cm, nb, m, s->sig_out, s->grad_dev);
+  s->stage_end(STG_SIGMA);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out6, s->sig_out, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
   s->release(y0);
   s->release(yd);
-  s->release(eta);
+  s->release(phi);
+  s->release(vs);
+  s->release(part);
+  s->release(cm);
   s->collect();
   return CY3Q_OK;
 }
@@ -787,6 +854,59 @@ int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, in
     s->collect();
   }
   if (iters) *iters = done;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_sigma_sq_grad(cy3q_solver *s, double *sigma_l2sq, double *grad,
+                              double *sigma_l1) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  double o[6];
+  ST_TRY(sigma_grad_impl(s, o));
+  if (sigma_l2sq) *sigma_l2sq = o[2];
+  if (sigma_l1) *sigma_l1 = o[0];
+  if (grad)
+    HIP_TRY(hipMemcpy(grad, s->grad_dev, sizeof(double) * 2 * s->nb * s->nb,
+                      hipMemcpyDeviceToHost));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_adam(cy3q_solver *s, double beta1, double beta2, double eps) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (!(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0 && eps > 0.0))
+    return fail(CY3Q_ERR_ARG, "Adam needs 0 <= beta < 1 and eps > 0");
+  s->adam_beta1 = beta1;
+  s->adam_beta2 = beta2;
+  s->adam_eps = eps;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_sigma_refine_analytic(cy3q_solver *s, int64_t n_iter, double lr,
+                                      double *history, int64_t *n_done) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_iter < 0) return fail(CY3Q_ERR_ARG, "n_iter must be >= 0");
+  HIP_TRY(hipSetDevice(s->device));
+  const int nb = s->nb;
+  const int64_t n2 = 2 * (int64_t)nb * nb;
+  int64_t done = 0;
+  if (n_iter > 0) {
+    double o[6];
+    ST_TRY(sigma_grad_impl(s, o));  // allocates the Adam state on first use
+    HIP_TRY(hipMemsetAsync(s->adam_m, 0, sizeof(double) * n2, s->st));
+    HIP_TRY(hipMemsetAsync(s->adam_v, 0, sizeof(double) * n2, s->st));
+    for (int64_t it = 0; it < n_iter; it++) {
+      if (it > 0) ST_TRY(sigma_grad_impl(s, o));
+      if (!std::isfinite(o[0])) break;
+      if (history) history[it] = o[0];
+      done = it + 1;
+      const double t1 = (double)(it + 1);
+      const double bc1 = 1.0 - std::pow(s->adam_beta1, t1), bc2 = 1.0 - std::pow(s->adam_beta2, t1);
+      launch_adam_step(s->st, s->g, nb, s->grad_dev, s->adam_m, s->adam_v,
+                       lr * (std::sqrt(bc2) / bc1), s->adam_beta1, s->adam_beta2, s->adam_eps);
+      ST_TRY(w_from_g(s));
+    }
+  }
+  if (n_done) *n_done = done;
   return CY3Q_OK;
 }
 

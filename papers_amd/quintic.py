@@ -71,6 +71,9 @@ def _load():
     L.cy3q_solver_donaldson_solve.argtypes = [vp, i64, dbl, C.POINTER(i64), vp]
     L.cy3q_solver_sigma.argtypes = [vp, C.POINTER(dbl)]
     L.cy3q_solver_sigma_ext.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl), vp]
+    L.cy3q_solver_sigma_sq_grad.argtypes = [vp, C.POINTER(dbl), vp, C.POINTER(dbl)]
+    L.cy3q_solver_sigma_refine_analytic.argtypes = [vp, i64, dbl, vp, C.POINTER(i64)]
+    L.cy3q_solver_set_adam.argtypes = [vp, dbl, dbl, dbl]
     L.cy3q_solver_get_h_block.argtypes = [vp, vp]
     L.cy3q_solver_set_h_block.argtypes = [vp, vp]
     L.cy3q_solver_get_t.argtypes = [vp, vp]
@@ -210,6 +213,26 @@ class QuinticSolver:
         eta = np.zeros(self.n_points) if want_eta else None
         _check(_load().cy3q_solver_sigma_ext(self._h, C.byref(s), C.byref(k), C.byref(l2), _ptr(eta)))
         return s.value, k.value, l2.value, eta
+
+    def sigma_squared_and_gradient(self):
+        """sigma_squared_and_gradient (quintic.rs:2714-2860): (sigma_L2^2, grad [2 N^2], sigma_L1).
+        grad[:N^2] = d/dh_re(a, b), diagonal halved; grad[N^2:] = d/dh_im(a, b) for a < b."""
+        s2, s1 = C.c_double(), C.c_double()
+        g = np.zeros(2 * self.n_basis * self.n_basis)
+        _check(_load().cy3q_solver_sigma_sq_grad(self._h, C.byref(s2), _ptr(g), C.byref(s1)))
+        return s2.value, g, s1.value
+
+    def sigma_refine_analytic(self, n_iter: int, lr: float) -> list:
+        """QuinticSolver::sigma_refine_analytic (quintic.rs:453-514); returns the sigma_L1
+        history (one value per Adam step, before the step)."""
+        hist = np.zeros(max(n_iter, 1))
+        done = C.c_int64()
+        _check(_load().cy3q_solver_sigma_refine_analytic(self._h, n_iter, lr, _ptr(hist),
+                                                         C.byref(done)))
+        return list(hist[: done.value])
+
+    def set_adam(self, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        _check(_load().cy3q_solver_set_adam(self._h, beta1, beta2, eps))
 
     # ---- workspace access ---------------------------------------------------------------
     @property

@@ -225,3 +225,65 @@ def test_published_sigma_fs_k4_seed100(oracle):
     # Same file, k=4 record, seed 100: sigma_fs_identity = 0.1405790096262153.
     P = oracle.Problem(4, 200000, 100)
     assert abs(P.sigma() / 0.1405790096262153 - 1) < 1e-12
+
+
+# ------------------------------------------------------------------------------------------
+# sigma_L2^2 analytic gradient and Adam (quintic.rs:2194-2860, 453-514)
+# ------------------------------------------------------------------------------------------
+def _herm_block(X, Y):
+    nb = X.shape[0]
+    G = X + 1j * Y
+    H = np.zeros((2 * nb, 2 * nb))
+    H[0::2, 0::2] = G.real
+    H[0::2, 1::2] = -G.imag
+    H[1::2, 0::2] = G.imag
+    H[1::2, 1::2] = G.real
+    return H
+
+
+def test_sigma_sq_grad_matches_finite_differences(oracle):
+    # The reference checks its analytic gradient against central differences
+    # (quintic.rs:5646-5696); the same check pins the restatement: along random Hermitian
+    # directions (X symmetric, Y antisymmetric) the derivative is
+    # sum_{a<=b} g_re(a,b) X_ab + sum_{a<b} g_im(a,b) Y_ab.
+    P = oracle.Problem(3, 2500, 7)
+    nb = P.nb
+    h, _ = oracle.donaldson_step(P.sv, P.w, P.h)
+    s2, g, s1 = P.sigma_sq_grad(h)
+    assert abs(s2 - P.sigma_l2sq(h)) < 1e-13 * s2
+    assert abs(s1 - P.sigma(h)) < 1e-13 * s1
+    gre, gim = g[: nb * nb].reshape(nb, nb), g[nb * nb:].reshape(nb, nb)
+    assert np.allclose(np.tril(gim), 0.0)
+    assert np.allclose(gre, gre.T)
+    rng = np.random.default_rng(3)
+    iu, iu1 = np.triu_indices(nb, 0), np.triu_indices(nb, 1)
+    for _ in range(3):
+        X = rng.standard_normal((nb, nb))
+        X = 0.5 * (X + X.T)
+        Y = rng.standard_normal((nb, nb))
+        Y = 0.5 * (Y - Y.T)
+        eps = 1e-6
+        D = _herm_block(X, Y)
+        fd = (P.sigma_l2sq(h + eps * D) - P.sigma_l2sq(h - eps * D)) / (2 * eps)
+        an = np.sum(gre[iu] * X[iu]) + np.sum(gim[iu1] * Y[iu1])
+        assert abs(fd - an) < 1e-6 * abs(an), (fd, an)
+
+
+def test_adam_first_step_closed_form(oracle):
+    # From zero moments the first Adam step is lr * g / (|g| + eps / sqrt(1 - beta2)) per
+    # parameter (scale = lr sqrt(1-b2)/(1-b1), m = (1-b1) g, v = (1-b2) g^2); a symmetric real
+    # gradient on one off-diagonal pair moves only Re G_01 = Re G_10, and the trace is unchanged.
+    nb = 4
+    h = np.eye(2 * nb)
+    g = np.zeros(2 * nb * nb)
+    g[0 * nb + 1] = g[1 * nb + 0] = 0.3
+    m, v = np.zeros_like(g), np.zeros_like(g)
+    lr = 1e-2
+    h2, t = oracle.adam_step(h, g, m, v, 0, lr)
+    assert t == 1
+    step = lr * 0.3 / (0.3 + 1e-8 / np.sqrt(1 - 0.999))
+    assert abs(h2[0, 2] - (-step)) < 1e-15 and abs(h2[2, 0] - (-step)) < 1e-15
+    assert abs(h2[1, 3] - (-step)) < 1e-15
+    d = h2 - h
+    d[0, 2] = d[2, 0] = d[1, 3] = d[3, 1] = 0.0
+    assert np.max(np.abs(d)) == 0.0
