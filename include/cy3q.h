@@ -140,6 +140,91 @@ int cy3q_solver_stage_times(cy3q_solver *s, double *ms, int64_t *count, int cap,
 const char *cy3q_stage_name(int stage);
 int cy3q_solver_synchronize(cy3q_solver *s);
 
+/* ============================================================================================
+ * Galerkin assembly: metric Laplacian and Yukawa triple overlaps.
+ * Host buffers in, host buffers out; every stage runs on `device` in FP64 (MFMA for the
+ * assembly contractions). Complex arrays are interleaved (re, im) doubles.
+ * ========================================================================================== */
+
+/* MetricLaplacianConfig (route34/metric_laplacian.rs:224-271) and its Default (:277-290). */
+typedef struct cy3q_laplacian_config {
+  uint32_t max_total_degree; /* test monomials of total degree 1..max (default 4) */
+  int64_t n_low_eigenvalues; /* default 15 */
+  int64_t jacobi_max_sweeps; /* default 128 */
+  double jacobi_tol;         /* default 1e-12 */
+  double mass_regularisation; /* eps of the (M + eps ||M||_F I)^-1 inverse, default 1e-10 */
+} cy3q_laplacian_config;
+void cy3q_laplacian_config_default(cy3q_laplacian_config *cfg);
+
+/* Number of test monomials of build_test_basis(max_total_degree) (metric_laplacian.rs:87-115)
+ * and, when exps != NULL, their exponents ([n][8]). */
+int cy3q_laplacian_basis(uint32_t max_total_degree, uint32_t *exps, int64_t cap, int64_t *n);
+
+/* compute_metric_laplacian_spectrum (route34/metric_laplacian.rs:296-418), the ambient form
+ * over a MetricBackground (route34/hym_hermitian.rs:257-294): points [n][8] complex, weights
+ * [n], omega [n] complex. M, K as build_mass_matrix / build_stiffness_matrix (:424-524), then the
+ * spectrum of the Hermitian part of (M + eps ||M||_F I)^-1 K, |lambda| ascending (:342-389).
+ * eig_low: min(n_low, basis_dim) values; eig_full ([basis_dim]), mass and stiffness
+ * ([basis_dim][basis_dim] complex) may be NULL. The Jacobi stops on the relative off-diagonal
+ * norm (jacobi_tol relative to ||A||_F) or after jacobi_max_sweeps sweeps. */
+int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
+                                   const double *omega, int64_t n,
+                                   const cy3q_laplacian_config *cfg, int device, double *eig_low,
+                                   double *eig_full, double *mass, double *stiffness,
+                                   int64_t *basis_dim);
+
+/* The metric Laplacian of the solver's current h on the quintic, with the intrinsic inverse
+ * metric g^{a b-bar} (the refinement metric_laplacian.rs:28-41 describes) on the test functions
+ * phi_ab = s_a conj(s_b) / |z|^(2 d), s the raw degree-d monomials (d = section_degree <= 4,
+ * alpha = a * C(d+4,4) + b). Measure w |Omega|^2 / sum w as build_mass_matrix. The generalized
+ * problem K v = lambda M v is solved in FP64 on the GPU by reduction with M^(-1/2) over the
+ * eigenvectors of M with eigenvalue > deflation_tol * max (the null-space deflation of
+ * metric_laplacian.rs:250-263). eig_low: the n_low smallest eigenvalues, ascending; *rank:
+ * dimension kept. mass/stiffness ([B][B] complex) may be NULL. */
+int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int64_t n_low,
+                                   double deflation_tol, int64_t max_sweeps, double *eig_low,
+                                   double *mass, double *stiffness, int64_t *basis_dim,
+                                   int64_t *rank);
+
+/* Hermitian eigensolver used by both (hermitian_jacobi_n, metric_laplacian.rs:616-693):
+ * a: [n][n] complex Hermitian; eig: n values ascending; vecs ([n][n] complex, column j for
+ * eig[j]) may be NULL. *sweeps: sweeps run. */
+int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double tol, int device,
+                         double *eig, double *vecs, int64_t *sweeps);
+
+/* sz_weights (route34/yukawa_overlap_real.rs:222-268): w / |Omega|^2 renormalised to sum one,
+ * zero for an unusable point; *uniformity = 1 / (1 + sd/mean). */
+int cy3q_yukawa_sz_weights(const double *weights, const double *omega, int64_t n, int device,
+                           double *sz, double *uniformity);
+
+/* triple_overlap_cpu (route34/yukawa_overlap_real.rs:271-315): modes [n_modes][n] complex,
+ * omega [n] complex, weights [n], h_v_diag [n_modes] or NULL (= 1). y: [n_modes]^3 complex,
+ * Y_ijk = sqrt(h_i h_j h_k) sum_p w_p conj(Omega_p) psi_i psi_j psi_k over usable points.
+ * n_modes <= 64. */
+int cy3q_yukawa_triple_overlap(const double *modes, const double *omega, const double *weights,
+                               const double *h_v_diag, int64_t n_modes, int64_t n, int device,
+                               double *y);
+
+/* compute_yukawa_couplings (route34/yukawa_overlap_real.rs:372-525) given h_v_diag: the SZ
+ * weights, the central tensor, the bootstrap standard deviation over n_bootstrap resamples and
+ * the N vs N/2 convergence ratio. Resample b draws n indices floor(u n) from the lib.rs LCG
+ * seeded with seed + b * 2654435761; the reference's ChaCha20 resampling stream is not
+ * reproduced. y: [n_modes]^3 complex, y_unc: [n_modes]^3. */
+int cy3q_yukawa_couplings(const double *modes, const double *omega, const double *weights,
+                          const double *h_v_diag, int64_t n_modes, int64_t n,
+                          int64_t n_bootstrap, uint64_t seed, int device, double *y,
+                          double *y_unc, double *uniformity, double *convergence_ratio);
+
+/* Triple overlaps of n_modes random holomorphic modes psi_a = sum_b C_ba s_b over the raw
+ * degree-d monomials s at the solver's points, with measure c_p = w_p |Omega_p|^2:
+ * Y_abc = sum_p c_p psi_a psi_b psi_c for the C(n_modes + 2, 3) triples a <= b <= c (in the
+ * order a, then b, then c ascending). C_ba = (g1 + i g2) / sqrt(2) with g Box-Muller normals of
+ * the lib.rs LCG seeded with coeff_seed, drawn b-major. y_triples: [n_triples] complex;
+ * coeffs ([nmono][n_modes] complex) may be NULL. */
+int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_modes,
+                                    uint64_t coeff_seed, double *y_triples, double *coeffs,
+                                    int64_t *n_triples);
+
 #ifdef __cplusplus
 }
 #endif

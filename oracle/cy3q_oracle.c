@@ -836,11 +836,11 @@ static double det3(const double g[3][3][2]) { /* :4782-4809 */
 #undef CMI
 }
 
-/* Per-point eta (r_per_point of :1311-1466); NaN for skipped points. */
-static double eta_point(const double *s, const double *d, const double *z, double log_om,
-                        const double *h, int64_t nb, double psi, double *scratch) {
+/* The ambient metric g_amb = ddK/K - dK dK^H / K^2 of one point (ambient_metric_5x5,
+ * :4474-4516, as inlined in r_per_point :1311-1466). Returns K floored at 1e-30. */
+static double ambient_ga(const double *s, const double *d, const double *h, int64_t nb,
+                         double *scratch, double ga[5][5][2]) {
     int64_t two_n = 2 * nb;
-    if (!isfinite(log_om)) return NAN;
     double k = 0.0;
     for (int64_t i = 0; i < two_n; i++) {
         double rs = 0.0;
@@ -890,7 +890,6 @@ static double eta_point(const double *s, const double *d, const double *z, doubl
             ddk[i][j][1] = si;
         }
     double ik = 1.0 / ks, ik2 = ik * ik;
-    double ga[5][5][2];
     for (int i = 0; i < 5; i++)
         for (int j = 0; j < 5; j++) {
             double t1r = ddk[i][j][0] * ik, t1i = ddk[i][j][1] * ik;
@@ -899,6 +898,15 @@ static double eta_point(const double *s, const double *d, const double *z, doubl
             ga[i][j][0] = t1r - pr * ik2;
             ga[i][j][1] = t1i - pi * ik2;
         }
+    return ks;
+}
+
+/* Per-point eta (r_per_point of :1311-1466); NaN for skipped points. */
+static double eta_point(const double *s, const double *d, const double *z, double log_om,
+                        const double *h, int64_t nb, double psi, double *scratch) {
+    if (!isfinite(log_om)) return NAN;
+    double ga[5][5][2];
+    ambient_ga(s, d, h, nb, scratch, ga);
     double g[10];
     deformed_gradient(z, psi, g);
     int chart, elim;
@@ -1348,4 +1356,657 @@ ORC_API int orc_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Complex helpers with num-complex 0.4's operation order (Mul, Div, norm = hypot, arg = atan2). */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    double re, im;
+} orc_c;
+static inline orc_c oc(double re, double im) { orc_c z = {re, im}; return z; }
+static inline orc_c c_mul(orc_c a, orc_c b) {
+    return oc(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+static inline orc_c c_add(orc_c a, orc_c b) { return oc(a.re + b.re, a.im + b.im); }
+static inline orc_c c_sub(orc_c a, orc_c b) { return oc(a.re - b.re, a.im - b.im); }
+static inline orc_c c_conj(orc_c a) { return oc(a.re, -a.im); }
+static inline double c_nsq(orc_c a) { return a.re * a.re + a.im * a.im; }
+static inline double c_norm(orc_c a) { return hypot(a.re, a.im); }
+static inline orc_c c_div(orc_c a, orc_c b) {
+    double ns = b.re * b.re + b.im * b.im;
+    double re = a.re * b.re + a.im * b.im;
+    double im = a.im * b.re - a.re * b.im;
+    return oc(re / ns, im / ns);
+}
+static inline orc_c c_scale(orc_c a, double s) { return oc(a.re * s, a.im * s); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Metric Laplacian, ambient (Bochner) form: route34/metric_laplacian.rs                       */
+/* ------------------------------------------------------------------------------------------ */
+/* monomials_4 (:118-130) and build_test_basis (:87-115): every bigraded monomial of total
+ * degree 1..max_deg in the 8 ambient coordinates, the constant excluded. exps: [nb][8]. */
+ORC_API int64_t orc_lap_basis(uint32_t max_deg, uint32_t *exps) {
+    int64_t cnt = 0;
+    for (uint32_t dz = 0; dz <= max_deg; dz++)
+        for (uint32_t dw = 0; dw <= max_deg - dz; dw++) {
+            if (dz == 0 && dw == 0) continue;
+            for (uint32_t a = 0; a <= dz; a++)
+                for (uint32_t b = 0; b <= dz - a; b++)
+                    for (uint32_t c = 0; c <= dz - a - b; c++) {
+                        uint32_t e = dz - a - b - c;
+                        for (uint32_t a2 = 0; a2 <= dw; a2++)
+                            for (uint32_t b2 = 0; b2 <= dw - a2; b2++)
+                                for (uint32_t c2 = 0; c2 <= dw - a2 - b2; c2++) {
+                                    uint32_t e2 = dw - a2 - b2 - c2;
+                                    if (exps) {
+                                        uint32_t *o = exps + 8 * cnt;
+                                        o[0] = a; o[1] = b; o[2] = c; o[3] = e;
+                                        o[4] = a2; o[5] = b2; o[6] = c2; o[7] = e2;
+                                    }
+                                    cnt++;
+                                }
+                    }
+        }
+    return cnt;
+}
+
+/* eval_monomial (:132-153): square-and-multiply per coordinate. */
+static orc_c eval_mono8(const double *pt, const uint32_t *e) {
+    orc_c acc = oc(1.0, 0.0);
+    for (int k = 0; k < 8; k++) {
+        uint32_t ee = e[k];
+        if (ee == 0) continue;
+        orc_c base = oc(pt[2 * k], pt[2 * k + 1]);
+        orc_c term = oc(1.0, 0.0);
+        while (ee > 0) {
+            if (ee & 1) term = c_mul(term, base);
+            ee >>= 1;
+            if (ee > 0) base = c_mul(base, base);
+        }
+        acc = c_mul(acc, term);
+    }
+    return acc;
+}
+
+/* build_mass_matrix (:424-471) and build_stiffness_matrix (:473-524) over the basis values
+ * evaluated as in compute_metric_laplacian_spectrum (:318-340). pts: [n][8][2], omega: [n][2].
+ * M, K: [nb][nb][2]. */
+ORC_API void orc_lap_ambient_mk(const double *pts, const double *w, const double *omega, int64_t n,
+                                const uint32_t *exps, int64_t nb, double *M, double *K) {
+    orc_c *bat = (orc_c *)calloc((size_t)(nb * n), sizeof(orc_c));
+    orc_c *dbat = (orc_c *)calloc((size_t)(nb * 8 * n), sizeof(orc_c));
+#pragma omp parallel for schedule(static)
+    for (int64_t al = 0; al < nb; al++) {
+        const uint32_t *ex = exps + 8 * al;
+        for (int64_t p = 0; p < n; p++) bat[al * n + p] = eval_mono8(pts + 16 * p, ex);
+        for (int i = 0; i < 8; i++) {
+            if (ex[i] == 0) continue; /* d_dz (:159-168) returns coefficient 0 */
+            uint32_t de[8];
+            memcpy(de, ex, sizeof(de));
+            double coeff = (double)de[i];
+            de[i] -= 1;
+            orc_c cz = oc(coeff, 0.0);
+            for (int64_t p = 0; p < n; p++)
+                dbat[(al * 8 + i) * n + p] = c_mul(cz, eval_mono8(pts + 16 * p, de));
+        }
+    }
+    orc_c *m = (orc_c *)calloc((size_t)(nb * nb), sizeof(orc_c));
+    orc_c *k = (orc_c *)calloc((size_t)(nb * nb), sizeof(orc_c));
+    double tw_m = 0.0, tw_k = 0.0;
+    /* The reference loops points outermost; rows alpha are independent, so each thread owns a
+     * row range and keeps the point order, giving identical sums. */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t al = 0; al < nb; al++) {
+        for (int64_t p = 0; p < n; p++) {
+            double wp = w[p];
+            if (!isfinite(wp) || wp <= 0.0) continue;
+            orc_c om = oc(omega[2 * p], omega[2 * p + 1]);
+            if (!isfinite(om.re) || !isfinite(om.im)) continue;
+            orc_c pw = oc(c_nsq(om) * wp, 0.0);
+            orc_c sa = bat[al * n + p];
+            if (c_nsq(sa) != 0.0) {
+                orc_c sac = c_conj(sa);
+                for (int64_t be = al; be < nb; be++) {
+                    orc_c inc = c_mul(c_mul(sac, bat[be * n + p]), pw);
+                    m[al * nb + be] = c_add(m[al * nb + be], inc);
+                }
+            }
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t al = 0; al < nb; al++) {
+        for (int64_t p = 0; p < n; p++) {
+            double wp = w[p];
+            if (!isfinite(wp) || wp <= 0.0) continue;
+            orc_c om = oc(omega[2 * p], omega[2 * p + 1]);
+            if (!isfinite(om.re) || !isfinite(om.im)) continue;
+            orc_c pw = oc(c_nsq(om) * wp, 0.0);
+            for (int i = 0; i < 8; i++) {
+                orc_c da = dbat[(al * 8 + i) * n + p];
+                if (c_nsq(da) == 0.0) continue;
+                orc_c dac = c_conj(da);
+                for (int64_t be = al; be < nb; be++) {
+                    orc_c db = dbat[(be * 8 + i) * n + p];
+                    if (c_nsq(db) == 0.0) continue;
+                    orc_c inc = c_mul(c_mul(dac, db), pw);
+                    k[al * nb + be] = c_add(k[al * nb + be], inc);
+                }
+            }
+        }
+    }
+    for (int64_t p = 0; p < n; p++) {
+        double wp = w[p];
+        if (!isfinite(wp) || wp <= 0.0) continue;
+        if (!isfinite(omega[2 * p]) || !isfinite(omega[2 * p + 1])) continue;
+        tw_m += wp;
+        tw_k += wp;
+    }
+    orc_c *mats[2] = {m, k};
+    double tws[2] = {tw_m, tw_k};
+    for (int q = 0; q < 2; q++) {
+        orc_c *x = mats[q];
+        if (tws[q] > 0.0) {
+            orc_c inv = oc(1.0 / tws[q], 0.0);
+            for (int64_t e = 0; e < nb * nb; e++) x[e] = c_mul(x[e], inv);
+        }
+        for (int64_t i = 0; i < nb; i++) {
+            orc_c z = x[i * nb + i];
+            x[i * nb + i] = oc(q == 0 ? z.re : fmax(z.re, 0.0), 0.0);
+            for (int64_t j = i + 1; j < nb; j++) x[j * nb + i] = c_conj(x[i * nb + j]);
+        }
+    }
+    memcpy(M, m, sizeof(orc_c) * nb * nb);
+    memcpy(K, k, sizeof(orc_c) * nb * nb);
+    free(bat);
+    free(dbat);
+    free(m);
+    free(k);
+}
+
+/* matmul_complex (:531-545). */
+static void c_matmul(const orc_c *a, const orc_c *b, int64_t n, orc_c *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) {
+            orc_c acc = oc(0.0, 0.0);
+            for (int64_t q = 0; q < n; q++) acc = c_add(acc, c_mul(a[i * n + q], b[q * n + j]));
+            out[i * n + j] = acc;
+        }
+}
+
+/* invert_hermitian_regularised (:547-613): Gauss-Jordan on [H + lambda I | I] with partial
+ * pivoting on |.|, lambda = max(eps ||H||_F, 1e-30). */
+ORC_API void orc_herm_inv_reg(const double *h_in, int64_t n, double eps, double *out_d) {
+    const orc_c *h = (const orc_c *)h_in;
+    orc_c *out = (orc_c *)out_d;
+    if (n == 0) return;
+    double fs = 0.0;
+    for (int64_t e = 0; e < n * n; e++) fs += c_nsq(h[e]);
+    double lambda = fmax(sqrt(fs) * eps, 1.0e-30);
+    int64_t tn = 2 * n;
+    orc_c *m = (orc_c *)calloc((size_t)(n * tn), sizeof(orc_c));
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < n; j++) m[i * tn + j] = h[i * n + j];
+        m[i * tn + i] = c_add(m[i * tn + i], oc(lambda, 0.0));
+        m[i * tn + n + i] = oc(1.0, 0.0);
+    }
+    for (int64_t kk = 0; kk < n; kk++) {
+        int64_t piv = kk;
+        double best = c_norm(m[kk * tn + kk]);
+        for (int64_t r = kk + 1; r < n; r++) {
+            double v = c_norm(m[r * tn + kk]);
+            if (v > best) {
+                best = v;
+                piv = r;
+            }
+        }
+        if (best < 1.0e-300) {
+            double inv = 1.0 / fmax(lambda, 1.0e-30);
+            for (int64_t e = 0; e < n * n; e++) out[e] = oc(0.0, 0.0);
+            for (int64_t i = 0; i < n; i++) out[i * n + i] = oc(inv, 0.0);
+            free(m);
+            return;
+        }
+        if (piv != kk)
+            for (int64_t j = 0; j < tn; j++) {
+                orc_c t = m[kk * tn + j];
+                m[kk * tn + j] = m[piv * tn + j];
+                m[piv * tn + j] = t;
+            }
+        orc_c pinv = c_div(oc(1.0, 0.0), m[kk * tn + kk]);
+        for (int64_t j = 0; j < tn; j++) m[kk * tn + j] = c_mul(m[kk * tn + j], pinv);
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < n; r++) {
+            if (r == kk) continue;
+            orc_c f = m[r * tn + kk];
+            if (c_norm(f) == 0.0) continue;
+            for (int64_t j = 0; j < tn; j++)
+                m[r * tn + j] = c_sub(m[r * tn + j], c_mul(f, m[kk * tn + j]));
+        }
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) out[i * n + j] = m[i * tn + n + j];
+    free(m);
+}
+
+/* hermitian_jacobi_n (:616-693): cyclic complex Jacobi; eigenvalues ascending, eigenvector
+ * columns aligned. evecs may be NULL. Returns the number of sweeps run. */
+ORC_API int64_t orc_herm_jacobi(const double *a_in, int64_t n, int64_t max_sweeps, double tol,
+                                double *eig, double *evecs_d) {
+    orc_c *a = (orc_c *)malloc(sizeof(orc_c) * n * n);
+    orc_c *v = (orc_c *)calloc((size_t)(n * n), sizeof(orc_c));
+    memcpy(a, a_in, sizeof(orc_c) * n * n);
+    for (int64_t i = 0; i < n; i++) v[i * n + i] = oc(1.0, 0.0);
+    int64_t sweeps = 0;
+    for (int64_t sw = 0; sw < max_sweeps; sw++) {
+        double off = 0.0;
+        for (int64_t p = 0; p < n; p++)
+            for (int64_t q = 0; q < n; q++)
+                if (p != q) off += c_nsq(a[p * n + q]);
+        if (sqrt(off) < tol) break;
+        sweeps++;
+        for (int64_t p = 0; p + 1 < n; p++)
+            for (int64_t q = p + 1; q < n; q++) {
+                orc_c apq = a[p * n + q];
+                if (c_norm(apq) < tol) continue;
+                double app = a[p * n + p].re, aqq = a[q * n + q].re;
+                double abs_pq = c_norm(apq);
+                double phi = abs_pq < 1.0e-300 ? 0.0 : atan2(apq.im, apq.re);
+                double th = fabs(app - aqq) < 1.0e-300 ? 0.78539816339744830962
+                                                        : 0.5 * atan(2.0 * abs_pq / (app - aqq));
+                double c = cos(th), s = sin(th);
+                orc_c csp = c_mul(oc(cos(phi), sin(phi)), oc(s, 0.0));
+                orc_c cc = oc(c, 0.0);
+                for (int64_t k2 = 0; k2 < n; k2++) {
+                    orc_c akp = a[k2 * n + p], akq = a[k2 * n + q];
+                    a[k2 * n + p] = c_add(c_mul(akp, cc), c_mul(akq, c_conj(csp)));
+                    a[k2 * n + q] = c_sub(c_mul(akq, cc), c_mul(akp, csp));
+                }
+                for (int64_t k2 = 0; k2 < n; k2++) {
+                    orc_c apk = a[p * n + k2], aqk = a[q * n + k2];
+                    a[p * n + k2] = c_add(c_mul(apk, cc), c_mul(aqk, csp));
+                    a[q * n + k2] = c_sub(c_mul(aqk, cc), c_mul(apk, c_conj(csp)));
+                }
+                a[p * n + q] = oc(0.0, 0.0);
+                a[q * n + p] = oc(0.0, 0.0);
+                a[p * n + p] = oc(a[p * n + p].re, 0.0);
+                a[q * n + q] = oc(a[q * n + q].re, 0.0);
+                for (int64_t k2 = 0; k2 < n; k2++) {
+                    orc_c vkp = v[k2 * n + p], vkq = v[k2 * n + q];
+                    v[k2 * n + p] = c_add(c_mul(vkp, cc), c_mul(vkq, c_conj(csp)));
+                    v[k2 * n + q] = c_sub(c_mul(vkq, cc), c_mul(vkp, csp));
+                }
+            }
+    }
+    /* Stable ascending sort of (index, value), as Rust's sort_by. */
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * n);
+    for (int64_t i = 0; i < n; i++) idx[i] = i;
+    for (int64_t i = 1; i < n; i++) {
+        int64_t t = idx[i];
+        double tv = a[t * n + t].re;
+        int64_t j = i;
+        while (j > 0 && a[idx[j - 1] * n + idx[j - 1]].re > tv) {
+            idx[j] = idx[j - 1];
+            j--;
+        }
+        idx[j] = t;
+    }
+    for (int64_t k2 = 0; k2 < n; k2++) {
+        eig[k2] = a[idx[k2] * n + idx[k2]].re;
+        if (evecs_d) {
+            orc_c *ev = (orc_c *)evecs_d;
+            for (int64_t i = 0; i < n; i++) ev[i * n + k2] = v[i * n + idx[k2]];
+        }
+    }
+    free(idx);
+    free(a);
+    free(v);
+    return sweeps;
+}
+
+/* compute_metric_laplacian_spectrum (:296-418) after the assembly: M^-1 K with the regularised
+ * inverse, Hermitian projection, Jacobi, then |lambda| sorted ascending. full: [nb]. */
+ORC_API void orc_lap_spectrum_from_mk(const double *M, const double *K, int64_t nb,
+                                      int64_t max_sweeps, double tol, double reg, double *full) {
+    orc_c *minv = (orc_c *)malloc(sizeof(orc_c) * nb * nb);
+    orc_c *l = (orc_c *)malloc(sizeof(orc_c) * nb * nb);
+    orc_herm_inv_reg(M, nb, reg, (double *)minv);
+    c_matmul(minv, (const orc_c *)K, nb, l);
+    for (int64_t i = 0; i < nb; i++) {
+        l[i * nb + i] = oc(l[i * nb + i].re, 0.0);
+        for (int64_t j = i + 1; j < nb; j++) {
+            orc_c avg = c_scale(c_add(l[i * nb + j], c_conj(l[j * nb + i])), 0.5);
+            l[i * nb + j] = avg;
+            l[j * nb + i] = c_conj(avg);
+        }
+    }
+    double *ev = (double *)malloc(sizeof(double) * nb);
+    orc_herm_jacobi((const double *)l, nb, max_sweeps, tol, ev, NULL);
+    for (int64_t i = 0; i < nb; i++) ev[i] = fabs(ev[i]);
+    /* stable sort by |lambda| */
+    for (int64_t i = 1; i < nb; i++) {
+        double t = ev[i];
+        int64_t j = i;
+        while (j > 0 && ev[j - 1] > t) {
+            ev[j] = ev[j - 1];
+            j--;
+        }
+        ev[j] = t;
+    }
+    memcpy(full, ev, sizeof(double) * nb);
+    free(ev);
+    free(minv);
+    free(l);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Metric Laplacian on the quintic with the intrinsic inverse metric g^{a b-bar} of the         */
+/* balanced h, on phi_ab = s_a conj(s_b) / |z|^(2d) with s the raw degree-d monomials.          */
+/* The reference states the ambient form only (metric_laplacian.rs:28-41 names the intrinsic    */
+/* form as the missing refinement); the measure (w |Omega|^2 / sum w) and the Hermitian pencil  */
+/* K v = lambda M v follow metric_laplacian.rs:424-524.                                          */
+/*   M[ab][a'b'] = sum_p c_p conj(phi_ab) phi_a'b' / sum w                                       */
+/*   K[ab][a'b'] = sum_p c_p sum_ij conj(F^ab_j) Y_ij F^a'b'_i / sum w                            */
+/* with c_p = w_p |Omega_p|^2, F_i = d phi / d conj(z_i), Y = T^H g_tan^-1 T the pulled-back      */
+/* inverse metric (compute_y_5x5, quintic.rs:2260-2301). Index alpha = a ns + b.                */
+/* ------------------------------------------------------------------------------------------ */
+ORC_API double orc_lap_quintic_mk(const double *pts, const double *w, const double *log_om,
+                                  const double *sv, const double *sd, const double *h, int64_t n,
+                                  int64_t nb, double psi, const uint32_t *smono, int64_t ns,
+                                  int sdeg, double *M, double *K) {
+    const int64_t B = ns * ns;
+    int nth = 1;
+#ifdef _OPENMP
+    nth = omp_get_max_threads();
+#endif
+    orc_c *pm = (orc_c *)calloc((size_t)(nth * B * B), sizeof(orc_c));
+    orc_c *pk = (orc_c *)calloc((size_t)(nth * B * B), sizeof(orc_c));
+    double *ptw = (double *)calloc((size_t)nth, sizeof(double));
+#pragma omp parallel
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double *scr = (double *)malloc(sizeof(double) * 12 * nb);
+        orc_c *phi = (orc_c *)malloc(sizeof(orc_c) * B);
+        orc_c *F = (orc_c *)malloc(sizeof(orc_c) * B * 5);
+        orc_c *YF = (orc_c *)malloc(sizeof(orc_c) * B * 5);
+        orc_c *sm = (orc_c *)malloc(sizeof(orc_c) * ns);
+        orc_c *G = (orc_c *)malloc(sizeof(orc_c) * ns * 5);
+        orc_c *mm = pm + (int64_t)t * B * B, *kk = pk + (int64_t)t * B * B;
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; p++) {
+            double wp = w[p], lo = log_om[p];
+            if (!isfinite(wp) || wp <= 0.0 || !isfinite(lo)) continue;
+            const double *z = pts + 10 * p;
+            double ga[5][5][2];
+            ambient_ga(sv + p * 2 * nb, sd + p * 10 * nb, h, nb, scr, ga);
+            double g[10];
+            deformed_gradient(z, psi, g);
+            int chart, elim;
+            chart_and_elim(z, g, &chart, &elim, NULL);
+            double fr[3][10];
+            affine_frame(g, chart, elim, fr);
+            double gt[3][3][2], gi[3][3][2];
+            project_tangent(ga, fr, gt);
+            if (!inv3_herm(gt, gi)) continue;
+            orc_c Y[5][5];
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    orc_c acc = oc(0.0, 0.0);
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) {
+                            orc_c ta = oc(fr[a][2 * i], -fr[a][2 * i + 1]);
+                            orc_c tb = oc(fr[b][2 * j], fr[b][2 * j + 1]);
+                            acc = c_add(acc, c_mul(c_mul(ta, oc(gi[a][b][0], gi[a][b][1])), tb));
+                        }
+                    Y[i][j] = acc;
+                }
+            double c = wp * exp(lo);
+            double r2 = 0.0;
+            for (int i = 0; i < 5; i++) r2 += z[2 * i] * z[2 * i] + z[2 * i + 1] * z[2 * i + 1];
+            double rd = pow(r2, (double)sdeg);
+            for (int64_t b = 0; b < ns; b++) {
+                const uint32_t *e = smono + 5 * b;
+                orc_c v = oc(1.0, 0.0);
+                for (int i = 0; i < 5; i++)
+                    for (uint32_t q = 0; q < e[i]; q++) v = c_mul(v, oc(z[2 * i], z[2 * i + 1]));
+                sm[b] = v;
+                for (int i = 0; i < 5; i++) {
+                    orc_c dv = oc(0.0, 0.0);
+                    if (e[i] > 0) {
+                        dv = oc((double)e[i], 0.0);
+                        for (int j = 0; j < 5; j++) {
+                            uint32_t ej = e[j] - (j == i ? 1u : 0u);
+                            for (uint32_t q = 0; q < ej; q++)
+                                dv = c_mul(dv, oc(z[2 * j], z[2 * j + 1]));
+                        }
+                    }
+                    orc_c corr = c_scale(c_mul(v, oc(z[2 * i], -z[2 * i + 1])), (double)sdeg / r2);
+                    G[b * 5 + i] = c_sub(dv, corr);
+                }
+            }
+            for (int64_t a = 0; a < ns; a++) {
+                orc_c u = c_scale(sm[a], 1.0 / rd);
+                for (int64_t b = 0; b < ns; b++) {
+                    int64_t al = a * ns + b;
+                    phi[al] = c_mul(u, c_conj(sm[b]));
+                    for (int i = 0; i < 5; i++) F[al * 5 + i] = c_mul(u, c_conj(G[b * 5 + i]));
+                }
+            }
+            for (int64_t be = 0; be < B; be++)
+                for (int j = 0; j < 5; j++) {
+                    orc_c acc = oc(0.0, 0.0);
+                    for (int i = 0; i < 5; i++) acc = c_add(acc, c_mul(Y[i][j], F[be * 5 + i]));
+                    YF[be * 5 + j] = acc;
+                }
+            for (int64_t al = 0; al < B; al++) {
+                orc_c pa = c_scale(c_conj(phi[al]), c);
+                orc_c fa[5];
+                for (int j = 0; j < 5; j++) fa[j] = c_scale(c_conj(F[al * 5 + j]), c);
+                orc_c *mrow = mm + al * B, *krow = kk + al * B;
+                for (int64_t be = 0; be < B; be++) {
+                    mrow[be] = c_add(mrow[be], c_mul(pa, phi[be]));
+                    orc_c acc = oc(0.0, 0.0);
+                    for (int j = 0; j < 5; j++) acc = c_add(acc, c_mul(fa[j], YF[be * 5 + j]));
+                    krow[be] = c_add(krow[be], acc);
+                }
+            }
+            ptw[t] += wp;
+        }
+        free(scr);
+        free(phi);
+        free(F);
+        free(YF);
+        free(sm);
+        free(G);
+    }
+    double tw = 0.0;
+    for (int t = 0; t < nth; t++) tw += ptw[t];
+    orc_c *Mo = (orc_c *)M, *Ko = (orc_c *)K;
+    for (int64_t e = 0; e < B * B; e++) {
+        orc_c sm_ = oc(0.0, 0.0), sk = oc(0.0, 0.0);
+        for (int t = 0; t < nth; t++) {
+            sm_ = c_add(sm_, pm[(int64_t)t * B * B + e]);
+            sk = c_add(sk, pk[(int64_t)t * B * B + e]);
+        }
+        Mo[e] = tw > 0.0 ? c_scale(sm_, 1.0 / tw) : sm_;
+        Ko[e] = tw > 0.0 ? c_scale(sk, 1.0 / tw) : sk;
+    }
+    free(pm);
+    free(pk);
+    free(ptw);
+    return tw;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Yukawa triple overlaps: route34/yukawa_overlap_real.rs                                      */
+/* ------------------------------------------------------------------------------------------ */
+/* sz_weights (:222-268): w / |Omega|^2 renormalised to sum one; returns the uniformity score. */
+ORC_API double orc_yuk_sz_weights(const double *w, const double *omega, int64_t n, double *sz) {
+    double total = 0.0;
+    int64_t bad = 0;
+    for (int64_t a = 0; a < n; a++) sz[a] = 0.0;
+    for (int64_t a = 0; a < n; a++) {
+        double wa = w[a];
+        if (!isfinite(wa) || wa <= 0.0) {
+            bad++;
+            continue;
+        }
+        double o2 = c_nsq(oc(omega[2 * a], omega[2 * a + 1]));
+        if (!isfinite(o2) || o2 <= 0.0) {
+            bad++;
+            continue;
+        }
+        double v = wa / o2;
+        sz[a] = v;
+        total += v;
+    }
+    if (total > 0.0) {
+        double inv = 1.0 / total;
+        for (int64_t a = 0; a < n; a++) sz[a] *= inv;
+    }
+    double n_eff = (double)(n - bad);
+    if (n_eff <= 1.0) return 0.0;
+    double mean = 0.0;
+    for (int64_t a = 0; a < n; a++) mean += sz[a];
+    mean /= (double)n;
+    double var = 0.0;
+    for (int64_t a = 0; a < n; a++) var += (sz[a] - mean) * (sz[a] - mean);
+    var /= (double)n;
+    double cv = mean > 0.0 ? sqrt(var) / mean : INFINITY;
+    double score = 1.0 / (1.0 + cv);
+    return score < 0.0 ? 0.0 : (score > 1.0 ? 1.0 : score);
+}
+
+/* triple_overlap_cpu (:271-315) when idx == NULL (check_psi = 1), bootstrap_one (:318-357) over
+ * the given resample indices otherwise (no psi finiteness check, as the reference).
+ * modes: [nm][n][2], y: [nm][nm][nm][2]. */
+ORC_API void orc_yuk_triple(const double *modes, const double *omega, const double *weights,
+                            const double *hdiag, int64_t nm, int64_t n, const int64_t *idx,
+                            int64_t n_idx, double *y) {
+    const orc_c *md = (const orc_c *)modes;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (int64_t i = 0; i < nm; i++)
+        for (int64_t j = 0; j < nm; j++)
+            for (int64_t k = 0; k < nm; k++) {
+                orc_c acc = oc(0.0, 0.0);
+                int64_t cnt = idx ? n_idx : n;
+                for (int64_t q = 0; q < cnt; q++) {
+                    int64_t p = idx ? idx[q] : q;
+                    double wv = weights[p];
+                    if (!isfinite(wv)) continue;
+                    orc_c om = oc(omega[2 * p], omega[2 * p + 1]);
+                    if (!isfinite(om.re) || !isfinite(om.im)) continue;
+                    orc_c pi_ = md[i * n + p], pj = md[j * n + p], pk = md[k * n + p];
+                    if (!idx && (!isfinite(pi_.re) || !isfinite(pj.re) || !isfinite(pk.re)))
+                        continue;
+                    acc = c_add(acc, c_mul(c_mul(c_mul(c_mul(c_conj(om), pi_), pj), pk),
+                                           oc(wv, 0.0)));
+                }
+                double sc = sqrt(hdiag ? hdiag[i] : 1.0) * sqrt(hdiag ? hdiag[j] : 1.0) *
+                            sqrt(hdiag ? hdiag[k] : 1.0);
+                orc_c r = c_mul(acc, oc(sc, 0.0));
+                int64_t o = (i * nm + j) * nm + k;
+                y[2 * o] = r.re;
+                y[2 * o + 1] = r.im;
+            }
+}
+
+/* Bootstrap resample indices: n draws of floor(u n) from the lib.rs LCG seeded with
+ * seed + b * 2654435761 (wrapping). The reference draws them from ChaCha20Rng::random_range
+ * (yukawa_overlap_real.rs:327-331), a third-party stream that is not restated here. */
+ORC_API void orc_yuk_bootstrap_indices(uint64_t seed, int64_t b, int64_t n, int64_t *idx) {
+    orc_lcg r;
+    lcg_init(&r, seed + (uint64_t)b * 2654435761ull);
+    for (int64_t q = 0; q < n; q++) {
+        int64_t p = (int64_t)(lcg_next_f64(&r) * (double)n);
+        idx[q] = p >= n ? n - 1 : p;
+    }
+}
+
+/* compute_yukawa_couplings (:372-525) without the HYM projection (h_v_diag given): SZ weights,
+ * central tensor, bootstrap standard deviation and the N vs N/2 convergence ratio. */
+ORC_API void orc_yuk_couplings(const double *modes, const double *omega, const double *w,
+                               const double *hdiag, int64_t nm, int64_t n, int64_t n_bs,
+                               uint64_t seed, double *y, double *unc, double *score_out,
+                               double *conv_out) {
+    double *sz = (double *)malloc(sizeof(double) * n);
+    double score = orc_yuk_sz_weights(w, omega, n, sz);
+    orc_yuk_triple(modes, omega, sz, hdiag, nm, n, NULL, 0, y);
+    int64_t n3 = nm * nm * nm;
+    int64_t nb_ = n_bs < 1 ? 1 : n_bs;
+    double *acc = (double *)calloc((size_t)n3, sizeof(double));
+    double *yb = (double *)malloc(sizeof(double) * 2 * n3);
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * n);
+    for (int64_t b = 0; b < nb_; b++) {
+        orc_yuk_bootstrap_indices(seed, b, n, idx);
+        orc_yuk_triple(modes, omega, sz, hdiag, nm, n, idx, n, yb);
+        for (int64_t e = 0; e < n3; e++) {
+            double dr = yb[2 * e] - y[2 * e], di = yb[2 * e + 1] - y[2 * e + 1];
+            acc[e] += dr * dr + di * di;
+        }
+    }
+    for (int64_t e = 0; e < n3; e++) unc[e] = sqrt(acc[e] / (double)nb_);
+    int64_t half = n / 2;
+    double conv = NAN;
+    if (half >= 8 && nm >= 1) {
+        double *hm = (double *)malloc(sizeof(double) * 2 * nm * half);
+        for (int64_t i = 0; i < nm; i++)
+            memcpy(hm + 2 * i * half, modes + 2 * i * n, sizeof(double) * 2 * half);
+        double tot = 0.0;
+        for (int64_t p = 0; p < half; p++) tot += sz[p];
+        double *hw = (double *)malloc(sizeof(double) * half);
+        for (int64_t p = 0; p < half; p++) hw[p] = tot > 0.0 ? sz[p] / tot : sz[p];
+        orc_yuk_triple(hm, omega, hw, hdiag, nm, half, NULL, 0, yb);
+        double md = 0.0, mf = 0.0;
+        for (int64_t e = 0; e < n3; e++) {
+            double dr = y[2 * e] - yb[2 * e], di = y[2 * e + 1] - yb[2 * e + 1];
+            double d = sqrt(dr * dr + di * di);
+            if (d > md) md = d;
+            double f = hypot(y[2 * e], y[2 * e + 1]);
+            if (f > mf) mf = f;
+        }
+        conv = mf > 0.0 ? md / mf : 0.0;
+        free(hm);
+        free(hw);
+    }
+    *score_out = score;
+    *conv_out = conv;
+    free(sz);
+    free(acc);
+    free(yb);
+    free(idx);
+}
+
+/* Per-point ambient metric and affine frame (for the independent numpy checks of the
+ * Laplacian stiffness). ga: [n][5][5][2], fr: [n][3][10], ok: [n] (1 when the tangent metric
+ * is invertible and the point is kept). */
+ORC_API void orc_point_metric(const double *pts, const double *sv, const double *sd,
+                              const double *h, int64_t n, int64_t nb, double psi, double *ga_out,
+                              double *fr_out, int32_t *ok) {
+#pragma omp parallel
+    {
+        double *scr = (double *)malloc(sizeof(double) * 12 * nb);
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; p++) {
+            const double *z = pts + 10 * p;
+            double ga[5][5][2];
+            ambient_ga(sv + p * 2 * nb, sd + p * 10 * nb, h, nb, scr, ga);
+            double g[10];
+            deformed_gradient(z, psi, g);
+            int chart, elim;
+            chart_and_elim(z, g, &chart, &elim, NULL);
+            double fr[3][10];
+            affine_frame(g, chart, elim, fr);
+            double gt[3][3][2], gi[3][3][2];
+            project_tangent(ga, fr, gt);
+            ok[p] = inv3_herm(gt, gi);
+            memcpy(ga_out + 50 * p, ga, sizeof(ga));
+            memcpy(fr_out + 30 * p, fr, sizeof(fr));
+        }
+        free(scr);
+    }
 }

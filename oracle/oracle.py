@@ -303,3 +303,168 @@ class Problem:
     def sigma_sq_grad(self, h=None):
         return sigma_sq_grad(self.pts, self.sv, self.sd, self.log_om, self.w,
                              self.h if h is None else h, self.psi)
+
+
+# ----------------------------------------------------------------------------------------------
+# Metric Laplacian (route34/metric_laplacian.rs) and Yukawa overlaps (route34/yukawa_overlap_real.rs)
+# ----------------------------------------------------------------------------------------------
+def _bind_galerkin(L):
+    i64, u64, u32, dbl, vp = C.c_int64, C.c_uint64, C.c_uint32, C.c_double, C.c_void_p
+    L.orc_lap_basis.argtypes = [u32, vp]
+    L.orc_lap_basis.restype = i64
+    L.orc_lap_ambient_mk.argtypes = [_dp, _dp, _dp, i64, _u32p, i64, _dp, _dp]
+    L.orc_herm_inv_reg.argtypes = [_dp, i64, dbl, _dp]
+    L.orc_herm_jacobi.argtypes = [_dp, i64, i64, dbl, _dp, vp]
+    L.orc_herm_jacobi.restype = i64
+    L.orc_lap_spectrum_from_mk.argtypes = [_dp, _dp, i64, i64, dbl, dbl, _dp]
+    L.orc_lap_quintic_mk.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, i64, i64, dbl, _u32p, i64,
+                                     C.c_int, _dp, _dp]
+    L.orc_lap_quintic_mk.restype = dbl
+    L.orc_point_metric.argtypes = [_dp, _dp, _dp, _dp, i64, i64, dbl, _dp, _dp, _i32p]
+    L.orc_yuk_sz_weights.argtypes = [_dp, _dp, i64, _dp]
+    L.orc_yuk_sz_weights.restype = dbl
+    L.orc_yuk_triple.argtypes = [_dp, _dp, _dp, vp, i64, i64, vp, i64, _dp]
+    L.orc_yuk_bootstrap_indices.argtypes = [u64, i64, i64, vp]
+    L.orc_yuk_couplings.argtypes = [_dp, _dp, _dp, vp, i64, i64, i64, u64, _dp, _dp,
+                                    C.POINTER(dbl), C.POINTER(dbl)]
+
+
+def _glib():
+    L = lib()
+    if not getattr(L, "_galerkin_bound", False):
+        _bind_galerkin(L)
+        L._galerkin_bound = True
+    return L
+
+
+def _c128(a):
+    """complex array -> contiguous float64 view [..., 2]."""
+    a = np.ascontiguousarray(a, np.complex128)
+    return a.view(np.float64).reshape(a.shape + (2,))
+
+
+def lap_basis(max_deg: int) -> np.ndarray:
+    """build_test_basis (metric_laplacian.rs:87-115): [nb, 8] exponents."""
+    n = _glib().orc_lap_basis(max_deg, None)
+    out = np.zeros((n, 8), np.uint32)
+    _glib().orc_lap_basis(max_deg, _ptr(out))
+    return out
+
+
+def lap_ambient_mk(pts8: np.ndarray, w: np.ndarray, omega: np.ndarray, exps: np.ndarray):
+    """build_mass_matrix / build_stiffness_matrix (metric_laplacian.rs:424-524).
+    pts8: complex [n, 8]; omega: complex [n]. Returns complex (M, K)."""
+    n, nb = pts8.shape[0], exps.shape[0]
+    M = np.zeros((nb, nb, 2))
+    K = np.zeros((nb, nb, 2))
+    _glib().orc_lap_ambient_mk(_c128(pts8).reshape(n, 16), np.ascontiguousarray(w, np.float64),
+                               _c128(omega).reshape(n, 2), n, np.ascontiguousarray(exps, np.uint32),
+                               nb, M, K)
+    return M[..., 0] + 1j * M[..., 1], K[..., 0] + 1j * K[..., 1]
+
+
+def herm_inv_reg(h: np.ndarray, eps: float) -> np.ndarray:
+    """invert_hermitian_regularised (metric_laplacian.rs:547-613)."""
+    n = h.shape[0]
+    out = np.zeros((n, n, 2))
+    _glib().orc_herm_inv_reg(_c128(h).reshape(n, n, 2), n, eps, out)
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def herm_jacobi(a: np.ndarray, max_sweeps: int = 128, tol: float = 1e-12, vectors: bool = False):
+    """hermitian_jacobi_n (metric_laplacian.rs:616-693): (eigenvalues ascending, vectors, sweeps)."""
+    n = a.shape[0]
+    ev = np.zeros(n)
+    vec = np.zeros((n, n, 2)) if vectors else None
+    sw = _glib().orc_herm_jacobi(_c128(a).reshape(n, n, 2), n, max_sweeps, tol, ev, _ptr(vec))
+    return ev, (None if vec is None else vec[..., 0] + 1j * vec[..., 1]), int(sw)
+
+
+def lap_spectrum_from_mk(M, K, max_sweeps: int = 128, tol: float = 1e-12, reg: float = 1e-10):
+    """The spectrum step of compute_metric_laplacian_spectrum (metric_laplacian.rs:342-389):
+    |eigenvalues| of the Hermitian part of (M + reg ||M||_F)^-1 K, ascending."""
+    nb = M.shape[0]
+    full = np.zeros(nb)
+    _glib().orc_lap_spectrum_from_mk(_c128(M).reshape(nb, nb, 2), _c128(K).reshape(nb, nb, 2), nb,
+                                     max_sweeps, tol, reg, full)
+    return full
+
+
+def lap_spectrum_ambient(pts8, w, omega, max_deg: int = 4, n_low: int = 15,
+                         max_sweeps: int = 128, tol: float = 1e-12, reg: float = 1e-10):
+    """compute_metric_laplacian_spectrum (metric_laplacian.rs:296-418): (low, full, M, K)."""
+    exps = lap_basis(max_deg)
+    M, K = lap_ambient_mk(pts8, w, omega, exps)
+    full = lap_spectrum_from_mk(M, K, max_sweeps, tol, reg)
+    return full[:min(n_low, len(full))], full, M, K
+
+
+def lap_quintic_mk(prob, h_block, sdeg: int = 3):
+    """Intrinsic-metric Galerkin matrices on phi_ab = s_a conj(s_b)/|z|^(2 sdeg) (raw degree-sdeg
+    monomials s) at the balanced h of `prob`. Returns complex (M, K, sum w)."""
+    smono = monomials(sdeg, False)
+    ns = smono.shape[0]
+    B = ns * ns
+    M = np.zeros((B, B, 2))
+    K = np.zeros((B, B, 2))
+    tw = _glib().orc_lap_quintic_mk(np.ascontiguousarray(prob.pts), np.ascontiguousarray(prob.w),
+                                    np.ascontiguousarray(prob.log_om), np.ascontiguousarray(prob.sv),
+                                    np.ascontiguousarray(prob.sd), np.ascontiguousarray(h_block),
+                                    prob.n, prob.nb, prob.psi, smono, ns, sdeg, M, K)
+    return M[..., 0] + 1j * M[..., 1], K[..., 0] + 1j * K[..., 1], tw
+
+
+def point_metric(prob, h_block):
+    """Per-point ambient metric g_amb (complex [n,5,5]), affine frame (complex [n,3,5]), ok flags."""
+    n = prob.n
+    ga = np.zeros((n, 5, 5, 2))
+    fr = np.zeros((n, 3, 10))
+    ok = np.zeros(n, np.int32)
+    _glib().orc_point_metric(np.ascontiguousarray(prob.pts), np.ascontiguousarray(prob.sv),
+                             np.ascontiguousarray(prob.sd), np.ascontiguousarray(h_block), n,
+                             prob.nb, prob.psi, ga, fr, ok)
+    frc = fr.reshape(n, 3, 5, 2)
+    return ga[..., 0] + 1j * ga[..., 1], frc[..., 0] + 1j * frc[..., 1], ok.astype(bool)
+
+
+def yuk_sz_weights(w, omega):
+    """sz_weights (yukawa_overlap_real.rs:222-268): (sz, uniformity score)."""
+    n = len(w)
+    sz = np.zeros(n)
+    score = _glib().orc_yuk_sz_weights(np.ascontiguousarray(w, np.float64),
+                                       _c128(omega).reshape(n, 2), n, sz)
+    return sz, score
+
+
+def yuk_triple(modes, omega, weights, hdiag=None, idx=None):
+    """triple_overlap_cpu (yukawa_overlap_real.rs:271-315), or bootstrap_one (:318-357) over
+    resample indices idx. modes: complex [n_modes, n]. Returns complex [nm, nm, nm]."""
+    nm, n = modes.shape
+    y = np.zeros((nm, nm, nm, 2))
+    hd = None if hdiag is None else np.ascontiguousarray(hdiag, np.float64)
+    ix = None if idx is None else np.ascontiguousarray(idx, np.int64)
+    _glib().orc_yuk_triple(_c128(modes).reshape(nm, n, 2), _c128(omega).reshape(n, 2),
+                           np.ascontiguousarray(weights, np.float64), _ptr(hd), nm, n, _ptr(ix),
+                           0 if ix is None else len(ix), y)
+    return y[..., 0] + 1j * y[..., 1]
+
+
+def yuk_bootstrap_indices(seed: int, b: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.int64)
+    _glib().orc_yuk_bootstrap_indices(seed, b, n, _ptr(out))
+    return out
+
+
+def yuk_couplings(modes, omega, w, hdiag=None, n_bootstrap: int = 64, seed: int = 0xBADDCAFEF00D):
+    """compute_yukawa_couplings (yukawa_overlap_real.rs:372-525) given h_v_diag:
+    (Y, uncertainty, uniformity score, convergence ratio)."""
+    nm, n = modes.shape
+    y = np.zeros((nm, nm, nm, 2))
+    unc = np.zeros((nm, nm, nm))
+    sc = C.c_double(0)
+    cv = C.c_double(0)
+    hd = None if hdiag is None else np.ascontiguousarray(hdiag, np.float64)
+    _glib().orc_yuk_couplings(_c128(modes).reshape(nm, n, 2), _c128(omega).reshape(n, 2),
+                              np.ascontiguousarray(w, np.float64), _ptr(hd), nm, n, n_bootstrap,
+                              seed, y, unc, C.byref(sc), C.byref(cv))
+    return y[..., 0] + 1j * y[..., 1], unc, sc.value, cv.value

@@ -33,7 +33,11 @@ SOURCES = {
     "k_split16.hip": [],
     "k_tpass.hip": [],
     "k_adjoint.hip": [],
+    "k_laplacian.hip": [],
+    "k_eigen.hip": [],
+    "k_yukawa.hip": [],
     "solver.cpp": ["-x", "hip"],
+    "galerkin.cpp": ["-x", "hip"],
 }
 
 

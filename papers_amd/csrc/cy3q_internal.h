@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <cstddef>
+#include <string>
 
 namespace cy3q {
 
@@ -130,6 +131,61 @@ void launch_grad_assemble(hipStream_t st, const double *cm, int nb, int64_t m_co
 void launch_adam_step(hipStream_t st, double *g_cplx, int nb, const double *grad, double *m,
                       double *v, double scale, double beta1, double beta2, double eps);
 
+// Metric Laplacian assembly (k_laplacian.hip).
+void launch_lap_metric(hipStream_t st, const double *pts, const double *y0, const double *yd,
+                       const double *w, const double *log_om, int64_t n, int64_t m_cols,
+                       double psi, double *pr, double *wv);
+void launch_lap_rows_quintic(hipStream_t st, const double *pts, const double *pr, int64_t n,
+                             const uint8_t *smono, int ns, int sdeg, int64_t mc, double *x0,
+                             double *x1);
+void launch_lap_rows_ambient(hipStream_t st, const double *pts, const double *w,
+                             const double *omega, int64_t n, const uint8_t *exps, int nb,
+                             int64_t mc, double *x0, double *x1, double *wv);
+int herk_max_buffers();
+void launch_herk_f64(hipStream_t st, const double *const *x, const int64_t *rows, double *const *c,
+                     int nbuf, int64_t mc);
+void launch_lap_finalize(hipStream_t st, const double *cmass, const double *const *ck, int nk,
+                         int B, int64_t mc, const double *tw, int clamp_k, double2 *M, double2 *K);
+
+// Hermitian Jacobi eigensolver and complex GEMM (k_eigen.hip).
+void launch_jacobi_round(hipStream_t st, const double2 *A, double2 *A2, const double2 *V,
+                         double2 *V2, const int *tag, int *tag2, int np, int want_v);
+int offnorm_parts();
+void launch_offnorm(hipStream_t st, const double2 *A, int np, double *part, double *out2);
+void launch_jacobi_init(hipStream_t st, const double2 *src, int n, int np, double shift,
+                        double2 *A, double2 *V, int *tag);
+void launch_diag_re(hipStream_t st, const double2 *A, int np, double *d);
+void launch_gather_cols(hipStream_t st, const double2 *V, int np, int n, const int *col,
+                        const double *scale, int r, double2 *out);
+void launch_herm_project(hipStream_t st, double2 *A, int n);
+void launch_zgemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C, int M, int N,
+                  int K, int lda, int ldb, int ldc, int opa, int opb);
+
+// Yukawa triple overlaps (k_yukawa.hip).
+int yukawa_max_modes();
+int64_t yukawa_chunks(int64_t n, int n_trip);
+void launch_yukawa(hipStream_t st, const double2 *modes, int64_t ld, const double *w,
+                   const double *omega, int nm, int64_t n, int check_psi, const short4 *trip,
+                   int n_trip, const double *hdiag, int64_t n_chunks, double2 *fac,
+                   unsigned long long *bad, double2 *partials, double2 *y_tri, double2 *y_full);
+void launch_sz_weights(hipStream_t st, const double *w, const double *omega, int64_t n, double *sz,
+                       double *out);
+void launch_scale_weights(hipStream_t st, const double *w, const int32_t *count,
+                          const double *total, int64_t n, double *out);
+void launch_modes_from_sections(hipStream_t st, const double *sv, int64_t n, int64_t m_cols,
+                                int nsec, const double2 *coef, int nm, double2 *modes,
+                                int64_t ld);
+void launch_measure_w_omega2(hipStream_t st, const double *w, const double *log_om, int64_t n,
+                             double *c);
+void launch_bootstrap_counts(hipStream_t st, uint64_t seed_b, int64_t n, int32_t *count);
+void launch_random_coeffs(hipStream_t st, uint64_t seed, int64_t count, double2 *out);
+void launch_bootstrap_acc(hipStream_t st, const double2 *yb, const double2 *y, int64_t n3,
+                          double *acc);
+void launch_bootstrap_final(hipStream_t st, const double *acc, int64_t n3, double nbs,
+                            double *unc);
+void launch_conv_ratio(hipStream_t st, const double2 *y, const double2 *yh, int64_t n3,
+                       double *out);
+
 // ---------------------------------------------------------------------------------------
 // SYRK work partition: lower 16x16 (FP64) or 32x32 (split16) output sub-tiles of the
 // M x M symmetric result, grouped into "parts" (one workgroup each per point chunk), each
@@ -150,4 +206,26 @@ struct SyrkPlan {
   // Row-group order of the per-wave lists lets a wave reuse the row fragment.
 };
 
+// ---------------------------------------------------------------------------------------
+// Host-side access to a solver's device workspace for the Galerkin entry points
+// (solver.cpp). The pointers stay owned by the solver.
+// ---------------------------------------------------------------------------------------
+struct SolverView {
+  hipStream_t st = nullptr;
+  int device = 0;
+  int64_t n = 0, m = 0;  // points, padded real columns of the section rows
+  int nb = 0;
+  double psi = 0.0;
+  const double *pts = nullptr, *w = nullptr, *log_om = nullptr, *sv = nullptr, *sd = nullptr;
+  const double *bmat = nullptr;  // real-ified W = L^-1 of the current h (W s gives K = |W s|^2)
+};
+
+}  // namespace cy3q
+
+struct cy3q_solver;
+namespace cy3q {
+// Records msg as the calling thread's last error and returns code.
+int set_error(int code, const std::string &msg);
+// Fills v for s (building the contraction operand of the current h first).
+int solver_view(cy3q_solver *s, SolverView *v);
 }  // namespace cy3q

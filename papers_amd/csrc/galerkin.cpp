@@ -1,0 +1,634 @@
+// Host side of the Galerkin entry points: metric Laplacian spectra, the Hermitian eigensolver
+// and the Yukawa triple overlaps (include/cy3q.h). The host orders the launches, allocates
+// device buffers and moves inputs and results. Every arithmetic stage is a HIP kernel
+// (k_laplacian.hip, k_eigen.hip, k_yukawa.hip). The host only sorts the final eigenvalues
+// and picks the columns that the deflation keeps.
+//
+// References: route34/metric_laplacian.rs (:296-693) and route34/yukawa_overlap_real.rs
+// (:222-525).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/cy3q.h"
+#include "cy3q_internal.h"
+
+using namespace cy3q;
+
+namespace {
+
+#define G_HIP(expr)                                                                          \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(CY3Q_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define G_TRY(expr)                  \
+  do {                               \
+    int rc_ = (expr);                \
+    if (rc_ != CY3Q_OK) return rc_;  \
+  } while (0)
+
+// Device allocations of one call, freed when the call returns. The stream is either borrowed
+// (a solver's) or owned (created on `device` for a stand-alone call).
+struct Ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  bool own = false;
+  std::vector<void *> mem;
+  ~Ctx() {
+    if (st) hipStreamSynchronize(st);
+    for (void *p : mem) hipFree(p);
+    if (own && st) hipStreamDestroy(st);
+  }
+  int open(int dev) {
+    device = dev;
+    G_HIP(hipSetDevice(dev));
+    G_HIP(hipStreamCreate(&st));
+    own = true;
+    return CY3Q_OK;
+  }
+  template <class T>
+  int alloc(T **p, size_t count) {
+    void *q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+      return set_error(CY3Q_ERR_HIP,
+                       "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    mem.push_back(q);
+    *p = (T *)q;
+    return CY3Q_OK;
+  }
+  template <class T>
+  int zeros(T **p, size_t count) {
+    G_TRY(alloc(p, count));
+    G_HIP(hipMemsetAsync(*p, 0, std::max<size_t>(count, 1) * sizeof(T), st));
+    return CY3Q_OK;
+  }
+  template <class T>
+  int upload(T **p, const T *src, size_t count) {
+    G_TRY(alloc(p, count));
+    G_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return CY3Q_OK;
+  }
+  void release(void *p) {
+    auto it = std::find(mem.begin(), mem.end(), p);
+    if (it != mem.end()) {
+      hipStreamSynchronize(st);
+      hipFree(p);
+      mem.erase(it);
+    }
+  }
+};
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// ------------------------------------------------------------------------------------------
+// Hermitian eigensolver: parallel Jacobi sweeps (k_jacobi_round) until the off-diagonal
+// Frobenius norm is at most tol * ||A||_F (or is zero), or max_sweeps sweeps. a: device
+// [n][n] complex; shift is added to the diagonal first. eig: ascending; vecs (device
+// [n][n], column j for eig[j]) when not null.
+// ------------------------------------------------------------------------------------------
+int heev(Ctx &c, const double2 *a, int n, int64_t max_sweeps, double tol, double shift,
+         std::vector<double> &eig, double2 *vecs, int64_t *sweeps_out) {
+  const int np = n + (n & 1);
+  double2 *A0, *A1, *V0 = nullptr, *V1 = nullptr;
+  int *t0, *t1;
+  double *part, *nrm, *diag;
+  G_TRY(c.alloc(&A0, (size_t)np * np));
+  G_TRY(c.alloc(&A1, (size_t)np * np));
+  if (vecs) {
+    G_TRY(c.alloc(&V0, (size_t)np * np));
+    G_TRY(c.alloc(&V1, (size_t)np * np));
+  }
+  G_TRY(c.alloc(&t0, np));
+  G_TRY(c.alloc(&t1, np));
+  G_TRY(c.alloc(&part, 2 * offnorm_parts()));
+  G_TRY(c.alloc(&nrm, 2));
+  G_TRY(c.alloc(&diag, np));
+  launch_jacobi_init(c.st, a, n, np, shift, A0, V0, t0);
+  int64_t sw = 0;
+  for (;;) {
+    double h2[2];
+    launch_offnorm(c.st, A0, np, part, nrm);
+    G_HIP(hipMemcpyAsync(h2, nrm, sizeof(h2), hipMemcpyDeviceToHost, c.st));
+    G_HIP(hipStreamSynchronize(c.st));
+    if (!std::isfinite(h2[0]) || !std::isfinite(h2[1]))
+      return set_error(CY3Q_ERR_ARG, "non-finite entries in the eigenproblem");
+    if (h2[0] == 0.0 || std::sqrt(h2[0]) <= tol * std::sqrt(h2[1]) || sw >= max_sweeps) break;
+    for (int r = 0; r < np - 1; r++) {
+      launch_jacobi_round(c.st, A0, A1, V0, V1, t0, t1, np, vecs ? 1 : 0);
+      std::swap(A0, A1);
+      std::swap(V0, V1);
+      std::swap(t0, t1);
+    }
+    G_HIP(hipGetLastError());
+    sw++;
+  }
+  std::vector<double> d(np);
+  std::vector<int> tag(np);
+  launch_diag_re(c.st, A0, np, diag);
+  G_HIP(hipMemcpyAsync(d.data(), diag, sizeof(double) * np, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipMemcpyAsync(tag.data(), t0, sizeof(int) * np, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  std::vector<int> slots;
+  for (int x = 0; x < np; x++)
+    if (tag[x] < n) slots.push_back(x);
+  std::stable_sort(slots.begin(), slots.end(), [&](int a_, int b_) { return d[a_] < d[b_]; });
+  eig.resize(n);
+  for (int j = 0; j < n; j++) eig[j] = d[slots[j]];
+  if (vecs) {
+    int *cols;
+    G_TRY(c.upload(&cols, slots.data(), slots.size()));
+    launch_gather_cols(c.st, V0, np, n, cols, nullptr, n, vecs);
+  }
+  if (sweeps_out) *sweeps_out = sw;
+  G_HIP(hipGetLastError());
+  return CY3Q_OK;
+}
+
+// Frobenius norm of a device [n][n] complex matrix.
+int frob(Ctx &c, const double2 *a, int n, double *out) {
+  double *part, *nrm;
+  G_TRY(c.alloc(&part, 2 * offnorm_parts()));
+  G_TRY(c.alloc(&nrm, 2));
+  launch_offnorm(c.st, a, n, part, nrm);
+  double h2[2];
+  G_HIP(hipMemcpyAsync(h2, nrm, sizeof(h2), hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  *out = std::sqrt(h2[1]);
+  return CY3Q_OK;
+}
+
+// Spectrum step of compute_metric_laplacian_spectrum (metric_laplacian.rs:342-389):
+// (M + lambda I)^-1 K with lambda = max(reg ||M||_F, 1e-30), through the eigenvectors of M, then
+// the Hermitian part, then its eigenvalues as |lambda| ascending.
+int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64_t max_sweeps,
+                       double tol, double reg, std::vector<double> &full) {
+  double fm;
+  G_TRY(frob(c, M, nb, &fm));
+  const double lambda = std::max(fm * reg, 1.0e-30);
+  double2 *U, *Y, *Minv, *L;
+  G_TRY(c.alloc(&U, (size_t)nb * nb));
+  G_TRY(c.alloc(&Y, (size_t)nb * nb));
+  G_TRY(c.alloc(&Minv, (size_t)nb * nb));
+  G_TRY(c.alloc(&L, (size_t)nb * nb));
+  std::vector<double> mu;
+  G_TRY(heev(c, M, nb, max_sweeps, tol, lambda, mu, U, nullptr));
+  std::vector<double> inv(nb);
+  for (int j = 0; j < nb; j++) inv[j] = 1.0 / mu[j];
+  std::vector<int> ident(nb);
+  std::iota(ident.begin(), ident.end(), 0);
+  double *d_inv;
+  int *d_id;
+  G_TRY(c.upload(&d_inv, inv.data(), nb));
+  G_TRY(c.upload(&d_id, ident.data(), nb));
+  launch_gather_cols(c.st, U, nb, nb, d_id, d_inv, nb, Y);           // U diag(1/mu)
+  launch_zgemm(c.st, Y, U, Minv, nb, nb, nb, nb, nb, nb, 0, 1);       // ... U^H
+  launch_zgemm(c.st, Minv, K, L, nb, nb, nb, nb, nb, nb, 0, 0);       // M^-1 K
+  launch_herm_project(c.st, L, nb);
+  std::vector<double> ev;
+  G_TRY(heev(c, L, nb, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+  full.resize(nb);
+  for (int j = 0; j < nb; j++) full[j] = std::fabs(ev[j]);
+  std::stable_sort(full.begin(), full.end());
+  return CY3Q_OK;
+}
+
+// Generalized K v = lambda M v by the canonical reduction X = U_keep diag(mu^-1/2):
+// C = X^H K X, the eigenvalues of its Hermitian part ascending.
+int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64_t max_sweeps,
+                         double tol, double defl, std::vector<double> &ev, int64_t *rank) {
+  double2 *U;
+  G_TRY(c.alloc(&U, (size_t)nb * nb));
+  std::vector<double> mu;
+  G_TRY(heev(c, M, nb, max_sweeps, tol, 0.0, mu, U, nullptr));
+  const double mmax = mu.empty() ? 0.0 : mu.back();
+  std::vector<int> keep;
+  std::vector<double> sc;
+  for (int j = 0; j < nb; j++)
+    if (mu[j] > defl * mmax && mu[j] > 0.0) {
+      keep.push_back(j);
+      sc.push_back(1.0 / std::sqrt(mu[j]));
+    }
+  const int r = (int)keep.size();
+  if (rank) *rank = r;
+  if (r == 0) return set_error(CY3Q_ERR_ARG, "mass matrix has no positive eigenvalue");
+  int *d_keep;
+  double *d_sc;
+  double2 *X, *T, *C;
+  G_TRY(c.upload(&d_keep, keep.data(), r));
+  G_TRY(c.upload(&d_sc, sc.data(), r));
+  G_TRY(c.alloc(&X, (size_t)nb * r));
+  G_TRY(c.alloc(&T, (size_t)nb * r));
+  G_TRY(c.alloc(&C, (size_t)r * r));
+  launch_gather_cols(c.st, U, nb, nb, d_keep, d_sc, r, X);
+  launch_zgemm(c.st, K, X, T, nb, r, nb, nb, r, r, 0, 0);  // K X
+  launch_zgemm(c.st, X, T, C, r, r, nb, r, r, r, 1, 0);    // X^H K X
+  launch_herm_project(c.st, C, r);
+  G_TRY(heev(c, C, r, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+  return CY3Q_OK;
+}
+
+// build_test_basis (metric_laplacian.rs:87-115): (dz, dw) with dz + dw <= max, (0, 0) excluded,
+// then monomials_4(dz) x monomials_4(dw), each in a, b, c ascending order (:118-130).
+std::vector<uint8_t> lap_basis(uint32_t max_deg) {
+  std::vector<uint8_t> out;
+  auto m4 = [](uint32_t d) {
+    std::vector<std::array<uint8_t, 4>> v;
+    for (uint32_t a = 0; a <= d; a++)
+      for (uint32_t b = 0; b <= d - a; b++)
+        for (uint32_t cc = 0; cc <= d - a - b; cc++)
+          v.push_back({(uint8_t)a, (uint8_t)b, (uint8_t)cc, (uint8_t)(d - a - b - cc)});
+    return v;
+  };
+  for (uint32_t dz = 0; dz <= max_deg; dz++)
+    for (uint32_t dw = 0; dw <= max_deg - dz; dw++) {
+      if (dz == 0 && dw == 0) continue;
+      auto zm = m4(dz), wm = m4(dw);
+      for (auto &a : zm)
+        for (auto &b : wm) {
+          out.insert(out.end(), a.begin(), a.end());
+          out.insert(out.end(), b.begin(), b.end());
+        }
+    }
+  return out;
+}
+
+// Raw degree-d monomials in 5 variables in the order of build_degree_k_quintic_monomials
+// (quintic.rs:3192-3209): e0 up, e1 up, e2 up, e3 up, e4 = d - sum.
+std::vector<uint8_t> raw_monomials(uint32_t d) {
+  std::vector<uint8_t> out;
+  const int k = (int)d;
+  for (int e0 = 0; e0 <= k; e0++)
+    for (int e1 = 0; e1 <= k - e0; e1++)
+      for (int e2 = 0; e2 <= k - e0 - e1; e2++)
+        for (int e3 = 0; e3 <= k - e0 - e1 - e2; e3++)
+          out.insert(out.end(), {(uint8_t)e0, (uint8_t)e1, (uint8_t)e2, (uint8_t)e3,
+                                 (uint8_t)(k - e0 - e1 - e2 - e3)});
+  return out;
+}
+
+std::vector<short4> triples(int nm) {
+  std::vector<short4> t;
+  for (int i = 0; i < nm; i++)
+    for (int j = i; j < nm; j++)
+      for (int k = j; k < nm; k++) t.push_back(make_short4((short)i, (short)j, (short)k, 0));
+  return t;
+}
+
+// Device Yukawa workspace for n points and nm modes.
+struct YukWs {
+  short4 *trip = nullptr;
+  int n_trip = 0;
+  int64_t n_chunks = 0;
+  double2 *fac = nullptr, *partials = nullptr;
+  unsigned long long *bad = nullptr;
+  int init(Ctx &c, int nm, int64_t n) {
+    std::vector<short4> t = triples(nm);
+    n_trip = (int)t.size();
+    G_TRY(c.upload(&trip, t.data(), t.size()));
+    n_chunks = yukawa_chunks(n, n_trip);
+    G_TRY(c.alloc(&fac, n));
+    G_TRY(c.alloc(&bad, n));
+    G_TRY(c.alloc(&partials, (size_t)n_chunks * n_trip));
+    return CY3Q_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void cy3q_laplacian_config_default(cy3q_laplacian_config *cfg) {
+  if (!cfg) return;
+  cfg->max_total_degree = 4;
+  cfg->n_low_eigenvalues = 15;
+  cfg->jacobi_max_sweeps = 128;
+  cfg->jacobi_tol = 1.0e-12;
+  cfg->mass_regularisation = 1.0e-10;
+}
+
+int cy3q_laplacian_basis(uint32_t max_total_degree, uint32_t *exps, int64_t cap, int64_t *n) {
+  if (max_total_degree > 8) return set_error(CY3Q_ERR_ARG, "max_total_degree > 8");
+  std::vector<uint8_t> b = lap_basis(max_total_degree);
+  const int64_t cnt = (int64_t)(b.size() / 8);
+  if (n) *n = cnt;
+  if (exps) {
+    if (cap < cnt) return set_error(CY3Q_ERR_ARG, "output capacity too small");
+    for (size_t i = 0; i < b.size(); i++) exps[i] = b[i];
+  }
+  return CY3Q_OK;
+}
+
+int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double tol, int device,
+                         double *eig, double *vecs, int64_t *sweeps) {
+  if (!a || !eig || n <= 0 || n > 8192) return set_error(CY3Q_ERR_ARG, "bad matrix arguments");
+  Ctx c;
+  G_TRY(c.open(device));
+  double2 *d_a, *d_v = nullptr;
+  G_TRY(c.upload(&d_a, reinterpret_cast<const double2 *>(a), (size_t)(n * n)));
+  if (vecs) G_TRY(c.alloc(&d_v, (size_t)(n * n)));
+  std::vector<double> ev;
+  G_TRY(heev(c, d_a, (int)n, max_sweeps, tol, 0.0, ev, d_v, sweeps));
+  std::memcpy(eig, ev.data(), sizeof(double) * n);
+  if (vecs)
+    G_HIP(hipMemcpyAsync(vecs, d_v, sizeof(double2) * n * n, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
+                                   const double *omega, int64_t n,
+                                   const cy3q_laplacian_config *cfg_in, int device,
+                                   double *eig_low, double *eig_full, double *mass,
+                                   double *stiffness, int64_t *basis_dim) {
+  cy3q_laplacian_config cfg;
+  cy3q_laplacian_config_default(&cfg);
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.max_total_degree > 6) return set_error(CY3Q_ERR_ARG, "max_total_degree > 6");
+  std::vector<uint8_t> ex = lap_basis(cfg.max_total_degree);
+  const int nb = (int)(ex.size() / 8);
+  if (basis_dim) *basis_dim = nb;
+  if (nb == 0 || n <= 0) return CY3Q_OK;  // the reference returns an empty spectrum (:305-316)
+  if (!points || !weights || !omega) return set_error(CY3Q_ERR_ARG, "null input");
+  Ctx c;
+  G_TRY(c.open(device));
+  const int64_t mc = round_up(2 * (int64_t)nb, 128);
+  const int nder = 8;
+  const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(1024, (int64_t)(3e8 / (9.0 * mc))));
+  double *d_pts, *d_w, *d_om, *x0, *x1, *wv, *tw, *cmass, *ck[8];
+  uint8_t *d_ex;
+  G_TRY(c.upload(&d_pts, points, (size_t)(16 * n)));
+  G_TRY(c.upload(&d_w, weights, (size_t)n));
+  G_TRY(c.upload(&d_om, omega, (size_t)(2 * n)));
+  G_TRY(c.upload(&d_ex, ex.data(), ex.size()));
+  G_TRY(c.zeros(&x0, (size_t)(chunk * mc)));
+  G_TRY(c.zeros(&x1, (size_t)(nder * chunk * mc)));
+  G_TRY(c.alloc(&wv, (size_t)n));
+  G_TRY(c.alloc(&tw, 1));
+  G_TRY(c.zeros(&cmass, (size_t)(mc * mc)));
+  for (int q = 0; q < nder; q++) G_TRY(c.zeros(&ck[q], (size_t)(mc * mc)));
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    const int64_t cn = std::min(chunk, n - p0);
+    launch_lap_rows_ambient(c.st, d_pts + 16 * p0, d_w + p0, d_om + 2 * p0, cn, d_ex, nb, mc, x0,
+                            x1, wv + p0);
+    const double *xs[9];
+    int64_t rows[9];
+    double *cs[9];
+    xs[0] = x0;
+    rows[0] = cn;
+    cs[0] = cmass;
+    for (int q = 0; q < nder; q++) {
+      xs[1 + q] = x1 + (int64_t)q * cn * mc;
+      rows[1 + q] = cn;
+      cs[1 + q] = ck[q];
+    }
+    launch_herk_f64(c.st, xs, rows, cs, 1 + nder, mc);
+  }
+  launch_sum_f64(c.st, wv, n, tw);
+  double2 *M, *K;
+  G_TRY(c.alloc(&M, (size_t)nb * nb));
+  G_TRY(c.alloc(&K, (size_t)nb * nb));
+  launch_lap_finalize(c.st, cmass, ck, nder, nb, mc, tw, 1, M, K);
+  G_HIP(hipGetLastError());
+  c.release(x0);
+  c.release(x1);
+  if (mass) G_HIP(hipMemcpyAsync(mass, M, sizeof(double2) * nb * nb, hipMemcpyDeviceToHost, c.st));
+  if (stiffness)
+    G_HIP(hipMemcpyAsync(stiffness, K, sizeof(double2) * nb * nb, hipMemcpyDeviceToHost, c.st));
+  std::vector<double> full;
+  G_TRY(reference_spectrum(c, M, K, nb, cfg.jacobi_max_sweeps, cfg.jacobi_tol,
+                           cfg.mass_regularisation, full));
+  const int64_t n_low = std::min<int64_t>(std::max<int64_t>(cfg.n_low_eigenvalues, 0), nb);
+  if (eig_low) std::memcpy(eig_low, full.data(), sizeof(double) * n_low);
+  if (eig_full) std::memcpy(eig_full, full.data(), sizeof(double) * nb);
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int64_t n_low,
+                                   double deflation_tol, int64_t max_sweeps, double *eig_low,
+                                   double *mass, double *stiffness, int64_t *basis_dim,
+                                   int64_t *rank) {
+  if (section_degree < 1 || section_degree > 4)
+    return set_error(CY3Q_ERR_ARG, "section_degree must be in 1..4");
+  SolverView v;
+  G_TRY(solver_view(s, &v));
+  Ctx c;
+  c.device = v.device;
+  c.st = v.st;
+  std::vector<uint8_t> sm = raw_monomials(section_degree);
+  const int ns = (int)(sm.size() / 5);
+  const int B = ns * ns;
+  if (basis_dim) *basis_dim = B;
+  const int64_t n = v.n, m = v.m;
+  const int64_t mc = round_up(2 * (int64_t)B, 128);
+  const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(256, (int64_t)(6e8 / (4.0 * mc))));
+  uint8_t *d_sm;
+  double *y0, *yd, *pr, *wv, *tw, *x0, *x1, *cmass, *ck[3];
+  G_TRY(c.upload(&d_sm, sm.data(), sm.size()));
+  G_TRY(c.alloc(&y0, (size_t)(chunk * m)));
+  G_TRY(c.alloc(&yd, (size_t)(chunk * 5 * m)));
+  G_TRY(c.alloc(&pr, (size_t)(32 * chunk)));
+  G_TRY(c.alloc(&wv, (size_t)n));
+  G_TRY(c.alloc(&tw, 1));
+  G_TRY(c.zeros(&x0, (size_t)(chunk * mc)));
+  G_TRY(c.zeros(&x1, (size_t)(3 * chunk * mc)));
+  G_TRY(c.zeros(&cmass, (size_t)(mc * mc)));
+  for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(mc * mc)));
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    const int64_t cn = std::min(chunk, n - p0);
+    launch_trigemm_f64(c.st, v.sv + p0 * m, cn, m, v.bmat, y0, 0, 0);
+    launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
+    launch_lap_metric(c.st, v.pts + 10 * p0, y0, yd, v.w + p0, v.log_om + p0, cn, m, v.psi, pr,
+                      wv + p0);
+    launch_lap_rows_quintic(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, mc, x0,
+                            x1);
+    const double *xs[4] = {x0, x1, x1 + cn * mc, x1 + 2 * cn * mc};
+    int64_t rows[4] = {cn, cn, cn, cn};
+    double *cs[4] = {cmass, ck[0], ck[1], ck[2]};
+    launch_herk_f64(c.st, xs, rows, cs, 4, mc);
+  }
+  launch_sum_f64(c.st, wv, n, tw);
+  double2 *M, *K;
+  G_TRY(c.alloc(&M, (size_t)B * B));
+  G_TRY(c.alloc(&K, (size_t)B * B));
+  launch_lap_finalize(c.st, cmass, ck, 3, B, mc, tw, 0, M, K);
+  G_HIP(hipGetLastError());
+  for (void *p : {(void *)x0, (void *)x1, (void *)y0, (void *)yd}) c.release(p);
+  if (mass) G_HIP(hipMemcpyAsync(mass, M, sizeof(double2) * B * B, hipMemcpyDeviceToHost, c.st));
+  if (stiffness)
+    G_HIP(hipMemcpyAsync(stiffness, K, sizeof(double2) * B * B, hipMemcpyDeviceToHost, c.st));
+  std::vector<double> ev;
+  G_TRY(generalized_spectrum(c, M, K, B, max_sweeps > 0 ? max_sweeps : 64, 1e-14,
+                             deflation_tol > 0.0 ? deflation_tol : 1e-10, ev, rank));
+  const int64_t nl = std::min<int64_t>(std::max<int64_t>(n_low, 0), (int64_t)ev.size());
+  if (eig_low) std::memcpy(eig_low, ev.data(), sizeof(double) * nl);
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_yukawa_sz_weights(const double *weights, const double *omega, int64_t n, int device,
+                           double *sz, double *uniformity) {
+  if (!weights || !omega || n <= 0) return set_error(CY3Q_ERR_ARG, "bad input");
+  Ctx c;
+  G_TRY(c.open(device));
+  double *d_w, *d_om, *d_sz, *d_out;
+  G_TRY(c.upload(&d_w, weights, (size_t)n));
+  G_TRY(c.upload(&d_om, omega, (size_t)(2 * n)));
+  G_TRY(c.alloc(&d_sz, (size_t)n));
+  G_TRY(c.alloc(&d_out, 1));
+  launch_sz_weights(c.st, d_w, d_om, n, d_sz, d_out);
+  if (sz) G_HIP(hipMemcpyAsync(sz, d_sz, sizeof(double) * n, hipMemcpyDeviceToHost, c.st));
+  if (uniformity) G_HIP(hipMemcpyAsync(uniformity, d_out, sizeof(double), hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_yukawa_triple_overlap(const double *modes, const double *omega, const double *weights,
+                               const double *h_v_diag, int64_t n_modes, int64_t n, int device,
+                               double *y) {
+  if (n_modes < 1 || n_modes > yukawa_max_modes())
+    return set_error(CY3Q_ERR_ARG, "n_modes must be in 1..64");
+  if (!modes || !omega || !weights || !y || n <= 0) return set_error(CY3Q_ERR_ARG, "bad input");
+  Ctx c;
+  G_TRY(c.open(device));
+  const int nm = (int)n_modes;
+  double2 *d_modes, *d_y;
+  double *d_om, *d_w, *d_h = nullptr;
+  G_TRY(c.upload(&d_modes, reinterpret_cast<const double2 *>(modes), (size_t)(nm * n)));
+  G_TRY(c.upload(&d_om, omega, (size_t)(2 * n)));
+  G_TRY(c.upload(&d_w, weights, (size_t)n));
+  if (h_v_diag) G_TRY(c.upload(&d_h, h_v_diag, (size_t)nm));
+  YukWs ws;
+  G_TRY(ws.init(c, nm, n));
+  G_TRY(c.alloc(&d_y, (size_t)nm * nm * nm));
+  launch_yukawa(c.st, d_modes, n, d_w, d_om, nm, n, 1, ws.trip, ws.n_trip, d_h, ws.n_chunks,
+                ws.fac, ws.bad, ws.partials, nullptr, d_y);
+  G_HIP(hipGetLastError());
+  G_HIP(hipMemcpyAsync(y, d_y, sizeof(double2) * nm * nm * nm, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_yukawa_couplings(const double *modes, const double *omega, const double *weights,
+                          const double *h_v_diag, int64_t n_modes, int64_t n,
+                          int64_t n_bootstrap, uint64_t seed, int device, double *y,
+                          double *y_unc, double *uniformity, double *convergence_ratio) {
+  if (n_modes < 1 || n_modes > yukawa_max_modes())
+    return set_error(CY3Q_ERR_ARG, "n_modes must be in 1..64");
+  if (!modes || !omega || !weights || n <= 0) return set_error(CY3Q_ERR_ARG, "bad input");
+  Ctx c;
+  G_TRY(c.open(device));
+  const int nm = (int)n_modes;
+  const int64_t n3 = (int64_t)nm * nm * nm;
+  double2 *d_modes, *d_y, *d_yb;
+  double *d_om, *d_w, *d_sz, *d_wb, *d_h = nullptr, *d_score, *d_acc, *d_unc, *d_tot, *d_conv;
+  int32_t *d_cnt;
+  G_TRY(c.upload(&d_modes, reinterpret_cast<const double2 *>(modes), (size_t)(nm * n)));
+  G_TRY(c.upload(&d_om, omega, (size_t)(2 * n)));
+  G_TRY(c.upload(&d_w, weights, (size_t)n));
+  if (h_v_diag) G_TRY(c.upload(&d_h, h_v_diag, (size_t)nm));
+  G_TRY(c.alloc(&d_sz, (size_t)n));
+  G_TRY(c.alloc(&d_wb, (size_t)n));
+  G_TRY(c.alloc(&d_score, 1));
+  G_TRY(c.alloc(&d_tot, 1));
+  G_TRY(c.alloc(&d_conv, 1));
+  G_TRY(c.alloc(&d_cnt, (size_t)n));
+  G_TRY(c.alloc(&d_y, (size_t)n3));
+  G_TRY(c.alloc(&d_yb, (size_t)n3));
+  G_TRY(c.zeros(&d_acc, (size_t)n3));
+  G_TRY(c.alloc(&d_unc, (size_t)n3));
+  YukWs ws;
+  G_TRY(ws.init(c, nm, n));
+  launch_sz_weights(c.st, d_w, d_om, n, d_sz, d_score);
+  launch_yukawa(c.st, d_modes, n, d_sz, d_om, nm, n, 1, ws.trip, ws.n_trip, d_h, ws.n_chunks,
+                ws.fac, ws.bad, ws.partials, nullptr, d_y);
+  const int64_t nbs = std::max<int64_t>(n_bootstrap, 1);
+  for (int64_t b = 0; b < nbs; b++) {
+    G_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * n, c.st));
+    launch_bootstrap_counts(c.st, seed + (uint64_t)b * 2654435761ull, n, d_cnt);
+    launch_scale_weights(c.st, d_sz, d_cnt, nullptr, n, d_wb);
+    launch_yukawa(c.st, d_modes, n, d_wb, d_om, nm, n, 0, ws.trip, ws.n_trip, d_h, ws.n_chunks,
+                  ws.fac, ws.bad, ws.partials, nullptr, d_yb);
+    launch_bootstrap_acc(c.st, d_yb, d_y, n3, d_acc);
+  }
+  launch_bootstrap_final(c.st, d_acc, n3, (double)nbs, d_unc);
+  double conv = NAN;
+  const int64_t half = n / 2;
+  if (half >= 8) {
+    launch_sum_f64(c.st, d_sz, half, d_tot);
+    launch_scale_weights(c.st, d_sz, nullptr, d_tot, half, d_wb);
+    launch_yukawa(c.st, d_modes, n, d_wb, d_om, nm, half, 1, ws.trip, ws.n_trip, d_h,
+                  yukawa_chunks(half, ws.n_trip), ws.fac, ws.bad, ws.partials, nullptr, d_yb);
+    launch_conv_ratio(c.st, d_y, d_yb, n3, d_conv);
+    G_HIP(hipMemcpyAsync(&conv, d_conv, sizeof(double), hipMemcpyDeviceToHost, c.st));
+  }
+  G_HIP(hipGetLastError());
+  if (y) G_HIP(hipMemcpyAsync(y, d_y, sizeof(double2) * n3, hipMemcpyDeviceToHost, c.st));
+  if (y_unc) G_HIP(hipMemcpyAsync(y_unc, d_unc, sizeof(double) * n3, hipMemcpyDeviceToHost, c.st));
+  if (uniformity)
+    G_HIP(hipMemcpyAsync(uniformity, d_score, sizeof(double), hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  if (convergence_ratio) *convergence_ratio = conv;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_modes,
+                                    uint64_t coeff_seed, double *y_triples, double *coeffs,
+                                    int64_t *n_triples) {
+  if (n_modes < 1 || n_modes > yukawa_max_modes())
+    return set_error(CY3Q_ERR_ARG, "n_modes must be in 1..64");
+  if (degree < 1 || degree > 12) return set_error(CY3Q_ERR_ARG, "degree must be in 1..12");
+  SolverView v;
+  G_TRY(solver_view(s, &v));
+  Ctx c;
+  c.device = v.device;
+  c.st = v.st;
+  const int nm = (int)n_modes;
+  std::vector<uint8_t> sm8 = raw_monomials(degree);
+  std::vector<uint32_t> sm(sm8.begin(), sm8.end());
+  const int nsec = (int)(sm.size() / 5);
+  const int64_t n = v.n;
+  const int64_t m5 = padded_cols(nsec);
+  const int64_t chunk = std::min<int64_t>(n, 65536);
+  uint32_t *d_mono;
+  double *sv5, *cw;
+  double2 *coef, *modes, *d_y;
+  G_TRY(c.upload(&d_mono, sm.data(), sm.size()));
+  G_TRY(c.alloc(&sv5, (size_t)(chunk * m5)));
+  G_TRY(c.alloc(&cw, (size_t)n));
+  G_TRY(c.alloc(&coef, (size_t)nsec * nm));
+  G_TRY(c.alloc(&modes, (size_t)nm * n));
+  launch_random_coeffs(c.st, coeff_seed, (int64_t)nsec * nm, coef);
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    const int64_t cn = std::min(chunk, n - p0);
+    launch_sections(c.st, v.pts + 10 * p0, cn, d_mono, nsec, (int)degree, m5, sv5, nullptr);
+    launch_modes_from_sections(c.st, sv5, cn, m5, nsec, coef, nm, modes + p0, n);
+  }
+  launch_measure_w_omega2(c.st, v.w, v.log_om, n, cw);
+  YukWs ws;
+  G_TRY(ws.init(c, nm, n));
+  G_TRY(c.alloc(&d_y, (size_t)ws.n_trip));
+  launch_yukawa(c.st, modes, n, cw, nullptr, nm, n, 0, ws.trip, ws.n_trip, nullptr, ws.n_chunks,
+                ws.fac, ws.bad, ws.partials, d_y, nullptr);
+  G_HIP(hipGetLastError());
+  if (n_triples) *n_triples = ws.n_trip;
+  if (y_triples)
+    G_HIP(hipMemcpyAsync(y_triples, d_y, sizeof(double2) * ws.n_trip, hipMemcpyDeviceToHost, c.st));
+  if (coeffs)
+    G_HIP(hipMemcpyAsync(coeffs, coef, sizeof(double2) * nsec * nm, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+}  // extern "C"

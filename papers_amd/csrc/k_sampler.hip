@@ -6,42 +6,9 @@
 // advanced 40*l steps. The lane reaches that state by affine jump-ahead, which makes every line
 // independent and reproduces the reference's sequential draws bit for bit. Accepted candidates
 // are compacted in (line, root) order by an exclusive scan, which keeps the reference's point order.
-#include "cy3q_internal.h"
+#include "cy3q_lcg.h"
 
 namespace cy3q {
-
-static constexpr uint64_t LCG_MUL = 6364136223846793005ULL;
-static constexpr uint64_t LCG_INC = 1442695040888963407ULL;
-
-struct Lcg {
-  uint64_t s;
-  __device__ double next_f64() {
-    s = s * LCG_MUL + LCG_INC;
-    uint64_t bits = (s >> 11) & ((1ULL << 53) - 1ULL);
-    return (double)bits / (double)(1ULL << 53);
-  }
-  __device__ double next_normal() {
-    double u1 = fmax(next_f64() + 1e-12, 1e-12);
-    double u2 = next_f64();
-    double r = sqrt(-2.0 * log(u1));
-    return r * cos(2.0 * M_PI * u2);
-  }
-};
-
-// State after `steps` applications of s -> MUL*s + INC, starting from s0.
-__device__ inline uint64_t lcg_jump(uint64_t s0, uint64_t steps) {
-  uint64_t acc_a = 1, acc_c = 0, a = LCG_MUL, c = LCG_INC;
-  while (steps) {
-    if (steps & 1ULL) {
-      acc_a = a * acc_a;
-      acc_c = a * acc_c + c;
-    }
-    c = a * c + c;
-    a = a * a;
-    steps >>= 1;
-  }
-  return acc_a * s0 + acc_c;
-}
 
 __device__ inline void quintic_value(const double *z, int dwork, double psi, double &fre,
                                      double &fim) {

@@ -640,6 +640,29 @@ int sigma_grad_impl(cy3q_solver *s, double *out6) {
 
 }  // namespace
 
+namespace cy3q {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+int solver_view(cy3q_solver *s, SolverView *v) {
+  if (!s || !v) return fail(CY3Q_ERR_ARG, "null solver");
+  HIP_TRY(hipSetDevice(s->device));
+  ensure_bmat(s);
+  v->st = s->st;
+  v->device = s->device;
+  v->n = s->n;
+  v->m = s->m;
+  v->nb = s->nb;
+  v->psi = s->psi;
+  v->pts = s->pts;
+  v->w = s->w;
+  v->log_om = s->log_om;
+  v->sv = s->sv;
+  v->sd = s->sd;
+  v->bmat = s->bmat;
+  return CY3Q_OK;
+}
+}  // namespace cy3q
+
 // ============================================================================================
 // C ABI
 // ============================================================================================

@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -40,3 +42,31 @@ def test_bad_arguments_fail_loudly(cy3q):
         q.QuinticSolver.new_with_sampler(3, 0, 1)
     with pytest.raises(q.Cy3qError):
         q.QuinticSolver._new(3, 100, 1, 0.5, q.SAMPLER_SZ)
+
+
+def test_laplacian_basis_table_bit_exact(cy3q, oracle):
+    # build_test_basis (metric_laplacian.rs:87-115) is host-side table code: bit-exact vs the oracle.
+    from papers_amd import galerkin as G
+    for deg in range(0, 6):
+        assert np.array_equal(G.build_test_basis(deg), oracle.lap_basis(deg))
+
+
+def test_laplacian_config_defaults(cy3q):
+    # MetricLaplacianConfig::default (metric_laplacian.rs:277-290)
+    from papers_amd import galerkin as G
+    c = G.LaplacianConfig.default()
+    assert (c.max_total_degree, c.n_low_eigenvalues, c.jacobi_max_sweeps) == (4, 15, 128)
+    assert (c.jacobi_tol, c.mass_regularisation) == (1e-12, 1e-10)
+
+
+def test_galerkin_bad_arguments_fail_loudly(cy3q):
+    # Argument checks run before any device call.
+    import pytest
+    from papers_amd import galerkin as G
+    from papers_amd import quintic as q
+    with pytest.raises(q.Cy3qError):
+        G.triple_overlap(np.zeros((65, 4), complex), np.ones(4, complex), np.ones(4))
+    with pytest.raises(q.Cy3qError):
+        G.compute_metric_laplacian_spectrum(np.zeros((4, 8), complex), np.ones(4),
+                                            np.ones(4, complex),
+                                            G.LaplacianConfig.default(max_total_degree=7))
