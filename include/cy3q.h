@@ -225,6 +225,11 @@ int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_m
                                     uint64_t coeff_seed, double *y_triples, double *coeffs,
                                     int64_t *n_triples);
 
+/* Device time per stage of the calling thread's last Galerkin call (HIP events on its stream):
+ * ms[i], count[i] for stage i < *n_stages, named by cy3q_galerkin_stage_name(i). */
+int cy3q_galerkin_stage_times(double *ms, int64_t *count, int cap, int *n_stages);
+const char *cy3q_galerkin_stage_name(int stage);
+
 #ifdef __cplusplus
 }
 #endif

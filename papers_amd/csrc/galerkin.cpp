@@ -91,6 +91,62 @@ struct Ctx {
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Per-stage device time of the calling thread's last Galerkin call (HIP events on its stream).
+enum GStage {
+  GS_ROWS = 0,   // per-point metric factor and row generation
+  GS_HERK,       // FP64 MFMA Hermitian rank-k accumulation
+  GS_FINALIZE,   // complex M, K from the real accumulators
+  GS_EIGEN_M,    // eigensolve of M (with vectors)
+  GS_REDUCE,     // M^-1/2 reduction GEMMs (or M^-1 K)
+  GS_EIGEN_C,    // eigensolve of the reduced pencil
+  GS_YUK_MODES,  // mode values psi = S C
+  GS_YUK_TRIPLES,
+  GS_COUNT
+};
+const char *kGStageNames[GS_COUNT] = {"rows", "herk", "finalize", "eigen_mass",
+                                      "reduce", "eigen_reduced", "yukawa_modes",
+                                      "yukawa_triples"};
+thread_local double g_stage_ms[GS_COUNT];
+thread_local int64_t g_stage_cnt[GS_COUNT];
+
+struct StageTimer {
+  hipStream_t st;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  explicit StageTimer(hipStream_t s) : st(s) {
+    for (int i = 0; i < GS_COUNT; i++) {
+      g_stage_ms[i] = 0.0;
+      g_stage_cnt[i] = 0;
+    }
+  }
+  void begin() {
+    hipEvent_t a = nullptr;
+    hipEventCreate(&a);
+    hipEventRecord(a, st);
+    ev.push_back({-1, {a, nullptr}});
+  }
+  void end(int id) {
+    hipEvent_t b = nullptr;
+    hipEventCreate(&b);
+    hipEventRecord(b, st);
+    ev.back().first = id;
+    ev.back().second.second = b;
+  }
+  ~StageTimer() {
+    for (auto &e : ev) {
+      if (e.second.second) {
+        hipEventSynchronize(e.second.second);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess && e.first >= 0) {
+          g_stage_ms[e.first] += ms;
+          g_stage_cnt[e.first] += 1;
+        }
+        hipEventDestroy(e.second.second);
+      }
+      hipEventDestroy(e.second.first);
+    }
+  }
+};
+
 // ------------------------------------------------------------------------------------------
 // Hermitian eigensolver: parallel Jacobi sweeps (k_jacobi_round) until the off-diagonal
 // Frobenius norm is at most tol * ||A||_F (or is zero), or max_sweeps sweeps. a: device
@@ -172,7 +228,7 @@ int frob(Ctx &c, const double2 *a, int n, double *out) {
 // (M + lambda I)^-1 K with lambda = max(reg ||M||_F, 1e-30), through the eigenvectors of M, then
 // the Hermitian part, then its eigenvalues as |lambda| ascending.
 int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64_t max_sweeps,
-                       double tol, double reg, std::vector<double> &full) {
+                       double tol, double reg, std::vector<double> &full, StageTimer &tm) {
   double fm;
   G_TRY(frob(c, M, nb, &fm));
   const double lambda = std::max(fm * reg, 1.0e-30);
@@ -182,7 +238,9 @@ int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64
   G_TRY(c.alloc(&Minv, (size_t)nb * nb));
   G_TRY(c.alloc(&L, (size_t)nb * nb));
   std::vector<double> mu;
+  tm.begin();
   G_TRY(heev(c, M, nb, max_sweeps, tol, lambda, mu, U, nullptr));
+  tm.end(GS_EIGEN_M);
   std::vector<double> inv(nb);
   for (int j = 0; j < nb; j++) inv[j] = 1.0 / mu[j];
   std::vector<int> ident(nb);
@@ -191,12 +249,16 @@ int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64
   int *d_id;
   G_TRY(c.upload(&d_inv, inv.data(), nb));
   G_TRY(c.upload(&d_id, ident.data(), nb));
+  tm.begin();
   launch_gather_cols(c.st, U, nb, nb, d_id, d_inv, nb, Y);           // U diag(1/mu)
   launch_zgemm(c.st, Y, U, Minv, nb, nb, nb, nb, nb, nb, 0, 1);       // ... U^H
   launch_zgemm(c.st, Minv, K, L, nb, nb, nb, nb, nb, nb, 0, 0);       // M^-1 K
   launch_herm_project(c.st, L, nb);
+  tm.end(GS_REDUCE);
   std::vector<double> ev;
+  tm.begin();
   G_TRY(heev(c, L, nb, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+  tm.end(GS_EIGEN_C);
   full.resize(nb);
   for (int j = 0; j < nb; j++) full[j] = std::fabs(ev[j]);
   std::stable_sort(full.begin(), full.end());
@@ -206,11 +268,14 @@ int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64
 // Generalized K v = lambda M v by the canonical reduction X = U_keep diag(mu^-1/2):
 // C = X^H K X, the eigenvalues of its Hermitian part ascending.
 int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64_t max_sweeps,
-                         double tol, double defl, std::vector<double> &ev, int64_t *rank) {
+                         double tol, double defl, std::vector<double> &ev, int64_t *rank,
+                         StageTimer &tm) {
   double2 *U;
   G_TRY(c.alloc(&U, (size_t)nb * nb));
   std::vector<double> mu;
+  tm.begin();
   G_TRY(heev(c, M, nb, max_sweeps, tol, 0.0, mu, U, nullptr));
+  tm.end(GS_EIGEN_M);
   const double mmax = mu.empty() ? 0.0 : mu.back();
   std::vector<int> keep;
   std::vector<double> sc;
@@ -230,11 +295,15 @@ int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int
   G_TRY(c.alloc(&X, (size_t)nb * r));
   G_TRY(c.alloc(&T, (size_t)nb * r));
   G_TRY(c.alloc(&C, (size_t)r * r));
+  tm.begin();
   launch_gather_cols(c.st, U, nb, nb, d_keep, d_sc, r, X);
   launch_zgemm(c.st, K, X, T, nb, r, nb, nb, r, r, 0, 0);  // K X
   launch_zgemm(c.st, X, T, C, r, r, nb, r, r, r, 1, 0);    // X^H K X
   launch_herm_project(c.st, C, r);
+  tm.end(GS_REDUCE);
+  tm.begin();
   G_TRY(heev(c, C, r, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+  tm.end(GS_EIGEN_C);
   return CY3Q_OK;
 }
 
@@ -308,6 +377,19 @@ struct YukWs {
 
 extern "C" {
 
+int cy3q_galerkin_stage_times(double *ms, int64_t *count, int cap, int *n_stages) {
+  if (n_stages) *n_stages = GS_COUNT;
+  for (int i = 0; i < GS_COUNT && i < cap; i++) {
+    if (ms) ms[i] = g_stage_ms[i];
+    if (count) count[i] = g_stage_cnt[i];
+  }
+  return CY3Q_OK;
+}
+
+const char *cy3q_galerkin_stage_name(int stage) {
+  return (stage >= 0 && stage < GS_COUNT) ? kGStageNames[stage] : "";
+}
+
 void cy3q_laplacian_config_default(cy3q_laplacian_config *cfg) {
   if (!cfg) return;
   cfg->max_total_degree = 4;
@@ -362,6 +444,7 @@ int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
   if (!points || !weights || !omega) return set_error(CY3Q_ERR_ARG, "null input");
   Ctx c;
   G_TRY(c.open(device));
+  StageTimer tm(c.st);
   const int64_t mc = round_up(2 * (int64_t)nb, 128);
   const int nder = 8;
   const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(1024, (int64_t)(3e8 / (9.0 * mc))));
@@ -379,8 +462,10 @@ int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
   for (int q = 0; q < nder; q++) G_TRY(c.zeros(&ck[q], (size_t)(mc * mc)));
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
+    tm.begin();
     launch_lap_rows_ambient(c.st, d_pts + 16 * p0, d_w + p0, d_om + 2 * p0, cn, d_ex, nb, mc, x0,
                             x1, wv + p0);
+    tm.end(GS_ROWS);
     const double *xs[9];
     int64_t rows[9];
     double *cs[9];
@@ -392,13 +477,17 @@ int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
       rows[1 + q] = cn;
       cs[1 + q] = ck[q];
     }
+    tm.begin();
     launch_herk_f64(c.st, xs, rows, cs, 1 + nder, mc);
+    tm.end(GS_HERK);
   }
   launch_sum_f64(c.st, wv, n, tw);
   double2 *M, *K;
   G_TRY(c.alloc(&M, (size_t)nb * nb));
   G_TRY(c.alloc(&K, (size_t)nb * nb));
+  tm.begin();
   launch_lap_finalize(c.st, cmass, ck, nder, nb, mc, tw, 1, M, K);
+  tm.end(GS_FINALIZE);
   G_HIP(hipGetLastError());
   c.release(x0);
   c.release(x1);
@@ -407,7 +496,7 @@ int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
     G_HIP(hipMemcpyAsync(stiffness, K, sizeof(double2) * nb * nb, hipMemcpyDeviceToHost, c.st));
   std::vector<double> full;
   G_TRY(reference_spectrum(c, M, K, nb, cfg.jacobi_max_sweeps, cfg.jacobi_tol,
-                           cfg.mass_regularisation, full));
+                           cfg.mass_regularisation, full, tm));
   const int64_t n_low = std::min<int64_t>(std::max<int64_t>(cfg.n_low_eigenvalues, 0), nb);
   if (eig_low) std::memcpy(eig_low, full.data(), sizeof(double) * n_low);
   if (eig_full) std::memcpy(eig_full, full.data(), sizeof(double) * nb);
@@ -426,6 +515,7 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   Ctx c;
   c.device = v.device;
   c.st = v.st;
+  StageTimer tm(c.st);
   std::vector<uint8_t> sm = raw_monomials(section_degree);
   const int ns = (int)(sm.size() / 5);
   const int B = ns * ns;
@@ -447,22 +537,28 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(mc * mc)));
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
+    tm.begin();
     launch_trigemm_f64(c.st, v.sv + p0 * m, cn, m, v.bmat, y0, 0, 0);
     launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
     launch_lap_metric(c.st, v.pts + 10 * p0, y0, yd, v.w + p0, v.log_om + p0, cn, m, v.psi, pr,
                       wv + p0);
     launch_lap_rows_quintic(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, mc, x0,
                             x1);
+    tm.end(GS_ROWS);
     const double *xs[4] = {x0, x1, x1 + cn * mc, x1 + 2 * cn * mc};
     int64_t rows[4] = {cn, cn, cn, cn};
     double *cs[4] = {cmass, ck[0], ck[1], ck[2]};
+    tm.begin();
     launch_herk_f64(c.st, xs, rows, cs, 4, mc);
+    tm.end(GS_HERK);
   }
   launch_sum_f64(c.st, wv, n, tw);
   double2 *M, *K;
   G_TRY(c.alloc(&M, (size_t)B * B));
   G_TRY(c.alloc(&K, (size_t)B * B));
+  tm.begin();
   launch_lap_finalize(c.st, cmass, ck, 3, B, mc, tw, 0, M, K);
+  tm.end(GS_FINALIZE);
   G_HIP(hipGetLastError());
   for (void *p : {(void *)x0, (void *)x1, (void *)y0, (void *)yd}) c.release(p);
   if (mass) G_HIP(hipMemcpyAsync(mass, M, sizeof(double2) * B * B, hipMemcpyDeviceToHost, c.st));
@@ -470,7 +566,7 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     G_HIP(hipMemcpyAsync(stiffness, K, sizeof(double2) * B * B, hipMemcpyDeviceToHost, c.st));
   std::vector<double> ev;
   G_TRY(generalized_spectrum(c, M, K, B, max_sweeps > 0 ? max_sweeps : 64, 1e-14,
-                             deflation_tol > 0.0 ? deflation_tol : 1e-10, ev, rank));
+                             deflation_tol > 0.0 ? deflation_tol : 1e-10, ev, rank, tm));
   const int64_t nl = std::min<int64_t>(std::max<int64_t>(n_low, 0), (int64_t)ev.size());
   if (eig_low) std::memcpy(eig_low, ev.data(), sizeof(double) * nl);
   G_HIP(hipStreamSynchronize(c.st));
@@ -512,8 +608,11 @@ int cy3q_yukawa_triple_overlap(const double *modes, const double *omega, const d
   YukWs ws;
   G_TRY(ws.init(c, nm, n));
   G_TRY(c.alloc(&d_y, (size_t)nm * nm * nm));
+  StageTimer tm(c.st);
+  tm.begin();
   launch_yukawa(c.st, d_modes, n, d_w, d_om, nm, n, 1, ws.trip, ws.n_trip, d_h, ws.n_chunks,
                 ws.fac, ws.bad, ws.partials, nullptr, d_y);
+  tm.end(GS_YUK_TRIPLES);
   G_HIP(hipGetLastError());
   G_HIP(hipMemcpyAsync(y, d_y, sizeof(double2) * nm * nm * nm, hipMemcpyDeviceToHost, c.st));
   G_HIP(hipStreamSynchronize(c.st));
@@ -601,6 +700,7 @@ int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_m
   const int64_t n = v.n;
   const int64_t m5 = padded_cols(nsec);
   const int64_t chunk = std::min<int64_t>(n, 65536);
+  StageTimer tm(c.st);
   uint32_t *d_mono;
   double *sv5, *cw;
   double2 *coef, *modes, *d_y;
@@ -609,6 +709,7 @@ int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_m
   G_TRY(c.alloc(&cw, (size_t)n));
   G_TRY(c.alloc(&coef, (size_t)nsec * nm));
   G_TRY(c.alloc(&modes, (size_t)nm * n));
+  tm.begin();
   launch_random_coeffs(c.st, coeff_seed, (int64_t)nsec * nm, coef);
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
@@ -616,11 +717,14 @@ int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_m
     launch_modes_from_sections(c.st, sv5, cn, m5, nsec, coef, nm, modes + p0, n);
   }
   launch_measure_w_omega2(c.st, v.w, v.log_om, n, cw);
+  tm.end(GS_YUK_MODES);
   YukWs ws;
   G_TRY(ws.init(c, nm, n));
   G_TRY(c.alloc(&d_y, (size_t)ws.n_trip));
+  tm.begin();
   launch_yukawa(c.st, modes, n, cw, nullptr, nm, n, 0, ws.trip, ws.n_trip, nullptr, ws.n_chunks,
                 ws.fac, ws.bad, ws.partials, d_y, nullptr);
+  tm.end(GS_YUK_TRIPLES);
   G_HIP(hipGetLastError());
   if (n_triples) *n_triples = ws.n_trip;
   if (y_triples)

@@ -41,6 +41,9 @@ def _lib():
         L.cy3q_yukawa_couplings.argtypes = [vp, vp, vp, vp, i64, i64, i64, u64, ip, vp, vp,
                                             C.POINTER(dbl), C.POINTER(dbl)]
         L.cy3q_solver_yukawa_random_modes.argtypes = [vp, u32, i64, u64, vp, vp, C.POINTER(i64)]
+        L.cy3q_galerkin_stage_times.argtypes = [vp, vp, ip, C.POINTER(ip)]
+        L.cy3q_galerkin_stage_name.argtypes = [ip]
+        L.cy3q_galerkin_stage_name.restype = C.c_char_p
         _bound = True
     return L
 
@@ -201,6 +204,17 @@ def solver_yukawa_random_modes(solver, degree: int = 5, n_modes: int = 20, coeff
     _check(_lib().cy3q_solver_yukawa_random_modes(solver._h, degree, n_modes, coeff_seed, _ptr(y),
                                                   _ptr(cf), C.byref(nt)))
     return y[:nt.value], cf
+
+
+def last_stage_times() -> dict:
+    """{stage: (device ms, launches)} of this thread's last Galerkin call."""
+    ms = np.zeros(16)
+    cnt = np.zeros(16, np.int64)
+    ns = C.c_int()
+    L = _lib()
+    _check(L.cy3q_galerkin_stage_times(_ptr(ms), _ptr(cnt), 16, C.byref(ns)))
+    return {L.cy3q_galerkin_stage_name(i).decode(): (float(ms[i]), int(cnt[i]))
+            for i in range(ns.value) if cnt[i]}
 
 
 def triple_index(n_modes: int) -> np.ndarray:
