@@ -98,6 +98,11 @@ int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, in
  * FP64 path: the sweep re-applies the exact FP64 T-operator to the fast path's metric, so the
  * returned h and residual are those of one FP64 iteration. */
 int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps);
+/* Tile parameters (the autotuner's knobs; no counterpart in the reference): point chunks per
+ * CU of the split-precision T pass (1..16) and the target workgroup count of the FP64 rank-k
+ * T pass (64..8192). 0 keeps a value. Results change only by summation order. */
+int cy3q_solver_set_tuning(cy3q_solver *s, int64_t tpass_wgs_per_cu, int64_t syrk_target_wgs);
+int cy3q_solver_get_tuning(cy3q_solver *s, int64_t *tpass_wgs_per_cu, int64_t *syrk_target_wgs);
 /* QuinticSolver::sigma (quintic.rs:423-425, 1301-1499). */
 int cy3q_solver_sigma(cy3q_solver *s, double *sigma);
 /* sigma with its side outputs: kappa, sigma_L2^2 = sum w (eta/kappa-1)^2 / sum w, and the
@@ -224,6 +229,41 @@ int cy3q_yukawa_couplings(const double *modes, const double *omega, const double
 int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_modes,
                                     uint64_t coeff_seed, double *y_triples, double *coeffs,
                                     int64_t *n_triples);
+
+/* ============================================================================================
+ * Batched instances (no counterpart in the reference, whose instances run one at a time; the
+ * nearest analogue is the multi-stream task orchestrator, cy3/src/orchestrator.rs:1-12).
+ * ========================================================================================== */
+typedef struct cy3q_instance_spec {
+  uint32_t k;       /* section degree */
+  int64_t n_points; /* sample size */
+  uint64_t seed;    /* sampler seed */
+  double psi;       /* Dwork modulus (0: Fermat sampler) */
+} cy3q_instance_spec;
+typedef struct cy3q_batch cy3q_batch;
+
+/* n instance specs from the lib.rs LCG seeded with `seed`, three draws u per instance:
+ * psi = 0.9 u1, k = 2 + floor(5 u2) (2..6), N_p = round(exp(ln 2e4 + u3 (ln 5e5 - ln 2e4)));
+ * sampler seed 42 + i. */
+int cy3q_batch_draw_specs(uint64_t seed, int n, cy3q_instance_spec *out);
+/* One solver per spec (QuinticSolver::new_with_psi_sz / new_with_sampler, then
+ * orthonormalise_basis_fs_gram), built concurrently; the options apply to every instance. */
+int cy3q_batch_new(const cy3q_instance_spec *specs, int n, const cy3q_options *opt,
+                   cy3q_batch **out);
+void cy3q_batch_free(cy3q_batch *b);
+int cy3q_batch_size(cy3q_batch *b, int *n);
+/* Borrowed handle of instance i (owned by the batch): sigma, h access, etc. */
+int cy3q_batch_solver(cy3q_batch *b, int i, cy3q_solver **s);
+/* h = I for every instance, then donaldson_solve(iters, 0) for each, one after another. */
+int cy3q_batch_solve_sequential(cy3q_batch *b, int64_t iters);
+/* The same solves, all instances captured into one hipGraph (h = I reset included) on first
+ * use for this `iters`, then one graph launch. Per-instance results equal the sequential ones
+ * bit for bit. */
+int cy3q_batch_solve_graph(cy3q_batch *b, int64_t iters);
+/* Times the tuning candidates per (k, quotient, round(log2 N_p)) and applies the fastest to
+ * every instance, reusing and updating the text cache at cache_path (may be NULL).
+ * *n_measured: shapes timed (not found in the cache). Leaves h = I. */
+int cy3q_batch_autotune(cy3q_batch *b, const char *cache_path, int64_t *n_measured);
 
 /* Device time per stage of the calling thread's last Galerkin call (HIP events on its stream):
  * ms[i], count[i] for stage i < *n_stages, named by cy3q_galerkin_stage_name(i). */

@@ -38,6 +38,7 @@ SOURCES = {
     "k_yukawa.hip": [],
     "solver.cpp": ["-x", "hip"],
     "galerkin.cpp": ["-x", "hip"],
+    "batch.cpp": ["-x", "hip"],
 }
 
 

@@ -228,4 +228,12 @@ namespace cy3q {
 int set_error(int code, const std::string &msg);
 // Fills v for s (building the contraction operand of the current h first).
 int solver_view(cy3q_solver *s, SolverView *v);
+// Batched and captured solves (batch.cpp): the solver's stream; set-up that must precede a
+// capture; donaldson_solve(iters, tol <= 0) as launches only; h = I as launches; the last
+// residual of the previous enqueued solve.
+hipStream_t solver_stream(cy3q_solver *s);
+int solver_prepare_enqueue(cy3q_solver *s, int64_t iters);
+int solver_enqueue_solve(cy3q_solver *s, int64_t iters);
+int solver_reset_identity(cy3q_solver *s);
+int solver_last_residual(cy3q_solver *s, int64_t iters, double *out);
 }  // namespace cy3q

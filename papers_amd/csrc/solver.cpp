@@ -127,6 +127,10 @@ struct cy3q_solver {
         *block_max = nullptr;
   double *tseg = nullptr;            // T-pass segment sums
   int64_t tp_chunk = 0, tp_chunks = 0;
+  int64_t tp_mult = 1;         // T-pass point chunks per CU (a tuning knob)
+  int64_t syrk_target = 1024;  // workgroups of the FP64 rank-k pass (a tuning knob)
+  double *batch_res = nullptr;  // residuals of captured solves
+  int64_t batch_res_cap = 0;
   int *row_ex = nullptr;
   _Float16 *xd = nullptr;
   float *xt = nullptr;
@@ -313,7 +317,7 @@ int finish_workspace(cy3q_solver *s) {
   launch_identity_cplx(s->st, nb, s->t);
   s->bmat_valid = false;
   if (s->m > 1024) return fail(CY3Q_ERR_ARG, "2N > 1024 (k > 9) is not supported");
-  ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, syrk_f64_max_tiles(s->m), 1024, 256, 16));
+  ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, syrk_f64_max_tiles(s->m), s->syrk_target, 256, 16));
   s->partials_cap = syrk_partials_count(s->plan64);
   ST_TRY(s->alloc(&s->partials, s->partials_cap));
   HIP_TRY(hipStreamSynchronize(s->st));
@@ -475,7 +479,8 @@ int ensure_split16(cy3q_solver *s) {
     int n_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
     n_cu = std::max(n_cu, 1);
-    s->tp_chunk = std::max<int64_t>((n + (int64_t)n_cu * 16 - 1) / ((int64_t)n_cu * 16), 1) * 16;
+    const int64_t slots = (int64_t)n_cu * s->tp_mult;
+    s->tp_chunk = std::max<int64_t>((n + slots * 16 - 1) / (slots * 16), 1) * 16;
     s->tp_chunks = (n + s->tp_chunk - 1) / s->tp_chunk;
     ST_TRY(s->alloc(&s->partials32, tpass_partials(m, s->tp_chunks)));
     ST_TRY(s->alloc(&s->tseg, tpass_seg_count(m, s->tp_chunks)));
@@ -661,6 +666,57 @@ int solver_view(cy3q_solver *s, SolverView *v) {
   v->bmat = s->bmat;
   return CY3Q_OK;
 }
+
+hipStream_t solver_stream(cy3q_solver *s) { return s->st; }
+
+// Everything a captured solve needs, allocated and built now: the split path's static operand
+// planes and plan, and a residual slot per iteration.
+int solver_prepare_enqueue(cy3q_solver *s, int64_t iters) {
+  HIP_TRY(hipSetDevice(s->device));
+  if (s->precision == CY3Q_PREC_SPLIT16) ST_TRY(ensure_split16(s));
+  if (s->batch_res_cap < iters) {
+    if (s->batch_res) s->release(s->batch_res);
+    s->batch_res = nullptr;
+    ST_TRY(s->alloc(&s->batch_res, std::max<int64_t>(iters, 1)));
+    s->batch_res_cap = std::max<int64_t>(iters, 1);
+  }
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+// The iterations of donaldson_solve(iters, tol <= 0) as kernel launches only: no allocation,
+// copy or synchronisation, so the sequence can be captured into a graph.
+int solver_enqueue_solve(cy3q_solver *s, int64_t iters) {
+  if (s->batch_res_cap < iters) return fail(CY3Q_ERR_STATE, "solver_prepare_enqueue first");
+  const int64_t n_ref = s->precision == CY3Q_PREC_FP64 ? 0 : std::min(s->refine, iters);
+  for (int64_t it = 0; it < iters; it++) {
+    double *keep = s->resid;
+    s->resid = s->batch_res + it;
+    const int prec = it >= iters - n_ref ? CY3Q_PREC_FP64 : s->precision;
+    int rc = step_in(s, prec);
+    s->resid = keep;
+    if (rc != CY3Q_OK) return rc;
+  }
+  return CY3Q_OK;
+}
+
+// h = I as kernel launches (G, W and T to the identity).
+int solver_reset_identity(cy3q_solver *s) {
+  launch_identity_cplx(s->st, s->nb, s->g);
+  launch_identity_cplx(s->st, s->nb, s->wmat);
+  launch_identity_cplx(s->st, s->nb, s->t);
+  s->bmat_valid = false;
+  s->bfrag_valid = false;
+  HIP_TRY(hipGetLastError());
+  return CY3Q_OK;
+}
+
+int solver_last_residual(cy3q_solver *s, int64_t iters, double *out) {
+  if (iters < 1 || s->batch_res_cap < iters) return fail(CY3Q_ERR_STATE, "no captured solve");
+  HIP_TRY(hipMemcpyAsync(out, s->batch_res + iters - 1, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
 }  // namespace cy3q
 
 // ============================================================================================
@@ -820,6 +876,51 @@ int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps) {
   if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
   if (sweeps < 0) return fail(CY3Q_ERR_ARG, "refinement sweeps must be >= 0");
   s->refine = sweeps;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_tuning(cy3q_solver *s, int64_t tpass_wgs_per_cu, int64_t syrk_target_wgs) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (tpass_wgs_per_cu < 0 || tpass_wgs_per_cu > 16 || syrk_target_wgs < 0 ||
+      (syrk_target_wgs > 0 && (syrk_target_wgs < 64 || syrk_target_wgs > 8192)))
+    return fail(CY3Q_ERR_ARG, "tuning out of range (1..16 chunks per CU, 64..8192 workgroups)");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  if (tpass_wgs_per_cu > 0 && tpass_wgs_per_cu != s->tp_mult) {
+    s->tp_mult = tpass_wgs_per_cu;
+    if (s->xd) {  // the split path is set up: re-plan its T-pass chunks
+      int n_cu = 0;
+      HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+      const int64_t slots = (int64_t)std::max(n_cu, 1) * s->tp_mult;
+      s->tp_chunk = std::max<int64_t>((s->n + slots * 16 - 1) / (slots * 16), 1) * 16;
+      s->tp_chunks = (s->n + s->tp_chunk - 1) / s->tp_chunk;
+      s->release(s->partials32);
+      s->release(s->tseg);
+      s->partials32 = nullptr;
+      s->tseg = nullptr;
+      ST_TRY(s->alloc(&s->partials32, tpass_partials(s->m, s->tp_chunks)));
+      ST_TRY(s->alloc(&s->tseg, tpass_seg_count(s->m, s->tp_chunks)));
+    }
+  }
+  if (syrk_target_wgs > 0 && syrk_target_wgs != s->syrk_target) {
+    s->syrk_target = syrk_target_wgs;
+    ST_TRY(build_syrk_plan(s, s->plan64, 16, 8, syrk_f64_max_tiles(s->m), s->syrk_target, 256, 16));
+    const int64_t need = syrk_partials_count(s->plan64);
+    if (need > s->partials_cap) {
+      s->release(s->partials);
+      s->partials = nullptr;
+      ST_TRY(s->alloc(&s->partials, need));
+      s->partials_cap = need;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_get_tuning(cy3q_solver *s, int64_t *tpass_wgs_per_cu, int64_t *syrk_target_wgs) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (tpass_wgs_per_cu) *tpass_wgs_per_cu = s->tp_mult;
+  if (syrk_target_wgs) *syrk_target_wgs = s->syrk_target;
   return CY3Q_OK;
 }
 

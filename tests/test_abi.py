@@ -70,3 +70,16 @@ def test_galerkin_bad_arguments_fail_loudly(cy3q):
         G.compute_metric_laplacian_spectrum(np.zeros((4, 8), complex), np.ones(4),
                                             np.ones(4, complex),
                                             G.LaplacianConfig.default(max_total_degree=7))
+
+
+def test_batch_specs_from_lcg(cy3q, oracle):
+    # psi = 0.9 u1, k = 2 + floor(5 u2), N_p = round(exp(ln 2e4 + u3 ln 25)) from the lib.rs LCG.
+    import math
+    from papers_amd.batch import draw_specs
+    specs = draw_specs(7, 32)
+    _, u, _ = oracle.lcg_stream(7, 96)
+    for i, (k, n, seed, psi) in enumerate(specs):
+        assert psi == 0.9 * u[3 * i]
+        assert k == 2 + min(4, math.floor(5 * u[3 * i + 1]))
+        assert n == round(math.exp(math.log(2e4) + u[3 * i + 2] * (math.log(5e5) - math.log(2e4))))
+        assert seed == 42 + i and 2 <= k <= 6 and 20000 <= n <= 500000
