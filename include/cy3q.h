@@ -93,6 +93,14 @@ int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual);
  * sweep below tol or after `refine` sweeps. */
 int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
                                 double *residuals);
+/* The legacy un-inverted iteration h <- T(h), trace-normalised, in FP64 on the solver's current
+ * sections: donaldson_iteration_quintic (quintic.rs:4140-4222) when weighted = 0 (all points,
+ * 1 / n_points), donaldson_iteration_quintic_weighted (:4256-4331) when weighted = 1 (valid
+ * weights, 1 / sum w). residual = ||H_new - H_old||_F. The solve starts from h = I, as
+ * donaldson_solve_quintic(_weighted) (:4224-4248, :4333-4357), and stops at residual < tol. */
+int cy3q_solver_donaldson_step_legacy(cy3q_solver *s, int weighted, double *residual);
+int cy3q_solver_donaldson_solve_legacy(cy3q_solver *s, int weighted, int64_t max_iter, double tol,
+                                       int64_t *iters, double *residuals);
 /* Number of FP64 T-operator refinement sweeps that end every split-precision donaldson_solve
  * (default 1; 0 turns refinement off). No counterpart in the reference, which has only the
  * FP64 path: the sweep re-applies the exact FP64 T-operator to the fast path's metric, so the

@@ -2010,3 +2010,70 @@ ORC_API void orc_point_metric(const double *pts, const double *sv, const double 
         free(scr);
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Legacy un-inverted Donaldson iteration: donaldson_iteration_quintic (quintic.rs:4140-4222)   */
+/* with w == NULL, donaldson_iteration_quintic_weighted (:4256-4331) otherwise. h_new: [2N][2N]. */
+/* ------------------------------------------------------------------------------------------ */
+ORC_API void orc_legacy_step(const double *sv, const double *w, const double *h, int64_t n,
+                             int64_t nb, double *h_new) {
+    int64_t two_n = 2 * nb;
+    double *inv_k = (double *)malloc(sizeof(double) * n);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; p++) {
+        const double *s = sv + p * two_n;
+        double k = 0.0;
+        for (int64_t i = 0; i < two_n; i++) {
+            double rs = 0.0;
+            for (int64_t j = 0; j < two_n; j++) rs += h[i * two_n + j] * s[j];
+            k += s[i] * rs;
+        }
+        inv_k[p] = 1.0 / fmax(k, 1e-30);
+    }
+    double *hre = (double *)calloc((size_t)(nb * nb), sizeof(double));
+    double *him = (double *)calloc((size_t)(nb * nb), sizeof(double));
+    double total_w = 0.0;
+    for (int64_t p = 0; p < n; p++) {
+        const double *s = sv + p * two_n;
+        double ik = inv_k[p];
+        if (w) {
+            double wp = w[p];
+            if (!isfinite(wp) || wp <= 0.0) continue;
+            total_w += wp;
+            ik = inv_k[p] * wp;
+        }
+        for (int64_t a = 0; a < nb; a++) {
+            double sar = s[2 * a], sai = s[2 * a + 1];
+            for (int64_t b = 0; b < nb; b++) {
+                double sbr = s[2 * b], sbi = s[2 * b + 1];
+                hre[a * nb + b] += (sar * sbr + sai * sbi) * ik;
+                him[a * nb + b] += (sar * sbi - sai * sbr) * ik;
+            }
+        }
+    }
+    double sc0 = w ? (total_w > 1e-12 ? 1.0 / total_w : 0.0) : 1.0 / (double)n;
+    for (int64_t e = 0; e < nb * nb; e++) {
+        hre[e] *= sc0;
+        him[e] *= sc0;
+    }
+    double tr = 0.0;
+    for (int64_t a = 0; a < nb; a++) tr += hre[a * nb + a];
+    if (tr > 1e-10) {
+        double sc = (double)nb / tr;
+        for (int64_t e = 0; e < nb * nb; e++) {
+            hre[e] *= sc;
+            him[e] *= sc;
+        }
+    }
+    for (int64_t a = 0; a < nb; a++)
+        for (int64_t b = 0; b < nb; b++) {
+            double r = hre[a * nb + b], i = him[a * nb + b];
+            h_new[(2 * a) * two_n + 2 * b] = r;
+            h_new[(2 * a) * two_n + 2 * b + 1] = -i;
+            h_new[(2 * a + 1) * two_n + 2 * b] = i;
+            h_new[(2 * a + 1) * two_n + 2 * b + 1] = r;
+        }
+    free(inv_k);
+    free(hre);
+    free(him);
+}

@@ -468,3 +468,32 @@ def yuk_couplings(modes, omega, w, hdiag=None, n_bootstrap: int = 64, seed: int 
                               np.ascontiguousarray(w, np.float64), _ptr(hd), nm, n, n_bootstrap,
                               seed, y, unc, C.byref(sc), C.byref(cv))
     return y[..., 0] + 1j * y[..., 1], unc, sc.value, cv.value
+
+
+def legacy_step(sv, h_block, w=None):
+    """donaldson_iteration_quintic (quintic.rs:4140-4222), or the _weighted form (:4256-4331)
+    when w is given: the un-inverted update h <- T(h), trace-normalised."""
+    L = lib()
+    if not getattr(L, "_legacy_bound", False):
+        L.orc_legacy_step.argtypes = [_dp, C.c_void_p, _dp, C.c_int64, C.c_int64, _dp]
+        L._legacy_bound = True
+    n, two_n = sv.shape
+    out = np.zeros((two_n, two_n))
+    wv = None if w is None else np.ascontiguousarray(w, np.float64)
+    L.orc_legacy_step(np.ascontiguousarray(sv), _ptr(wv), np.ascontiguousarray(h_block), n,
+                      two_n // 2, out)
+    return out
+
+
+def legacy_solve(sv, max_iter: int, tol: float, w=None):
+    """donaldson_solve_quintic(_weighted) (quintic.rs:4224-4248, 4333-4357): from h = I."""
+    h = identity_block(sv.shape[1] // 2)
+    res = []
+    for _ in range(max_iter):
+        hn = legacy_step(sv, h, w)
+        r = float(np.sqrt(np.sum((hn - h) ** 2)))
+        res.append(r)
+        h = hn
+        if r < tol:
+            break
+    return h, res

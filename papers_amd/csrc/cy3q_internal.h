@@ -101,6 +101,8 @@ void launch_cplx_to_block(hipStream_t st, const double *g_cplx, int nb, double *
 void launch_block_to_cplx(hipStream_t st, const double *h_block, int nb, double *g_cplx);
 void launch_identity_cplx(hipStream_t st, int nb, double *g_cplx);
 void launch_hermitian_part(hipStream_t st, double *g_cplx, int nb);
+void launch_legacy_replace(hipStream_t st, const double *t, int nb, double *g, double *resid);
+void launch_fill_f64(hipStream_t st, double *x, int64_t n, double v);
 
 // N x N solve (k_hsolve.hip): T from xx; inverse Cholesky factor W = L^-1; G = W^H W.
 void launch_t_from_xx(hipStream_t st, const double *xx, int64_t m_cols, int nb,

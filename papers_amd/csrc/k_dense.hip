@@ -160,4 +160,41 @@ void launch_hermitian_part(hipStream_t st, double *g_cplx, int nb) {
                      g_cplx, nb);
 }
 
+// ---- legacy un-inverted update (donaldson_iteration_quintic, quintic.rs:4140-4222) --------------
+// resid = sqrt(2 sum |t - g|^2) (the 2N x 2N block difference), then g = t. One workgroup, fixed
+// summation order.
+__global__ void __launch_bounds__(1024) k_legacy_replace(const double *__restrict__ t, int nb,
+                                                         double *__restrict__ g,
+                                                         double *__restrict__ resid) {
+  __shared__ double sh[1024];
+  const int64_t tot = (int64_t)nb * nb;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < tot; e += 1024) {
+    const double dr = t[2 * e] - g[2 * e], di = t[2 * e + 1] - g[2 * e + 1];
+    acc += 2.0 * (dr * dr + di * di);
+    g[2 * e] = t[2 * e];
+    g[2 * e + 1] = t[2 * e + 1];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *resid = sqrt(sh[0]);
+}
+
+void launch_legacy_replace(hipStream_t st, const double *t, int nb, double *g, double *resid) {
+  hipLaunchKernelGGL(k_legacy_replace, dim3(1), dim3(1024), 0, st, t, nb, g, resid);
+}
+
+__global__ void k_fill_f64(double *__restrict__ x, int64_t n, double v) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] = v;
+}
+
+void launch_fill_f64(hipStream_t st, double *x, int64_t n, double v) {
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, v);
+}
+
 }  // namespace cy3q

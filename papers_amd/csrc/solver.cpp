@@ -130,6 +130,7 @@ struct cy3q_solver {
   int64_t tp_mult = 1;         // T-pass point chunks per CU (a tuning knob)
   int64_t syrk_target = 1024;  // workgroups of the FP64 rank-k pass (a tuning knob)
   double *batch_res = nullptr;  // residuals of captured solves
+  double *ones = nullptr, *n_as_w = nullptr;  // unit weights and their sum (legacy iteration)
   int64_t batch_res_cap = 0;
   int *row_ex = nullptr;
   _Float16 *xd = nullptr;
@@ -645,6 +646,38 @@ int sigma_grad_impl(cy3q_solver *s, double *out6) {
 
 }  // namespace
 
+namespace {
+// donaldson_iteration_quintic(_weighted) (quintic.rs:4140-4331): h <- T(h) with no inversion.
+// The FP64 D pass and T pass give T = sum c_p conj(s_a) s_b with c_p = w_p / K_p over the valid
+// weights (weighted), or c_p = 1 / K_p over all points with sum 1 / n_points (unweighted); then
+// the trace normalisation, G = T, the block residual and W from the new G.
+int legacy_step(cy3q_solver *s, int weighted, double *residual) {
+  const int64_t n = s->n, m = s->m;
+  if (!weighted && !s->ones) {
+    ST_TRY(s->alloc(&s->ones, n));
+    ST_TRY(s->alloc(&s->n_as_w, 1));
+    launch_fill_f64(s->st, s->ones, n, 1.0);
+    launch_sum_f64(s->st, s->ones, n, s->n_as_w);
+  }
+  const double *wv = weighted ? s->w : s->ones;
+  ensure_bmat(s);
+  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
+  launch_point_coeffs_f64(s->st, wv, s->dpart, (int)(m / 32), n, s->coef);
+  launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
+  launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
+  launch_t_from_xx(s->st, s->xx, m, s->nb, weighted ? s->sum_w : s->n_as_w, 1.0, nullptr, 1,
+                   s->scale_buf, s->t, s->abuf, s->rbuf);
+  launch_legacy_replace(s->st, s->t, s->nb, s->g, s->resid);
+  HIP_TRY(hipGetLastError());
+  ST_TRY(w_from_g(s));
+  if (residual) {
+    HIP_TRY(hipMemcpyAsync(residual, s->resid, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+  }
+  return CY3Q_OK;
+}
+}  // namespace
+
 namespace cy3q {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 
@@ -876,6 +909,31 @@ int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps) {
   if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
   if (sweeps < 0) return fail(CY3Q_ERR_ARG, "refinement sweeps must be >= 0");
   s->refine = sweeps;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_donaldson_step_legacy(cy3q_solver *s, int weighted, double *residual) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  HIP_TRY(hipSetDevice(s->device));
+  return legacy_step(s, weighted, residual);
+}
+
+int cy3q_solver_donaldson_solve_legacy(cy3q_solver *s, int weighted, int64_t max_iter, double tol,
+                                       int64_t *iters, double *residuals) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (max_iter < 0) return fail(CY3Q_ERR_ARG, "max_iter must be >= 0");
+  HIP_TRY(hipSetDevice(s->device));
+  launch_identity_cplx(s->st, s->nb, s->g);  // init_h_block_identity
+  ST_TRY(w_from_g(s));
+  int64_t done = 0;
+  for (int64_t it = 0; it < max_iter; it++) {
+    double r = 0.0;
+    ST_TRY(legacy_step(s, weighted, &r));
+    if (residuals) residuals[it] = r;
+    done = it + 1;
+    if (r < tol) break;
+  }
+  if (iters) *iters = done;
   return CY3Q_OK;
 }
 

@@ -66,6 +66,8 @@ def _load():
     L.cy3q_solver_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(u32), C.POINTER(dbl)]
     L.cy3q_solver_set_precision.argtypes = [vp, ip]
     L.cy3q_solver_set_refine.argtypes = [vp, i64]
+    L.cy3q_solver_donaldson_step_legacy.argtypes = [vp, ip, C.POINTER(dbl)]
+    L.cy3q_solver_donaldson_solve_legacy.argtypes = [vp, ip, i64, dbl, C.POINTER(i64), vp]
     L.cy3q_solver_orthonormalise_fs_gram.argtypes = [vp]
     L.cy3q_solver_donaldson_step.argtypes = [vp, C.POINTER(dbl)]
     L.cy3q_solver_donaldson_solve.argtypes = [vp, i64, dbl, C.POINTER(i64), vp]
@@ -202,6 +204,21 @@ class QuinticSolver:
         _check(_load().cy3q_solver_donaldson_solve(self._h, max_iter, tol, C.byref(it), _ptr(res)))
         self.donaldson_residuals = list(res[: it.value])
         return it.value
+
+    def donaldson_iteration_legacy(self, weighted: bool = True) -> float:
+        """donaldson_iteration_quintic(_weighted) (quintic.rs:4140-4331) on this solver's h and
+        sections, in place: h <- T(h) without inversion. Returns the block residual."""
+        r = C.c_double()
+        _check(_load().cy3q_solver_donaldson_step_legacy(self._h, int(weighted), C.byref(r)))
+        return r.value
+
+    def donaldson_solve_legacy(self, max_iter: int, tol: float, weighted: bool = True) -> list:
+        """donaldson_solve_quintic(_weighted) (quintic.rs:4224-4357): from h = I; residuals."""
+        res = np.zeros(max(max_iter, 1))
+        it = C.c_int64()
+        _check(_load().cy3q_solver_donaldson_solve_legacy(self._h, int(weighted), max_iter, tol,
+                                                          C.byref(it), _ptr(res)))
+        return list(res[: it.value])
 
     def sigma(self) -> float:
         s = C.c_double()

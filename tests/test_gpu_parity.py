@@ -176,3 +176,16 @@ def test_run_twice_bit_identical(cy3q):
         out.append((s.h_block, s.sigma()))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_legacy_iteration_matches_oracle(cy3q, oracle, weighted):
+    # donaldson_solve_quintic(_weighted) (quintic.rs:4224-4357): h and the residual history.
+    P = oracle.Problem(3, 3000, 21)
+    s = cy3q.QuinticSolver.from_points(3, P.pts)
+    s.orthonormalise_basis_fs_gram()
+    res = s.donaldson_solve_legacy(6, 0.0, weighted=weighted)
+    h_ref, res_ref = oracle.legacy_solve(P.sv, 6, 0.0, P.w if weighted else None)
+    assert len(res) == 6
+    assert np.abs(np.array(res) - np.array(res_ref)).max() <= 1e-10 * max(res_ref)
+    assert np.linalg.norm(s.h_block - h_ref) / np.linalg.norm(h_ref) < 1e-11

@@ -287,3 +287,31 @@ def test_adam_first_step_closed_form(oracle):
     d = h2 - h
     d[0, 2] = d[2, 0] = d[1, 3] = d[3, 1] = 0.0
     assert np.max(np.abs(d)) == 0.0
+
+
+def test_legacy_weighted_step_is_the_uninverted_T(oracle):
+    # donaldson_iteration_quintic_weighted (quintic.rs:4256-4331) builds the same trace-normalised
+    # T as the fixed step (quintic.rs:1771-1812), without the inversion: two independent oracle
+    # code paths must agree.
+    import numpy as np
+    P = oracle.Problem(2, 1500, 9)
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((P.nb, P.nb)) + 1j * rng.standard_normal((P.nb, P.nb))
+    G = a @ a.conj().T / P.nb + np.eye(P.nb)
+    h = np.zeros((2 * P.nb, 2 * P.nb))
+    h[0::2, 0::2] = G.real
+    h[0::2, 1::2] = -G.imag
+    h[1::2, 0::2] = G.imag
+    h[1::2, 1::2] = G.real
+    hl = oracle.legacy_step(P.sv, h, P.w)
+    _, _, (tre, tim, _) = oracle.donaldson_step(P.sv, P.w, h, want_t=True)
+    assert np.abs(hl[0::2, 0::2] - tre).max() <= 1e-13 * np.abs(tre).max()
+    assert np.abs(hl[1::2, 0::2] - tim).max() <= 1e-13 * np.abs(tre).max()
+    # unweighted: the brute-force formula with uniform weights 1/n
+    hu = oracle.legacy_step(P.sv, h, None)
+    s = P.sv[:, 0::2] + 1j * P.sv[:, 1::2]
+    K = np.einsum("pa,ab,pb->p", s.conj(), G, s).real
+    T = np.einsum("p,pa,pb->ab", 1.0 / K, s.conj(), s) / P.n
+    T *= P.nb / np.trace(T).real
+    assert np.abs(hu[0::2, 0::2] - T.real).max() <= 1e-12 * np.abs(T).max()
+    assert np.abs(hu[1::2, 0::2] - T.imag).max() <= 1e-12 * np.abs(T).max()
