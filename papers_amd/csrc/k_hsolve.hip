@@ -14,8 +14,11 @@
 //                 sweep (one barrier per column): W_JJ = L_JJ^-1.
 //   k_elim_update (one workgroup per tile): L_IJ = A_IJ L_JJ^-H, then A_IK -= L_IJ L_KJ^H for
 //                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J, where R is the updated right
-//                 half and W_JK = L_JJ^-1 R_JK (K < J) is formed in the same kernel.
+//                 half and W_JK = L_JJ^-1 R_JK (K < J) is formed in the same kernel. The tile
+//                 products run on FP64 MFMA.
 //   k_elim_lastrow: the last block row of W.
+// The chain is latency-bound (one panel and one update per 32 columns), so every kernel fetches
+// all its tiles at once and keeps LDS traffic low.
 // A non-positive or non-finite pivot sets status = ST_ERR_NOT_PD. Later kernels then return
 // early, and the gram kernel installs the reference's identity fallback (G = W = I).
 // Complex matrices are row-major [n][n] of (re, im).
@@ -239,61 +242,108 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
   store_tile(w, n, j0, j0, e, tid, 256);
 }
 
-// 32 x 32 x 32 tile products with four outputs per thread accumulated together, so the LDS reads
-// of one step serve four independent FMA chains. out[q] for q = tid + 256u, u = 0..3.
-// prod_nn: C = A B over m <= mmax(i, k); prod_nc: C = A conj(B)^T.
-__device__ inline void tile_prod(const Cz (*a)[TB + 1], const Cz (*b)[TB + 1], int tid, int mode,
-                                 double acc[4][2]) {
-  // mode 0: sum_m a[i][m] b[m][k]          (A B)
-  // mode 1: sum_m a[i][m] conj(b[k][m])    (A B^H)
-  // mode 2: sum_{m<=i} a[i][m] b[m][k]     (lower A times B)
-  // mode 3: sum_{m<=k} a[i][m] conj(b[k][m])  (A times upper B^H, B lower)
-  int iu[4], ku[4];
+// ---------------------------------------------------------------------------------------------
+// 32 x 32 x 32 complex tile products on v_mfma_f64_16x16x4f64. Wave w computes the 16 x 16
+// quadrant (qr, qc) = (w / 2, w % 2) of C = op(A) op(B), op 0 = as stored, 1 = conjugate
+// transpose, as four real products per 4-deep k step (re += ar br - ai bi, im += ar bi + ai br).
+// MFMA f64 layout: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k; output element e of lane l
+// is (row 4e + l/16, col l%16) of the quadrant.
+// ---------------------------------------------------------------------------------------------
+__device__ inline void cgemm32(const Cz (*A)[TB + 1], int opa, const Cz (*B)[TB + 1], int opb,
+                               int qr, int qc, int lane, double4_t &cr, double4_t &ci) {
+  const int li = lane & 15, lk = lane >> 4;
+  const int ra = 16 * qr + li, cb = 16 * qc + li;
+#pragma unroll
+  for (int kk = 0; kk < TB / 4; kk++) {
+    const int kx = 4 * kk + lk;
+    Cz av = opa ? A[kx][ra] : A[ra][kx];
+    if (opa) av.i = -av.i;
+    Cz bv = opb ? B[cb][kx] : B[kx][cb];
+    if (opb) bv.i = -bv.i;
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(av.r, bv.r, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.i, bv.i, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.r, bv.i, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.i, bv.r, ci, 0, 0, 0);
+  }
+}
+
+// Register staging of a tile: 4 elements per thread (q = tid + 256 u), zero outside the matrix.
+struct TileRegs {
+  double2 v[4];
+};
+__device__ inline void tile_fetch(TileRegs &t, const double *__restrict__ src, int n, int r0,
+                                  int c0, int tid) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int q = tid + 256 * u, rr = q / TB, cc = q % TB;
+    t.v[u] = double2{0.0, 0.0};
+    if (r0 + rr < n && c0 + cc < n)
+      t.v[u] = *reinterpret_cast<const double2 *>(src + 2 * ((int64_t)(r0 + rr) * n + c0 + cc));
+  }
+}
+__device__ inline void tile_fetch_packed(TileRegs &t, const double *__restrict__ src, int tid) {
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    t.v[u] = *reinterpret_cast<const double2 *>(src + 2 * (tid + 256 * u));
+}
+__device__ inline void tile_put(Cz (*dst)[TB + 1], const TileRegs &t, int tid) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int q = tid + 256 * u;
-    iu[u] = q / TB;
-    ku[u] = q % TB;
-    acc[u][0] = acc[u][1] = 0.0;
+    dst[q / TB][q % TB] = Cz{t.v[u].x, t.v[u].y};
   }
-  for (int mm = 0; mm < TB; mm++) {
+}
+// Quadrant results back into an LDS tile (dst = acc) or into global memory (dst = base - acc).
+__device__ inline void quad_to_lds(Cz (*dst)[TB + 1], const double4_t &cr, const double4_t &ci,
+                                   int qr, int qc, int lane) {
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int i = iu[u], k = ku[u];
-      Cz p;
-      if (mode == 0 || mode == 2) {
-        if (mode == 2 && mm > i) continue;
-        p = cmul(a[i][mm], b[mm][k]);
-      } else {
-        if (mode == 3 && mm > k) continue;
-        p = cmulc(a[i][mm], b[k][mm]);
-      }
-      acc[u][0] += p.r;
-      acc[u][1] += p.i;
+  for (int e4 = 0; e4 < 4; e4++) dst[16 * qr + 4 * e4 + (lane >> 4)][16 * qc + (lane & 15)] =
+      Cz{cr[e4], ci[e4]};
+}
+__device__ inline void quad_sub_store(double *__restrict__ dst, int n, int r0, int c0,
+                                      const Cz (*base)[TB + 1], const double4_t &cr,
+                                      const double4_t &ci, int qr, int qc, int lane) {
+#pragma unroll
+  for (int e4 = 0; e4 < 4; e4++) {
+    const int rr = 16 * qr + 4 * e4 + (lane >> 4), cc = 16 * qc + (lane & 15);
+    if (r0 + rr < n && c0 + cc < n) {
+      double2 v;
+      v.x = base[rr][cc].r - cr[e4];
+      v.y = base[rr][cc].i - ci[e4];
+      *reinterpret_cast<double2 *>(dst + 2 * ((int64_t)(r0 + rr) * n + c0 + cc)) = v;
+    }
+  }
+}
+__device__ inline void quad_store(double *__restrict__ dst, int n, int r0, int c0,
+                                  const double4_t &cr, const double4_t &ci, int qr, int qc,
+                                  int lane) {
+#pragma unroll
+  for (int e4 = 0; e4 < 4; e4++) {
+    const int rr = 16 * qr + 4 * e4 + (lane >> 4), cc = 16 * qc + (lane & 15);
+    if (r0 + rr < n && c0 + cc < n) {
+      double2 v;
+      v.x = cr[e4];
+      v.y = ci[e4];
+      *reinterpret_cast<double2 *>(dst + 2 * ((int64_t)(r0 + rr) * n + c0 + cc)) = v;
     }
   }
 }
 
-// L_IJ = A_IJ L_JJ^-H into dst (x_ik = sum_{m <= k} a_im conj(e_km)).
-__device__ inline void panel_row(Cz (*dst)[TB + 1], const Cz (*aij)[TB + 1],
-                                 const Cz (*e)[TB + 1], int tid) {
-  double acc[4][2];
-  tile_prod(aij, e, tid, 3, acc);
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int q = tid + 256 * u;
-    dst[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
-  }
-}
-
+// One workgroup per tile of step J. E = L_JJ^-1 is lower triangular with zeros above the
+// diagonal, so the triangular products are plain tile products.
+//   type 1 (J < K <= I):  A_IK -= L_IJ L_KJ^H with L_XJ = A_XJ E^H;
+//   type 2 (K <= J < I):  R_IK -= L_IJ W_JK with W_JK = E R_JK (K < J; the workgroup with
+//                         I = J + 1 also stores it) or W_JJ = E.
+// All tiles a workgroup needs are fetched at once, so the kernel pays one memory latency.
 __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
                                                      double *__restrict__ r,
                                                      const double *__restrict__ einv,
                                                      double *__restrict__ w, int n, int jblk,
                                                      const int32_t *__restrict__ status) {
-  __shared__ Cz e[TB][TB + 1], x[TB][TB + 1], li[TB][TB + 1], y[TB][TB + 1];
+  __shared__ Cz e[TB][TB + 1], t1[TB][TB + 1], t2[TB][TB + 1], t3[TB][TB + 1];
   if (*status != ST_OK) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int qr = wave >> 1, qc = wave & 1;
   const int nblk = (n + TB - 1) / TB;
   const int mrem = nblk - jblk - 1;
   const int n_type1 = mrem * (mrem + 1) / 2;
@@ -316,67 +366,47 @@ __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
     type1 = false;
   }
   const int j0 = jblk * TB;
-  for (int q = tid; q < TB * TB; q += 256) {
-    const double2 v = *reinterpret_cast<const double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q));
-    e[q / TB][q % TB] = Cz{v.x, v.y};
+  TileRegs te, ta, tb, tc;
+  tile_fetch_packed(te, einv + 2 * (int64_t)jblk * TB * TB, tid);
+  tile_fetch(ta, a, n, I * TB, j0, tid);
+  const bool need_b = type1 ? (K != I) : (K < jblk);
+  if (need_b) {
+    if (type1)
+      tile_fetch(tb, a, n, K * TB, j0, tid);
+    else
+      tile_fetch(tb, r, n, j0, K * TB, tid);
   }
-  load_tile(x, a, n, I * TB, j0, false, tid, 256);
+  if (type1)
+    tile_fetch(tc, a, n, I * TB, K * TB, tid);
+  else
+    tile_fetch(tc, r, n, I * TB, K * TB, tid);
+  tile_put(e, te, tid);
+  tile_put(t1, ta, tid);
+  if (need_b) tile_put(t2, tb, tid);
+  tile_put(t3, tc, tid);
   __syncthreads();
-  panel_row(li, x, e, tid);  // L_IJ
+  double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  double4_t dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+  // L_IJ = A_IJ E^H (and L_KJ = A_KJ E^H, or W_JK = E R_JK).
+  cgemm32(t1, 0, e, 1, qr, qc, lane, cr, ci);
+  if (need_b) {
+    if (type1)
+      cgemm32(t2, 0, e, 1, qr, qc, lane, dr, di);
+    else
+      cgemm32(e, 0, t2, 0, qr, qc, lane, dr, di);
+  }
   __syncthreads();
+  quad_to_lds(t1, cr, ci, qr, qc, lane);
+  if (need_b) quad_to_lds(t2, dr, di, qr, qc, lane);
+  if (!type1 && need_b && I == jblk + 1) quad_store(w, n, j0, K * TB, dr, di, qr, qc, lane);
+  __syncthreads();
+  double4_t xr = {0.0, 0.0, 0.0, 0.0}, xi = {0.0, 0.0, 0.0, 0.0};
   if (type1) {
-    // L_KJ into x (I == K reuses L_IJ), then A_IK -= L_IJ L_KJ^H.
-    if (K != I) {
-      load_tile(y, a, n, K * TB, j0, false, tid, 256);
-      __syncthreads();
-      panel_row(x, y, e, tid);
-    } else {
-      for (int q = tid; q < TB * TB; q += 256) x[q / TB][q % TB] = li[q / TB][q % TB];
-    }
-    __syncthreads();
-    load_tile(y, a, n, I * TB, K * TB, false, tid, 256);
-    __syncthreads();
-    double acc[4][2];
-    tile_prod(li, x, tid, 1, acc);
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int q = tid + 256 * u;
-      y[q / TB][q % TB].r -= acc[u][0];
-      y[q / TB][q % TB].i -= acc[u][1];
-    }
-    __syncthreads();
-    store_tile(a, n, I * TB, K * TB, y, tid, 256);
+    cgemm32(t1, 0, need_b ? t2 : t1, 1, qr, qc, lane, xr, xi);  // L_IJ L_KJ^H
+    quad_sub_store(a, n, I * TB, K * TB, t3, xr, xi, qr, qc, lane);
   } else {
-    // R_IK -= L_IJ W_JK. For K < J, W_JK = L_JJ^-1 R_JK is formed here (R_JK is final after
-    // step J-1); the workgroup with I = J + 1 also stores it. W_JJ came from the panel.
-    if (K < jblk) {
-      load_tile(y, r, n, j0, K * TB, false, tid, 256);
-      __syncthreads();
-      double acc[4][2];
-      tile_prod(e, y, tid, 2, acc);
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int q = tid + 256 * u;
-        x[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
-      }
-      __syncthreads();
-      if (I == jblk + 1) store_tile(w, n, j0, K * TB, x, tid, 256);
-    } else {
-      load_tile(x, w, n, j0, K * TB, false, tid, 256);
-    }
-    load_tile(y, r, n, I * TB, K * TB, false, tid, 256);
-    __syncthreads();
-    double acc[4][2];
-    tile_prod(li, x, tid, 0, acc);
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int q = tid + 256 * u;
-      y[q / TB][q % TB].r -= acc[u][0];
-      y[q / TB][q % TB].i -= acc[u][1];
-    }
-    __syncthreads();
-    store_tile(r, n, I * TB, K * TB, y, tid, 256);
+    cgemm32(t1, 0, need_b ? t2 : e, 0, qr, qc, lane, xr, xi);  // L_IJ W_JK
+    quad_sub_store(r, n, I * TB, K * TB, t3, xr, xi, qr, qc, lane);
   }
 }
 
@@ -385,25 +415,20 @@ __global__ void __launch_bounds__(256) k_elim_lastrow(const double *__restrict__
                                                       const double *__restrict__ einv,
                                                       double *__restrict__ w, int n,
                                                       const int32_t *__restrict__ status) {
-  __shared__ Cz e[TB][TB + 1], y[TB][TB + 1], x[TB][TB + 1];
+  __shared__ Cz e[TB][TB + 1], y[TB][TB + 1];
   if (*status != ST_OK) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int qr = wave >> 1, qc = wave & 1;
   const int jblk = (n + TB - 1) / TB - 1, j0 = jblk * TB, K = blockIdx.x;
-  for (int q = tid; q < TB * TB; q += 256) {
-    const double2 v = *reinterpret_cast<const double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q));
-    e[q / TB][q % TB] = Cz{v.x, v.y};
-  }
-  load_tile(y, r, n, j0, K * TB, false, tid, 256);
+  TileRegs te, ty;
+  tile_fetch_packed(te, einv + 2 * (int64_t)jblk * TB * TB, tid);
+  tile_fetch(ty, r, n, j0, K * TB, tid);
+  tile_put(e, te, tid);
+  tile_put(y, ty, tid);
   __syncthreads();
-  double acc[4][2];
-  tile_prod(e, y, tid, 2, acc);
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int q = tid + 256 * u;
-    x[q / TB][q % TB] = Cz{acc[u][0], acc[u][1]};
-  }
-  __syncthreads();
-  store_tile(w, n, j0, K * TB, x, tid, 256);
+  double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  cgemm32(e, 0, y, 0, qr, qc, lane, cr, ci);
+  quad_store(w, n, j0, K * TB, cr, ci, qr, qc, lane);
 }
 
 // Runs the elimination on (a, r) prepared by launch_t_from_xx or launch_elim_init; W -> w.
@@ -423,47 +448,50 @@ void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv,
 }
 
 // ---------------------------------------------------------------------------------------------
-// G = W^H W per 32 x 32 tile: G_IK = sum_{J >= max(I, K)} W_JI^H W_JK. With residual != nullptr
-// the tile also yields 2 sum |G_new - G_old|^2 into partial[tile] and G_new replaces g in place;
-// otherwise G_new is written to g. If status shows a failed factorisation, G = W = I (the
-// reference's identity fallback, quintic.rs:1874-1883). Each tile reads and writes only its own
-// G entries, so the in-place update has no race.
+// G = W^H W per 32 x 32 tile: G_IK = sum_{J >= max(I, K)} W_JI^H W_JK on f64 MFMA, with the next
+// pair of W tiles fetched into registers while the current pair is multiplied. With
+// residual != nullptr the tile also yields 2 sum |G_new - G_old|^2 into partial[tile] and G_new
+// replaces g in place; otherwise G_new is written to g. If status shows a failed factorisation,
+// G = W = I (the reference's identity fallback, quintic.rs:1874-1883). Each tile reads and
+// writes only its own G entries, so the in-place update has no race. G_ab and G_ba come from
+// the same products in the same order, so G is exactly Hermitian.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_gram_tiles(double *__restrict__ w, int n,
                                                     double *__restrict__ g,
                                                     const int32_t *__restrict__ status,
                                                     double *__restrict__ partial) {
-  __shared__ Cz wi[TB][TB + 1], wk[TB][TB + 1];
+  __shared__ Cz wi[2][TB][TB + 1], wk[2][TB][TB + 1];
   __shared__ double red[256];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int qr = wave >> 1, qc = wave & 1;
   const int nblk = (n + TB - 1) / TB;
   const int I = blockIdx.x / nblk, K = blockIdx.x % nblk;
   const bool fail = status && *status != ST_OK;
-  double acc[4][2];
-  for (int u = 0; u < 4; u++) acc[u][0] = acc[u][1] = 0.0;
+  double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
   if (!fail) {
-    for (int J = max(I, K); J < nblk; J++) {
-      load_tile(wi, w, n, J * TB, I * TB, false, tid, 256);
-      load_tile(wk, w, n, J * TB, K * TB, false, tid, 256);
+    const int jb = max(I, K);
+    TileRegs ti, tk;
+    tile_fetch(ti, w, n, jb * TB, I * TB, tid);
+    tile_fetch(tk, w, n, jb * TB, K * TB, tid);
+    for (int J = jb; J < nblk; J++) {
+      const int buf = (J - jb) & 1;
+      tile_put(wi[buf], ti, tid);
+      tile_put(wk[buf], tk, tid);
       __syncthreads();
-      for (int mm = 0; mm < TB; mm++) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int q = tid + 256 * u, i = q / TB, k = q % TB;
-          const Cz p = cmulc(wk[mm][k], wi[mm][i]);  // conj(W_mi) W_mk
-          acc[u][0] += p.r;
-          acc[u][1] += p.i;
-        }
+      if (J + 1 < nblk) {
+        tile_fetch(ti, w, n, (J + 1) * TB, I * TB, tid);
+        tile_fetch(tk, w, n, (J + 1) * TB, K * TB, tid);
       }
-      __syncthreads();
+      cgemm32(wi[buf], 1, wk[buf], 0, qr, qc, lane, cr, ci);  // W_JI^H W_JK
     }
   }
   double res = 0.0;
-  for (int u = 0; u < 4; u++) {
-    const int q = tid + 256 * u, i = q / TB, k = q % TB;
-    const int gi = I * TB + i, gk = K * TB + k;
+#pragma unroll
+  for (int e4 = 0; e4 < 4; e4++) {
+    const int gi = I * TB + 16 * qr + 4 * e4 + (lane >> 4);
+    const int gk = K * TB + 16 * qc + (lane & 15);
     if (gi >= n || gk >= n) continue;
-    double nr = acc[u][0], ni = acc[u][1];
+    double nr = cr[e4], ni = ci[e4];
     if (fail) {
       nr = gi == gk ? 1.0 : 0.0;
       ni = 0.0;
