@@ -196,13 +196,15 @@ __device__ inline void y5_from_frame(const double fr[3][10], const double gi[3][
 }
 
 // The 61 per-point dot products of the contracted vectors y0 = W s and y_i = W ds/dz_i, summed
-// over the 64 lanes of a wave (every lane ends with the totals): [0] K, [1..10] dK_i =
-// conj(y0) y_i, [11..60] ddK_ij = conj(y_j) y_i.
-__device__ inline void point_dots(const double *__restrict__ r0, const double *__restrict__ rd,
-                                  int64_t m, int lane, double acc[61]) {
+// over the L lanes that share the point (sl = lane index within the group; every lane of the
+// group ends with the totals): [0] K, [1..10] dK_i = conj(y0) y_i, [11..60] ddK_ij =
+// conj(y_j) y_i. L is a power of two up to 64.
+template <int L>
+__device__ inline void point_dots_l(const double *__restrict__ r0, const double *__restrict__ rd,
+                                    int64_t m, int sl, double acc[61]) {
 #pragma unroll
   for (int q = 0; q < 61; q++) acc[q] = 0.0;
-  for (int64_t c = 2 * lane; c < m; c += 128) {
+  for (int64_t c = 2 * sl; c < m; c += 2 * L) {
     double sr = r0[c], si = r0[c + 1];
     double vr[5], vi[5];
 #pragma unroll
@@ -227,9 +229,16 @@ __device__ inline void point_dots(const double *__restrict__ r0, const double *_
 #pragma unroll
   for (int q = 0; q < 61; q++) {
     double v = acc[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
     acc[q] = v;
   }
+}
+
+// One point per 64-lane wave.
+__device__ inline void point_dots(const double *__restrict__ r0, const double *__restrict__ rd,
+                                  int64_t m, int lane, double acc[61]) {
+  point_dots_l<64>(r0, rd, m, lane, acc);
 }
 
 __device__ inline void unpack_dots(const double acc[61], double dk[5][2], double ddk[5][5][2]) {

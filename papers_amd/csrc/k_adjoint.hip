@@ -22,10 +22,13 @@
 
 namespace cy3q {
 
-// One wavefront per point. pts/log_om/w/sv/sd: the solver's point data from point p0; y0/yd: the
-// W-contracted rows of the same points; sig: the sigma reduction (kappa, sum w, sum of the
-// gamma numerators). Writes phi [n][5][m] and vs [n][m] (zero rows for points that do not
-// contribute).
+// GR_PPW points per wavefront: the GR_L = 64 / GR_PPW lanes of a point split its dot products
+// and its output rows, and the first lane of each group does the point's algebra. pts/log_om/
+// w/sv/sd: the solver's point data from point p0; y0/yd: the W-contracted rows of the same
+// points; sig: the sigma reduction (kappa, sum w, sum of the gamma numerators). Writes
+// phi [n][5][m] and vs [n][m] (zero rows for points that do not contribute).
+static constexpr int GR_PPW = 8, GR_L = 64 / GR_PPW;
+
 __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pts,
                                                    const double *__restrict__ y0,
                                                    const double *__restrict__ yd,
@@ -37,16 +40,17 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
                                                    const double *__restrict__ sig,
                                                    double *__restrict__ phi,
                                                    double *__restrict__ vs) {
-  // Per wave: [0] gamma/K, [1] gamma/K^2, [2] beta/2, [3..52] Y (Y[j][i] at 3 + 2 (5 j + i)),
+  // Per point: [0] gamma/K, [1] gamma/K^2, [2] beta/2, [3..52] Y (Y[j][i] at 3 + 2 (5 j + i)),
   // [53..62] v_i, [63..72] w_j.
-  __shared__ double cf[4][73];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t p = (int64_t)blockIdx.x * 4 + wave;
+  __shared__ double cf[4 * GR_PPW][73];
+  const int lane = threadIdx.x & 63, sl = lane % GR_L;
+  const int slot = (threadIdx.x >> 6) * GR_PPW + lane / GR_L;
+  const int64_t p = (int64_t)blockIdx.x * 4 * GR_PPW + slot;
   const bool in = p < n;
   double acc[61];
-  if (in) point_dots(y0 + p * m, yd + p * 5 * m, m, lane, acc);
-  if (lane == 0) {
-    double *c = cf[wave];
+  point_dots_l<GR_L>(y0 + (in ? p : 0) * m, yd + (in ? p : 0) * 5 * m, in ? m : 0, sl, acc);
+  if (sl == 0) {
+    double *c = cf[slot];
     for (int q = 0; q < 73; q++) c[q] = 0.0;
     const double kappa = sig[1], tw = sig[3], sa2 = sig[5];
     const double wp = in ? w[p] : 0.0, lo = in ? log_om[p] : NAN;
@@ -108,11 +112,11 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
   }
   __syncthreads();
   if (!in) return;
-  const double *c = cf[wave];
+  const double *c = cf[slot];
   const double gk = c[0], gk2 = c[1], hb = c[2];
   const double *sp = sv + p * m, *dp = sd + p * 5 * m;
   double *ph = phi + p * 5 * m, *vp = vs + p * m;
-  for (int64_t col = 2 * lane; col < m; col += 128) {
+  for (int64_t col = 2 * sl; col < m; col += 2 * GR_L) {
     double dr[5], di[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -152,8 +156,9 @@ void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const
                       int64_t n, int64_t m_cols, double psi, const double *sig, double *phi,
                       double *vs) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_grad_rows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, pts, y0, yd,
-                     log_om, w, sv, sd, n, m_cols, psi, sig, phi, vs);
+  const int64_t per_wg = 4 * GR_PPW;
+  hipLaunchKernelGGL(k_grad_rows, dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256), 0, st,
+                     pts, y0, yd, log_om, w, sv, sd, n, m_cols, psi, sig, phi, vs);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -25,8 +25,11 @@ namespace cy3q {
 // Per-point factor of the intrinsic form: pr[p][0] = sqrt(w |Omega|^2) (0 for a skipped point),
 // pr[p][1] = w (0 when skipped), pr[p][2..31] = R (3 x 5 complex); wv[p] = pr[p][1]. A point
 // is skipped when w is not finite and positive, log|Omega|^2 is not finite, g_tan is not
-// invertible, or g_tan^-1 fails the Cholesky test. One wave per point.
+// invertible, or g_tan^-1 fails the Cholesky test. LM_PPW points per wave (LM_L lanes each for
+// the dot products, one lane each for the algebra).
 // ---------------------------------------------------------------------------------------------
+static constexpr int LM_PPW = 8, LM_L = 64 / LM_PPW;
+
 __global__ void __launch_bounds__(256) k_lap_metric(const double *__restrict__ pts,
                                                     const double *__restrict__ y0,
                                                     const double *__restrict__ yd,
@@ -35,12 +38,12 @@ __global__ void __launch_bounds__(256) k_lap_metric(const double *__restrict__ p
                                                     int64_t m, double psi,
                                                     double *__restrict__ pr,
                                                     double *__restrict__ wv) {
-  const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= n) return;
+  const int lane = threadIdx.x & 63, sl = lane % LM_L;
+  const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * LM_PPW + lane / LM_L;
+  const bool in = p < n;
   double acc[61];
-  point_dots(y0 + p * m, yd + p * 5 * m, m, lane, acc);
-  if (lane != 0) return;
+  point_dots_l<LM_L>(y0 + (in ? p : 0) * m, yd + (in ? p : 0) * 5 * m, in ? m : 0, sl, acc);
+  if (sl != 0 || !in) return;
   double *out = pr + 32 * p;
   for (int q = 0; q < 32; q++) out[q] = 0.0;
   wv[p] = 0.0;
@@ -95,8 +98,9 @@ void launch_lap_metric(hipStream_t st, const double *pts, const double *y0, cons
                        const double *w, const double *log_om, int64_t n, int64_t m_cols,
                        double psi, double *pr, double *wv) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_lap_metric, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, pts, y0, yd, w,
-                     log_om, n, m_cols, psi, pr, wv);
+  const int64_t per_wg = 4 * LM_PPW;
+  hipLaunchKernelGGL(k_lap_metric, dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256), 0, st,
+                     pts, y0, yd, w, log_om, n, m_cols, psi, pr, wv);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -8,25 +8,30 @@
 // contracted vectors y0 = W s and y_i = W ds/dz_i. Those come from the triangular contraction,
 // so the per-point kernel does no N^2 work:
 //   K = s^H G s = |y0|^2,  dK_i = (G s)^H ds_i = y0^H y_i,  ddK_ij = ds_j^H G ds_i = y_j^H y_i.
-// One wavefront per point: the lanes split the N-length dot products, the results are reduced
-// with cross-lane shuffles, and lane 0 does the 5x5 -> 3x3 algebra in FP64 registers.
+// Several points per wavefront: a group of lanes splits each point's N-length dot products, the
+// results are reduced with cross-lane shuffles, and one lane per point does the 5x5 -> 3x3
+// algebra in FP64 registers.
 #include "cy3q_point.h"
 
 namespace cy3q {
 
-// y0: [n][m] contracted sections, yd: [n][5][m] contracted derivatives.
+// y0: [n][m] contracted sections, yd: [n][5][m] contracted derivatives. SE_PPW points per wave:
+// the SE_L = 64 / SE_PPW lanes of a point split its dot products, then the first lane of each
+// group does the 5x5 -> 3x3 algebra, so SE_PPW points' algebra runs side by side.
+static constexpr int SE_PPW = 8, SE_L = 64 / SE_PPW;
+
 __global__ void __launch_bounds__(256) k_sigma_eta(const double *__restrict__ pts,
                                                    const double *__restrict__ y0,
                                                    const double *__restrict__ yd,
                                                    const double *__restrict__ log_om, int64_t n,
                                                    int64_t m, double psi,
                                                    double *__restrict__ eta) {
-  const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= n) return;
+  const int lane = threadIdx.x & 63, sl = lane % SE_L;
+  const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * SE_PPW + lane / SE_L;
+  const bool in = p < n;
   double acc[61];
-  point_dots(y0 + p * m, yd + p * 5 * m, m, lane, acc);
-  if (lane != 0) return;
+  point_dots_l<SE_L>(y0 + (in ? p : 0) * m, yd + (in ? p : 0) * 5 * m, in ? m : 0, sl, acc);
+  if (sl != 0 || !in) return;
   const double lo = log_om[p];
   if (!isfinite(lo)) {
     eta[p] = NAN;
@@ -49,8 +54,9 @@ __global__ void __launch_bounds__(256) k_sigma_eta(const double *__restrict__ pt
 void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const double *yd,
                       const double *log_om, int64_t n, int64_t m_cols, double psi, double *eta) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_sigma_eta, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, pts, y0, yd,
-                     log_om, n, m_cols, psi, eta);
+  const int64_t per_wg = 4 * SE_PPW;
+  hipLaunchKernelGGL(k_sigma_eta, dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256), 0, st,
+                     pts, y0, yd, log_om, n, m_cols, psi, eta);
 }
 
 // Two deterministic single-workgroup passes (quintic.rs:1469-1498):

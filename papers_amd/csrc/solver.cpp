@@ -141,6 +141,12 @@ struct cy3q_solver {
 
   // sigma^2 gradient and Adam (allocated on first use)
   double *grad_dev = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  // sigma / gradient workspaces (allocated on first use and kept): the contracted rows of every
+  // point when they fit in kRowsResidentBytes (else per chunk), eta, and the gradient rows.
+  double *y0all = nullptr, *ydall = nullptr, *eta_buf = nullptr;
+  bool rows_resident = false;
+  double *g_phi = nullptr, *g_vs = nullptr, *g_part = nullptr, *g_cm = nullptr;
+  int64_t g_chunk = 0, g_groups = 0;
   double adam_beta1 = 0.9, adam_beta2 = 0.999, adam_eps = 1e-8;
 
   // profiling
@@ -550,34 +556,53 @@ int w_from_g(cy3q_solver *s) {
   return CY3Q_OK;
 }
 
-// Contract W with the section rows and the derivative rows in point chunks, then eta per
-// point and the two-pass reduction.
-// eta for every point (device, [n]) and the two-pass reduction into s->sig_out.
+// Contracted rows y0 = W s and y_i = W ds_i of every point stay resident when they fit in this
+// many bytes; otherwise they are recomputed per chunk of the sigma and gradient passes.
+constexpr double kRowsResidentBytes = 48e9;
+constexpr int64_t kRowChunk = 65536;
+
+int ensure_sigma_ws(cy3q_solver *s) {
+  if (!s->eta_buf) ST_TRY(s->alloc(&s->eta_buf, s->n));
+  if (!s->y0all) {
+    const double bytes = 8.0 * 6.0 * (double)s->n * (double)s->m;
+    s->rows_resident = bytes <= kRowsResidentBytes;
+    const int64_t rows = s->rows_resident ? s->n : std::min<int64_t>(s->n, kRowChunk);
+    ST_TRY(s->alloc(&s->y0all, rows * s->m));
+    ST_TRY(s->alloc(&s->ydall, rows * 5 * s->m));
+  }
+  return CY3Q_OK;
+}
+
+// eta for every point (device, [n]) and the two-pass reduction into s->sig_out. With resident
+// rows, y0 and yd hold every point's contracted rows afterwards.
 int sigma_device(cy3q_solver *s, double *eta) {
   const int64_t n = s->n, m = s->m;
+  if (!eta) return fail(CY3Q_ERR_STATE, "sigma: no eta buffer");
   ensure_bmat(s);
-  const int64_t chunk = std::min<int64_t>(n, 65536);
-  double *y0 = nullptr, *yd = nullptr;
-  ST_TRY(s->alloc(&y0, chunk * m));
-  ST_TRY(s->alloc(&yd, chunk * 5 * m));
-  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
-    int64_t cn = std::min(chunk, n - p0);
-    launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, y0, 0, 0);
-    launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, yd, 0, 0);
-    launch_sigma_eta(s->st, s->pts + 10 * p0, y0, yd, s->log_om + p0, cn, m, s->psi, eta + p0);
+  ST_TRY(ensure_sigma_ws(s));
+  if (s->rows_resident) {
+    launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->y0all, 0, 0);
+    launch_trigemm_f64(s->st, s->sd, n * 5, m, s->bmat, s->ydall, 0, 0);
+    launch_sigma_eta(s->st, s->pts, s->y0all, s->ydall, s->log_om, n, m, s->psi, eta);
+  } else {
+    for (int64_t p0 = 0; p0 < n; p0 += kRowChunk) {
+      const int64_t cn = std::min(kRowChunk, n - p0);
+      launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, s->y0all, 0, 0);
+      launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, s->ydall, 0, 0);
+      launch_sigma_eta(s->st, s->pts + 10 * p0, s->y0all, s->ydall, s->log_om + p0, cn, m, s->psi,
+                       eta + p0);
+    }
   }
   launch_sigma_reduce(s->st, eta, s->w, n, s->sig_out);
-  HIP_TRY(hipStreamSynchronize(s->st));
-  s->release(y0);
-  s->release(yd);
+  HIP_TRY(hipGetLastError());
   return CY3Q_OK;
 }
 
 int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
   const int64_t n = s->n;
   s->stage_begin(STG_SIGMA);
-  double *eta = nullptr;
-  ST_TRY(s->alloc(&eta, n));
+  ST_TRY(ensure_sigma_ws(s));
+  double *eta = s->eta_buf;
   ST_TRY(sigma_device(s, eta));
   s->stage_end(STG_SIGMA);
   HIP_TRY(hipGetLastError());
@@ -585,7 +610,6 @@ int sigma_impl(cy3q_solver *s, double *out5, double *eta_host) {
   if (eta_host)
     HIP_TRY(hipMemcpyAsync(eta_host, eta, n * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
-  s->release(eta);
   s->collect();
   return CY3Q_OK;
 }
@@ -603,25 +627,30 @@ int sigma_grad_impl(cy3q_solver *s, double *out6) {
     ST_TRY(s->alloc(&s->adam_v, 2 * (int64_t)nb * nb));
   }
   s->stage_begin(STG_SIGMA);
-  double *eta = nullptr;
-  ST_TRY(s->alloc(&eta, n));
-  ST_TRY(sigma_device(s, eta));
-  s->release(eta);
-  const int64_t chunk = std::min<int64_t>(n, 65536);
-  double *y0 = nullptr, *yd = nullptr, *phi = nullptr, *vs = nullptr, *part = nullptr,
-         *cm = nullptr;
-  const int64_t groups = gemm_tn_groups(5 * chunk, m);
-  ST_TRY(s->alloc(&y0, chunk * m));
-  ST_TRY(s->alloc(&yd, chunk * 5 * m));
-  ST_TRY(s->alloc(&phi, chunk * 5 * m));
-  ST_TRY(s->alloc(&vs, chunk * m));
-  ST_TRY(s->alloc(&part, groups * m * m));
-  ST_TRY(s->alloc(&cm, m * m));
+  ST_TRY(ensure_sigma_ws(s));  // allocates eta_buf before it is passed on
+  ST_TRY(sigma_device(s, s->eta_buf));
+  const int64_t chunk = std::min<int64_t>(n, kRowChunk);
+  if (!s->g_phi) {
+    s->g_chunk = chunk;
+    s->g_groups = gemm_tn_groups(5 * chunk, m);
+    ST_TRY(s->alloc(&s->g_phi, chunk * 5 * m));
+    ST_TRY(s->alloc(&s->g_vs, chunk * m));
+    ST_TRY(s->alloc(&s->g_part, s->g_groups * m * m));
+    ST_TRY(s->alloc(&s->g_cm, m * m));
+  }
+  double *phi = s->g_phi, *vs = s->g_vs, *part = s->g_part, *cm = s->g_cm;
+  const int64_t groups = s->g_groups;
   HIP_TRY(hipMemsetAsync(cm, 0, sizeof(double) * m * m, s->st));
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
-    launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, y0, 0, 0);
-    launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, yd, 0, 0);
+    const double *y0 = s->y0all, *yd = s->ydall;
+    if (s->rows_resident) {
+      y0 += p0 * m;
+      yd += p0 * 5 * m;
+    } else {
+      launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, s->y0all, 0, 0);
+      launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, s->ydall, 0, 0);
+    }
     launch_grad_rows(s->st, s->pts + 10 * p0, y0, yd, s->log_om + p0, s->w + p0, s->sv + p0 * m,
                      s->sd + p0 * 5 * m, cn, m, s->psi, s->sig_out, phi, vs);
     launch_gemm_tn_acc(s->st, s->sd + p0 * 5 * m, phi, cn * 5, m,
@@ -634,12 +663,6 @@ int sigma_grad_impl(cy3q_solver *s, double *out6) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out6, s->sig_out, 6 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
-  s->release(y0);
-  s->release(yd);
-  s->release(phi);
-  s->release(vs);
-  s->release(part);
-  s->release(cm);
   s->collect();
   return CY3Q_OK;
 }
