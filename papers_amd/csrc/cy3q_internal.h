@@ -160,6 +160,7 @@ void launch_diag_re(hipStream_t st, const double2 *A, int np, double *d);
 void launch_gather_cols(hipStream_t st, const double2 *V, int np, int n, const int *col,
                         const double *scale, int r, double2 *out);
 void launch_herm_project(hipStream_t st, double2 *A, int n);
+void launch_add_diag(hipStream_t st, double2 *a, int n, double shift);
 void launch_zgemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C, int M, int N,
                   int K, int lda, int ldb, int ldc, int opa, int opb);
 

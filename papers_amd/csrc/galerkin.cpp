@@ -96,14 +96,14 @@ enum GStage {
   GS_ROWS = 0,   // per-point metric factor and row generation
   GS_HERK,       // FP64 MFMA Hermitian rank-k accumulation
   GS_FINALIZE,   // complex M, K from the real accumulators
-  GS_EIGEN_M,    // eigensolve of M (with vectors)
+  GS_EIGEN_M,    // factorisation of M: inverse Cholesky, or eigenvectors when M is singular
   GS_REDUCE,     // M^-1/2 reduction GEMMs (or M^-1 K)
   GS_EIGEN_C,    // eigensolve of the reduced pencil
   GS_YUK_MODES,  // mode values psi = S C
   GS_YUK_TRIPLES,
   GS_COUNT
 };
-const char *kGStageNames[GS_COUNT] = {"rows", "herk", "finalize", "eigen_mass",
+const char *kGStageNames[GS_COUNT] = {"rows", "herk", "finalize", "factor_mass",
                                       "reduce", "eigen_reduced", "yukawa_modes",
                                       "yukawa_triples"};
 thread_local double g_stage_ms[GS_COUNT];
@@ -224,6 +224,28 @@ int frob(Ctx &c, const double2 *a, int n, double *out) {
   return CY3Q_OK;
 }
 
+// W = L^-1 with M + shift I = L L^H (FP64 tiled elimination of k_hsolve.hip). *ok = false when
+// the matrix is not positive definite (then W is not usable).
+int inverse_cholesky(Ctx &c, const double2 *M, int n, double shift, double2 *W, bool *ok) {
+  const int nblk = (n + 31) / 32;
+  double *a, *r, *einv;
+  int32_t *status;
+  G_TRY(c.alloc(&a, (size_t)2 * n * n));
+  G_TRY(c.alloc(&r, (size_t)2 * n * n));
+  G_TRY(c.alloc(&einv, (size_t)nblk * 32 * 32 * 2));
+  G_TRY(c.alloc(&status, 4));
+  G_HIP(hipMemsetAsync(W, 0, sizeof(double2) * n * n, c.st));  // the upper triangle stays zero
+  launch_elim_init(c.st, reinterpret_cast<const double *>(M), n, a, r);
+  if (shift != 0.0) launch_add_diag(c.st, reinterpret_cast<double2 *>(a), n, shift);
+  launch_inverse_cholesky(c.st, a, r, einv, reinterpret_cast<double *>(W), n, status);
+  int32_t h = -1;
+  G_HIP(hipMemcpyAsync(&h, status, sizeof(h), hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  *ok = h == 0;
+  for (void *p : {(void *)a, (void *)r, (void *)einv, (void *)status}) c.release(p);
+  return CY3Q_OK;
+}
+
 // Spectrum step of compute_metric_laplacian_spectrum (metric_laplacian.rs:342-389):
 // (M + lambda I)^-1 K with lambda = max(reg ||M||_F, 1e-30), through the eigenvectors of M, then
 // the Hermitian part, then its eigenvalues as |lambda| ascending.
@@ -237,6 +259,27 @@ int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64
   G_TRY(c.alloc(&Y, (size_t)nb * nb));
   G_TRY(c.alloc(&Minv, (size_t)nb * nb));
   G_TRY(c.alloc(&L, (size_t)nb * nb));
+  // (M + lambda I)^-1 = W^H W from the inverse Cholesky factor; the eigenvectors of M are the
+  // fallback when the factorisation meets a non-positive pivot.
+  bool chol_ok = false;
+  tm.begin();
+  G_TRY(inverse_cholesky(c, M, nb, lambda, U, &chol_ok));
+  tm.end(GS_EIGEN_M);
+  if (chol_ok) {
+    tm.begin();
+    launch_zgemm(c.st, U, U, Minv, nb, nb, nb, nb, nb, nb, 1, 0);  // W^H W
+    launch_zgemm(c.st, Minv, K, L, nb, nb, nb, nb, nb, nb, 0, 0);  // M^-1 K
+    launch_herm_project(c.st, L, nb);
+    tm.end(GS_REDUCE);
+    std::vector<double> ev;
+    tm.begin();
+    G_TRY(heev(c, L, nb, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+    tm.end(GS_EIGEN_C);
+    full.resize(nb);
+    for (int j = 0; j < nb; j++) full[j] = std::fabs(ev[j]);
+    std::stable_sort(full.begin(), full.end());
+    return CY3Q_OK;
+  }
   std::vector<double> mu;
   tm.begin();
   G_TRY(heev(c, M, nb, max_sweeps, tol, lambda, mu, U, nullptr));
@@ -272,6 +315,29 @@ int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int
                          StageTimer &tm) {
   double2 *U;
   G_TRY(c.alloc(&U, (size_t)nb * nb));
+  // Positive definite M: C = W K W^H with W = chol(M)^-1, whose eigenvalues are those of the
+  // pencil. Otherwise the canonical reduction below deflates M's null space.
+  {
+    bool chol_ok = false;
+    tm.begin();
+    G_TRY(inverse_cholesky(c, M, nb, 0.0, U, &chol_ok));
+    tm.end(GS_EIGEN_M);
+    if (chol_ok) {
+      double2 *T, *C;
+      G_TRY(c.alloc(&T, (size_t)nb * nb));
+      G_TRY(c.alloc(&C, (size_t)nb * nb));
+      tm.begin();
+      launch_zgemm(c.st, U, K, T, nb, nb, nb, nb, nb, nb, 0, 0);  // W K
+      launch_zgemm(c.st, T, U, C, nb, nb, nb, nb, nb, nb, 0, 1);  // W K W^H
+      launch_herm_project(c.st, C, nb);
+      tm.end(GS_REDUCE);
+      if (rank) *rank = nb;
+      tm.begin();
+      G_TRY(heev(c, C, nb, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+      tm.end(GS_EIGEN_C);
+      return CY3Q_OK;
+    }
+  }
   std::vector<double> mu;
   tm.begin();
   G_TRY(heev(c, M, nb, max_sweeps, tol, 0.0, mu, U, nullptr));

@@ -326,4 +326,14 @@ void launch_zgemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C
                      dim3(256), 0, st, A, B, C, M, N, K, lda, ldb, ldc, opa, opb);
 }
 
+// a[i][i] += shift (real), i < n.
+__global__ void k_add_diag(double2 *__restrict__ a, int n, double shift) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[(int64_t)i * n + i].x += shift;
+}
+
+void launch_add_diag(hipStream_t st, double2 *a, int n, double shift) {
+  hipLaunchKernelGGL(k_add_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, shift);
+}
+
 }  // namespace cy3q
