@@ -158,13 +158,17 @@ void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_
 }
 
 // ---------------------------------------------------------------------------------------------
-// D pass. Workgroup: 128 points (8 point tiles), 8 waves. Waves 0..6 compute: wave w owns the
-// column groups cglist[w][..] (round-robin, so the groups still active at each k-step are spread
-// evenly) and keeps their accumulators for all 8 tiles. Wave 7 only moves data: it streams the
-// A fragments (16 KB per k-step) straight from HBM into a DP_RING-slot LDS ring with
-// global_load_lds_dwordx4, DP_RING - 1 k-steps ahead. Its vector-memory counter tracks these
-// copies alone, so it can wait for exactly the slot needed next, while the compute waves' counters
-// track only their B-fragment loads (register prefetch one k-step ahead). One barrier per k-step.
+// D pass. A block is 128 points (8 point tiles). The grid is persistent: one workgroup per CU
+// (the LDS ring fills most of a CU's 160 KB), each looping over blocks blockIdx.x,
+// blockIdx.x + gridDim.x, ... Waves 0..6 compute: wave w owns the column groups cglist[w][..]
+// (round-robin, so the groups still active at each k-step are spread evenly) and keeps their
+// accumulators for all 8 tiles of the current block. Wave 7 only moves data: it streams the A
+// fragments (16 KB per k-step) straight from HBM into a DP_RING-slot LDS ring with
+// global_load_lds_dwordx4, DP_RING - 1 k-steps ahead and across block boundaries, so the next
+// block's first k-steps are already resident when the current block's reduction runs and no
+// block waits for a cold HBM fetch. Its vector-memory counter tracks these copies alone, while
+// the compute waves' counters track only their B-fragment loads (register prefetch one k-step
+// ahead, also across blocks). One barrier per k-step and one per block (the row reduction).
 // The three split products of one accumulator are spread out so consecutive MFMAs write
 // different accumulators.
 // Output: dsum[p] = sum_j Y_pj^2 (FP64), for the row-scaled x and the 2^-f scaled W.
@@ -175,8 +179,10 @@ static constexpr int DP_PTS = 16 * DP_TILES;
 static constexpr int DP_WAVES = 8;
 static constexpr int DP_CW = DP_WAVES - 1;  // compute waves
 static constexpr int DP_MAXCG = 4;
-static constexpr int DP_RING = 4;
+static constexpr int DP_RING = 8;  // a power of two
 static constexpr int DP_SLOT = DP_TILES * 2 * 64;  // half8 per ring slot (16 KB)
+static constexpr int DP_LOADS = DP_TILES * 2;      // global_load_lds per k-step and lane
+static_assert((DP_RING & (DP_RING - 1)) == 0, "ring index uses a mask");
 
 int dpass_split16_waves() { return DP_CW; }
 int dpass_split16_max_cg() { return DP_MAXCG; }
@@ -186,6 +192,24 @@ int dpass_split16_max_cg() { return DP_MAXCG; }
 #define DP_WAIT_VM(v) \
   __builtin_amdgcn_s_waitcnt(((v)&15) | (((v) >> 4) << 14) | (7 << 4) | (15 << 8))
 
+// The data mover's wait before the barrier of k-step g: the copies of k-step g must have
+// landed, while those of the `ahead` later k-steps already issued may stay in flight. Copies
+// complete in issue order for the counter, so allowing ahead * DP_LOADS outstanding is exact;
+// the counter field holds at most 63, and beyond that the wait is merely earlier than needed.
+__device__ __forceinline__ void dp_mover_wait(int64_t ahead) {
+  static_assert(DP_LOADS == 16, "wait immediates assume 16 copies per k-step");
+  if (ahead >= 4) {
+    DP_WAIT_VM(63);
+  } else if (ahead == 3) {
+    DP_WAIT_VM(48);
+  } else if (ahead == 2) {
+    DP_WAIT_VM(32);
+  } else if (ahead == 1) {
+    DP_WAIT_VM(16);
+  } else {
+    DP_WAIT_VM(0);
+  }
+}
 
 // Largest number of active column groups (cg >= 2 ks) held by any compute wave at k-step ks,
 // for the round-robin assignment cg -> wave cg % DP_CW.
@@ -203,7 +227,7 @@ __host__ __device__ constexpr int dp_slots(int n_cg, int ks) {
 // compiler's memory-counter waits are exact (no loop-carried merging of pending loads).
 template <int NKS>
 __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *__restrict__ xd,
-                                                                 int64_t n,
+                                                                 int64_t n, int64_t nblk,
                                                                  const half8_t *__restrict__ bfrag,
                                                                  const int16_t *__restrict__ cglist,
                                                                  double *__restrict__ dsum) {
@@ -211,40 +235,36 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
   __shared__ double red[DP_CW][DP_PTS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t tile0 = (int64_t)blockIdx.x * DP_TILES;
   constexpr int n_ks = NKS, n_cg = 2 * NKS;
+  const int64_t blk0 = blockIdx.x, bstride = gridDim.x;
+  const int64_t nmine = blk0 < nblk ? (nblk - 1 - blk0) / bstride + 1 : 0;
 
   if (wave == DP_CW) {
-    // Data mover. Slot (ks + DP_RING - 1) % DP_RING was last read in k-step ks - 1, which every
-    // compute wave finished before the barrier of k-step ks.
-    const half8_t *src = xd + (tile0 * n_ks * 2) * 64 + lane;
-    auto issue = [&](int ks) {
-      half8_t *slot = sa[ks % DP_RING];
+    // Data mover. Global k-step g is k-step g % n_ks of this workgroup's block g / n_ks. Slot
+    // (g + DP_RING - 1) % DP_RING was last read in k-step g - 1, which every compute wave
+    // finished before the barrier of k-step g.
+    const int64_t total = nmine * n_ks;
+    auto issue = [&](int64_t g) {
+      const int64_t blk = blk0 + (g / n_ks) * bstride;
+      const int ks = (int)(g % n_ks);
+      half8_t *slot = sa[(int)(g & (DP_RING - 1))];
+      const half8_t *src = xd + (blk * DP_TILES * n_ks * 2) * 64 + lane;
 #pragma unroll
-      for (int b = 0; b < DP_TILES * 2; b++) {
+      for (int b = 0; b < DP_LOADS; b++) {
         const int rg = b >> 1, pl = b & 1;
         __builtin_amdgcn_global_load_lds(
             (const void *)(src + (((int64_t)rg * n_ks + ks) * 2 + pl) * 64),
             (__attribute__((address_space(3))) void *)(slot + b * 64), 16, 0, 0);
       }
     };
-#pragma unroll
-    for (int u = 0; u < DP_RING - 1 && u < n_ks; u++) issue(u);
-#pragma unroll
-    for (int ks = 0; ks < n_ks; ks++) {
-      // k-steps already issued after ks: min(ks + DP_RING - 2, n_ks - 1) - ks.
-      const int ahead = min(DP_RING - 2, n_ks - 1 - ks);
-      if (ahead >= 2)
-        DP_WAIT_VM(2 * DP_TILES * 2);
-      else if (ahead == 1)
-        DP_WAIT_VM(DP_TILES * 2);
-      else
-        DP_WAIT_VM(0);
+    for (int64_t g = 0; g < DP_RING - 1 && g < total; g++) issue(g);
+    for (int64_t g = 0; g < total; g++) {
+      dp_mover_wait(min((int64_t)(DP_RING - 2), total - 1 - g));
       // A bare barrier: a workgroup fence here would wait for every copy in flight.
       __builtin_amdgcn_s_barrier();
-      if (ks + DP_RING - 1 < n_ks) issue(ks + DP_RING - 1);
+      if (g + DP_RING - 1 < total) issue(g + DP_RING - 1);
+      if (g % n_ks == n_ks - 1) __builtin_amdgcn_s_barrier();  // the block's row reduction
     }
-    __syncthreads();
     return;
   }
 
@@ -258,37 +278,37 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
 #pragma unroll
   for (int c = 0; c < DP_MAXCG; c++) cgs[c] = cglist[wave * DP_MAXCG + c];
   float4_t acc[DP_TILES][DP_MAXCG];
-#pragma unroll
-  for (int r = 0; r < DP_TILES; r++)
-#pragma unroll
-    for (int c = 0; c < DP_MAXCG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
 
   // Two B register sets, alternating by k-step parity (no register copies, so the compiler's
-  // wait for the set in use leaves the other set's loads in flight).
+  // wait for the set in use leaves the other set's loads in flight). The last k-step of a block
+  // prefetches k-step 0 of the next one into the free set.
   half8_t b0h[DP_MAXCG], b0l[DP_MAXCG], b1h[DP_MAXCG], b1l[DP_MAXCG];
   constexpr int64_t zero_frag = (int64_t)n_ks * n_cg;
+  // The B fragments are the same for every block. An opaque copy of their base, renewed per
+  // block, keeps the compiler from hoisting all n_ks * DP_MAXCG fragment addresses out of the
+  // block loop (they would not fit in registers alongside the accumulators).
+  const half8_t *bfp = bfrag;
   auto load_b = [&](int ks, half8_t *hh, half8_t *ll) {
 #pragma unroll
     for (int c = 0; c < DP_MAXCG; c++) {
       if (c < DP_MAXCG - dp_slots(n_cg, ks)) continue;
       const int64_t frag = cgs[c] >= 2 * ks ? (int64_t)ks * n_cg + cgs[c] : zero_frag;
-      hh[c] = bfrag[(frag * 2 + 0) * 64 + lane];
-      ll[c] = bfrag[(frag * 2 + 1) * 64 + lane];
+      hh[c] = bfp[(frag * 2 + 0) * 64 + lane];
+      ll[c] = bfp[(frag * 2 + 1) * 64 + lane];
     }
   };
   // MFMAs with swapped operands: B as the row operand and X as the column operand give the
   // transpose Y^T, so that element e of lane l is column 16 cg + 4 (l / 16) + e of point
   // 16 r + l % 16 (the fragment layouts of both operands serve either role).
-  auto step = [&](int ks, const half8_t *bh, const half8_t *bl) {
-    const half8_t *sl = sa[ks % DP_RING];
+  auto step = [&](int slot, int ks, const half8_t *bh, const half8_t *bl) {
     const int c0 = DP_MAXCG - dp_slots(n_cg, ks);
 #pragma unroll
     for (int r = 0; r < DP_TILES; r += 2) {
       if (c0 >= DP_MAXCG) break;
-      const half8_t ah0 = sl[(r * 2 + 0) * 64 + lane];
-      const half8_t al0 = sl[(r * 2 + 1) * 64 + lane];
-      const half8_t ah1 = sl[(r * 2 + 2) * 64 + lane];
-      const half8_t al1 = sl[(r * 2 + 3) * 64 + lane];
+      const half8_t ah0 = sa[slot][(r * 2 + 0) * 64 + lane];
+      const half8_t al0 = sa[slot][(r * 2 + 1) * 64 + lane];
+      const half8_t ah1 = sa[slot][(r * 2 + 2) * 64 + lane];
+      const half8_t al1 = sa[slot][(r * 2 + 3) * 64 + lane];
 #pragma unroll
       for (int pr = 0; pr < 3; pr++) {
 #pragma unroll
@@ -302,69 +322,98 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
       }
     }
   };
+  // One block: the k-steps (even ones on set x, odd ones on set y), then the row sums.
+  auto block = [&](int64_t i, half8_t *xh, half8_t *xl, half8_t *yh, half8_t *yl) {
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r++)
+#pragma unroll
+      for (int c = 0; c < DP_MAXCG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    const int slot0 = (int)((i * n_ks) & (DP_RING - 1));
+    asm volatile("" : "+s"(bfp));
+    const bool more = i + 1 < nmine;
+#pragma unroll
+    for (int ks = 0; ks < n_ks; ks += 2) {
+      __syncthreads();
+      if (ks + 1 < n_ks)
+        load_b(ks + 1, yh, yl);
+      else if (more)
+        load_b(0, yh, yl);
+      step((slot0 + ks) & (DP_RING - 1), ks, xh, xl);
+      if (ks + 1 >= n_ks) break;
+      __syncthreads();
+      if (ks + 2 < n_ks)
+        load_b(ks + 2, xh, xl);
+      else if (more)
+        load_b(0, xh, xl);
+      step((slot0 + ks + 1) & (DP_RING - 1), ks + 1, yh, yl);
+    }
+    // Row sums of squares (FP64). Lane l holds, for tile r, point 16 r + l % 16 and columns
+    // 4 (l / 16) + e of every slot (inactive slots are still zero): sum them in the lane, then
+    // over the four 16-lane rows in two halving exchanges (lanes l ^ 32, then l ^ 16), after
+    // which lane l owns the complete sums of tiles 4 (l / 32) + 2 ((l / 16) % 2) + {0, 1}.
+    double part[DP_TILES];
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r++) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int c = 0; c < DP_MAXCG; c++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const double y = (double)acc[r][c][e];
+          sacc += y * y;
+        }
+      part[r] = sacc;
+    }
+    const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
+    double h4[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double send = up32 ? part[q] : part[4 + q];
+      const double keep = up32 ? part[4 + q] : part[q];
+      h4[q] = keep + __shfl_xor(send, 32);
+    }
+    double h2[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const double send = up16 ? h4[q] : h4[2 + q];
+      const double keep = up16 ? h4[2 + q] : h4[q];
+      h2[q] = keep + __shfl_xor(send, 16);
+    }
+    // red is rewritten only after the next block's first barrier, which the reading waves
+    // reach after their reads.
+    const int rbase = 4 * (lane >> 5) + 2 * ((lane >> 4) & 1);
+    red[wave][(rbase + 0) * 16 + (lane & 15)] = h2[0];
+    red[wave][(rbase + 1) * 16 + (lane & 15)] = h2[1];
+    __syncthreads();
+    if (tid < DP_PTS) {
+      double sacc = 0.0;
+      for (int w = 0; w < DP_CW; w++) sacc += red[w][tid];
+      const int64_t p = (blk0 + i * bstride) * DP_PTS + tid;
+      if (p < n) dsum[p] = sacc;
+    }
+  };
   load_b(0, b0h, b0l);
-#pragma unroll
-  for (int ks = 0; ks < n_ks; ks += 2) {
-    __syncthreads();
-    if (ks + 1 < n_ks) load_b(ks + 1, b1h, b1l);
-    step(ks, b0h, b0l);
-    if (ks + 1 >= n_ks) break;
-    __syncthreads();
-    if (ks + 2 < n_ks) load_b(ks + 2, b0h, b0l);
-    step(ks + 1, b1h, b1l);
-  }
-  // Row sums of squares (FP64). Lane l holds, for tile r, point 16 r + l % 16 and columns
-  // 4 (l / 16) + e of every slot (inactive slots are still zero): sum them in the lane, then over
-  // the four 16-lane rows in two halving exchanges (lanes l ^ 32, then l ^ 16), after which lane
-  // l owns the complete sums of tiles 4 (l / 32) + 2 ((l / 16) % 2) + {0, 1}.
-  double part[DP_TILES];
-#pragma unroll
-  for (int r = 0; r < DP_TILES; r++) {
-    double sacc = 0.0;
-#pragma unroll
-    for (int c = 0; c < DP_MAXCG; c++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const double y = (double)acc[r][c][e];
-        sacc += y * y;
-      }
-    part[r] = sacc;
-  }
-  const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
-  double h4[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const double send = up32 ? part[i] : part[4 + i];
-    const double keep = up32 ? part[4 + i] : part[i];
-    h4[i] = keep + __shfl_xor(send, 32);
-  }
-  double h2[2];
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    const double send = up16 ? h4[i] : h4[2 + i];
-    const double keep = up16 ? h4[2 + i] : h4[i];
-    h2[i] = keep + __shfl_xor(send, 16);
-  }
-  const int rbase = 4 * (lane >> 5) + 2 * ((lane >> 4) & 1);
-  red[wave][(rbase + 0) * 16 + (lane & 15)] = h2[0];
-  red[wave][(rbase + 1) * 16 + (lane & 15)] = h2[1];
-  __syncthreads();
-  if (tid < DP_PTS) {
-    double sacc = 0.0;
-    for (int w = 0; w < DP_CW; w++) sacc += red[w][tid];
-    const int64_t p = tile0 * 16 + tid;
-    if (p < n) dsum[p] = sacc;
+  // With an odd n_ks the sets swap roles from one block to the next.
+  for (int64_t i = 0; i < nmine;) {
+    block(i, b0h, b0l, b1h, b1l);
+    if (++i >= nmine) break;
+    if constexpr (NKS % 2 == 0) continue;
+    block(i, b1h, b1l, b0h, b0l);
+    ++i;
   }
 }
 
+// n_cu: the device's compute units; the grid is one workgroup per CU (at most one block each).
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
-                          const half8_t *bfrag, const int16_t *cglist, double *dsum) {
+                          const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum) {
   if (n <= 0) return;
-  const dim3 grid((unsigned)((n + DP_PTS - 1) / DP_PTS)), block(64 * DP_WAVES);
+  const int64_t nblk = (n + DP_PTS - 1) / DP_PTS;
+  const int64_t wgs = std::min<int64_t>(nblk, std::max(n_cu, 1));
+  const dim3 grid((unsigned)wgs), block(64 * DP_WAVES);
   switch (m_cols / 32) {
-#define DP_CASE(k)                                                                       \
-  case k:                                                                                \
-    hipLaunchKernelGGL(k_dpass_split16<k>, grid, block, 0, st, xd, n, bfrag, cglist, dsum); \
+#define DP_CASE(k)                                                                             \
+  case k:                                                                                      \
+    hipLaunchKernelGGL(k_dpass_split16<k>, grid, block, 0, st, xd, n, nblk, bfrag, cglist, dsum); \
     break;
     DP_CASE(1) DP_CASE(2) DP_CASE(3) DP_CASE(4) DP_CASE(5) DP_CASE(6) DP_CASE(7)
     DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(13) DP_CASE(14)

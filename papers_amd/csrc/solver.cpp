@@ -127,6 +127,7 @@ struct cy3q_solver {
         *block_max = nullptr;
   double *tseg = nullptr;            // T-pass segment sums
   int64_t tp_chunk = 0, tp_chunks = 0;
+  int n_cu = 1;                // compute units of the device (persistent D-pass grid)
   int64_t tp_mult = 1;         // T-pass point chunks per CU (a tuning knob)
   int64_t syrk_target = 1024;  // workgroups of the FP64 rank-k pass (a tuning knob)
   double *batch_res = nullptr;  // residuals of captured solves
@@ -486,6 +487,7 @@ int ensure_split16(cy3q_solver *s) {
     int n_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
     n_cu = std::max(n_cu, 1);
+    s->n_cu = n_cu;
     const int64_t slots = (int64_t)n_cu * s->tp_mult;
     s->tp_chunk = std::max<int64_t>((n + slots * 16 - 1) / (slots * 16), 1) * 16;
     s->tp_chunks = (n + s->tp_chunk - 1) / s->tp_chunk;
@@ -509,7 +511,8 @@ int t_step_split16(cy3q_solver *s) {
     s->bfrag_valid = true;
   }
   s->stage_begin(STG_DPASS);
-  launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->dsum);
+  launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->n_cu,
+                       s->dsum);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
   launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->block_max,
