@@ -110,8 +110,9 @@ void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t 
 //  xd: D-pass A fragments of v_mfma_f32_16x16x32_f16. For point tile t (16 points), k-step ks
 //      (32 columns), plane h (hi, lo), lane l: half8 of X[16t + l%16][32ks + 8(l/16) + j].
 //      Index ((t * n_ks + ks) * 2 + h) * 64 + l.
-//  xt: T-pass stage records (FP32). For stage s (16 points), column c: the 16 floats
-//      X[16s + j][c], j = 0..15. Index (s * m + c) * 16 + j.
+//  xt: T-pass stage records (FP32). For stage s (16 points), quad q (4 points), column c: the
+//      4 floats X[16s + 4q + j][c], j = 0..3. Index ((s * 4 + q) * m + c) * 4 + j, so that one
+//      16-byte load per lane over consecutive columns reads a contiguous run.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__ sv, int64_t n,
                                                       int64_t m, int *__restrict__ ex,
@@ -145,7 +146,7 @@ __global__ void k_split_planes(const double *__restrict__ sv, const int *__restr
   const int lane = (int)(p % 16) + 16 * (int)((c % 32) / 8), j = (int)(c % 8);
   xd[((((t * n_ks + ks) * 2 + 0) * 64 + lane) * 8) + j] = h;
   xd[((((t * n_ks + ks) * 2 + 1) * 64 + lane) * 8) + j] = l;
-  xt[((p / 16) * m + c) * 16 + p % 16] = x;
+  xt[(((p / 16) * 4 + (p % 16) / 4) * m + c) * 4 + p % 4] = x;
 }
 
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,

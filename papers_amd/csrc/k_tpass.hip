@@ -3,8 +3,9 @@
 // the fp16 MFMA (v_mfma_f32_32x32x16_f16) with FP32 accumulation within a point chunk.
 //
 // Staging. One workgroup (8 waves) per CU streams one point chunk, 16 points per stage. Thread c
-// owns column c: it loads the column's 16 FP32 values of the next stage (and the stage's 16
-// coefficients) while the current stage's MFMAs run, then scales by 2^g sqrt(c'), splits z into
+// owns column c: it loads the column's 16 FP32 values of the next stage (four 16-byte loads,
+// each contiguous across the wave's columns) and the stage's 16 coefficients while the current
+// stage's MFMAs run, then scales by 2^g sqrt(c'), splits z into
 // fp16 hi/lo and writes the transposed tile zT[plane][column][16 points] to the other LDS
 // buffer. Two waves share each SIMD, so one wave's conversion overlaps the other's MFMAs.
 //
@@ -169,12 +170,28 @@ __device__ __forceinline__ void tq_store(float *__restrict__ out, const float16_
   }
 }
 
+// LDS ring of raw stage records: TQ_RING slots, each one stage's FP32 record (4 quads x M
+// columns x 4 points, the global layout) and its 16 coefficients (replicated over 64 floats).
+static constexpr int TQ_RING = 3;
+
+// s_waitcnt with vmcnt = v (0..63), lgkmcnt = 0 and expcnt at its maximum (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
+#define TQ_WAIT_VM_LGKM0(v) __builtin_amdgcn_s_waitcnt(((v)&15) | (((v) >> 4) << 14) | (7 << 4))
+
 template <int NT, int W>
-__device__ __forceinline__ void tq_wave(_Float16 *zt, const float4_t *__restrict__ xt,
+__device__ __forceinline__ void tq_wave(_Float16 *zt, float4_t *ring, float *uring,
+                                        const float4_t *__restrict__ xt,
                                         const float *__restrict__ u, int64_t p_begin,
                                         int64_t p_end, float *__restrict__ out) {
   constexpr int M = 32 * NT;
   constexpr int64_t PLANE = (int64_t)M * TQ_LD;
+  // Wave TQ_WAVES - 1 holds no column (M <= 416 < 448) and doubles as the data mover: per stage
+  // it issues G 1 KB copies of the record and one of the coefficients into the ring.
+  constexpr bool MOVER = W == TQ_WAVES - 1;
+  constexpr int G = 2 * NT;               // 1 KB granules per stage record (64 M bytes)
+  constexpr int COPIES = G + 1;           // copies per stage
+  static_assert(M <= 64 * (TQ_WAVES - 1), "the mover wave must hold no column");
+  static_assert(2 * COPIES <= 63, "ring waits must fit the vmcnt field");
   const int tid = threadIdx.x, lane = tid & 63;
   float16_t acc[TQ_MAXT];
 #pragma unroll
@@ -186,26 +203,46 @@ __device__ __forceinline__ void tq_wave(_Float16 *zt, const float4_t *__restrict
   const int64_t stage0 = p_begin / TQ_PTS;
   const int col = tid;
 
-  // This thread's column of the next stage (16 FP32 values) and the stage's 16 scaled
-  // coefficients (wave-uniform: one scalar load), both requested before the current stage's
-  // MFMAs so that their latency is covered.
-  float4_t rec[4];
-  float us[TQ_PTS];
-  auto fetch = [&](int s) {
-    if (col < M) {
-      const float4_t *src = xt + ((stage0 + s) * M + col) * 4;
-      rec[0] = src[0];
-      rec[1] = src[1];
-      rec[2] = src[2];
-      rec[3] = src[3];
-    }
-    const float *us_src = u + p_begin + (int64_t)s * TQ_PTS;
+  // Mover: stage t's record and coefficients into ring slot t % TQ_RING. Each lane copies 16
+  // bytes per granule (lane l to byte 16 l of the granule), and 4 bytes of coefficients (lane l
+  // gets coefficient l % 16).
+  auto issue = [&](int t) {
+    const char *src = reinterpret_cast<const char *>(xt + (stage0 + t) * 4 * M) + 16 * lane;
+    char *dst = reinterpret_cast<char *>(ring + (t % TQ_RING) * 4 * M);
 #pragma unroll
-    for (int j = 0; j < TQ_PTS; j++) us[j] = us_src[j];
+    for (int g = 0; g < G; g++)
+      __builtin_amdgcn_global_load_lds((const void *)(src + 1024 * g),
+                                       (__attribute__((address_space(3))) void *)(dst + 1024 * g),
+                                       16, 0, 0);
+    __builtin_amdgcn_global_load_lds(
+        (const void *)(u + p_begin + (int64_t)t * TQ_PTS + (lane & (TQ_PTS - 1))),
+        (__attribute__((address_space(3))) void *)(uring + (t % TQ_RING) * 64), 4, 0, 0);
   };
-  auto store = [&](int buf) {
+  // Mover: wait until the copies of every stage up to `need` have landed, given that stages up
+  // to `last` have been issued (copies complete in issue order for the counter), and the
+  // wave's own LDS reads too; then a bare barrier (a workgroup fence would wait for every copy
+  // in flight). The empty asm statements keep the compiler from moving LDS accesses across.
+  auto mover_sync = [&](int need, int last) {
+    asm volatile("" ::: "memory");
+    const int later = last - need;
+    if (later >= 2)
+      TQ_WAIT_VM_LGKM0(2 * COPIES);
+    else if (later == 1)
+      TQ_WAIT_VM_LGKM0(COPIES);
+    else
+      TQ_WAIT_VM_LGKM0(0);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  // Converting waves: stage t from its ring slot, z = u x split into fp16 hi/lo, into zt[buf].
+  auto convert = [&](int t, int buf) {
     _Float16 *hi = zt + (int64_t)buf * 2 * PLANE, *lo = hi + PLANE;
+    const float4_t *rs = ring + (t % TQ_RING) * 4 * M;
+    const float *us = uring + (t % TQ_RING) * 64;
     if (col < M) {
+      float4_t rec[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) rec[q] = rs[q * M + col];
 #pragma unroll
       for (int hf = 0; hf < 2; hf++) {
         half8_t zh, zl;
@@ -223,20 +260,47 @@ __device__ __forceinline__ void tq_wave(_Float16 *zt, const float4_t *__restrict
     }
   };
 
-  if (n_stage > 0) {
-    fetch(0);
-    store(0);
+  // Prologue: stages 0 .. TQ_RING - 1 in flight; stage 0 converted into zt[0]; stage 1 landed.
+  const int first = n_stage < TQ_RING ? n_stage : TQ_RING;
+  if constexpr (MOVER) {
+    for (int t = 0; t < first; t++) issue(t);
+    mover_sync(0, first - 1);
+  } else {
+    __syncthreads();
+    if (n_stage > 0) convert(0, 0);
   }
-  __syncthreads();
+  if constexpr (MOVER)
+    mover_sync(1, first - 1);
+  else
+    __syncthreads();
+  // Stage s: the mover first refills the slot of stage s (converted in stage s - 1) with stage
+  // s + TQ_RING; all waves run their MFMAs on zt[s % 2]; the converting waves fill zt[1 - s % 2]
+  // from stage s + 1 (landed before this stage's barrier, and zt[1 - s % 2] was last read
+  // before it); the mover then waits for stage s + 2, which the next stage converts.
+  // Waves w and w + 4 share a SIMD. Waves 4..6 convert before their MFMAs and waves 0..3
+  // after, so one wave's conversion runs under its partner's MFMAs.
+  constexpr bool CONVERT_FIRST = W >= TQ_WAVES / 2 && !MOVER;
   // MFMA operand position of this lane: column l % 32 of a tile, points 8 (l / 32) .. + 7.
   const int lofs = (lane & 31) * TQ_LD + 8 * (lane >> 5);
   for (int s = 0; s < n_stage; s++) {
     const int buf = s & 1;
-    if (s + 1 < n_stage) fetch(s + 1);
+    if constexpr (MOVER) {
+      if (s + TQ_RING < n_stage) issue(s + TQ_RING);
+    }
+    if constexpr (CONVERT_FIRST) {
+      if (s + 1 < n_stage) convert(s + 1, buf ^ 1);
+    }
     const _Float16 *hi = zt + (int64_t)buf * 2 * PLANE + lofs, *lo = hi + PLANE;
     tq_run<NT, W, 0>(hi, lo, acc);
-    if (s + 1 < n_stage) store(buf ^ 1);
-    __syncthreads();
+    if constexpr (MOVER) {
+      const int last = s + TQ_RING < n_stage ? s + TQ_RING : n_stage - 1;
+      mover_sync(s + 2 < last ? s + 2 : last, last);
+    } else {
+      if constexpr (!CONVERT_FIRST) {
+        if (s + 1 < n_stage) convert(s + 1, buf ^ 1);
+      }
+      __syncthreads();
+    }
   }
   tq_store<NT, W, 0>(out, acc, lane);
 }
@@ -249,20 +313,22 @@ k_tpass(const float4_t *__restrict__ xt, const float *__restrict__ u, int64_t n,
   static_assert(tq_progs_valid(NT), "T-pass wave programs must cover the lower triangle once");
   constexpr int M = 32 * NT;
   __shared__ __attribute__((aligned(16))) _Float16 zt[2 * 2 * M * TQ_LD];
+  __shared__ float4_t ring[TQ_RING * 4 * M];
+  __shared__ float uring[TQ_RING * 64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t chunk_id = blockIdx.x;
   const int64_t p_begin = chunk_id * chunk;
   const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
   float *out = partials + chunk_id * (int64_t)(NT * (NT + 1) / 2) * 1024;
   switch (wave) {
-    case 0: tq_wave<NT, 0>(zt, xt, u, p_begin, p_end, out); break;
-    case 1: tq_wave<NT, 1>(zt, xt, u, p_begin, p_end, out); break;
-    case 2: tq_wave<NT, 2>(zt, xt, u, p_begin, p_end, out); break;
-    case 3: tq_wave<NT, 3>(zt, xt, u, p_begin, p_end, out); break;
-    case 4: tq_wave<NT, 4>(zt, xt, u, p_begin, p_end, out); break;
-    case 5: tq_wave<NT, 5>(zt, xt, u, p_begin, p_end, out); break;
-    case 6: tq_wave<NT, 6>(zt, xt, u, p_begin, p_end, out); break;
-    default: tq_wave<NT, 7>(zt, xt, u, p_begin, p_end, out); break;
+    case 0: tq_wave<NT, 0>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 1: tq_wave<NT, 1>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 2: tq_wave<NT, 2>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 3: tq_wave<NT, 3>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 4: tq_wave<NT, 4>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 5: tq_wave<NT, 5>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    case 6: tq_wave<NT, 6>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
+    default: tq_wave<NT, 7>(zt, ring, uring, xt, u, p_begin, p_end, out); break;
   }
 }
 
