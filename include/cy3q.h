@@ -133,6 +133,32 @@ int cy3q_solver_sigma_sq_grad(cy3q_solver *s, double *sigma_l2sq, double *grad,
 int cy3q_solver_sigma_refine_analytic(cy3q_solver *s, int64_t n_iter, double lr,
                                       double *history, int64_t *n_done);
 int cy3q_solver_set_adam(cy3q_solver *s, double beta1, double beta2, double eps);
+/* QuinticSolver::sigma_refine_lbfgs_gpu (quintic.rs:678-880, 901-1094): L-BFGS on sigma_L2^2
+ * over (h_re, h_im) with the m_history newest curvature pairs (1..63), a backtracking Armijo
+ * line search (c1 = 1e-4, at most 30 halvings, first trial step 1, or 1/|g| for a steepest-
+ * descent step), a steepest-descent retry when the line search fails, and the Hermitian
+ * projection plus trace renormalisation of apply_lbfgs_step (:2003-2048) at every trial point.
+ * The point with the lowest sigma_L1 is kept at the end. history: n_iter + 1 doubles (sigma_L1
+ * at the start and after each accepted step) or NULL; *n_done: entries written. A trial point
+ * whose G is not positive definite is rejected by the line search. */
+int cy3q_solver_sigma_refine_lbfgs(cy3q_solver *s, int64_t n_iter, int64_t m_history,
+                                   double *history, int64_t *n_done);
+/* QuinticSolver::sigma_refine_analytic_with_restarts (quintic.rs:545-582) and its GPU form
+ * (:1145-1182): Adam from the current G, then n_restarts more runs, each from the best G so far
+ * plus a Hermitian perturbation (perturb_h_block_hermitian, :1195-1223: uniform in
+ * [-scale, scale) from the lib.rs LCG with `seed`, real on the diagonal, re and im for a < b)
+ * and the trace renormalisation. The best G (lowest sigma_L1) is kept. history: n_restarts + 1
+ * doubles (sigma_L1 after each run) or NULL. */
+int cy3q_solver_sigma_refine_analytic_restarts(cy3q_solver *s, int64_t n_restarts,
+                                               int64_t n_iter_per_restart, double lr,
+                                               double perturb_scale, uint64_t seed,
+                                               double *best_sigma, double *history);
+/* QuinticSolver::sigma_refine_lbfgs_with_restarts_gpu (quintic.rs:1100-1139): the restart loop
+ * above around cy3q_solver_sigma_refine_lbfgs. */
+int cy3q_solver_sigma_refine_lbfgs_restarts(cy3q_solver *s, int64_t n_restarts,
+                                            int64_t n_iter_per_restart, int64_t m_history,
+                                            double perturb_scale, uint64_t seed,
+                                            double *best_sigma, double *history);
 
 /* h_block access (4 N^2 doubles). set_h_block takes the Hermitian part. */
 int cy3q_solver_get_h_block(cy3q_solver *s, double *h_block);

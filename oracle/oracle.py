@@ -497,3 +497,215 @@ def legacy_solve(sv, max_iter: int, tol: float, w=None):
         if r < tol:
             break
     return h, res
+
+
+# ----------------------------------------------------------------------------------------------
+# L-BFGS sigma refinement and random restarts (quintic.rs:545-582, 678-880, 1100-1223, 2003-2048)
+# ----------------------------------------------------------------------------------------------
+def _unpack_h(h_block):
+    """h_re[a][b] = h[2a][2b], h_im[a][b] = -h[2a][2b+1] (quintic.rs:5128-5140)."""
+    h = np.asarray(h_block, dtype=np.float64)
+    return h[0::2, 0::2].copy(), -h[0::2, 1::2].copy()
+
+
+def _pack_h(hre, him):
+    """pack_h_block_workspace (quintic.rs:2148-2161)."""
+    nb = hre.shape[0]
+    h = np.zeros((2 * nb, 2 * nb))
+    h[0::2, 0::2] = hre
+    h[0::2, 1::2] = -him
+    h[1::2, 0::2] = him
+    h[1::2, 1::2] = hre
+    return h
+
+
+def renorm_trace(h_block):
+    """renormalise_h_trace_workspace (quintic.rs:3169-3182): sequential trace of the even
+    diagonal, scale to N when it exceeds 1e-10."""
+    h = np.array(h_block, dtype=np.float64)
+    nb = h.shape[0] // 2
+    tr = 0.0
+    for a in range(nb):
+        tr += h[2 * a, 2 * a]
+    if tr > 1e-10:
+        h *= nb / tr
+    return h
+
+
+def lbfgs_apply_step(h_block, d, step: float):
+    """apply_lbfgs_step (quintic.rs:2003-2048) on a copy: h_re += step d_re for every (a, b),
+    then the (a, b)/(b, a) average; h_im(a, b) += step d_im(a, b), h_im(b, a) -= the same, a < b;
+    zero h_im diagonal; pack; trace renormalisation."""
+    hre, him = _unpack_h(h_block)
+    nb = hre.shape[0]
+    n2 = nb * nb
+    d = np.asarray(d, dtype=np.float64)
+    hre += step * d[:n2].reshape(nb, nb)
+    for a in range(nb):
+        for b in range(a + 1, nb):
+            avg = 0.5 * (hre[a, b] + hre[b, a])
+            hre[a, b] = avg
+            hre[b, a] = avg
+    for a in range(nb):
+        for b in range(a + 1, nb):
+            upd = step * d[n2 + a * nb + b]
+            him[a, b] += upd
+            him[b, a] -= upd
+    for a in range(nb):
+        him[a, a] = 0.0
+    return renorm_trace(_pack_h(hre, him))
+
+
+def lbfgs_direction(g, pairs):
+    """The two-loop recursion of quintic.rs:746-776: d = -H g from the curvature pairs
+    (s, y, rho = 1/(s.y)) listed oldest first, with H0 = gamma I, gamma = (s.y)/(y.y) of the
+    newest pair (1 when y.y <= 1e-30 or rho is not finite, or without pairs)."""
+    q = np.array(g, dtype=np.float64)
+    m = len(pairs)
+    alphas = [0.0] * m
+    for k in range(m - 1, -1, -1):
+        s, y, rho = pairs[k]
+        a = rho * float(np.dot(s, q))
+        alphas[k] = a
+        q -= a * y
+    gamma = 1.0
+    if m > 0:
+        _, y_l, rho_l = pairs[-1]
+        yy = float(np.dot(y_l, y_l))
+        if yy > 1e-30 and np.isfinite(rho_l):
+            gamma = 1.0 / (rho_l * yy)
+    r = q * gamma
+    for k in range(m):
+        s, y, rho = pairs[k]
+        beta = rho * float(np.dot(y, r))
+        r += s * (alphas[k] - beta)
+    return -r
+
+
+def lbfgs_minimize(x0, fgs, apply_step, n_iter: int, m_history: int, c1: float = 1e-4,
+                   max_backtracks: int = 30):
+    """The optimiser of sigma_refine_lbfgs_gpu_legacy (quintic.rs:678-880) on a generic
+    objective: fgs(x) -> (f, grad, score), apply_step(x, d, step) -> the trial point. Armijo
+    backtracking from step 1 (or 1/|g| for steepest descent) with halving, a steepest-descent
+    retry when the L-BFGS line search fails, pairs kept when s.y > 1e-12, and the point with the
+    lowest score kept. Returns (score history, best x, trial points that were evaluated)."""
+    f, g, score = fgs(x0)
+    hist = []
+    trials = []
+    if not np.isfinite(score):
+        return hist, x0, trials
+    hist.append(score)
+    best, best_x = score, x0
+    x = x0
+    pairs = []
+    it = 0
+    while it < n_iter:
+        d = lbfgs_direction(g, pairs)
+        gd = float(np.dot(g, d))
+        used = len(pairs) > 0
+        if not np.isfinite(gd) or gd >= 0.0:
+            pairs = []
+            d = -g
+            gd = float(np.dot(g, d))
+            used = False
+            if gd >= 0.0:
+                break
+        g_norm = float(np.sqrt(np.dot(g, g)))
+        step = 1.0 if used else (1.0 / g_norm if g_norm > 1e-30 else 1.0)
+        x_save = x
+        accepted = False
+        for _ in range(max_backtracks):
+            x = apply_step(x_save, d, step)
+            f2, g2, s2 = fgs(x)
+            trials.append(x)
+            if np.isfinite(f2) and f2 <= f + c1 * step * gd:
+                accepted = True
+                break
+            step *= 0.5
+        if not accepted and used:
+            pairs = []
+            d = -g
+            gd = float(np.dot(g, d))
+            sd = 1.0 / g_norm if g_norm > 1e-30 else 1.0
+            for _ in range(max_backtracks):
+                x = apply_step(x_save, d, sd)
+                f2, g2, s2 = fgs(x)
+                trials.append(x)
+                if np.isfinite(f2) and f2 <= f + c1 * sd * gd:
+                    accepted = True
+                    step = sd
+                    break
+                sd *= 0.5
+        if not accepted:
+            x = x_save
+            break
+        s_k = step * d
+        y_k = g2 - g
+        sy = float(np.dot(s_k, y_k))
+        if sy > 1e-12:
+            pairs.append((s_k, y_k, 1.0 / sy))
+            if len(pairs) > m_history:
+                pairs.pop(0)
+        if np.isfinite(s2) and s2 < best:
+            best, best_x = s2, x
+        f, g = f2, g2
+        hist.append(s2)
+        it += 1
+    return hist, best_x, trials
+
+
+def sigma_refine_lbfgs(prob, h_block, n_iter: int, m_history: int):
+    """QuinticSolver::sigma_refine_lbfgs_gpu(_legacy) (quintic.rs:678-880): L-BFGS on sigma_L2^2
+    over (h_re, h_im), scored by sigma_L1. Returns (history, best h_block, min over every
+    evaluated trial point of the smallest eigenvalue of its complex G)."""
+    def fgs(h):
+        s2, g, s1 = prob.sigma_sq_grad(h)
+        return s2, g, s1
+    hist, h_best, trials = lbfgs_minimize(np.array(h_block, dtype=np.float64), fgs,
+                                          lbfgs_apply_step, n_iter, m_history)
+    min_eig = np.inf
+    for h in trials:
+        hre, him = _unpack_h(h)
+        min_eig = min(min_eig, float(np.linalg.eigvalsh(hre + 1j * him).min()))
+    return hist, h_best, min_eig
+
+
+def perturb_h_hermitian(h_block, scale: float, draws):
+    """perturb_h_block_hermitian (quintic.rs:1195-1223): for each a, the diagonal h_re gets
+    scale (2u - 1); for each b > a, h_re(a, b) and h_re(b, a) get scale (2u - 1) and h_im(a, b)
+    gets +scale (2u' - 1), h_im(b, a) the opposite, with u, u' the next LCG draws (`draws`
+    iterates over next_f64 values)."""
+    hre, him = _unpack_h(h_block)
+    nb = hre.shape[0]
+    for a in range(nb):
+        hre[a, a] += scale * (next(draws) * 2.0 - 1.0)
+        for b in range(a + 1, nb):
+            dr = scale * (next(draws) * 2.0 - 1.0)
+            di = scale * (next(draws) * 2.0 - 1.0)
+            hre[a, b] += dr
+            hre[b, a] += dr
+            him[a, b] += di
+            him[b, a] -= di
+    return _pack_h(hre, him)
+
+
+def refine_with_restarts(prob, h_block, n_restarts: int, refine, perturb_scale: float,
+                         seed: int):
+    """The restart loop of sigma_refine_analytic_with_restarts (quintic.rs:545-582) and
+    sigma_refine_lbfgs_with_restarts_gpu (:1100-1139): refine(h) -> h after one run. Returns
+    (best sigma_L1, history of sigma_L1 after each run, best h_block)."""
+    nb = h_block.shape[0] // 2
+    h = refine(np.array(h_block, dtype=np.float64))
+    best = prob.sigma(h)
+    best_h = h
+    history = [best]
+    _, u, _ = lcg_stream(seed, max(1, n_restarts * nb * nb))
+    draws = iter(u.tolist())
+    for _ in range(n_restarts):
+        h = renorm_trace(perturb_h_hermitian(best_h, perturb_scale, draws))
+        h = refine(h)
+        trial = prob.sigma(h)
+        history.append(trial)
+        if np.isfinite(trial) and trial < best:
+            best, best_h = trial, h
+    return best, history, best_h

@@ -33,6 +33,7 @@ SOURCES = {
     "k_split16.hip": [],
     "k_tpass.hip": [],
     "k_adjoint.hip": [],
+    "k_lbfgs.hip": [],
     "k_laplacian.hip": [],
     "k_eigen.hip": [],
     "k_yukawa.hip": [],

@@ -24,6 +24,7 @@
 
 #include "../../include/cy3q.h"
 #include "cy3q_internal.h"
+#include "cy3q_lcg.h"
 
 using namespace cy3q;
 
@@ -72,15 +73,6 @@ namespace {
     if (rc_ != CY3Q_OK) return rc_;  \
   } while (0)
 
-// The lib.rs LCG (cy3/src/lib.rs:222-245) on the host: 53-bit doubles.
-struct HostLcg {
-  uint64_t s;
-  explicit HostLcg(uint64_t seed) : s(seed * 6364136223846793005ULL + 1ULL) {}
-  double next() {
-    s = s * 6364136223846793005ULL + 1442695040888963407ULL;
-    return (double)((s >> 11) & ((1ULL << 53) - 1ULL)) / (double)(1ULL << 53);
-  }
-};
 
 int solve_timed(cy3q_solver *s, int64_t iters, double *ms) {
   B_TRY(cy3q_solver_synchronize(s));
@@ -103,10 +95,10 @@ extern "C" {
 
 int cy3q_batch_draw_specs(uint64_t seed, int n, cy3q_instance_spec *out) {
   if (n < 0 || (n > 0 && !out)) return set_error(CY3Q_ERR_ARG, "bad arguments");
-  HostLcg r(seed);
+  Lcg r{lcg_seed(seed)};
   const double l0 = std::log(2.0e4), l1 = std::log(5.0e5);
   for (int i = 0; i < n; i++) {
-    const double u_psi = r.next(), u_k = r.next(), u_n = r.next();
+    const double u_psi = r.next_f64(), u_k = r.next_f64(), u_n = r.next_f64();
     out[i].psi = 0.9 * u_psi;
     out[i].k = 2u + (uint32_t)std::min(4.0, std::floor(5.0 * u_k));
     out[i].n_points = (int64_t)std::llround(std::exp(l0 + u_n * (l1 - l0)));

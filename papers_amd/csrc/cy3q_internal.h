@@ -132,6 +132,24 @@ void launch_grad_assemble(hipStream_t st, const double *cm, int nb, int64_t m_co
                           const double *sig, double *grad);
 void launch_adam_step(hipStream_t st, double *g_cplx, int nb, const double *grad, double *m,
                       double *v, double scale, double beta1, double beta2, double eps);
+// Trace renormalisation of G to tr Re G = nb (when the trace exceeds 1e-10).
+void launch_trace_renorm(hipStream_t st, double *g_cplx, int nb);
+
+// L-BFGS vectors (k_lbfgs.hip). The curvature pairs in use, oldest first: slot[k] indexes the
+// pair's rows of the s and y stores, rho[k] = 1 / (s.y).
+constexpr int LBFGS_MAX_PAIRS = 64;
+struct LbfgsPairs {
+  int count;
+  int slot[LBFGS_MAX_PAIRS];
+  double rho[LBFGS_MAX_PAIRS];
+};
+void launch_lbfgs_direction(hipStream_t st, const double *g, const double *s_all,
+                            const double *y_all, const LbfgsPairs &hist, int64_t n2, double *d,
+                            double *out);
+void launch_lbfgs_apply(hipStream_t st, const double *g0, int nb, const double *d, double step,
+                        double *g);
+void launch_lbfgs_pair(hipStream_t st, const double *d, double step, const double *g_new,
+                       const double *g_old, int64_t n2, double *s_out, double *y_out, double *out);
 
 // Metric Laplacian assembly (k_laplacian.hip).
 void launch_lap_metric(hipStream_t st, const double *pts, const double *y0, const double *yd,

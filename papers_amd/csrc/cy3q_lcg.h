@@ -1,6 +1,7 @@
-// The reference crate's LCG (cy3/src/lib.rs:222-245) on the device: the 64-bit multiplier
-// and increment of lib.rs:230-237, 53-bit doubles, and Box-Muller normals (:239-244), plus an
-// O(log n) jump-ahead so that draw q of a stream can be taken by thread q.
+// The reference crate's LCG (cy3/src/lib.rs:222-245) for the device and the host: the 64-bit
+// multiplier and increment of lib.rs:230-237, the seeding of LCG::new, 53-bit doubles, and
+// Box-Muller normals (:239-244), plus an O(log n) jump-ahead so that draw q of a stream can be
+// taken by thread q.
 #pragma once
 #include "cy3q_internal.h"
 
@@ -9,9 +10,12 @@ namespace cy3q {
 static constexpr uint64_t LCG_MUL = 6364136223846793005ULL;
 static constexpr uint64_t LCG_INC = 1442695040888963407ULL;
 
+// LCG::new(seed): the state after seeding (lib.rs:230-233).
+__host__ __device__ inline uint64_t lcg_seed(uint64_t seed) { return seed * LCG_MUL + 1ULL; }
+
 struct Lcg {
   uint64_t s;
-  __device__ double next_f64() {
+  __host__ __device__ double next_f64() {
     s = s * LCG_MUL + LCG_INC;
     uint64_t bits = (s >> 11) & ((1ULL << 53) - 1ULL);
     return (double)bits / (double)(1ULL << 53);

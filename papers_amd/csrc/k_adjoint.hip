@@ -342,12 +342,16 @@ __global__ void __launch_bounds__(256) k_trace_renorm(double *__restrict__ g, in
   for (int64_t e = threadIdx.x; e < 2 * (int64_t)nb * nb; e += 256) g[e] *= sc;
 }
 
+void launch_trace_renorm(hipStream_t st, double *g_cplx, int nb) {
+  hipLaunchKernelGGL(k_trace_renorm, dim3(1), dim3(256), 0, st, g_cplx, nb);
+}
+
 void launch_adam_step(hipStream_t st, double *g_cplx, int nb, const double *grad, double *m,
                       double *v, double scale, double beta1, double beta2, double eps) {
   const int64_t nn = (int64_t)nb * nb;
   hipLaunchKernelGGL(k_adam_herm, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, g_cplx, nb,
                      grad, m, v, scale, beta1, beta2, eps);
-  hipLaunchKernelGGL(k_trace_renorm, dim3(1), dim3(256), 0, st, g_cplx, nb);
+  launch_trace_renorm(st, g_cplx, nb);
 }
 
 }  // namespace cy3q

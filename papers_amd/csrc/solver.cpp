@@ -17,6 +17,7 @@
 
 #include "../../include/cy3q.h"
 #include "cy3q_internal.h"
+#include "cy3q_lcg.h"
 
 using namespace cy3q;
 
@@ -542,8 +543,8 @@ int step_in(cy3q_solver *s, int precision) {
 int one_step(cy3q_solver *s) { return step_in(s, s->precision); }
 
 // W from an arbitrary Hermitian PD G: W1 = chol(G)^-1, G^-1 = W1^H W1, W = chol(G^-1)^-1,
-// so that W^H W = G.
-int w_from_g(cy3q_solver *s) {
+// so that W^H W = G. *pd = false when either factorisation finds G not positive definite.
+int w_from_g_pd(cy3q_solver *s, bool *pd) {
   const int nb = s->nb;
   launch_elim_init(s->st, s->g, nb, s->abuf, s->rbuf);
   launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wnew, nb, s->status);
@@ -555,7 +556,14 @@ int w_from_g(cy3q_solver *s) {
   int32_t st2[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(st2, s->status, sizeof(st2), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
-  if (st2[0] != 0 || st2[1] != 0) return fail(CY3Q_ERR_ARG, "h_block is not positive definite");
+  *pd = st2[0] == 0 && st2[1] == 0;
+  return CY3Q_OK;
+}
+
+int w_from_g(cy3q_solver *s) {
+  bool pd = false;
+  ST_TRY(w_from_g_pd(s, &pd));
+  if (!pd) return fail(CY3Q_ERR_ARG, "h_block is not positive definite");
   return CY3Q_OK;
 }
 
@@ -1116,6 +1124,284 @@ int cy3q_solver_sigma_refine_analytic(cy3q_solver *s, int64_t n_iter, double lr,
   }
   if (n_done) *n_done = done;
   return CY3Q_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// sigma_L2^2 (*f), its gradient (into s->grad_dev) and sigma_L1 at the current G. A G with no
+// W (not positive definite) gives f = +inf and sigma_L1 = NaN, which every line search rejects.
+int refine_eval(cy3q_solver *s, double *f, double *sigma_l1) {
+  bool pd = false;
+  ST_TRY(w_from_g_pd(s, &pd));
+  if (!pd) {
+    *f = INFINITY;
+    *sigma_l1 = NAN;
+    return CY3Q_OK;
+  }
+  double o[6];
+  ST_TRY(sigma_grad_impl(s, o));
+  *f = o[2];
+  *sigma_l1 = o[0];
+  return CY3Q_OK;
+}
+
+// Device scalars of the L-BFGS kernels, read back after each launch.
+int read_scalars(cy3q_solver *s, const double *dev, int count, double *host) {
+  HIP_TRY(hipMemcpyAsync(host, dev, sizeof(double) * count, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int copy_dev(cy3q_solver *s, double *dst, const double *src, int64_t count) {
+  HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, s->st));
+  return CY3Q_OK;
+}
+
+// sigma_refine_lbfgs_gpu_legacy (quintic.rs:678-880) with the vectors on the device. The pair
+// stores hold m_history + 1 slots, so a candidate pair can be formed before it is known whether
+// it enters the history (s.y > 1e-12), without overwriting the oldest pair.
+int lbfgs_run(cy3q_solver *s, int64_t n_iter, int m_history, double *history, int64_t *n_done) {
+  const int nb = s->nb;
+  const int64_t n2 = 2 * (int64_t)nb * nb;  // gradient entries = doubles of the complex G
+  const int n_slots = m_history + 1;
+  double *s_all = nullptr, *y_all = nullptr, *d = nullptr, *g_cur = nullptr, *g_save = nullptr,
+         *g_best = nullptr, *scal = nullptr;
+  auto cleanup = [&] {
+    for (double *p : {s_all, y_all, d, g_cur, g_save, g_best, scal})
+      if (p) s->release(p);
+  };
+  auto run = [&]() -> int {
+    ST_TRY(s->alloc(&s_all, (size_t)n_slots * n2));
+    ST_TRY(s->alloc(&y_all, (size_t)n_slots * n2));
+    ST_TRY(s->alloc(&d, n2));
+    ST_TRY(s->alloc(&g_cur, n2));
+    ST_TRY(s->alloc(&g_save, n2));
+    ST_TRY(s->alloc(&g_best, n2));
+    ST_TRY(s->alloc(&scal, 4));
+    int64_t written = 0;
+    double f_cur = 0.0, sig = 0.0;
+    ST_TRY(refine_eval(s, &f_cur, &sig));
+    if (!std::isfinite(sig)) {
+      *n_done = 0;
+      return CY3Q_OK;
+    }
+    if (history) history[written] = sig;
+    written++;
+    double best = sig;
+    ST_TRY(copy_dev(s, g_best, s->g, n2));
+    ST_TRY(copy_dev(s, g_cur, s->grad_dev, n2));
+    LbfgsPairs hist{};
+    hist.count = 0;
+    const double c1 = 1e-4;
+    for (int64_t iter = 0; iter < n_iter;) {
+      double sc[2];
+      launch_lbfgs_direction(s->st, g_cur, s_all, y_all, hist, n2, d, scal);
+      ST_TRY(read_scalars(s, scal, 2, sc));
+      double gd = sc[0], gg = sc[1];
+      bool used_lbfgs = hist.count > 0;
+      if (!std::isfinite(gd) || gd >= 0.0) {
+        hist.count = 0;
+        launch_lbfgs_direction(s->st, g_cur, s_all, y_all, hist, n2, d, scal);
+        ST_TRY(read_scalars(s, scal, 2, sc));
+        gd = sc[0];
+        gg = sc[1];
+        used_lbfgs = false;
+        if (gd >= 0.0) break;  // zero gradient: converged
+      }
+      const double g_norm = std::sqrt(gg);
+      double step = used_lbfgs ? 1.0 : (g_norm > 1e-30 ? 1.0 / g_norm : 1.0);
+      ST_TRY(copy_dev(s, g_save, s->g, n2));
+      bool accepted = false;
+      double f_new = 0.0, sig_new = 0.0;
+      for (int ls = 0; ls < 30; ls++) {
+        launch_lbfgs_apply(s->st, g_save, nb, d, step, s->g);
+        ST_TRY(refine_eval(s, &f_new, &sig_new));
+        if (std::isfinite(f_new) && f_new <= f_cur + c1 * step * gd) {
+          accepted = true;
+          break;
+        }
+        step *= 0.5;
+      }
+      if (!accepted && used_lbfgs) {
+        // The line search along the L-BFGS direction failed: drop the history and retry this
+        // iteration along -g (quintic.rs:810-841).
+        hist.count = 0;
+        launch_lbfgs_direction(s->st, g_cur, s_all, y_all, hist, n2, d, scal);
+        ST_TRY(read_scalars(s, scal, 2, sc));
+        gd = sc[0];
+        const double gn = std::sqrt(sc[1]);
+        double sd_step = gn > 1e-30 ? 1.0 / gn : 1.0;
+        for (int ls = 0; ls < 30; ls++) {
+          launch_lbfgs_apply(s->st, g_save, nb, d, sd_step, s->g);
+          ST_TRY(refine_eval(s, &f_new, &sig_new));
+          if (std::isfinite(f_new) && f_new <= f_cur + c1 * sd_step * gd) {
+            accepted = true;
+            step = sd_step;
+            break;
+          }
+          sd_step *= 0.5;
+        }
+      }
+      if (!accepted) {  // neither direction gives a decrease: converged
+        ST_TRY(copy_dev(s, s->g, g_save, n2));
+        break;
+      }
+      // Candidate pair in a free slot; it enters the history when s.y > 1e-12, evicting the
+      // oldest pair beyond m_history.
+      int slot = 0;
+      while (std::find(hist.slot, hist.slot + hist.count, slot) != hist.slot + hist.count) slot++;
+      launch_lbfgs_pair(s->st, d, step, s->grad_dev, g_cur, n2, s_all + (int64_t)slot * n2,
+                        y_all + (int64_t)slot * n2, scal + 2);
+      double sy = 0.0;
+      ST_TRY(read_scalars(s, scal + 2, 1, &sy));
+      if (sy > 1e-12) {
+        if (hist.count == m_history) {
+          for (int k = 1; k < hist.count; k++) {
+            hist.slot[k - 1] = hist.slot[k];
+            hist.rho[k - 1] = hist.rho[k];
+          }
+          hist.count--;
+        }
+        hist.slot[hist.count] = slot;
+        hist.rho[hist.count] = 1.0 / sy;
+        hist.count++;
+      }
+      if (std::isfinite(sig_new) && sig_new < best) {
+        best = sig_new;
+        ST_TRY(copy_dev(s, g_best, s->g, n2));
+      }
+      f_cur = f_new;
+      ST_TRY(copy_dev(s, g_cur, s->grad_dev, n2));
+      if (history) history[written] = sig_new;
+      written++;
+      iter++;
+    }
+    // The best iterate (lowest sigma_L1) with its W.
+    ST_TRY(copy_dev(s, s->g, g_best, n2));
+    ST_TRY(w_from_g(s));
+    *n_done = written;
+    return CY3Q_OK;
+  };
+  const int rc = run();
+  HIP_TRY(hipStreamSynchronize(s->st));
+  cleanup();
+  return rc;
+}
+
+// The restart loop of sigma_refine_analytic_with_restarts (quintic.rs:545-582) and
+// sigma_refine_lbfgs_with_restarts_gpu (:1100-1139) around `refine`, with
+// perturb_h_block_hermitian (:1195-1223) on G (h_re = Re G, h_im = Im G) in the LCG's draw
+// order: for each a the diagonal, then re and im of each (a, b), b > a.
+template <typename F>
+int restarts_run(cy3q_solver *s, int64_t n_restarts, double scale, uint64_t seed, F refine,
+                 double *best_sigma, double *history) {
+  const int nb = s->nb;
+  const int64_t n2 = 2 * (int64_t)nb * nb;
+  double *g_best = nullptr;
+  ST_TRY(s->alloc(&g_best, n2));
+  auto run = [&]() -> int {
+    double o[5];
+    ST_TRY(refine());
+    ST_TRY(sigma_impl(s, o, nullptr));
+    double best = o[0];
+    ST_TRY(copy_dev(s, g_best, s->g, n2));
+    if (history) history[0] = best;
+    Lcg rng{lcg_seed(seed)};
+    std::vector<double> hg(n2);
+    for (int64_t r = 0; r < n_restarts; r++) {
+      HIP_TRY(hipMemcpyAsync(hg.data(), g_best, sizeof(double) * n2, hipMemcpyDeviceToHost,
+                             s->st));
+      HIP_TRY(hipStreamSynchronize(s->st));
+      for (int a = 0; a < nb; a++) {
+        const int64_t aa = (int64_t)a * nb + a;
+        hg[2 * aa] += scale * (rng.next_f64() * 2.0 - 1.0);
+        for (int b = a + 1; b < nb; b++) {
+          const double dr = scale * (rng.next_f64() * 2.0 - 1.0);
+          const double di = scale * (rng.next_f64() * 2.0 - 1.0);
+          const int64_t ab = (int64_t)a * nb + b, ba = (int64_t)b * nb + a;
+          hg[2 * ab] += dr;
+          hg[2 * ba] += dr;
+          hg[2 * ab + 1] += di;
+          hg[2 * ba + 1] -= di;
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(s->g, hg.data(), sizeof(double) * n2, hipMemcpyHostToDevice, s->st));
+      launch_trace_renorm(s->st, s->g, nb);
+      bool pd = false;
+      ST_TRY(w_from_g_pd(s, &pd));
+      double trial = NAN;
+      if (pd) {  // a perturbation that leaves G without a W counts as a failed restart
+        ST_TRY(refine());
+        ST_TRY(sigma_impl(s, o, nullptr));
+        trial = o[0];
+      }
+      if (history) history[r + 1] = trial;
+      if (std::isfinite(trial) && trial < best) {
+        best = trial;
+        ST_TRY(copy_dev(s, g_best, s->g, n2));
+      }
+    }
+    ST_TRY(copy_dev(s, s->g, g_best, n2));
+    ST_TRY(w_from_g(s));
+    if (best_sigma) *best_sigma = best;
+    return CY3Q_OK;
+  };
+  const int rc = run();
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(g_best);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cy3q_solver_sigma_refine_lbfgs(cy3q_solver *s, int64_t n_iter, int64_t m_history,
+                                   double *history, int64_t *n_done) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_iter < 0) return fail(CY3Q_ERR_ARG, "n_iter must be >= 0");
+  if (m_history < 1 || m_history > LBFGS_MAX_PAIRS - 1)
+    return fail(CY3Q_ERR_ARG, "m_history must be in 1..63");
+  HIP_TRY(hipSetDevice(s->device));
+  int64_t done = 0;
+  ST_TRY(lbfgs_run(s, n_iter, (int)m_history, history, &done));
+  if (n_done) *n_done = done;
+  return CY3Q_OK;
+}
+
+int cy3q_solver_sigma_refine_analytic_restarts(cy3q_solver *s, int64_t n_restarts,
+                                               int64_t n_iter_per_restart, double lr,
+                                               double perturb_scale, uint64_t seed,
+                                               double *best_sigma, double *history) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_restarts < 0 || n_iter_per_restart < 0)
+    return fail(CY3Q_ERR_ARG, "restart and iteration counts must be >= 0");
+  HIP_TRY(hipSetDevice(s->device));
+  return restarts_run(
+      s, n_restarts, perturb_scale, seed,
+      [&] { return cy3q_solver_sigma_refine_analytic(s, n_iter_per_restart, lr, nullptr, nullptr); },
+      best_sigma, history);
+}
+
+int cy3q_solver_sigma_refine_lbfgs_restarts(cy3q_solver *s, int64_t n_restarts,
+                                            int64_t n_iter_per_restart, int64_t m_history,
+                                            double perturb_scale, uint64_t seed,
+                                            double *best_sigma, double *history) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_restarts < 0 || n_iter_per_restart < 0)
+    return fail(CY3Q_ERR_ARG, "restart and iteration counts must be >= 0");
+  if (m_history < 1 || m_history > LBFGS_MAX_PAIRS - 1)
+    return fail(CY3Q_ERR_ARG, "m_history must be in 1..63");
+  HIP_TRY(hipSetDevice(s->device));
+  return restarts_run(
+      s, n_restarts, perturb_scale, seed,
+      [&] {
+        int64_t done = 0;
+        return lbfgs_run(s, n_iter_per_restart, (int)m_history, nullptr, &done);
+      },
+      best_sigma, history);
 }
 
 int cy3q_solver_sigma_ext(cy3q_solver *s, double *sigma_l1, double *kappa, double *sigma_l2sq,

@@ -76,6 +76,11 @@ def _load():
     L.cy3q_solver_sigma_sq_grad.argtypes = [vp, C.POINTER(dbl), vp, C.POINTER(dbl)]
     L.cy3q_solver_sigma_refine_analytic.argtypes = [vp, i64, dbl, vp, C.POINTER(i64)]
     L.cy3q_solver_set_adam.argtypes = [vp, dbl, dbl, dbl]
+    L.cy3q_solver_sigma_refine_lbfgs.argtypes = [vp, i64, i64, vp, C.POINTER(i64)]
+    L.cy3q_solver_sigma_refine_analytic_restarts.argtypes = [vp, i64, i64, dbl, dbl, u64,
+                                                             C.POINTER(dbl), vp]
+    L.cy3q_solver_sigma_refine_lbfgs_restarts.argtypes = [vp, i64, i64, i64, dbl, u64,
+                                                          C.POINTER(dbl), vp]
     L.cy3q_solver_get_h_block.argtypes = [vp, vp]
     L.cy3q_solver_set_h_block.argtypes = [vp, vp]
     L.cy3q_solver_get_t.argtypes = [vp, vp]
@@ -85,8 +90,6 @@ def _load():
     L.cy3q_solver_set_profiling.argtypes = [vp, ip]
     L.cy3q_solver_stage_times.argtypes = [vp, vp, vp, ip, C.POINTER(ip)]
     L.cy3q_solver_synchronize.argtypes = [vp]
-    for name in dir(L):
-        pass
     _lib = L
     return L
 
@@ -250,6 +253,41 @@ class QuinticSolver:
 
     def set_adam(self, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
         _check(_load().cy3q_solver_set_adam(self._h, beta1, beta2, eps))
+
+    def sigma_refine_lbfgs_gpu(self, n_iter: int, m_history: int) -> list:
+        """QuinticSolver::sigma_refine_lbfgs_gpu (quintic.rs:678-1094): L-BFGS on sigma_L2^2 with
+        Armijo backtracking and a steepest-descent retry; the lowest-sigma_L1 point is kept.
+        Returns the sigma_L1 history (the start, then one value per accepted step)."""
+        hist = np.zeros(n_iter + 1)
+        done = C.c_int64()
+        _check(_load().cy3q_solver_sigma_refine_lbfgs(self._h, n_iter, m_history, _ptr(hist),
+                                                      C.byref(done)))
+        return list(hist[: done.value])
+
+    def sigma_refine_analytic_with_restarts(self, n_restarts: int, n_iter_per_restart: int,
+                                            lr: float, perturb_scale: float, seed: int):
+        """QuinticSolver::sigma_refine_analytic_with_restarts (quintic.rs:545-582): Adam, then
+        n_restarts Adam runs from LCG-perturbed copies of the best h. Returns (best sigma_L1,
+        history of n_restarts + 1 values); the best h stays in the solver."""
+        hist = np.zeros(n_restarts + 1)
+        best = C.c_double()
+        _check(_load().cy3q_solver_sigma_refine_analytic_restarts(
+            self._h, n_restarts, n_iter_per_restart, lr, perturb_scale, seed, C.byref(best),
+            _ptr(hist)))
+        return best.value, list(hist)
+
+    # The reference's GPU form (quintic.rs:1145-1182) runs the same loop.
+    sigma_refine_analytic_with_restarts_gpu = sigma_refine_analytic_with_restarts
+
+    def sigma_refine_lbfgs_with_restarts_gpu(self, n_restarts: int, n_iter_per_restart: int,
+                                             m_history: int, perturb_scale: float, seed: int):
+        """QuinticSolver::sigma_refine_lbfgs_with_restarts_gpu (quintic.rs:1100-1139)."""
+        hist = np.zeros(n_restarts + 1)
+        best = C.c_double()
+        _check(_load().cy3q_solver_sigma_refine_lbfgs_restarts(
+            self._h, n_restarts, n_iter_per_restart, m_history, perturb_scale, seed,
+            C.byref(best), _ptr(hist)))
+        return best.value, list(hist)
 
     # ---- workspace access ---------------------------------------------------------------
     @property

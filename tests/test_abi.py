@@ -42,6 +42,15 @@ def test_bad_arguments_fail_loudly(cy3q):
         q.QuinticSolver.new_with_sampler(3, 0, 1)
     with pytest.raises(q.Cy3qError):
         q.QuinticSolver._new(3, 100, 1, 0.5, q.SAMPLER_SZ)
+    # The refinement drivers check their handle before touching the device.
+    L = q._load()
+    import ctypes as C
+    best = C.c_double()
+    assert L.cy3q_solver_sigma_refine_lbfgs(None, 5, 4, None, None) == -1
+    assert L.cy3q_solver_sigma_refine_analytic_restarts(None, 1, 1, 1e-3, 0.01, 1,
+                                                        C.byref(best), None) == -1
+    assert L.cy3q_solver_sigma_refine_lbfgs_restarts(None, 1, 1, 4, 0.01, 1, C.byref(best),
+                                                     None) == -1
 
 
 def test_laplacian_basis_table_bit_exact(cy3q, oracle):
