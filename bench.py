@@ -299,6 +299,15 @@ def bench_adjoint(args) -> int:
     s.synchronize()
     t_adam = time.perf_counter() - t0
     _log(f"sigma_refine_analytic: {t_adam:.2f} s, sigma {hist[0]:.5f} -> {hist[-1]:.5f}")
+    # (e) sigma_refine_lbfgs_gpu: 30 L-BFGS iterations with 10 pairs from the same start.
+    s.h_block = h0
+    s.synchronize()
+    t0 = time.perf_counter()
+    hist_lb = s.sigma_refine_lbfgs_gpu(30, 10)
+    s.synchronize()
+    t_lbfgs = time.perf_counter() - t0
+    _log(f"sigma_refine_lbfgs_gpu: {t_lbfgs:.2f} s, {len(hist_lb) - 1} accepted steps, "
+         f"sigma {hist_lb[0]:.5f} -> {min(hist_lb):.5f}")
     n = s.n_points
     out = {
         "metric": "sigma_L2^2 gradient evaluations per second, k=5 quotient (N=125), 400k pts",
@@ -309,7 +318,8 @@ def bench_adjoint(args) -> int:
         "dtype": "f64",
         "data": "synthetic: SZ line-intersection sample of the Fermat quintic, LCG seed 42",
         "config": {"workload": f"sigma^2 adjoint at the 10-iteration balanced h, k={k}, N={nb}, "
-                               f"N_p={n}; FD check 20 directions; 200 Adam steps lr 1e-3",
+                               f"N_p={n}; FD check 20 directions; 200 Adam steps lr 1e-3; "
+                               f"30 L-BFGS iterations, 10 pairs",
                    "k": k, "n_basis": nb, "n_points": n},
         "grad_ms": t_grad * 1e3,
         "fd_max_rel_err": max(fd_err), "fd_median_rel_err": float(np.median(fd_err)),
@@ -319,6 +329,9 @@ def bench_adjoint(args) -> int:
         "sigma_refine_analytic_s": t_adam,
         "sigma_refine_analytic_sigma_first_last": [hist[0], hist[-1]] if hist else None,
         "adam_steps_per_s": 200.0 / t_adam,
+        "lbfgs_30_iters_m10_s": t_lbfgs,
+        "lbfgs_accepted_steps": len(hist_lb) - 1,
+        "lbfgs_sigma_first_best": [hist_lb[0], min(hist_lb)] if hist_lb else None,
     }
     print(json.dumps(out))
     return 0
