@@ -36,9 +36,11 @@ extern "C" {
 #define CY3Q_ERR_EMPTY (-4)  /* no point sampled: the reference returns None (quintic.rs:203-205) */
 #define CY3Q_ERR_STATE (-5)  /* call out of order */
 
-/* Sampler kinds (cy3/src/quintic.rs:3613-3616; NewtonProjection is not provided). */
-#define CY3Q_SAMPLER_SZ 1       /* sample_quintic_points_sz, quintic.rs:3633-3785 */
-#define CY3Q_SAMPLER_SZ_DWORK 2 /* sample_deformed_quintic_points_sz, quintic.rs:3810-3983 */
+/* Sampler kinds (cy3/src/quintic.rs:3613-3616). */
+#define CY3Q_SAMPLER_NEWTON 0       /* sample_quintic_points, quintic.rs:3568-3595 */
+#define CY3Q_SAMPLER_SZ 1           /* sample_quintic_points_sz, quintic.rs:3633-3785 */
+#define CY3Q_SAMPLER_SZ_DWORK 2     /* sample_deformed_quintic_points_sz, quintic.rs:3810-3983 */
+#define CY3Q_SAMPLER_NEWTON_DWORK 3 /* sample_deformed_quintic_points, quintic.rs:3448-3482 */
 
 /* Arithmetic of the T-operator contractions. */
 #define CY3Q_PREC_FP64 0    /* plain FP64 MFMA */
@@ -51,11 +53,13 @@ typedef struct cy3q_options {
   int device;        /* HIP device ordinal */
   int dk_iters;      /* Durand–Kerner sweeps per line (reference: 50) */
   int newton_polish; /* FP64 Newton steps per root after DK (reference: 0) */
+  double newton_tol; /* tolerance of the Newton-projection samplers (the reference's newton_tol
+                        argument; default 1e-12) */
 } cy3q_options;
 
 typedef struct cy3q_solver cy3q_solver;
 
-/* Fills the defaults: raw basis, FP64, device 0, 50 DK sweeps, no polish. */
+/* Fills the defaults: raw basis, FP64, device 0, 50 DK sweeps, no polish, newton_tol 1e-12. */
 void cy3q_options_default(cy3q_options *opt);
 const char *cy3q_version(void);
 const char *cy3q_last_error(void);
@@ -65,10 +69,14 @@ const char *cy3q_last_error(void);
 int cy3q_n_sections(uint32_t k, int quotient, int64_t *n);
 int cy3q_monomials(uint32_t k, int quotient, uint32_t *out, int64_t cap, int64_t *n);
 
-/* QuinticSolver::new_with_sampler(k, n, seed, tol, ShiffmanZelditch) when sampler is
- * CY3Q_SAMPLER_SZ (psi must be 0), QuinticSolver::new_with_psi_sz(k, n, seed, psi) when
- * CY3Q_SAMPLER_SZ_DWORK (quintic.rs:241-322). Samples, weights, log|Omega|^2 and sections are
- * computed on the GPU. h starts at the identity. */
+/* The QuinticSolver constructors (quintic.rs:174-322), by sampler:
+ *   CY3Q_SAMPLER_NEWTON        QuinticSolver::new(k, n, seed, tol), equivalently
+ *                              new_with_sampler(.., NewtonProjection) (psi must be 0);
+ *   CY3Q_SAMPLER_SZ            new_with_sampler(k, n, seed, tol, ShiffmanZelditch) (psi 0);
+ *   CY3Q_SAMPLER_SZ_DWORK      new_with_psi_sz(k, n, seed, psi);
+ *   CY3Q_SAMPLER_NEWTON_DWORK  new_with_psi(k, n, seed, tol, psi).
+ * tol is opt->newton_tol. Samples, weights, log|Omega|^2 and sections are computed on the GPU.
+ * h starts at the identity. */
 int cy3q_solver_new(uint32_t k, int64_t n_points, uint64_t seed, double psi, int sampler,
                     const cy3q_options *opt, cy3q_solver **out);
 /* Same workspace built from caller-supplied points ([n][10], host memory). */
@@ -170,6 +178,17 @@ int cy3q_solver_get_t(cy3q_solver *s, double *t_cplx);
 int cy3q_solver_get_points(cy3q_solver *s, double *points);
 int cy3q_solver_get_weights(cy3q_solver *s, double *weights, double *log_omega);
 int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *section_derivs);
+/* Replaces the solver's weights (n doubles, host memory), as the reference's callers do by
+ * writing QuinticSolver::weights (e.g. quintic.rs:6631-6642). Later steps, sigma and gradients
+ * use them; the FS Gram already applied is not redone. */
+int cy3q_solver_set_weights(cy3q_solver *s, const double *weights);
+
+/* monge_ampere_residual_quintic(_weighted) (quintic.rs:4961-5049): sigma_L1 of the Fermat
+ * quintic metric h_block (2N x 2N block over the raw degree-k monomials, the legacy pipeline's
+ * basis) at the given points ([n][10], host), with the CY weights 1/|grad f|^2, or the given
+ * weights (n doubles) when weights is not NULL. NaN where the reference returns NaN. */
+int cy3q_monge_ampere_residual(uint32_t k, const double *points, int64_t n, const double *h_block,
+                               const double *weights, int device, double *sigma);
 
 /* Per-stage device time (milliseconds, from HIP events on the solver's stream) accumulated
  * since profiling was switched on: out[i] for stage i, count[i] launches. Stage ids are listed in

@@ -365,6 +365,80 @@ ORC_API int64_t orc_sample_sz(int64_t n, uint64_t seed, int dwork, double psi, i
     return np;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Newton-projection sampler (quintic.rs:3334-3482, 3568-3595)                                 */
+/* ------------------------------------------------------------------------------------------ */
+/* newton_project_to_quintic (:3334-3389) and newton_project_to_deformed_quintic (:3395-3443):
+ * identical at psi = 0, where the deformed forms return the Fermat values. Returns 1 and the
+ * projected z in place on acceptance. */
+static int newton_project(double *z, double psi, double tol, int max_iter) {
+    for (int it = 0; it < max_iter; it++) {
+        double f_re, f_im;
+        deformed_fermat_quintic(z, psi, &f_re, &f_im);
+        double f_mag_sq = f_re * f_re + f_im * f_im;
+        if (f_mag_sq < tol * tol) return 1;
+        double grad[10];
+        deformed_gradient(z, psi, grad);
+        double grad_norm_sq = 0.0;
+        for (int k = 0; k < 10; k++) grad_norm_sq += grad[k] * grad[k];
+        if (grad_norm_sq < 1e-14) return 0; /* singular locus */
+        double scale = -1.0 / grad_norm_sq;
+        for (int j = 0; j < 5; j++) {
+            double gr = grad[2 * j], gi = grad[2 * j + 1];
+            double dr = scale * (f_re * gr + f_im * gi);
+            double di = scale * (f_im * gr - f_re * gi);
+            z[2 * j] += dr;
+            z[2 * j + 1] += di;
+        }
+        double nrm_sq = 0.0;
+        for (int k = 0; k < 10; k++) nrm_sq += z[k] * z[k];
+        double nrm = sqrt(nrm_sq);
+        if (nrm < 1e-12) return 0;
+        for (int k = 0; k < 10; k++) z[k] /= nrm;
+    }
+    double f_re, f_im;
+    deformed_fermat_quintic(z, psi, &f_re, &f_im);
+    double t10 = 10.0 * tol;
+    return f_re * f_re + f_im * f_im < t10 * t10;
+}
+
+ORC_API int orc_newton_project(double *z, double psi, double tol, int max_iter) {
+    return newton_project(z, psi, tol, max_iter);
+}
+
+/* dwork = 0: sample_quintic_points (:3568-3595, max_tries = 4 n); dwork = 1:
+ * sample_deformed_quintic_points (:3448-3482, max_tries = 8 n). Each try draws 10 normals,
+ * normalises onto S^9 (a norm below 1e-12 skips the try) and keeps the projected point on
+ * success. out holds n * 10 doubles; returns the number of points written. acc_out (optional,
+ * max_tries ints) records each try's outcome. */
+ORC_API int64_t orc_sample_newton(int64_t n, uint64_t seed, int dwork, double psi, double tol,
+                                  double *out, int64_t *tries_out, int32_t *acc_out) {
+    orc_lcg rng;
+    lcg_init(&rng, seed);
+    int64_t max_tries = n * (dwork ? 8 : 4);
+    int64_t tried = 0, np = 0;
+    while (np < n && tried < max_tries) {
+        tried++;
+        double z[10];
+        for (int k = 0; k < 10; k++) z[k] = lcg_next_normal(&rng);
+        double nrm_sq = 0.0;
+        for (int k = 0; k < 10; k++) nrm_sq += z[k] * z[k];
+        double nrm = sqrt(nrm_sq);
+        int ok = 0;
+        if (!(nrm < 1e-12)) {
+            for (int k = 0; k < 10; k++) z[k] /= nrm;
+            ok = newton_project(z, psi, tol, 50);
+        }
+        if (acc_out) acc_out[tried - 1] = ok;
+        if (ok) {
+            memcpy(out + 10 * np, z, sizeof(double) * 10);
+            np++;
+        }
+    }
+    if (tries_out) *tries_out = tried;
+    return np;
+}
+
 /* Per-line candidate generator (test helper): the 5 candidates of each line in order with
  * accept flags; used to check a line-parallel sampler line by line. */
 ORC_API void orc_sz_lines(int64_t n_lines, uint64_t seed, int dwork, double psi, int dk_iters,

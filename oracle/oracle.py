@@ -48,6 +48,10 @@ def lib():
     L.orc_quintic_eval.argtypes = [_dp, i64, dbl, vp, vp]
     L.orc_sample_sz.argtypes = [i64, u64, C.c_int, dbl, C.c_int, _dp, C.POINTER(i64)]
     L.orc_sample_sz.restype = i64
+    L.orc_sample_newton.argtypes = [i64, u64, C.c_int, dbl, dbl, _dp, C.POINTER(i64), C.c_void_p]
+    L.orc_sample_newton.restype = i64
+    L.orc_newton_project.argtypes = [_dp, dbl, dbl, C.c_int]
+    L.orc_newton_project.restype = C.c_int
     L.orc_sz_lines.argtypes = [i64, u64, C.c_int, dbl, C.c_int, _dp, _i32p]
     L.orc_weights_logomega.argtypes = [_dp, i64, dbl, vp, vp, vp]
     L.orc_sections.argtypes = [_dp, i64, _u32p, i64, _dp, vp]
@@ -112,6 +116,27 @@ def sample_sz(n: int, seed: int, psi: float = 0.0, dwork: bool | None = None, dk
     tried = C.c_int64(0)
     got = lib().orc_sample_sz(n, seed, int(dwork), psi, dk_iters, out, C.byref(tried))
     return out[:got].copy(), int(tried.value)
+
+
+def sample_newton(n: int, seed: int, psi: float = 0.0, tol: float = 1e-12,
+                  dwork: bool | None = None):
+    """dwork=False: sample_quintic_points (QuinticSolver::new); dwork=True:
+    sample_deformed_quintic_points (QuinticSolver::new_with_psi). Returns (points, tries,
+    per-try accept flags)."""
+    if dwork is None:
+        dwork = psi != 0.0
+    out = np.zeros((n, 10))
+    tried = C.c_int64(0)
+    acc = np.zeros(n * (8 if dwork else 4), np.int32)
+    got = lib().orc_sample_newton(n, seed, int(dwork), psi, tol, out, C.byref(tried), _ptr(acc))
+    return out[:got].copy(), int(tried.value), acc[: tried.value].copy()
+
+
+def newton_project(z, psi: float = 0.0, tol: float = 1e-12, max_iter: int = 50):
+    """newton_project_to_(deformed_)quintic (quintic.rs:3334-3443): (accepted, projected z)."""
+    zz = np.array(z, dtype=np.float64)
+    ok = lib().orc_newton_project(zz, psi, tol, max_iter)
+    return bool(ok), zz
 
 
 def sz_lines(n_lines: int, seed: int, psi: float = 0.0, dwork: bool = False, dk_iters: int = 50):

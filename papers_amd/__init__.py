@@ -9,11 +9,14 @@ cannot load, importing ``papers_amd.quintic`` raises.
 from .quintic import (  # noqa: F401
     PREC_FP64,
     PREC_SPLIT16,
+    SAMPLER_NEWTON,
+    SAMPLER_NEWTON_DWORK,
     SAMPLER_SZ,
     SAMPLER_SZ_DWORK,
     Cy3qError,
     QuinticSolver,
     build_degree_k_quintic_monomials,
     lib_path,
+    monge_ampere_residual_quintic,
     n_sections_quintic,
 )

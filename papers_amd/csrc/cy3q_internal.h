@@ -43,6 +43,8 @@ enum {
 // cand: [n_lines][5][10] doubles, acc: [n_lines][5] int32 accept flags.
 void launch_sz_lines(hipStream_t st, uint64_t seed, int64_t line0, int64_t n_lines, int dwork,
                      double psi, int dk_iters, int newton_polish, double *cand, int32_t *acc);
+void launch_newton_tries(hipStream_t st, uint64_t seed, int64_t try0, int64_t n_tries, double psi,
+                         double tol, double *cand, int32_t *acc);
 // Counts accepted candidates per block of `block` lines; writes block sums.
 void launch_scan_accept(hipStream_t st, const int32_t *acc, int64_t n_cand, int64_t *pos,
                         int64_t *block_sums, int64_t *scratch_sums, int64_t *total);

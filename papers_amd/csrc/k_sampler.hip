@@ -6,7 +6,12 @@
 // advanced 40*l steps. The lane reaches that state by affine jump-ahead, which makes every line
 // independent and reproduces the reference's sequential draws bit for bit. Accepted candidates
 // are compacted in (line, root) order by an exclusive scan, which keeps the reference's point order.
+//
+// The Newton-projection sampler (sample_quintic_points, quintic.rs:3568-3595, and
+// sample_deformed_quintic_points, :3448-3482) runs one try per lane the same way: a try consumes
+// 20 draws (10 normals), so try t starts 20*t draws into the stream.
 #include "cy3q_lcg.h"
+#include "cy3q_point.h"
 
 namespace cy3q {
 
@@ -211,6 +216,74 @@ void launch_sz_lines(hipStream_t st, uint64_t seed, int64_t line0, int64_t n_lin
   int64_t blocks = (n_lines + 255) / 256;
   hipLaunchKernelGGL(k_sz_lines, dim3((unsigned)blocks), dim3(256), 0, st, seed, line0, n_lines,
                      dwork, psi, dk_iters, newton_polish, cand, acc);
+}
+
+// ---------------------------------------------------------------------------------------
+// Newton projection (newton_project_to_quintic, quintic.rs:3334-3389, and its deformed form
+// newton_project_to_deformed_quintic, :3395-3443; psi = 0 is the Fermat form exactly): try t
+// draws 10 normals, normalises them onto S^9 (a norm below 1e-12 consumes the try with no point)
+// and takes up to 50 steps z <- (z - f conj(grad f)/|grad f|^2)/|.|, accepting at |f| < tol
+// inside the loop or |f| < 10 tol after it, and rejecting at |grad f|^2 < 1e-14 or a vanishing
+// norm. Operation order and divisions as the reference; the file builds with
+// -ffp-contract=off.
+// ---------------------------------------------------------------------------------------
+__device__ inline bool newton_project(double *z, double psi, double tol) {
+  const int dwork = psi != 0.0;
+  for (int it = 0; it < 50; it++) {
+    double fre, fim;
+    quintic_value(z, dwork, psi, fre, fim);
+    if (fre * fre + fim * fim < tol * tol) return true;
+    double g[10];
+    grad_quintic(z, psi, g);
+    double gn = 0.0;
+    for (int k = 0; k < 10; k++) gn += g[k] * g[k];
+    if (gn < 1e-14) return false;
+    const double scale = -1.0 / gn;
+    for (int j = 0; j < 5; j++) {
+      const double gr = g[2 * j], gi = g[2 * j + 1];
+      const double dr = scale * (fre * gr + fim * gi);
+      const double di = scale * (fim * gr - fre * gi);
+      z[2 * j] += dr;
+      z[2 * j + 1] += di;
+    }
+    double nsq = 0.0;
+    for (int k = 0; k < 10; k++) nsq += z[k] * z[k];
+    const double nrm = sqrt(nsq);
+    if (nrm < 1e-12) return false;
+    for (int k = 0; k < 10; k++) z[k] /= nrm;
+  }
+  double fre, fim;
+  quintic_value(z, dwork, psi, fre, fim);
+  const double t10 = 10.0 * tol;
+  return fre * fre + fim * fim < t10 * t10;
+}
+
+__global__ void __launch_bounds__(256) k_newton_tries(uint64_t seed, int64_t try0, int64_t n_tries,
+                                                      double psi, double tol,
+                                                      double *__restrict__ cand,
+                                                      int32_t *__restrict__ acc) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_tries) return;
+  Lcg rng{lcg_jump(lcg_seed(seed), 20ULL * (uint64_t)(try0 + l))};
+  double z[10];
+  for (int k = 0; k < 10; k++) z[k] = rng.next_normal();
+  double nsq = 0.0;
+  for (int k = 0; k < 10; k++) nsq += z[k] * z[k];
+  const double nrm = sqrt(nsq);
+  bool ok = false;
+  if (!(nrm < 1e-12)) {
+    for (int k = 0; k < 10; k++) z[k] /= nrm;
+    ok = newton_project(z, psi, tol);
+  }
+  acc[l] = ok ? 1 : 0;
+  for (int q = 0; q < 10; q++) cand[10 * l + q] = z[q];
+}
+
+void launch_newton_tries(hipStream_t st, uint64_t seed, int64_t try0, int64_t n_tries, double psi,
+                         double tol, double *cand, int32_t *acc) {
+  if (n_tries <= 0) return;
+  hipLaunchKernelGGL(k_newton_tries, dim3((unsigned)((n_tries + 255) / 256)), dim3(256), 0, st,
+                     seed, try0, n_tries, psi, tol, cand, acc);
 }
 
 // ---------------------------------------------------------------------------------------

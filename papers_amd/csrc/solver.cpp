@@ -406,6 +406,47 @@ int sample_points(cy3q_solver *s, int64_t n_target, uint64_t seed, int dwork) {
   return CY3Q_OK;
 }
 
+// sample_quintic_points / sample_deformed_quintic_points (quintic.rs:3448-3482, 3568-3595):
+// tries in order until n_target points are accepted or the reference's try cap (4 n, 8 n for the
+// deformed form) is reached. Each batch of tries is projected in parallel, then the accepted
+// points are compacted in try order.
+int sample_points_newton(cy3q_solver *s, int64_t n_target, uint64_t seed, double tol, int dwork) {
+  const int64_t max_tries = n_target * (dwork ? 8 : 4);
+  int64_t batch = std::max<int64_t>(1024, n_target + n_target / 16 + 64);
+  batch = std::min(batch, max_tries);
+  double *cand = nullptr;
+  int32_t *acc = nullptr;
+  int64_t *pos = nullptr, *bsum = nullptr, *total = nullptr;
+  ST_TRY(s->alloc(&s->pts, n_target * 10));
+  ST_TRY(s->alloc(&cand, batch * 10));
+  ST_TRY(s->alloc(&acc, batch));
+  ST_TRY(s->alloc(&pos, batch));
+  ST_TRY(s->alloc(&bsum, (batch + 1023) / 1024 + 1));
+  ST_TRY(s->alloc(&total, 1));
+  int64_t have = 0, try0 = 0;
+  s->stage_begin(STG_SAMPLE);
+  while (have < n_target && try0 < max_tries) {
+    const int64_t nt = std::min(batch, max_tries - try0);
+    launch_newton_tries(s->st, seed, try0, nt, s->psi, tol, cand, acc);
+    launch_scan_accept(s->st, acc, nt, pos, bsum, nullptr, total);
+    launch_compact_points(s->st, cand, acc, pos, nt, have, n_target, s->pts);
+    HIP_TRY(hipGetLastError());
+    int64_t got = 0;
+    HIP_TRY(hipMemcpyAsync(&got, total, sizeof(int64_t), hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    have = std::min(n_target, have + got);
+    try0 += nt;
+  }
+  s->stage_end(STG_SAMPLE);
+  s->release(cand);
+  s->release(acc);
+  s->release(pos);
+  s->release(bsum);
+  s->release(total);
+  s->n = have;
+  return CY3Q_OK;
+}
+
 // T-operator pieces -------------------------------------------------------------------------
 // B(W) for the triangular contraction, rebuilt lazily after W changes.
 void ensure_bmat(cy3q_solver *s) {
@@ -798,6 +839,7 @@ void cy3q_options_default(cy3q_options *opt) {
   opt->device = 0;
   opt->dk_iters = 50;
   opt->newton_polish = 0;
+  opt->newton_tol = 1e-12;
 }
 
 const char *cy3q_version(void) { return "cy3q 0.1 (gfx950)"; }
@@ -824,13 +866,20 @@ int cy3q_monomials(uint32_t k, int quotient, uint32_t *out, int64_t cap, int64_t
 int cy3q_solver_new(uint32_t k, int64_t n_points, uint64_t seed, double psi, int sampler,
                     const cy3q_options *opt, cy3q_solver **out) {
   if (n_points <= 0) return fail(CY3Q_ERR_ARG, "n_points must be > 0");
-  if (sampler != CY3Q_SAMPLER_SZ && sampler != CY3Q_SAMPLER_SZ_DWORK)
-    return fail(CY3Q_ERR_ARG, "unsupported sampler (NewtonProjection is not provided)");
-  if (sampler == CY3Q_SAMPLER_SZ && psi != 0.0)
-    return fail(CY3Q_ERR_ARG, "the Fermat SZ sampler takes psi = 0; use CY3Q_SAMPLER_SZ_DWORK");
+  if (sampler != CY3Q_SAMPLER_SZ && sampler != CY3Q_SAMPLER_SZ_DWORK &&
+      sampler != CY3Q_SAMPLER_NEWTON && sampler != CY3Q_SAMPLER_NEWTON_DWORK)
+    return fail(CY3Q_ERR_ARG, "unknown sampler");
+  if ((sampler == CY3Q_SAMPLER_SZ || sampler == CY3Q_SAMPLER_NEWTON) && psi != 0.0)
+    return fail(CY3Q_ERR_ARG, "the Fermat samplers take psi = 0; use a Dwork sampler");
+  const bool newton = sampler == CY3Q_SAMPLER_NEWTON || sampler == CY3Q_SAMPLER_NEWTON_DWORK;
+  const double tol = opt ? opt->newton_tol : 1e-12;
+  if (newton && !(tol > 0.0 && std::isfinite(tol)))
+    return fail(CY3Q_ERR_ARG, "newton_tol must be finite and > 0");
   cy3q_solver *s = nullptr;
   ST_TRY(make_solver(k, psi, opt, out, &s));
-  int rc = sample_points(s, n_points, seed, sampler == CY3Q_SAMPLER_SZ_DWORK ? 1 : 0);
+  const int dwork = sampler == CY3Q_SAMPLER_SZ_DWORK || sampler == CY3Q_SAMPLER_NEWTON_DWORK;
+  int rc = newton ? sample_points_newton(s, n_points, seed, tol, dwork)
+                  : sample_points(s, n_points, seed, dwork);
   if (rc == CY3Q_OK && s->n == 0) rc = fail(CY3Q_ERR_EMPTY, "no point was accepted");
   if (rc == CY3Q_OK) rc = finish_workspace(s);
   if (rc != CY3Q_OK) {
@@ -1474,6 +1523,32 @@ int cy3q_solver_get_weights(cy3q_solver *s, double *weights, double *log_omega) 
                            s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
   return CY3Q_OK;
+}
+
+int cy3q_solver_set_weights(cy3q_solver *s, const double *weights) {
+  if (!s || !weights) return fail(CY3Q_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * s->n, hipMemcpyHostToDevice, s->st));
+  launch_valid_weights(s->st, s->w, s->n, s->wvalid);
+  launch_sum_f64(s->st, s->wvalid, s->n, s->sum_w);
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_monge_ampere_residual(uint32_t k, const double *points, int64_t n, const double *h_block,
+                               const double *weights, int device, double *sigma) {
+  if (!points || !h_block || !sigma) return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (n <= 0) return fail(CY3Q_ERR_EMPTY, "no points");
+  cy3q_options opt;
+  cy3q_options_default(&opt);
+  opt.device = device;
+  cy3q_solver *s = nullptr;
+  ST_TRY(cy3q_solver_new_from_points(k, points, n, 0.0, &opt, &s));
+  int rc = cy3q_solver_set_h_block(s, h_block);
+  if (rc == CY3Q_OK && weights) rc = cy3q_solver_set_weights(s, weights);
+  if (rc == CY3Q_OK) rc = cy3q_solver_sigma(s, sigma);
+  cy3q_solver_free(s);
+  return rc;
 }
 
 int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *section_derivs) {

@@ -18,8 +18,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libcy3q.so")
 
+SAMPLER_NEWTON = 0
 SAMPLER_SZ = 1
 SAMPLER_SZ_DWORK = 2
+SAMPLER_NEWTON_DWORK = 3
 PREC_FP64 = 0
 PREC_SPLIT16 = 1
 
@@ -34,7 +36,7 @@ class Cy3qError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("quotient", C.c_int), ("precision", C.c_int), ("device", C.c_int),
-                ("dk_iters", C.c_int), ("newton_polish", C.c_int)]
+                ("dk_iters", C.c_int), ("newton_polish", C.c_int), ("newton_tol", C.c_double)]
 
 
 def lib_path() -> str:
@@ -87,6 +89,8 @@ def _load():
     L.cy3q_solver_get_points.argtypes = [vp, vp]
     L.cy3q_solver_get_weights.argtypes = [vp, vp, vp]
     L.cy3q_solver_get_sections.argtypes = [vp, vp, vp]
+    L.cy3q_solver_set_weights.argtypes = [vp, vp]
+    L.cy3q_monge_ampere_residual.argtypes = [u32, vp, i64, vp, vp, ip, C.POINTER(dbl)]
     L.cy3q_solver_set_profiling.argtypes = [vp, ip]
     L.cy3q_solver_stage_times.argtypes = [vp, vp, vp, ip, C.POINTER(ip)]
     L.cy3q_solver_synchronize.argtypes = [vp]
@@ -119,11 +123,13 @@ def build_degree_k_quintic_monomials(k: int, quotient: bool = False) -> np.ndarr
     return out
 
 
-def _options(quotient=False, precision=PREC_FP64, device=0, dk_iters=50, newton_polish=0):
+def _options(quotient=False, precision=PREC_FP64, device=0, dk_iters=50, newton_polish=0,
+             newton_tol=1e-12):
     o = Options()
     _load().cy3q_options_default(C.byref(o))
     o.quotient, o.precision, o.device = int(quotient), int(precision), int(device)
     o.dk_iters, o.newton_polish = int(dk_iters), int(newton_polish)
+    o.newton_tol = float(newton_tol)
     return o
 
 
@@ -139,10 +145,24 @@ class QuinticSolver:
 
     # ---- constructors -------------------------------------------------------------------
     @classmethod
+    def new(cls, k_degree: int, n_points: int, seed: int, newton_tol: float = 1e-12, **opts):
+        """QuinticSolver::new (quintic.rs:174-182): the Newton-projection sampler."""
+        return cls._new(k_degree, n_points, seed, 0.0, SAMPLER_NEWTON, newton_tol=newton_tol,
+                        **opts)
+
+    @classmethod
+    def new_with_psi(cls, k_degree: int, n_points: int, seed: int, newton_tol: float, psi: float,
+                     **opts):
+        """QuinticSolver::new_with_psi (quintic.rs:184-221): Newton projection onto the Dwork
+        quintic at psi."""
+        return cls._new(k_degree, n_points, seed, psi, SAMPLER_NEWTON_DWORK,
+                        newton_tol=newton_tol, **opts)
+
+    @classmethod
     def new_with_sampler(cls, k_degree: int, n_points: int, seed: int, newton_tol: float = 1e-12,
                          sampler: int = SAMPLER_SZ, **opts):
-        """QuinticSolver::new_with_sampler (quintic.rs:281-322). Only the SZ sampler exists."""
-        return cls._new(k_degree, n_points, seed, 0.0, sampler, **opts)
+        """QuinticSolver::new_with_sampler (quintic.rs:281-322): SAMPLER_NEWTON or SAMPLER_SZ."""
+        return cls._new(k_degree, n_points, seed, 0.0, sampler, newton_tol=newton_tol, **opts)
 
     @classmethod
     def new_with_psi_sz(cls, k_degree: int, n_points: int, seed: int, psi: float, **opts):
@@ -343,5 +363,26 @@ class QuinticSolver:
         L = _load()
         return {L.cy3q_stage_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(ns.value)}
 
+    def set_weights(self, weights: np.ndarray):
+        """Replace QuinticSolver::weights (later steps, sigma and gradients use them)."""
+        w = np.ascontiguousarray(weights, np.float64)
+        if w.shape != (self.n_points,):
+            raise ValueError(f"weights must have shape ({self.n_points},)")
+        _check(_load().cy3q_solver_set_weights(self._h, _ptr(w)))
+
     def synchronize(self):
         _check(_load().cy3q_solver_synchronize(self._h))
+
+
+def monge_ampere_residual_quintic(points: np.ndarray, h_block: np.ndarray, k_degree: int,
+                                  weights: np.ndarray | None = None, device: int = 0) -> float:
+    """monge_ampere_residual_quintic(_weighted) (quintic.rs:4961-5049): sigma_L1 of the Fermat
+    quintic metric h_block over the raw degree-k monomials at the given points, with the CY
+    weights 1/|grad f|^2 or the given weights."""
+    pts = np.ascontiguousarray(points, np.float64)
+    h = np.ascontiguousarray(h_block, np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    out = C.c_double()
+    _check(_load().cy3q_monge_ampere_residual(k_degree, _ptr(pts), pts.shape[0], _ptr(h), _ptr(w),
+                                              device, C.byref(out)))
+    return out.value

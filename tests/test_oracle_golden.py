@@ -315,3 +315,67 @@ def test_legacy_weighted_step_is_the_uninverted_T(oracle):
     T *= P.nb / np.trace(T).real
     assert np.abs(hu[0::2, 0::2] - T.real).max() <= 1e-12 * np.abs(T).max()
     assert np.abs(hu[1::2, 0::2] - T.imag).max() <= 1e-12 * np.abs(T).max()
+
+
+# --------------------------------------------------------------------------------------------
+# Newton-projection sampler (quintic.rs:3334-3482, 3568-3595) and the legacy Monge-Ampere
+# residual pipeline (quintic.rs:4224-4250, 4961-5049)
+# --------------------------------------------------------------------------------------------
+def test_newton_projects_random_to_quintic(oracle):
+    # The reference's newton_projects_random_to_quintic (quintic.rs:5497-5519): the first LCG
+    # draws of seed 42, projected with tol 1e-10, leave |f|^2 < 1e-15.
+    pts, tried, _ = oracle.sample_newton(1, 42, tol=1e-10)
+    assert tried == 1 and pts.shape == (1, 10)
+    f, _ = oracle.quintic_eval(pts)
+    assert f[0, 0] ** 2 + f[0, 1] ** 2 < 1e-15
+
+
+def test_sampled_points_lie_on_quintic(oracle):
+    # The reference's sampled_points_lie_on_quintic (quintic.rs:5521-5534).
+    pts, _, _ = oracle.sample_newton(50, 7, tol=1e-10)
+    assert pts.shape == (50, 10)
+    f, _ = oracle.quintic_eval(pts)
+    assert np.all(f[:, 0] ** 2 + f[:, 1] ** 2 < 1e-15)
+    assert np.allclose(np.sum(pts * pts, axis=1), 1.0, atol=1e-14)
+
+
+def test_newton_draw_order_and_normalisation(oracle):
+    # With tol = 10 every start point is accepted before any step (|f| <= sum |z_j|^5 <= 1 on
+    # the unit sphere), so the sampler returns try t's 10 normals (LCG draws 20t .. 20t + 19,
+    # Box-Muller pairs) divided by their norm, bit for bit.
+    n = 6
+    pts, tried, acc = oracle.sample_newton(n, 99, tol=10.0)
+    assert tried == n and acc.all()
+    _, _, normals = oracle.lcg_stream(99, 10 * n)
+    z = normals.reshape(n, 10)
+    nrm = np.sqrt(np.array([sum(float(v) * float(v) for v in row) for row in z]))
+    assert np.array_equal(pts, z / nrm[:, None])
+
+
+def test_newton_projection_fixed_point_and_dwork(oracle):
+    # A point on the variety is returned unchanged; the Dwork form at psi = 0 is the Fermat one.
+    z = np.array([1, 0, 0, 0, 0, 0, 0, 0, -1, 0], float) / np.sqrt(2.0)
+    ok, zp = oracle.newton_project(z, 0.0, 1e-12)
+    assert ok and np.array_equal(zp, z)
+    a, _, _ = oracle.sample_newton(40, 5, psi=0.0, dwork=False)
+    b, _, _ = oracle.sample_newton(40, 5, psi=0.0, dwork=True)
+    assert np.array_equal(a, b)
+    # Dwork psi = 0.5: every point satisfies the deformed equation.
+    c, _, _ = oracle.sample_newton(200, 5, psi=0.5)
+    f, _ = oracle.quintic_eval(c, 0.5)
+    assert np.all(np.hypot(f[:, 0], f[:, 1]) < 1e-11)
+
+
+def test_monge_ampere_pipeline_in_published_bounds(oracle):
+    # monge_ampere_residual_quintic_in_published_bounds_at_k2 (quintic.rs:5556-5580): Newton
+    # points (1500, seed 23, tol 1e-10), raw k=2 basis, 30 un-inverted Donaldson iterations
+    # (tol 1e-6), then sigma_L1 with the CY weights inside PublishedQuinticBounds k=2
+    # ([0.05, 1.0], quintic.rs:5089-5098).
+    pts, _, _ = oracle.sample_newton(1500, 23, tol=1e-10)
+    mono = oracle.monomials(2)
+    sv, sd = oracle.sections(pts, mono)
+    w, log_om, _ = oracle.weights_logomega(pts)
+    h, res = oracle.legacy_solve(sv, 30, 1e-6)
+    assert res[-1] < res[0]
+    s = oracle.sigma(pts, sv, sd, log_om, w, h)
+    assert np.isfinite(s) and 0.05 <= s <= 1.0, s
