@@ -182,6 +182,11 @@ int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *sec
  * writing QuinticSolver::weights (e.g. quintic.rs:6631-6642). Later steps, sigma and gradients
  * use them; the FS Gram already applied is not redone. */
 int cy3q_solver_set_weights(cy3q_solver *s, const double *weights);
+/* Arithmetic of the Galerkin rank-k updates of cy3q_solver_laplacian_spectrum: CY3Q_PREC_FP64
+ * (default; the reference's FP64 sums, compute_metric_laplacian_spectrum, metric_laplacian.rs:
+ * 296-418) or CY3Q_PREC_SPLIT16 (fp16 hi/lo split MFMA with FP32 partials over at most 4096
+ * points, summed in FP64). The eigensolve is FP64 either way. No reference counterpart. */
+int cy3q_solver_set_galerkin_precision(cy3q_solver *s, int precision);
 
 /* monge_ampere_residual_quintic(_weighted) (quintic.rs:4961-5049): sigma_L1 of the Fermat
  * quintic metric h_block (2N x 2N block over the raw degree-k monomials, the legacy pipeline's

@@ -35,6 +35,7 @@ SOURCES = {
     "k_adjoint.hip": [],
     "k_lbfgs.hip": [],
     "k_laplacian.hip": [],
+    "k_herk_split.hip": [],
     "k_eigen.hip": [],
     "k_yukawa.hip": [],
     "solver.cpp": ["-x", "hip"],

@@ -164,6 +164,18 @@ void launch_lap_rows_ambient(hipStream_t st, const double *pts, const double *w,
                              const double *omega, int64_t n, const uint8_t *exps, int nb,
                              int64_t mc, double *x0, double *x1, double *wv);
 int herk_max_buffers();
+// Split-precision rank-k update of the Galerkin assembly (k_herk_split.hip).
+// One power-of-two range for the rows of two arrays: scale[0] applied, scale[1] restores products.
+void launch_rows_scale_pair(hipStream_t st, const double *a, int64_t na, const double *b,
+                            int64_t nb, double *part, double *scale);
+int rows_scale_parts();
+void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t stages, int64_t mc,
+                       const double *scale, half8_t *xh, half8_t *xl);
+void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,
+                         double *const *c, int nbuf, int64_t stages, int64_t mc, int nparts,
+                         const double *scale);
+void launch_sum_parts(hipStream_t st, double *c, int64_t len, int nparts);
+int herk_split_tile();
 void launch_herk_f64(hipStream_t st, const double *const *x, const int64_t *rows, double *const *c,
                      int nbuf, int64_t mc);
 void launch_lap_finalize(hipStream_t st, const double *cmass, const double *const *ck, int nk,
@@ -241,6 +253,7 @@ struct SolverView {
   double psi = 0.0;
   const double *pts = nullptr, *w = nullptr, *log_om = nullptr, *sv = nullptr, *sd = nullptr;
   const double *bmat = nullptr;  // real-ified W = L^-1 of the current h (W s gives K = |W s|^2)
+  int galerkin_precision = 0;    // CY3Q_PREC_* of the Galerkin rank-k updates
 };
 
 }  // namespace cy3q

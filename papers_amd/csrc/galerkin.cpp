@@ -94,18 +94,19 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // Per-stage device time of the calling thread's last Galerkin call (HIP events on its stream).
 enum GStage {
   GS_ROWS = 0,   // per-point metric factor and row generation
-  GS_HERK,       // FP64 MFMA Hermitian rank-k accumulation
+  GS_HERK,       // Hermitian rank-k accumulation (FP64 MFMA, or the fp16 split)
   GS_FINALIZE,   // complex M, K from the real accumulators
   GS_EIGEN_M,    // factorisation of M: inverse Cholesky, or eigenvectors when M is singular
   GS_REDUCE,     // M^-1/2 reduction GEMMs (or M^-1 K)
   GS_EIGEN_C,    // eigensolve of the reduced pencil
   GS_YUK_MODES,  // mode values psi = S C
   GS_YUK_TRIPLES,
+  GS_SPLIT,      // row range and fp16 hi/lo planes of the split rank-k update
   GS_COUNT
 };
 const char *kGStageNames[GS_COUNT] = {"rows", "herk", "finalize", "factor_mass",
                                       "reduce", "eigen_reduced", "yukawa_modes",
-                                      "yukawa_triples"};
+                                      "yukawa_triples", "split_rows"};
 thread_local double g_stage_ms[GS_COUNT];
 thread_local int64_t g_stage_cnt[GS_COUNT];
 
@@ -587,10 +588,24 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   const int B = ns * ns;
   if (basis_dim) *basis_dim = B;
   const int64_t n = v.n, m = v.m;
-  const int64_t mc = round_up(2 * (int64_t)B, 128);
+  const bool split = v.galerkin_precision == CY3Q_PREC_SPLIT16;
+  // The split update tiles the columns by 256; the FP64 one by 128.
+  const int64_t mc = round_up(2 * (int64_t)B, split ? herk_split_tile() : 128);
   const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(256, (int64_t)(6e8 / (4.0 * mc))));
+  // Split path: the point parts of each tile's sum (one accumulator copy each), enough
+  // workgroups for about three per CU.
+  int nparts = 1;
+  if (split) {
+    int n_cu = 0;
+    G_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+    const int64_t nt = mc / herk_split_tile(), tiles = 4 * nt * (nt + 1) / 2;
+    nparts = (int)std::min<int64_t>(8, std::max<int64_t>(1, (3 * (int64_t)n_cu + tiles - 1) / tiles));
+  }
+  const int64_t stages = (chunk + 15) / 16;
   uint8_t *d_sm;
   double *y0, *yd, *pr, *wv, *tw, *x0, *x1, *cmass, *ck[3];
+  double *part = nullptr, *scale = nullptr;
+  half8_t *planes = nullptr;
   G_TRY(c.upload(&d_sm, sm.data(), sm.size()));
   G_TRY(c.alloc(&y0, (size_t)(chunk * m)));
   G_TRY(c.alloc(&yd, (size_t)(chunk * 5 * m)));
@@ -599,8 +614,13 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   G_TRY(c.alloc(&tw, 1));
   G_TRY(c.zeros(&x0, (size_t)(chunk * mc)));
   G_TRY(c.zeros(&x1, (size_t)(3 * chunk * mc)));
-  G_TRY(c.zeros(&cmass, (size_t)(mc * mc)));
-  for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(mc * mc)));
+  G_TRY(c.zeros(&cmass, (size_t)(nparts * mc * mc)));
+  for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(nparts * mc * mc)));
+  if (split) {
+    G_TRY(c.alloc(&part, (size_t)rows_scale_parts()));
+    G_TRY(c.alloc(&scale, 2));
+    G_TRY(c.alloc(&planes, (size_t)(4 * 2 * stages * mc * 2)));  // 4 buffers x hi/lo planes
+  }
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
     tm.begin();
@@ -614,9 +634,35 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     const double *xs[4] = {x0, x1, x1 + cn * mc, x1 + 2 * cn * mc};
     int64_t rows[4] = {cn, cn, cn, cn};
     double *cs[4] = {cmass, ck[0], ck[1], ck[2]};
+    if (split) {
+      // One power-of-two range for the chunk's rows (x0 and x1 are adjacent buffers of the
+      // same layout, so both are covered by scanning each), then the fp16 hi/lo planes.
+      const int64_t cst = (cn + 15) / 16;
+      tm.begin();
+      launch_rows_scale_pair(c.st, x0, cn * mc, x1, 3 * cn * mc, part, scale);
+      const half8_t *ph[4], *pl[4];
+      for (int q = 0; q < 4; q++) {
+        half8_t *h = planes + (int64_t)q * 2 * stages * mc * 2, *l = h + stages * mc * 2;
+        launch_split_rows(c.st, xs[q], cn, cst, mc, scale, h, l);
+        ph[q] = h;
+        pl[q] = l;
+      }
+      tm.end(GS_SPLIT);
+      tm.begin();
+      launch_herk_split16(c.st, ph, pl, cs, 4, cst, mc, nparts, scale);
+      tm.end(GS_HERK);
+    } else {
+      tm.begin();
+      launch_herk_f64(c.st, xs, rows, cs, 4, mc);
+      tm.end(GS_HERK);
+    }
+  }
+  if (split) {
     tm.begin();
-    launch_herk_f64(c.st, xs, rows, cs, 4, mc);
+    launch_sum_parts(c.st, cmass, mc * mc, nparts);
+    for (int q = 0; q < 3; q++) launch_sum_parts(c.st, ck[q], mc * mc, nparts);
     tm.end(GS_HERK);
+    for (void *p : {(void *)part, (void *)scale, (void *)planes}) c.release(p);
   }
   launch_sum_f64(c.st, wv, n, tw);
   double2 *M, *K;

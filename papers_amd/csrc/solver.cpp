@@ -129,6 +129,7 @@ struct cy3q_solver {
   double *tseg = nullptr;            // T-pass segment sums
   int64_t tp_chunk = 0, tp_chunks = 0;
   int n_cu = 1;                // compute units of the device (persistent D-pass grid)
+  int galerkin_precision = CY3Q_PREC_FP64;  // Galerkin rank-k updates
   int64_t tp_mult = 1;         // T-pass point chunks per CU (a tuning knob)
   int64_t syrk_target = 1024;  // workgroups of the FP64 rank-k pass (a tuning knob)
   double *batch_res = nullptr;  // residuals of captured solves
@@ -772,6 +773,7 @@ int solver_view(cy3q_solver *s, SolverView *v) {
   v->sv = s->sv;
   v->sd = s->sd;
   v->bmat = s->bmat;
+  v->galerkin_precision = s->galerkin_precision;
   return CY3Q_OK;
 }
 
@@ -1522,6 +1524,14 @@ int cy3q_solver_get_weights(cy3q_solver *s, double *weights, double *log_omega) 
     HIP_TRY(hipMemcpyAsync(log_omega, s->log_om, sizeof(double) * s->n, hipMemcpyDeviceToHost,
                            s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
+  return CY3Q_OK;
+}
+
+int cy3q_solver_set_galerkin_precision(cy3q_solver *s, int precision) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (precision != CY3Q_PREC_FP64 && precision != CY3Q_PREC_SPLIT16)
+    return fail(CY3Q_ERR_ARG, "precision must be CY3Q_PREC_FP64 or CY3Q_PREC_SPLIT16");
+  s->galerkin_precision = precision;
   return CY3Q_OK;
 }
 

@@ -36,6 +36,7 @@ def _lib():
         L.cy3q_solver_laplacian_spectrum.argtypes = [vp, u32, i64, dbl, i64, vp, vp, vp,
                                                      C.POINTER(i64), C.POINTER(i64)]
         L.cy3q_hermitian_eigen.argtypes = [vp, i64, i64, dbl, ip, vp, vp, C.POINTER(i64)]
+        L.cy3q_solver_set_galerkin_precision.argtypes = [vp, ip]
         L.cy3q_yukawa_sz_weights.argtypes = [vp, vp, i64, ip, vp, C.POINTER(dbl)]
         L.cy3q_yukawa_triple_overlap.argtypes = [vp, vp, vp, vp, i64, i64, ip, vp]
         L.cy3q_yukawa_couplings.argtypes = [vp, vp, vp, vp, i64, i64, i64, u64, ip, vp, vp,
@@ -129,9 +130,12 @@ def hermitian_eigen(a, max_sweeps: int = 128, tol: float = 1e-14, device: int = 
 
 def solver_laplacian_spectrum(solver, section_degree: int = 3, n_low: int = 50,
                               deflation_tol: float = 1e-10, max_sweeps: int = 64,
-                              want_matrices: bool = False):
+                              want_matrices: bool = False, precision: int | None = None):
     """Intrinsic-metric Laplacian of the solver's current h on phi_ab = s_a conj(s_b)/|z|^(2d):
-    (lowest eigenvalues, rank kept, M, K)."""
+    (lowest eigenvalues, rank kept, M, K). precision: PREC_FP64 or PREC_SPLIT16 for the rank-k
+    assembly (None keeps the solver's setting, FP64 by default); the eigensolve is FP64."""
+    if precision is not None:
+        _check(_lib().cy3q_solver_set_galerkin_precision(solver._h, int(precision)))
     ns = (section_degree + 1) * (section_degree + 2) * (section_degree + 3) * (section_degree + 4) // 24
     B = ns * ns
     low = np.zeros(max(n_low, 1))

@@ -120,6 +120,30 @@ def test_quintic_laplacian_matches_oracle(cy3q, oracle, sdeg, k, n, psi):
     assert abs(low[0]) <= 1e-10 * ref.max()
 
 
+@pytest.mark.parametrize("sdeg,k,n,psi", [(1, 2, 1500, 0.0), (3, 3, 20000, 0.5)])
+def test_quintic_laplacian_split_assembly(cy3q, oracle, sdeg, k, n, psi):
+    # The fp16 split rank-k assembly against the FP64 one on the same solver: M and K to 1e-6 of
+    # their largest entry (about 22-bit products, FP32 partials over at most 4096 points), the
+    # eigenvalues to 1e-5 relative, and bit-identical repeats.
+    from papers_amd import galerkin as G
+    from papers_amd.quintic import PREC_FP64, PREC_SPLIT16
+    P, s, h = _quintic(cy3q, oracle, k, n, 42, psi)
+    low64, r64, M64, K64 = G.solver_laplacian_spectrum(s, sdeg, n_low=20, want_matrices=True,
+                                                        precision=PREC_FP64)
+    low16, r16, M16, K16 = G.solver_laplacian_spectrum(s, sdeg, n_low=20, want_matrices=True,
+                                                        precision=PREC_SPLIT16)
+    print(f"sdeg={sdeg}: M {_relmax(M16, M64):.2e}, K {_relmax(K16, K64):.2e}, "
+          f"eig {np.max(np.abs(low16[1:] - low64[1:]) / np.abs(low64[1:])):.2e}")
+    assert r16 == r64 == M64.shape[0]
+    assert _relmax(M16, M64) <= 1e-6
+    assert _relmax(K16, K64) <= 1e-6
+    assert np.all(np.abs(low16[1:] - low64[1:]) <= 1e-5 * np.abs(low64[1:]))
+    assert abs(low16[0]) <= 1e-6 * low64.max()
+    low16b, _, M16b, _ = G.solver_laplacian_spectrum(s, sdeg, n_low=20, want_matrices=True,
+                                                     precision=PREC_SPLIT16)
+    assert np.array_equal(M16b, M16) and np.array_equal(low16b, low16)
+
+
 # ------------------------------------------------------------------------------------------
 # Yukawa
 # ------------------------------------------------------------------------------------------
