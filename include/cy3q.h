@@ -240,9 +240,11 @@ int cy3q_metric_laplacian_spectrum(const double *points, const double *weights,
  * metric g^{a b-bar} (the refinement metric_laplacian.rs:28-41 describes) on the test functions
  * phi_ab = s_a conj(s_b) / |z|^(2 d), s the raw degree-d monomials (d = section_degree <= 4,
  * alpha = a * C(d+4,4) + b). Measure w |Omega|^2 / sum w as build_mass_matrix. The generalized
- * problem K v = lambda M v is solved in FP64 on the GPU by reduction with M^(-1/2) over the
- * eigenvectors of M with eigenvalue > deflation_tol * max (the null-space deflation of
- * metric_laplacian.rs:250-263). eig_low: the n_low smallest eigenvalues, ascending; *rank:
+ * problem K v = lambda M v is solved in FP64 on the GPU: C = W K W^H with W = chol(M)^-1, or,
+ * when M is not positive definite, with M^(-1/2) over the eigenvectors of M with eigenvalue >
+ * deflation_tol * max (the null-space deflation of metric_laplacian.rs:250-263; max_sweeps
+ * bounds that Jacobi eigensolve of M). The n_low lowest eigenvalues of C come from Householder
+ * tridiagonalisation and bisection. eig_low: the n_low smallest eigenvalues, ascending; *rank:
  * dimension kept. mass/stiffness ([B][B] complex) may be NULL. */
 int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int64_t n_low,
                                    double deflation_tol, int64_t max_sweeps, double *eig_low,

@@ -37,6 +37,7 @@ SOURCES = {
     "k_laplacian.hip": [],
     "k_herk_split.hip": [],
     "k_eigen.hip": [],
+    "k_tridiag.hip": [],
     "k_yukawa.hip": [],
     "solver.cpp": ["-x", "hip"],
     "galerkin.cpp": ["-x", "hip"],

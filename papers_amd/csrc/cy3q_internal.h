@@ -192,6 +192,13 @@ void launch_diag_re(hipStream_t st, const double2 *A, int np, double *d);
 void launch_gather_cols(hipStream_t st, const double2 *V, int np, int n, const int *col,
                         const double *scale, int r, double2 *out);
 void launch_herm_project(hipStream_t st, double2 *A, int n);
+// Householder tridiagonalisation (in place; d: n, e: n - 1, v and p: n double2, scal: 3) and
+// bisection of the lowest n_want eigenvalues of the tridiagonal (k_tridiag.hip).
+int tridiag_smem_ok(int n);
+void launch_hh_tridiag(hipStream_t st, double2 *A, int n, double2 *v, double2 *p, double *scal,
+                       double *d, double *e);
+void launch_tridiag_bisect(hipStream_t st, const double *d, const double *e, int n, int n_want,
+                           double *ev);
 void launch_add_diag(hipStream_t st, double2 *a, int n, double shift);
 void launch_zgemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C, int M, int N,
                   int K, int lda, int ldb, int ldc, int opa, int opb);

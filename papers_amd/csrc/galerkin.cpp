@@ -311,9 +311,39 @@ int reference_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64
 
 // Generalized K v = lambda M v by the canonical reduction X = U_keep diag(mu^-1/2):
 // C = X^H K X, the eigenvalues of its Hermitian part ascending.
+// The lowest n_want eigenvalues (ascending) of a Hermitian matrix: Householder
+// tridiagonalisation of a copy, then bisection.
+int tridiag_eigvals(Ctx &c, const double2 *a, int n, int n_want, std::vector<double> &ev) {
+  if (!tridiag_smem_ok(n)) return set_error(CY3Q_ERR_ARG, "eigenproblem too large");
+  n_want = std::max(0, std::min(n_want, n));
+  double2 *A, *v, *p;
+  double *scal, *d, *e, *dev;
+  G_TRY(c.alloc(&A, (size_t)n * n));
+  G_TRY(c.alloc(&v, (size_t)n));
+  G_TRY(c.alloc(&p, (size_t)n));
+  G_TRY(c.alloc(&scal, 3));
+  G_TRY(c.alloc(&d, (size_t)n));
+  G_TRY(c.alloc(&e, (size_t)n));
+  G_TRY(c.alloc(&dev, (size_t)std::max(n_want, 1)));
+  G_HIP(hipMemcpyAsync(A, a, sizeof(double2) * n * n, hipMemcpyDeviceToDevice, c.st));
+  launch_hh_tridiag(c.st, A, n, v, p, scal, d, e);
+  launch_tridiag_bisect(c.st, d, e, n, n_want, dev);
+  G_HIP(hipGetLastError());
+  ev.assign(n_want, 0.0);
+  if (n_want > 0)
+    G_HIP(hipMemcpyAsync(ev.data(), dev, sizeof(double) * n_want, hipMemcpyDeviceToHost, c.st));
+  G_HIP(hipStreamSynchronize(c.st));
+  for (double x : ev)
+    if (!std::isfinite(x)) return set_error(CY3Q_ERR_ARG, "non-finite entries in the eigenproblem");
+  for (void *q : {(void *)A, (void *)v, (void *)p, (void *)scal, (void *)d, (void *)e, (void *)dev})
+    c.release(q);
+  return CY3Q_OK;
+}
+
+// The lowest n_want eigenvalues of the pencil (K, M).
 int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int64_t max_sweeps,
-                         double tol, double defl, std::vector<double> &ev, int64_t *rank,
-                         StageTimer &tm) {
+                         double tol, double defl, int n_want, std::vector<double> &ev,
+                         int64_t *rank, StageTimer &tm) {
   double2 *U;
   G_TRY(c.alloc(&U, (size_t)nb * nb));
   // Positive definite M: C = W K W^H with W = chol(M)^-1, whose eigenvalues are those of the
@@ -334,7 +364,7 @@ int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int
       tm.end(GS_REDUCE);
       if (rank) *rank = nb;
       tm.begin();
-      G_TRY(heev(c, C, nb, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+      G_TRY(tridiag_eigvals(c, C, nb, n_want, ev));
       tm.end(GS_EIGEN_C);
       return CY3Q_OK;
     }
@@ -369,7 +399,7 @@ int generalized_spectrum(Ctx &c, const double2 *M, const double2 *K, int nb, int
   launch_herm_project(c.st, C, r);
   tm.end(GS_REDUCE);
   tm.begin();
-  G_TRY(heev(c, C, r, max_sweeps, tol, 0.0, ev, nullptr, nullptr));
+  G_TRY(tridiag_eigvals(c, C, r, n_want, ev));
   tm.end(GS_EIGEN_C);
   return CY3Q_OK;
 }
@@ -678,7 +708,9 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     G_HIP(hipMemcpyAsync(stiffness, K, sizeof(double2) * B * B, hipMemcpyDeviceToHost, c.st));
   std::vector<double> ev;
   G_TRY(generalized_spectrum(c, M, K, B, max_sweeps > 0 ? max_sweeps : 64, 1e-14,
-                             deflation_tol > 0.0 ? deflation_tol : 1e-10, ev, rank, tm));
+                             deflation_tol > 0.0 ? deflation_tol : 1e-10,
+                             (int)std::min<int64_t>(std::max<int64_t>(n_low, 0), B), ev, rank,
+                             tm));
   const int64_t nl = std::min<int64_t>(std::max<int64_t>(n_low, 0), (int64_t)ev.size());
   if (eig_low) std::memcpy(eig_low, ev.data(), sizeof(double) * nl);
   G_HIP(hipStreamSynchronize(c.st));
