@@ -306,7 +306,7 @@ def bench_adjoint(args) -> int:
         loss = sigma_squared(Gr, Gi, s)
         loss.backward()
         opt.step()
-        hist_torch.append(float(loss))
+        hist_torch.append(float(loss.detach()))
     t_adam_torch = time.perf_counter() - t0
     _log(f"Adam (Cholesky, torch): {t_adam_torch:.2f} s, sigma^2 {hist_torch[0]:.4e} -> {hist_torch[-1]:.4e}")
     # (d) sigma_refine_analytic: the reference's (h_re, h_im) Adam, all on the device.

@@ -120,8 +120,10 @@ void launch_gram_residual(hipStream_t st, double *w, int nb, double *g, const in
 void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const double *yd,
                       const double *log_om, int64_t n, int64_t m_cols, double psi,
                       double *eta);
+// sigma reduction into out6 (see k_sigma.hip); scratch holds sigma_reduce_scratch() doubles.
+int sigma_reduce_scratch();
 void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
-                         double *out5);
+                         double *scratch, double *out6);
 // Adjoint of sigma_L2^2 and Adam (k_adjoint.hip).
 void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const double *yd,
                       const double *log_om, const double *w, const double *sv, const double *sd,

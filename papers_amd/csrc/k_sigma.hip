@@ -59,76 +59,127 @@ void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const
                      pts, y0, yd, log_om, n, m_cols, psi, eta);
 }
 
-// Two deterministic single-workgroup passes (quintic.rs:1469-1498):
+// Two deterministic passes (quintic.rs:1469-1498), each over fixed blocks of points followed by
+// one workgroup that sums the block partials in block order:
 //   pass 1: total_w = sum w, weighted = sum w eta, count over valid (eta finite > 0, w finite > 0)
-//   pass 2: sum w |eta/kappa - 1|, sum w (eta/kappa - 1)^2
+//   pass 2: sum w |eta/kappa - 1|, sum w (eta/kappa - 1)^2, sum w 2 (eta/kappa - 1)(eta/kappa)
 // out[0] = sigma_L1, out[1] = kappa, out[2] = sigma_L2^2 (NaN where the reference returns NaN),
 // out[3] = total_w, out[4] = count, out[5] = sum w 2 (eta/kappa - 1)(eta/kappa).
-__global__ void __launch_bounds__(1024) k_sigma_reduce(const double *__restrict__ eta,
-                                                       const double *__restrict__ w, int64_t n,
-                                                       double *__restrict__ out) {
-  __shared__ double s0[1024], s1[1024], s2[1024];
-  double tw = 0.0, we = 0.0, cnt = 0.0;
-  for (int64_t p = threadIdx.x; p < n; p += 1024) {
-    double e = eta[p], wp = w[p];
-    if (!isfinite(e) || e <= 0.0 || !isfinite(wp) || wp <= 0.0) continue;
-    tw += wp;
-    we += wp * e;
-    cnt += 1.0;
-  }
-  s0[threadIdx.x] = tw;
-  s1[threadIdx.x] = we;
-  s2[threadIdx.x] = cnt;
+static constexpr int SR_BLOCKS = 512;
+static constexpr int SR_THREADS = 256;
+
+__device__ inline void sr_sum3(double &a, double &b, double &c, double (*sh)[SR_THREADS]) {
+  const int t = threadIdx.x;
+  sh[0][t] = a;
+  sh[1][t] = b;
+  sh[2][t] = c;
   __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (threadIdx.x < (unsigned)off) {
-      s0[threadIdx.x] += s0[threadIdx.x + off];
-      s1[threadIdx.x] += s1[threadIdx.x + off];
-      s2[threadIdx.x] += s2[threadIdx.x + off];
+  for (int off = SR_THREADS / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      sh[0][t] += sh[0][t + off];
+      sh[1][t] += sh[1][t + off];
+      sh[2][t] += sh[2][t + off];
     }
     __syncthreads();
   }
-  const double total_w = s0[0], wsum = s1[0], count = s2[0];
-  __syncthreads();
-  const bool ok = count > 0.0 && total_w >= 1e-12;
-  const double kappa = ok ? wsum / total_w : NAN;
-  const bool ok2 = ok && fabs(kappa) >= 1e-30;
-  double ad = 0.0, sq = 0.0, ga = 0.0;
-  if (ok2) {
-    for (int64_t p = threadIdx.x; p < n; p += 1024) {
-      double e = eta[p], wp = w[p];
-      if (!isfinite(e) || e <= 0.0 || !isfinite(wp) || wp <= 0.0) continue;
-      double r = e / kappa - 1.0;
-      ad += wp * fabs(r);
-      sq += wp * r * r;
-      ga += wp * 2.0 * r * (e / kappa);
+  a = sh[0][0];
+  b = sh[1][0];
+  c = sh[2][0];
+}
+
+__device__ inline bool sr_valid(double e, double wp) {
+  return isfinite(e) && e > 0.0 && isfinite(wp) && wp > 0.0;
+}
+
+// Block b covers points [b * chunk, (b + 1) * chunk). stats (pass 2 only): total_w, wsum, count.
+template <int PASS>
+__global__ void __launch_bounds__(SR_THREADS) k_sigma_part(const double *__restrict__ eta,
+                                                           const double *__restrict__ w, int64_t n,
+                                                           int64_t chunk,
+                                                           const double *__restrict__ stats,
+                                                           double *__restrict__ part) {
+  __shared__ double sh[3][SR_THREADS];
+  const int64_t p0 = (int64_t)blockIdx.x * chunk;
+  const int64_t p1 = p0 + chunk < n ? p0 + chunk : n;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (PASS == 1) {
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += SR_THREADS) {
+      const double e = eta[p], wp = w[p];
+      if (!sr_valid(e, wp)) continue;
+      a += wp;
+      b += wp * e;
+      c += 1.0;
+    }
+  } else {
+    const double total_w = stats[0], count = stats[2];
+    const bool ok = count > 0.0 && total_w >= 1e-12;
+    const double kappa = ok ? stats[1] / total_w : NAN;
+    if (ok && fabs(kappa) >= 1e-30) {
+      for (int64_t p = p0 + threadIdx.x; p < p1; p += SR_THREADS) {
+        const double e = eta[p], wp = w[p];
+        if (!sr_valid(e, wp)) continue;
+        const double r = e / kappa - 1.0;
+        a += wp * fabs(r);
+        b += wp * r * r;
+        c += wp * 2.0 * r * (e / kappa);
+      }
     }
   }
-  s0[threadIdx.x] = ad;
-  s1[threadIdx.x] = sq;
-  s2[threadIdx.x] = ga;
-  __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (threadIdx.x < (unsigned)off) {
-      s0[threadIdx.x] += s0[threadIdx.x + off];
-      s1[threadIdx.x] += s1[threadIdx.x + off];
-      s2[threadIdx.x] += s2[threadIdx.x + off];
-    }
-    __syncthreads();
-  }
+  sr_sum3(a, b, c, sh);
   if (threadIdx.x == 0) {
-    out[0] = ok2 ? s0[0] / total_w : NAN;
-    out[1] = ok2 ? kappa : NAN;
-    out[2] = ok2 ? s1[0] / total_w : NAN;
-    out[3] = total_w;
-    out[4] = count;
-    out[5] = ok2 ? s2[0] : NAN;  // sum w 2 (eta/kappa - 1)(eta/kappa), for the sigma^2 gradient
+    part[3 * blockIdx.x + 0] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
   }
 }
 
+// Block partials in block order. Pass 1 writes stats; pass 2 writes out (with stats).
+template <int PASS>
+__global__ void __launch_bounds__(SR_THREADS) k_sigma_final(const double *__restrict__ part,
+                                                            int nparts, double *__restrict__ stats,
+                                                            double *__restrict__ out) {
+  __shared__ double sh[3][SR_THREADS];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += SR_THREADS) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  sr_sum3(a, b, c, sh);
+  if (threadIdx.x != 0) return;
+  if (PASS == 1) {
+    stats[0] = a;
+    stats[1] = b;
+    stats[2] = c;
+    return;
+  }
+  const double total_w = stats[0], count = stats[2];
+  const bool ok = count > 0.0 && total_w >= 1e-12;
+  const double kappa = ok ? stats[1] / total_w : NAN;
+  const bool ok2 = ok && fabs(kappa) >= 1e-30;
+  out[0] = ok2 ? a / total_w : NAN;
+  out[1] = ok2 ? kappa : NAN;
+  out[2] = ok2 ? b / total_w : NAN;
+  out[3] = total_w;
+  out[4] = count;
+  out[5] = ok2 ? c : NAN;  // sum w 2 (eta/kappa - 1)(eta/kappa), for the sigma^2 gradient
+}
+
+int sigma_reduce_scratch() { return 3 * SR_BLOCKS + 3; }
+
+// scratch: sigma_reduce_scratch() doubles.
 void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
-                         double *out5) {
-  hipLaunchKernelGGL(k_sigma_reduce, dim3(1), dim3(1024), 0, st, eta, w, n, out5);
+                         double *scratch, double *out6) {
+  const int64_t chunk = (n + SR_BLOCKS - 1) / SR_BLOCKS > 0 ? (n + SR_BLOCKS - 1) / SR_BLOCKS : 1;
+  double *part = scratch, *stats = scratch + 3 * SR_BLOCKS;
+  hipLaunchKernelGGL(k_sigma_part<1>, dim3(SR_BLOCKS), dim3(SR_THREADS), 0, st, eta, w, n, chunk,
+                     stats, part);
+  hipLaunchKernelGGL(k_sigma_final<1>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS, stats,
+                     out6);
+  hipLaunchKernelGGL(k_sigma_part<2>, dim3(SR_BLOCKS), dim3(SR_THREADS), 0, st, eta, w, n, chunk,
+                     stats, part);
+  hipLaunchKernelGGL(k_sigma_final<2>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS, stats,
+                     out6);
 }
 
 }  // namespace cy3q

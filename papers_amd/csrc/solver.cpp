@@ -114,7 +114,7 @@ struct cy3q_solver {
   // h state: G = W^H W, W lower triangular
   double *g = nullptr, *wmat = nullptr, *t = nullptr, *wnew = nullptr, *gnew = nullptr;
   double *bmat = nullptr, *dpart = nullptr, *coef = nullptr, *xx = nullptr, *partials = nullptr;
-  double *resid = nullptr, *sig_out = nullptr;
+  double *resid = nullptr, *sig_out = nullptr, *sig_scratch = nullptr;
   // elimination workspace of the N x N solve
   double *abuf = nullptr, *rbuf = nullptr, *einv = nullptr, *partial = nullptr,
          *scale_buf = nullptr;
@@ -315,6 +315,7 @@ int finish_workspace(cy3q_solver *s) {
   ST_TRY(s->alloc(&s->dpart, (m / 32) * n));
   ST_TRY(s->alloc(&s->resid, 1));
   ST_TRY(s->alloc(&s->sig_out, 8));
+  ST_TRY(s->alloc(&s->sig_scratch, sigma_reduce_scratch()));
   ST_TRY(s->alloc(&s->status, 4));
   const int nblk = (nb + 31) / 32;
   ST_TRY(s->alloc(&s->abuf, 2 * nn));
@@ -646,7 +647,7 @@ int sigma_device(cy3q_solver *s, double *eta) {
                        eta + p0);
     }
   }
-  launch_sigma_reduce(s->st, eta, s->w, n, s->sig_out);
+  launch_sigma_reduce(s->st, eta, s->w, n, s->sig_scratch, s->sig_out);
   HIP_TRY(hipGetLastError());
   return CY3Q_OK;
 }
