@@ -163,67 +163,96 @@ void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const
 
 // ---------------------------------------------------------------------------------------------
 // C_g = sum over the rows r of group g of U[r]^T V[r] (U, V: [rows][m] row-major, FP64). A
-// workgroup computes a 64 x 64 tile of C (4 waves, 32 x 32 each, v_mfma_f64_16x16x4f64) over its
-// rows, staged through LDS in blocks of 16 rows. Output: part[g][m][m].
+// workgroup computes a 128 x 128 tile of C over its rows: 4 waves, each a 64 x 64 quadrant as
+// 4 x 4 tiles of v_mfma_f64_16x16x4f64, 16-row stages in an LDS double buffer filled from
+// registers prefetched one stage ahead. Output: part[g][m][m] (columns beyond m are not written).
 // v_mfma_f64_16x16x4f64 layout: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k,
 // D[i][j] at lane j + 16 (i % 4), element i / 4.
 // ---------------------------------------------------------------------------------------------
-static constexpr int GT = 64;   // output tile edge
-static constexpr int GKB = 16;  // rows per LDS block
+static constexpr int GT = 128;  // output tile edge
+static constexpr int GKB = 16;  // rows per stage
+static constexpr int GLD = GT + 2;
 
 __global__ void __launch_bounds__(256) k_gemm_tn_f64(const double *__restrict__ u,
                                                      const double *__restrict__ v, int64_t rows,
                                                      int64_t m, int64_t rows_per_group,
                                                      double *__restrict__ part) {
-  __shared__ double su[GKB][GT], sv_[GKB][GT];
+  __shared__ double sa[2][GKB][GLD], sb[2][GKB][GLD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t c0 = (int64_t)blockIdx.x * GT, d0 = (int64_t)blockIdx.y * GT;
   const int64_t g = blockIdx.z;
   const int64_t r0 = g * rows_per_group;
   const int64_t r1 = r0 + rows_per_group < rows ? r0 + rows_per_group : rows;
-  double4_t acc[2][2];
+  const int wr = 64 * (wave >> 1), wc = 64 * (wave & 1);
+  double4_t acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const int wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
-  const int lrow = tid / 16, lcol = (tid % 16) * 4;
-  for (int64_t rb = r0; rb < r1; rb += GKB) {
-    const int64_t r = rb + lrow;
+    for (int c = 0; c < 4; c++) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const int64_t n_stage = r1 > r0 ? (r1 - r0 + GKB - 1) / GKB : 0;
+  // Thread tid moves pairs e = tid + 256 q: row e >> 6 of the stage, columns 2 (e & 63) + {0, 1}
+  // (m is even, so a pair is inside or outside the matrix as a whole).
+  double2 ra[4], rb[4];
+  auto fetch = [&](int64_t s) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int64_t cu = c0 + lcol + q, cv = d0 + lcol + q;
-      su[lrow][lcol + q] = (r < r1 && cu < m) ? u[r * m + cu] : 0.0;
-      sv_[lrow][lcol + q] = (r < r1 && cv < m) ? v[r * m + cv] : 0.0;
+      const int e = tid + 256 * q;
+      const int64_t row = r0 + s * GKB + (e >> 6);
+      const int cc = (e & 63) * 2;
+      const bool in = row < r1;
+      ra[q] = (in && c0 + cc < m) ? *reinterpret_cast<const double2 *>(u + row * m + c0 + cc)
+                                  : make_double2(0.0, 0.0);
+      rb[q] = (in && d0 + cc < m) ? *reinterpret_cast<const double2 *>(v + row * m + d0 + cc)
+                                  : make_double2(0.0, 0.0);
     }
-    __syncthreads();
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      const int r = e >> 6, cc = (e & 63) * 2;
+      sa[buf][r][cc] = ra[q].x;
+      sa[buf][r][cc + 1] = ra[q].y;
+      sb[buf][r][cc] = rb[q].x;
+      sb[buf][r][cc + 1] = rb[q].y;
+    }
+  };
+  if (n_stage > 0) {
+    fetch(0);
+    put(0);
+  }
+  __syncthreads();
+  for (int64_t s = 0; s < n_stage; s++) {
+    const int buf = (int)(s & 1);
+    if (s + 1 < n_stage) fetch(s + 1);
 #pragma unroll
     for (int kk = 0; kk < GKB / 4; kk++) {
-      const int kr = 4 * kk + lane / 16;
-      double a[2], b[2];
+      const int kr = 4 * kk + (lane >> 4);
+      double av[4], bv[4];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        a[t] = su[kr][wr + 16 * t + lane % 16];
-        b[t] = sv_[kr][wc + 16 * t + lane % 16];
+      for (int q = 0; q < 4; q++) {
+        av[q] = sa[buf][kr][wr + 16 * q + (lane & 15)];
+        bv[q] = sb[buf][kr][wc + 16 * q + (lane & 15)];
       }
 #pragma unroll
-      for (int ta = 0; ta < 2; ta++)
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int tb = 0; tb < 2; tb++)
-          acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
+        for (int c = 0; c < 4; c++)
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
     }
+    if (s + 1 < n_stage) put(buf ^ 1);
     __syncthreads();
   }
   double *out = part + g * m * m;
 #pragma unroll
-  for (int ta = 0; ta < 2; ta++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int tb = 0; tb < 2; tb++)
+    for (int c = 0; c < 4; c++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int64_t i = c0 + wr + 16 * ta + 4 * e + lane / 16;
-        const int64_t j = d0 + wc + 16 * tb + lane % 16;
-        if (i < m && j < m) out[i * m + j] = acc[ta][tb][e];
+        const int64_t i = c0 + wr + 16 * a + 4 * e + (lane >> 4);
+        const int64_t j = d0 + wc + 16 * c + (lane & 15);
+        if (i < m && j < m) out[i * m + j] = acc[a][c][e];
       }
 }
 
