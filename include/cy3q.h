@@ -61,6 +61,8 @@ typedef struct cy3q_solver cy3q_solver;
 
 /* Fills the defaults: raw basis, FP64, device 0, 50 DK sweeps, no polish, newton_tol 1e-12. */
 void cy3q_options_default(cy3q_options *opt);
+/* Library version, the clang/hipcc version that built it and the GPU target (recorded in the
+ * run manifest; the reference's build.rs records RUSTC_VERSION for it, cy3/build.rs). */
 const char *cy3q_version(void);
 const char *cy3q_last_error(void);
 

@@ -845,7 +845,7 @@ void cy3q_options_default(cy3q_options *opt) {
   opt->newton_tol = 1e-12;
 }
 
-const char *cy3q_version(void) { return "cy3q 0.1 (gfx950)"; }
+const char *cy3q_version(void) { return "cy3q 0.1 (gfx950; clang " __clang_version__ ")"; }
 const char *cy3q_last_error(void) { return g_last_error.c_str(); }
 
 int cy3q_n_sections(uint32_t k, int quotient, int64_t *n) {

@@ -98,6 +98,11 @@ def _load():
     return L
 
 
+def library_version() -> str:
+    """cy3q_version(): the library version, the compiler that built it and its GPU target."""
+    return _load().cy3q_version().decode()
+
+
 def _check(rc: int):
     if rc != OK:
         raise Cy3qError(rc, _load().cy3q_last_error().decode())
