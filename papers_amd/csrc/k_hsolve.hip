@@ -11,7 +11,8 @@
 // W comes from one blocked elimination of the augmented matrix [T | I] in 32 x 32 complex tiles.
 // For each block column J:
 //   k_elim_panel  (1 workgroup): factors the diagonal tile and inverts it in the same 32-step
-//                 sweep (one barrier per column): W_JJ = L_JJ^-1.
+//                 sweep (one barrier per column; the tile lives in registers and only the
+//                 pivot column and row go through LDS): W_JJ = L_JJ^-1.
 //   k_elim_update (one workgroup per tile): L_IJ = A_IJ L_JJ^-H, then A_IK -= L_IJ L_KJ^H for
 //                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J, where R is the updated right
 //                 half and W_JK = L_JJ^-1 R_JK (K < J) is formed in the same kernel. The tile
@@ -163,15 +164,24 @@ void launch_elim_init(hipStream_t st, const double *src, int nb, double *a, doub
 // ---------------------------------------------------------------------------------------------
 // Panel of block column J.
 // ---------------------------------------------------------------------------------------------
+// 1 / sqrt(x) for a positive normal x: the hardware estimate and two Newton steps
+// y <- y + y (1 - x y^2) / 2 (each squares the relative error), to about 1 ulp.
+__device__ inline double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double t = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, t, y);
+  t = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, t, y);
+}
+
 __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a,
                                                     const double *__restrict__ r, int n, int jblk,
                                                     double *__restrict__ einv,
                                                     double *__restrict__ w,
                                                     int32_t *__restrict__ status) {
-  __shared__ Cz d[TB][TB + 1];   // A_JJ, factored in place (lower part)
-  __shared__ Cz rr[TB][TB + 1];  // right half of the tile, starts as I
+  __shared__ Cz colb[2][TB];     // column j of the left tile at step j (double-buffered)
+  __shared__ Cz rowb[2][TB];     // row j of the right tile at step j
   __shared__ Cz e[TB][TB + 1];   // L_JJ^-1
-  __shared__ int bad;
   const int tid = threadIdx.x;
   const int j0 = jblk * TB;
   if (jblk == 0) {
@@ -179,55 +189,68 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
   } else if (*status != ST_OK) {
     return;
   }
-  if (tid == 0) bad = 0;
-  load_tile(d, a, n, j0, j0, true, tid, 256);
-  for (int q = tid; q < TB * TB; q += 256) {
-    const int i = q / TB, k = q % TB;
-    rr[i][k] = Cz{i == k ? 1.0 : 0.0, 0.0};
-    e[i][k] = Cz{0.0, 0.0};
+  // Thread tid owns elements (i, k) = (tid / 32 + 8 u, tid % 32), u = 0..3, of the left tile
+  // A_JJ (identity outside the matrix) and of the right tile (starts as I), in registers.
+  const int k = tid % TB, i0 = tid / TB;
+  Cz dv[4], rv[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int i = i0 + 8 * u;
+    dv[u] = Cz{i == k ? 1.0 : 0.0, 0.0};
+    if (j0 + i < n && j0 + k < n) {
+      const double2 v =
+          *reinterpret_cast<const double2 *>(a + 2 * ((int64_t)(j0 + i) * n + j0 + k));
+      dv[u] = Cz{v.x, v.y};
+    }
+    rv[u] = Cz{i == k ? 1.0 : 0.0, 0.0};
   }
-  __syncthreads();
+  for (int q = tid; q < TB * TB; q += 256) e[q / TB][q % TB] = Cz{0.0, 0.0};
   // Step j: l_jj = sqrt(d_jj); row j of L^-1 is rr_j / l_jj; for i > j the trailing left part
-  // loses d_ij conj(d_kj) / d_jj and the right part loses d_ij rr_j / d_jj. Column j of d and
-  // row j of rr are only read in step j, so one barrier per step suffices. Each thread owns
-  // fixed elements (at most 4 of the 32 x 32 left tile and 4 of the right): it gathers all its
-  // operands first, then writes, so the LDS reads of a step overlap.
+  // loses d_ij conj(d_kj) / d_jj (j < k <= i) and the right part loses d_ij rr_jk / d_jj
+  // (k <= j). The owners publish column j of d and row j of rr, then one barrier; every thread
+  // reads the pivot and its operands from those two short vectors and updates its registers.
+  // Buffer j & 1 is rewritten at step j + 2, after the barrier of step j + 1, so its readers
+  // at step j are done.
+  bool ok = true;
+#pragma unroll
   for (int j = 0; j < TB; j++) {
-    const double piv = d[j][j].r;
+    const int b = j & 1;
+    if (k == j) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) colb[b][i0 + 8 * u] = dv[u];
+    }
+    if (i0 == j % 8) rowb[b][k] = rv[j / 8];
+    __syncthreads();
+    const double piv = colb[b][j].r;
     if (!(piv > 0.0) || !isfinite(piv)) {
-      if (tid == 0) bad = 1;
+      ok = false;  // the same pivot for every thread: all leave at the same step
       break;
     }
-    const double inv_l = 1.0 / sqrt(piv), ipiv = 1.0 / piv;
-    if (tid <= j) e[j][tid] = Cz{rr[j][tid].r * inv_l, rr[j][tid].i * inv_l};
-    Cz dl[4], dj[4], rj[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int q = tid + 256 * u, i = q / TB, k = q % TB;
-      // left: rows i > j, columns j < k <= i; right: rows i > j, columns k <= j
-      dl[u] = d[i][j];
-      dj[u] = d[k][j];
-      rj[u] = rr[j][k];
+    const double inv_l = rsqrt_f64(piv), ipiv = inv_l * inv_l;
+    if (tid <= j) {
+      const Cz x = rowb[b][tid];
+      e[j][tid] = Cz{x.r * inv_l, x.i * inv_l};
     }
+    const Cz ck = colb[b][k], rk = rowb[b][k];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int q = tid + 256 * u, i = q / TB, k = q % TB;
+      const int i = i0 + 8 * u;
       if (i <= j) continue;
+      const Cz dl = colb[b][i];
       if (k > j) {
         if (k > i) continue;
-        const Cz p = cmulc(dl[u], dj[u]);
-        d[i][k].r -= p.r * ipiv;
-        d[i][k].i -= p.i * ipiv;
+        const Cz p = cmulc(dl, ck);
+        dv[u].r -= p.r * ipiv;
+        dv[u].i -= p.i * ipiv;
       } else {
-        const Cz p = cmul(dl[u], rj[u]);
-        rr[i][k].r -= p.r * ipiv;
-        rr[i][k].i -= p.i * ipiv;
+        const Cz p = cmul(dl, rk);
+        rv[u].r -= p.r * ipiv;
+        rv[u].i -= p.i * ipiv;
       }
     }
-    __syncthreads();
   }
   __syncthreads();
-  if (bad) {
+  if (!ok) {
     if (tid == 0) *status = ST_ERR_NOT_PD;
     return;
   }
