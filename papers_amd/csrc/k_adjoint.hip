@@ -257,13 +257,22 @@ __global__ void __launch_bounds__(256) k_gemm_tn_f64(const double *__restrict__ 
 }
 
 // acc[i] += sum_g part[g][i], in group order.
+// acc += sum of the group partials. Eight interleaved running sums (group g into sum g % 8,
+// then the eight in a fixed tree) keep eight loads in flight per thread; the order is fixed, so
+// the result is the same run to run.
 __global__ void k_gemm_accumulate(const double *__restrict__ part, int64_t n_groups, int64_t mm,
                                   double *__restrict__ acc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= mm) return;
-  double s = acc[i];
-  for (int64_t g = 0; g < n_groups; g++) s += part[g * mm + i];
-  acc[i] = s;
+  double r[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int64_t g = 0;
+  for (; g + 8 <= n_groups; g += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) r[u] += part[(g + u) * mm + i];
+  }
+  for (int u = 0; g < n_groups; g++, u++) r[u] += part[g * mm + i];
+  const double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  acc[i] += s;
 }
 
 int64_t gemm_tn_groups(int64_t rows, int64_t m_cols) {
