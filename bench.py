@@ -216,10 +216,10 @@ def bench_batch(args) -> int:
         "unit": "instance T-op iters/s",
         "higher_is_better": True,
         "n_gpus": 1,
-        "dtype": "fp16-split/fp32-acc (fp64 N x N and refinement sweep)",
+        "dtype": "fp16-split/fp32-acc (fp64 N x N; refinement sweep: fp64 T pass)",
         "data": "synthetic: SZ samples of 32 Dwork quintics, specs from the lib.rs LCG seed 7",
         "config": {"workload": "32 instances, psi~U[0,0.9], k~U{2..6}, N_p~logU[2e4,5e5], "
-                               "quotient bases, 15 iterations (last one FP64)",
+                               "quotient bases, 15 iterations (the last one with the FP64 T pass)",
                    "instances": [list(sp) for sp in specs],
                    "total_points": int(sum(sp[1] for sp in specs))},
         "graph_s": t_graph, "sequential_s": t_seq, "speedup_graph_vs_sequential": t_seq / t_graph,
@@ -414,9 +414,10 @@ def main() -> int:
     N, n = s.n_basis, s.n_points
     K, W = args.steps, args.warmup
 
-    def run(prec, refine=1):
+    def run(prec, refine=1, kind=P.REFINE_T_FP64):
         s.set_precision(prec)
         s.set_refine(refine)
+        s.set_refine_kind(kind)
         s.h_block = h0
         if W:
             s.donaldson_solve(W, 0.0)
@@ -433,7 +434,7 @@ def main() -> int:
     _log("FP64 path")
     t64, G64, sig64 = run(P.PREC_FP64)
     _log(f"FP64: {t64 / K * 1e3:.3f} ms/iter, sigma {sig64!r}")
-    _log("split16 path + FP64 refinement sweep")
+    _log("split16 path + refinement sweep (split D pass, FP64 T pass)")
     t16, G16, sig16 = run(P.PREC_SPLIT16, refine=1)
     _log(f"split16+refine: {t16 / K * 1e3:.3f} ms/iter, sigma {sig16!r}")
     rel_G = float(np.linalg.norm(G16 - G64) / np.linalg.norm(G64))
@@ -441,6 +442,10 @@ def main() -> int:
     _log("split16 path, no refinement")
     t16n, G16n, sig16n = run(P.PREC_SPLIT16, refine=0)
     _log(f"split16 only: {t16n / K * 1e3:.3f} ms/iter, sigma {sig16n!r}")
+    _log("split16 path + a whole FP64 step as the refinement sweep")
+    t16t, G16t, sig16t = run(P.PREC_SPLIT16, refine=1, kind=P.REFINE_STEP_FP64)
+    _log(f"split16+FP64-step refine: {t16t / K * 1e3:.3f} ms/iter, sigma {sig16t!r}")
+    s.set_refine_kind(P.REFINE_T_FP64)
 
     # Kernel-level timing of the split path (HIP events around each stage on the solver's
     # stream), in a separate run so the events do not perturb the timed region above. The
@@ -505,7 +510,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16-split/fp32-acc (fp64 N x N and refinement sweep)",
+        "dtype": "fp16-split/fp32-acc (fp64 N x N; refinement sweep: fp64 T pass)",
         "data": "synthetic: SZ line-intersection sample of the Fermat quintic, LCG seed 42",
         "config": {"workload": f"quintic Donaldson T-operator, k={k} quotient basis N={N}, "
                                f"N_p={n} points, FS-Gram orthonormalised, h0=I",
@@ -516,6 +521,10 @@ def main() -> int:
                               "speedup_vs_fp64": t64 / t16n,
                               "sigma_rel_err": float(abs(sig16n / sig64 - 1.0)),
                               "G_rel_err": float(np.linalg.norm(G16n - G64) / np.linalg.norm(G64))},
+        "split16_refine_fp64_step": {"value": K / t16t, "ms_per_step": t16t / K * 1e3,
+                               "speedup_vs_fp64": t64 / t16t,
+                               "sigma_rel_err": float(abs(sig16t / sig64 - 1.0)),
+                               "G_rel_err": float(np.linalg.norm(G16t - G64) / np.linalg.norm(G64))},
         "sigma_fp64": sig64,
         "sigma_split16": sig16,
         "sigma_rel_err": rel_sig,

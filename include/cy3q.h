@@ -97,8 +97,8 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s);
 int cy3q_solver_donaldson_step(cy3q_solver *s, double *residual);
 /* QuinticSolver::donaldson_solve (quintic.rs:433-443). With tol <= 0 no early stop is taken
  * and the iterations run back to back with no host synchronisation. residuals: max_iter
- * entries or NULL. In CY3Q_PREC_SPLIT16 the last min(refine, max_iter) iterations are FP64
- * refinement sweeps (see cy3q_solver_set_refine); with tol > 0 a fast iteration below tol
+ * entries or NULL. In CY3Q_PREC_SPLIT16 the last min(refine, max_iter) iterations are
+ * refinement sweeps (cy3q_solver_set_refine, _set_refine_kind); with tol > 0 a fast iteration below tol
  * ends the fast phase early, the refinement sweeps follow, and the solve stops at the first
  * sweep below tol or after `refine` sweeps. */
 int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, int64_t *iters,
@@ -111,11 +111,20 @@ int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, in
 int cy3q_solver_donaldson_step_legacy(cy3q_solver *s, int weighted, double *residual);
 int cy3q_solver_donaldson_solve_legacy(cy3q_solver *s, int weighted, int64_t max_iter, double tol,
                                        int64_t *iters, double *residuals);
-/* Number of FP64 T-operator refinement sweeps that end every split-precision donaldson_solve
- * (default 1; 0 turns refinement off). No counterpart in the reference, which has only the
- * FP64 path: the sweep re-applies the exact FP64 T-operator to the fast path's metric, so the
- * returned h and residual are those of one FP64 iteration. */
+/* Number of refinement sweeps that end every split-precision donaldson_solve (default 1; 0
+ * turns refinement off). No counterpart in the reference, which has only the FP64 path. A sweep
+ * re-applies the T-operator to the fast path's metric with the T pass in FP64 (and, for
+ * CY3Q_REFINE_STEP_FP64, the D pass too), so the returned h and residual are those of that
+ * iteration. */
 int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps);
+
+/* What one refinement sweep is. CY3Q_REFINE_T_FP64 (the default): the split D pass, then the
+ * FP64 T pass and the FP64 N x N solve (a final FP64 T-iteration correction: the D pass's
+ * rounding is independent per point and averages out in T). CY3Q_REFINE_STEP_FP64: a whole
+ * FP64 Donaldson step. No counterpart in the reference. */
+#define CY3Q_REFINE_STEP_FP64 0
+#define CY3Q_REFINE_T_FP64 1
+int cy3q_solver_set_refine_kind(cy3q_solver *s, int kind);
 /* Tile parameters (the autotuner's knobs; no counterpart in the reference): point chunks per
  * CU of the split-precision T pass (1..16) and the target workgroup count of the FP64 rank-k
  * T pass (64..8192). 0 keeps a value. Results change only by summation order. */

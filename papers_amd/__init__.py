@@ -9,6 +9,8 @@ cannot load, importing ``papers_amd.quintic`` raises.
 from .quintic import (  # noqa: F401
     PREC_FP64,
     PREC_SPLIT16,
+    REFINE_STEP_FP64,
+    REFINE_T_FP64,
     SAMPLER_NEWTON,
     SAMPLER_NEWTON_DWORK,
     SAMPLER_SZ,

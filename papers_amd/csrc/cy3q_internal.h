@@ -80,6 +80,8 @@ int dpass_split16_waves();
 int dpass_split16_max_cg();
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum);
+void launch_coeffs_f64_from_split(hipStream_t st, const double *w, const double *dsum,
+                                  const float *rs2, const double *bscale, int64_t n, double *c);
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
                            const double *bscale, int64_t n, float *u, float *block_max,
                            float *zscale, double *tscale);

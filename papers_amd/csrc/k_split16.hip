@@ -456,6 +456,32 @@ __global__ void __launch_bounds__(256) k_coeffs_split16(const double *__restrict
   if (threadIdx.x == 0) block_max[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
+// FP64 T-pass coefficients from the split D pass (the refinement sweep of CY3Q_REFINE_T_FP64):
+// c_p = w_p / max(D_p, 1e-30) with D_p = dsum_p * bscale / rs2_p, for valid w, else 0
+// (quintic.rs:1757, 1776-1781).
+__global__ void k_coeffs_f64_from_split(const double *__restrict__ w,
+                                        const double *__restrict__ dsum,
+                                        const float *__restrict__ rs2,
+                                        const double *__restrict__ bscale, int64_t n,
+                                        double *__restrict__ c) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double wp = w[p];
+  double cv = 0.0;
+  if (isfinite(wp) && wp > 0.0) {
+    const double d = dsum[p] * bscale[0] / (double)rs2[p];
+    cv = (1.0 / fmax(d, 1e-30)) * wp;
+  }
+  c[p] = cv;
+}
+
+void launch_coeffs_f64_from_split(hipStream_t st, const double *w, const double *dsum,
+                                  const float *rs2, const double *bscale, int64_t n, double *c) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_coeffs_f64_from_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     w, dsum, rs2, bscale, n, c);
+}
+
 // Max over the block maxima -> zscale[0] = 2^g (float, applied to z) and tscale[0] = 4^-g
 // (double, applied to T), with g chosen so that max u 2^g < 2^14; the row-scaled |x| < 1, so
 // every z stays below 2^14.

@@ -103,6 +103,7 @@ struct cy3q_solver {
   double psi = 0.0;
   int precision = CY3Q_PREC_FP64;
   int64_t refine = 1;  // FP64 sweeps that end a split-precision donaldson_solve
+  int refine_kind = CY3Q_REFINE_T_FP64;  // what one refinement sweep is
   std::vector<uint32_t> mono;
   std::vector<void *> allocs;
 
@@ -571,9 +572,46 @@ int t_step_split16(cy3q_solver *s) {
   return CY3Q_OK;
 }
 
+// Refinement sweep of kind CY3Q_REFINE_T_FP64: the split D pass, then the FP64 T pass on
+// FP64 coefficients. The D pass's rounding is independent from point to point and averages
+// out in T; the T pass's does not (it sets the split path's fixed point).
+int t_step_refine_t64(cy3q_solver *s) {
+  const int64_t n = s->n, m = s->m;
+  ST_TRY(ensure_split16(s));
+  if (!s->bfrag_valid) {
+    launch_build_bfrag16(s->st, s->wmat, s->nb, m, s->bfrag, s->bscale);
+    s->bfrag_valid = true;
+  }
+  s->stage_begin(STG_DPASS);
+  launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->n_cu,
+                       s->dsum);
+  s->stage_end(STG_DPASS);
+  s->stage_begin(STG_COEFF);
+  launch_coeffs_f64_from_split(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->coef);
+  s->stage_end(STG_COEFF);
+  s->stage_begin(STG_TPASS);
+  launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
+  s->stage_end(STG_TPASS);
+  s->stage_begin(STG_TREDUCE);
+  launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
+  s->stage_end(STG_TREDUCE);
+  return CY3Q_OK;
+}
+
+// Step codes: CY3Q_PREC_FP64, CY3Q_PREC_SPLIT16, or PREC_REFINE_T64 (the sweep above).
+constexpr int PREC_REFINE_T64 = 100;
+
+// The step code of one refinement sweep of s.
+int refine_step_code(const cy3q_solver *s) {
+  return s->refine_kind == CY3Q_REFINE_T_FP64 ? PREC_REFINE_T64 : CY3Q_PREC_FP64;
+}
+
 int step_in(cy3q_solver *s, int precision) {
   if (precision == CY3Q_PREC_FP64) {
     ST_TRY(t_step_fp64(s));
+    ST_TRY(finish_step(s, 1.0, nullptr));
+  } else if (precision == PREC_REFINE_T64) {
+    ST_TRY(t_step_refine_t64(s));
     ST_TRY(finish_step(s, 1.0, nullptr));
   } else {
     ST_TRY(t_step_split16(s));
@@ -803,7 +841,7 @@ int solver_enqueue_solve(cy3q_solver *s, int64_t iters) {
   for (int64_t it = 0; it < iters; it++) {
     double *keep = s->resid;
     s->resid = s->batch_res + it;
-    const int prec = it >= iters - n_ref ? CY3Q_PREC_FP64 : s->precision;
+    const int prec = it >= iters - n_ref ? refine_step_code(s) : s->precision;
     int rc = step_in(s, prec);
     s->resid = keep;
     if (rc != CY3Q_OK) return rc;
@@ -998,6 +1036,14 @@ int cy3q_solver_set_refine(cy3q_solver *s, int64_t sweeps) {
   return CY3Q_OK;
 }
 
+int cy3q_solver_set_refine_kind(cy3q_solver *s, int kind) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (kind != CY3Q_REFINE_STEP_FP64 && kind != CY3Q_REFINE_T_FP64)
+    return fail(CY3Q_ERR_ARG, "kind must be CY3Q_REFINE_STEP_FP64 or CY3Q_REFINE_T_FP64");
+  s->refine_kind = kind;
+  return CY3Q_OK;
+}
+
 int cy3q_solver_donaldson_step_legacy(cy3q_solver *s, int weighted, double *residual) {
   if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
   HIP_TRY(hipSetDevice(s->device));
@@ -1078,7 +1124,7 @@ int cy3q_solver_donaldson_solve(cy3q_solver *s, int64_t max_iter, double tol, in
   const int64_t n_ref = s->precision == CY3Q_PREC_FP64 ? 0 : std::min(s->refine, max_iter);
   const int64_t n_fast = max_iter - n_ref;
   auto prec_of = [&](int64_t it, bool fast_done) {
-    return (it >= n_fast || fast_done) ? CY3Q_PREC_FP64 : s->precision;
+    return (it >= n_fast || fast_done) ? refine_step_code(s) : s->precision;
   };
   int64_t done = 0;
   if (tol > 0.0) {

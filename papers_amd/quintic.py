@@ -24,6 +24,8 @@ SAMPLER_SZ_DWORK = 2
 SAMPLER_NEWTON_DWORK = 3
 PREC_FP64 = 0
 PREC_SPLIT16 = 1
+REFINE_STEP_FP64 = 0
+REFINE_T_FP64 = 1
 
 OK, ERR_ARG, ERR_HIP, ERR_NOT_PD, ERR_EMPTY, ERR_STATE = 0, -1, -2, -3, -4, -5
 
@@ -68,6 +70,7 @@ def _load():
     L.cy3q_solver_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(u32), C.POINTER(dbl)]
     L.cy3q_solver_set_precision.argtypes = [vp, ip]
     L.cy3q_solver_set_refine.argtypes = [vp, i64]
+    L.cy3q_solver_set_refine_kind.argtypes = [vp, ip]
     L.cy3q_solver_donaldson_step_legacy.argtypes = [vp, ip, C.POINTER(dbl)]
     L.cy3q_solver_donaldson_solve_legacy.argtypes = [vp, ip, i64, dbl, C.POINTER(i64), vp]
     L.cy3q_solver_orthonormalise_fs_gram.argtypes = [vp]
@@ -214,6 +217,11 @@ class QuinticSolver:
     def set_refine(self, sweeps: int):
         """FP64 T-operator sweeps that end a split-precision donaldson_solve (default 1)."""
         _check(_load().cy3q_solver_set_refine(self._h, sweeps))
+
+    def set_refine_kind(self, kind: int):
+        """REFINE_T_FP64 (split D pass, FP64 T pass: the default) or REFINE_STEP_FP64 (a
+        whole FP64 step)."""
+        _check(_load().cy3q_solver_set_refine_kind(self._h, kind))
 
     def orthonormalise_basis_fs_gram(self):
         rc = _load().cy3q_solver_orthonormalise_fs_gram(self._h)

@@ -51,6 +51,7 @@ def test_bad_arguments_fail_loudly(cy3q):
                                                         C.byref(best), None) == -1
     assert L.cy3q_solver_sigma_refine_lbfgs_restarts(None, 1, 1, 4, 0.01, 1, C.byref(best),
                                                      None) == -1
+    assert L.cy3q_solver_set_refine_kind(None, cy3q.REFINE_T_FP64) == -1
 
 
 def test_laplacian_basis_table_bit_exact(cy3q, oracle):

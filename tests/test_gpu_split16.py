@@ -58,13 +58,15 @@ def test_split16_switch_precision_midway(cy3q):
 
 
 def test_refinement_sweep_is_one_fp64_step(cy3q):
-    # A split-precision solve of K iterations with one refinement sweep equals K - 1 split
-    # iterations followed by one FP64 T-operator step, bit for bit (deterministic reductions).
+    # A split-precision solve of K iterations with one whole-step refinement sweep equals K - 1
+    # split iterations followed by one FP64 T-operator step, bit for bit (deterministic
+    # reductions).
     mk = lambda: cy3q.QuinticSolver.new_with_sampler(4, 30000, 5, precision=cy3q.PREC_SPLIT16)
     a, b = mk(), mk()
     for s in (a, b):
         s.orthonormalise_basis_fs_gram()
     a.set_refine(1)
+    a.set_refine_kind(cy3q.REFINE_STEP_FP64)
     assert a.donaldson_solve(4, 0.0) == 4
     b.set_refine(0)
     b.donaldson_solve(3, 0.0)
@@ -100,3 +102,35 @@ def test_refinement_sweep_improves_fixed_point(cy3q):
     b.donaldson_solve(2, 0.0)
     e_ref = _rel(b.h_block, g42)
     assert e_ref < e_split
+
+
+def test_refinement_kinds(cy3q):
+    # Both sweep kinds move the settled split iterate towards the FP64 iterate of the same
+    # index; the default (split D pass, FP64 T pass) is deterministic run to run and lands
+    # within 1e-6 of the whole-step sweep.
+    a = cy3q.QuinticSolver.new_with_sampler(5, 60000, 11, quotient=True)
+    a.orthonormalise_basis_fs_gram()
+    a.donaldson_solve(42, 0.0)
+    g42 = a.h_block
+    b = cy3q.QuinticSolver.new_with_sampler(5, 60000, 11, quotient=True, precision=cy3q.PREC_SPLIT16)
+    b.orthonormalise_basis_fs_gram()
+    b.set_refine(0)
+    b.donaldson_solve(40, 0.0)
+    h40 = b.h_block
+    b.donaldson_solve(2, 0.0)
+    e_split = _rel(b.h_block, g42)
+    out = {}
+    for kind in (cy3q.REFINE_T_FP64, cy3q.REFINE_T_FP64, cy3q.REFINE_STEP_FP64):
+        b.h_block = h40
+        b.set_refine(2)
+        b.set_refine_kind(kind)
+        b.donaldson_solve(2, 0.0)
+        out.setdefault(kind, []).append(b.h_block)
+    t64, step = out[cy3q.REFINE_T_FP64], out[cy3q.REFINE_STEP_FP64][0]
+    assert np.array_equal(t64[0], t64[1])
+    assert _rel(t64[0], g42) < e_split
+    assert _rel(step, g42) < e_split
+    assert _rel(t64[0], step) < 1e-6
+    with pytest.raises(cy3q.Cy3qError):
+        b.set_refine_kind(7)
+
