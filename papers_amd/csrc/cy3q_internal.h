@@ -239,6 +239,7 @@ void launch_conv_ratio(hipStream_t st, const double2 *y, const double2 *yh, int6
 // ---------------------------------------------------------------------------------------
 struct SyrkPlan {
   int64_t m_cols = 0;     // M
+  int prog13 = 0;         // 1: k_syrk13_f64 (M = 416, fixed wave programs)
   int tile = 16;          // sub-tile edge (16 for FP64 MFMA, 32 for split16)
   int n_groups = 0;       // M / tile
   int n_parts = 0;        // workgroups per point chunk

@@ -256,13 +256,300 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
   }
 }
 
+// ----------------------------------------------------------------------------------------
+// Weighted SYRK for M = 416 (13 column groups of 32: k = 6 with the quotient basis) with
+// wave programs fixed at compile time. The 91 lower 32 x 32 tiles go to 16 waves in two parts
+// (workgroups of 8 waves per point chunk): part 0 holds tile rows 0..8 (and copies column
+// groups 0..8 only), part 1 rows 9..12. A wave's sub-blocks are rectangles or lower triangles
+// of tiles; it holds the row fragments of a sub-block and streams its column fragments, 4 f64
+// 16 x 16 MFMAs per tile and 4-point k step, the column fragments scaled by c_p. At most 6 tiles
+// per wave (192 accumulator VGPRs), and every wave runs its own copy of the stage loop, so no
+// accumulators are merged across programs.
+// Output: partials[chunk][tile (I (I + 1) / 2 + J)][2][2][256], in MFMA output order.
+// ----------------------------------------------------------------------------------------
+struct S13Sub {
+  int r0, nr, c0, nc, tri;
+};
+struct S13Prog {
+  int n;
+  S13Sub s[2];
+};
+static constexpr int S13_KP = 16;
+static constexpr int S13_NT = 13;
+static constexpr int S13_M = 32 * S13_NT;
+static constexpr int S13_LD = S13_M + 2;
+
+__host__ __device__ constexpr S13Prog s13_prog(int part, int w) {
+  constexpr S13Prog tab[2][8] = {
+      {{1, {{0, 3, 0, 3, 1}, {}}}, {1, {{3, 3, 3, 3, 1}, {}}}, {1, {{6, 3, 6, 3, 1}, {}}},
+       {1, {{3, 3, 0, 2, 0}, {}}}, {1, {{3, 3, 2, 1, 0}, {}}}, {1, {{6, 3, 0, 2, 0}, {}}},
+       {1, {{6, 3, 2, 2, 0}, {}}}, {1, {{6, 3, 4, 2, 0}, {}}}},
+      {{1, {{9, 3, 9, 3, 1}, {}}}, {1, {{9, 3, 0, 2, 0}, {}}}, {1, {{9, 3, 2, 2, 0}, {}}},
+       {1, {{9, 3, 4, 2, 0}, {}}}, {1, {{9, 3, 6, 2, 0}, {}}},
+       {2, {{9, 3, 8, 1, 0}, {12, 1, 12, 1, 1}}}, {1, {{12, 1, 0, 6, 0}, {}}},
+       {1, {{12, 1, 6, 6, 0}, {}}}}};
+  return tab[part][w];
+}
+__host__ __device__ constexpr int s13_tiles(const S13Sub &b) {
+  return b.nr <= 0 ? 0 : (b.tri ? b.nr * (b.nr + 1) / 2 : b.nr * b.nc);
+}
+__host__ __device__ constexpr int s13_sub_off(int part, int w, int k) {
+  int o = 0;
+  const S13Prog p = s13_prog(part, w);
+  for (int q = 0; q < k; q++) o += s13_tiles(p.s[q]);
+  return o;
+}
+// Every lower tile in exactly one program, at most 6 tiles per wave, part 0 within groups 0..8.
+__host__ __device__ constexpr bool s13_valid() {
+  int cover[S13_NT * S13_NT] = {};
+  for (int part = 0; part < 2; part++)
+    for (int w = 0; w < 8; w++) {
+      const S13Prog p = s13_prog(part, w);
+      int t = 0;
+      for (int k = 0; k < p.n; k++) {
+        const S13Sub b = p.s[k];
+        if (b.tri && (b.r0 != b.c0 || b.nr != b.nc)) return false;
+        t += s13_tiles(b);
+        for (int i = 0; i < b.nr; i++)
+          for (int j = 0; j < b.nc; j++) {
+            if (b.tri && j > i) continue;
+            const int I = b.r0 + i, J = b.c0 + j;
+            if (I >= S13_NT || J > I || (part == 0 && I > 8)) return false;
+            cover[I * S13_NT + J]++;
+          }
+      }
+      if (t > 6) return false;
+    }
+  for (int i = 0; i < S13_NT; i++)
+    for (int j = 0; j <= i; j++)
+      if (cover[i * S13_NT + j] != 1) return false;
+  return true;
+}
+static_assert(s13_valid(), "FP64 SYRK wave programs must cover the lower triangle once");
+
+// One sub-block for one 4-point k step. rowp: this lane's LDS position in the k step's row
+// (point 4 kk + lane / 16, column lane % 16).
+template <int R0, int NR, int C0, int NC, int TRI, int OFF>
+__device__ __forceinline__ void s13_sub(const double *rowp, double cp, double4_t (&acc)[24]) {
+  double a[NR][2];
+#pragma unroll
+  for (int i = 0; i < NR; i++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) a[i][h] = rowp[(R0 + i) * 32 + 16 * h];
+#pragma unroll
+  for (int j = 0; j < NC; j++) {
+    double b[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) b[h] = cp * rowp[(C0 + j) * 32 + 16 * h];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+      if (TRI && i < j) continue;
+      const int t = OFF + (TRI ? i * (i + 1) / 2 + j : i * NC + j);
+#pragma unroll
+      for (int ha = 0; ha < 2; ha++)
+#pragma unroll
+        for (int hb = 0; hb < 2; hb++)
+          acc[4 * t + 2 * ha + hb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              a[i][ha], b[hb], acc[4 * t + 2 * ha + hb], 0, 0, 0);
+    }
+  }
+}
+
+template <int PART, int W, int K>
+__device__ __forceinline__ void s13_run(const double *rowp, double cp, double4_t (&acc)[24]) {
+  constexpr S13Prog P = s13_prog(PART, W);
+  if constexpr (K < P.n) {
+    constexpr S13Sub b = P.s[K];
+    s13_sub<b.r0, b.nr, b.c0, b.nc, b.tri, s13_sub_off(PART, W, K)>(rowp, cp, acc);
+    s13_run<PART, W, K + 1>(rowp, cp, acc);
+  }
+}
+
+template <int PART, int W, int K>
+__device__ __forceinline__ void s13_store(double *__restrict__ out, const double4_t (&acc)[24],
+                                          int lane) {
+  constexpr S13Prog P = s13_prog(PART, W);
+  if constexpr (K < P.n) {
+    constexpr S13Sub b = P.s[K];
+    constexpr int OFF = s13_sub_off(PART, W, K);
+#pragma unroll
+    for (int i = 0; i < b.nr; i++)
+#pragma unroll
+      for (int j = 0; j < b.nc; j++) {
+        if (b.tri && j > i) continue;
+        const int t = OFF + (b.tri ? i * (i + 1) / 2 + j : i * b.nc + j);
+        const int I = b.r0 + i, J = b.c0 + j;
+        double *o = out + (int64_t)(I * (I + 1) / 2 + J) * 1024;
+#pragma unroll
+        for (int ha = 0; ha < 2; ha++)
+#pragma unroll
+          for (int hb = 0; hb < 2; hb++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) o[(ha * 2 + hb) * 256 + e * 64 + lane] = acc[4 * t + 2 * ha + hb][e];
+      }
+    s13_store<PART, W, K + 1>(out, acc, lane);
+  }
+}
+
+template <int PART, int W>
+__device__ __forceinline__ void s13_wave(const double *sxb, const double *scb, int lane,
+                                         double4_t (&acc)[24]) {
+#pragma unroll
+  for (int kk = 0; kk < S13_KP / 4; kk++) {
+    const int kr = 4 * kk + (lane >> 4);
+    s13_run<PART, W, 0>(sxb + kr * S13_LD + (lane & 15), scb[kr], acc);
+  }
+}
+
+// One stage: S13_KP point rows of X (row stride S13_LDB bytes in LDS) copied straight from
+// global memory into LDS with 16-byte global_load_lds copies, 1 KB per copy and wave (lane l to
+// byte 16 l of the copy), so no staging registers are live across the MFMAs. Copy g of a stage
+// belongs to wave g % 8. Bytes of a copy past a row's 416 columns (the row padding) or past the
+// stage fetch a valid element and are never read; rows past the chunk's last point fetch that
+// point again and get coefficient 0. PART 0 copies only its column groups 0..8.
+static constexpr int S13_LDB = S13_LD * 8;                                   // bytes per LDS row
+static constexpr int S13_STAGE_B = (S13_KP * S13_LDB + 1023) / 1024 * 1024;  // bytes per buffer
+static constexpr int S13_COPIES = S13_STAGE_B / 1024;
+static constexpr int S13_CPW = (S13_COPIES + 7) / 8;  // copies per wave and stage
+
+// The whole stage loop of wave W of part PART: each wave has its own copy of the loop, so its
+// accumulators never merge with another program's.
+template <int PART, int W>
+__device__ __forceinline__ void s13_body(const double *__restrict__ x, const double *__restrict__ cw,
+                                         int64_t n, int64_t chunk_id, int64_t chunk,
+                                         double *__restrict__ partials, char *sx,
+                                         double (*sc)[S13_KP]) {
+  constexpr int ROWB = PART == 0 ? 9 * 32 * 8 : S13_M * 8;  // bytes copied per row
+  const int tid = threadIdx.x, lane = tid & 63;
+  constexpr int wave = W;
+  const int64_t p_begin = chunk_id * chunk;
+  const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
+  const int n_stage = (int)((p_end - p_begin + S13_KP - 1) / S13_KP);
+  // Per copy slot: the LDS row and the byte offset within the source row (-1: padding).
+  int crow[S13_CPW], cbyte[S13_CPW];
+#pragma unroll
+  for (int u = 0; u < S13_CPW; u++) {
+    const int g = wave + 8 * u;
+    const int o = g * 1024 + lane * 16;
+    const int r = o / S13_LDB, b = o % S13_LDB;
+    const bool live = g < S13_COPIES && r < S13_KP && b < ROWB;
+    crow[u] = live ? r : 0;
+    cbyte[u] = live ? b : (g < S13_COPIES ? 0 : -1);  // -1: no copy for this slot
+  }
+  auto issue = [&](int st, int buf) {
+    const int64_t p0 = p_begin + (int64_t)st * S13_KP;
+    const int last = (int)((p_end - p0 < S13_KP ? p_end - p0 : S13_KP) - 1);
+    const char *xb = reinterpret_cast<const char *>(x + p0 * S13_M);
+    char *dst = sx + buf * S13_STAGE_B;
+#pragma unroll
+    for (int u = 0; u < S13_CPW; u++) {
+      if (wave + 8 * u < S13_COPIES) {
+        const int r = crow[u] < last ? crow[u] : last;
+        const int b = cbyte[u] < 0 ? 0 : cbyte[u];
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(xb + (int64_t)r * (S13_M * 8) + b),
+            (__attribute__((address_space(3))) void *)(dst + (wave + 8 * u) * 1024), 16, 0, 0);
+      }
+    }
+  };
+  double4_t acc[24];
+#pragma unroll
+  for (int t = 0; t < 24; t++) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  double rc = 0.0;
+  auto fetch_c = [&](int st) {
+    const int64_t p0 = p_begin + (int64_t)st * S13_KP;
+    if (tid < S13_KP) rc = p0 + tid < p_end ? cw[p0 + tid] : 0.0;
+  };
+  if (n_stage > 0) {
+    issue(0, 0);
+    fetch_c(0);
+    if (tid < S13_KP) sc[0][tid] = rc;
+  }
+  // vmcnt(0), lgkmcnt(0): this wave's copies and LDS writes have landed; then the barrier.
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0x70);
+  __syncthreads();
+  asm volatile("" ::: "memory");
+  for (int st = 0; st < n_stage; st++) {
+    const int buf = st & 1;
+    if (st + 1 < n_stage) {
+      issue(st + 1, buf ^ 1);
+      fetch_c(st + 1);
+    }
+    s13_wave<PART, W>(reinterpret_cast<const double *>(sx + buf * S13_STAGE_B), sc[buf], lane,
+                      acc);
+    if (st + 1 < n_stage && tid < S13_KP) sc[buf ^ 1][tid] = rc;
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x70);
+    __syncthreads();
+    asm volatile("" ::: "memory");
+  }
+  s13_store<PART, W, 0>(partials + chunk_id * (int64_t)(91 * 1024), acc, lane);
+}
+
+__global__ void __launch_bounds__(512) k_syrk13_f64(const double *__restrict__ x,
+                                                    const double *__restrict__ cw, int64_t n,
+                                                    int64_t chunk, int64_t n_chunks,
+                                                    double *__restrict__ partials) {
+  __shared__ __attribute__((aligned(1024))) char sx[2 * S13_STAGE_B];
+  __shared__ double sc[2][S13_KP];
+  // XCD-aware: the two parts of a chunk go to one XCD and share its L2.
+  const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
+  const int part = (int)(qq & 1);
+  const int64_t chunk_id = (qq >> 1) * 8 + xcd;
+  if (chunk_id >= n_chunks) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  switch (part * 8 + wave) {
+    case 0: s13_body<0, 0>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 1: s13_body<0, 1>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 2: s13_body<0, 2>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 3: s13_body<0, 3>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 4: s13_body<0, 4>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 5: s13_body<0, 5>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 6: s13_body<0, 6>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 7: s13_body<0, 7>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 8: s13_body<1, 0>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 9: s13_body<1, 1>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 10: s13_body<1, 2>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 11: s13_body<1, 3>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 12: s13_body<1, 4>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 13: s13_body<1, 5>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    case 14: s13_body<1, 6>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+    default: s13_body<1, 7>(x, cw, n, chunk_id, chunk, partials, sx, sc); break;
+  }
+}
+
+// Chunk sums of the 91 tile partials, in chunk order, into the full 416 x 416 matrix (a diagonal
+// tile keeps its lower elements and mirrors them, so every element has one writer).
+__global__ void k_syrk13_reduce(const double *__restrict__ partials, int64_t n_chunks,
+                                double *__restrict__ xx) {
+  const int64_t per_chunk = 91 * 1024;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per_chunk) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < n_chunks; c++) s += partials[c * per_chunk + idx];
+  const int t = (int)(idx / 1024), el = (int)(idx % 1024);
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= t) I++;
+  const int J = t - I * (I + 1) / 2;
+  const int q = el / 256, e = (el % 256) / 64, ln = el % 64;
+  const int r = 16 * (q >> 1) + 4 * e + ln / 16, c = 16 * (q & 1) + ln % 16;
+  if (I == J && r < c) return;
+  const int64_t gi = 32 * (int64_t)I + r, gj = 32 * (int64_t)J + c;
+  xx[gi * S13_M + gj] = s;
+  xx[gj * S13_M + gi] = s;
+}
+
 int syrk_f64_max_tiles(int64_t m_cols) { return m_cols <= 448 ? 22 : 12; }
 
 void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, const double *c,
                      int64_t n, double *partials) {
   int64_t n_ch8 = (plan.n_chunks + 7) / 8 * 8;
   int64_t grid = n_ch8 * plan.n_parts;
-  if (plan.m_cols <= 448) {
+  if (plan.prog13) {
+    hipLaunchKernelGGL(k_syrk13_f64, dim3((unsigned)(n_ch8 * 2)), dim3(512), 0, st, x, c, n,
+                       plan.chunk, plan.n_chunks, partials);
+  } else if (plan.m_cols <= 448) {
     size_t lds = sizeof(double) * 2 * (16 * (plan.m_cols + 1) + 16);
     hipLaunchKernelGGL((k_syrk_f64<22, 7, 16>), dim3((unsigned)grid), dim3(512), lds, st, x, c, n,
                        plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks,
@@ -310,6 +597,11 @@ __global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t
 
 void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *partials,
                         double *xx_full) {
+  if (plan.prog13) {
+    hipLaunchKernelGGL(k_syrk13_reduce, dim3((unsigned)((91 * 1024 + 255) / 256)), dim3(256), 0,
+                       st, partials, plan.n_chunks, xx_full);
+    return;
+  }
   int64_t per_chunk = (int64_t)plan.n_parts * plan.waves * plan.max_tiles * plan.tile * plan.tile;
   hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)((per_chunk + 255) / 256)), dim3(256), 0, st,
                      partials, plan.d_tiles, plan.n_parts, plan.waves, plan.max_tiles,

@@ -239,6 +239,25 @@ int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_t
                     int64_t target_wgs, int64_t min_chunk, int64_t kp, bool pad_repeat = false,
                     int fixed_tiles = 0) {
   pl.m_cols = s->m;
+  if (tile == 16 && !pad_repeat && fixed_tiles == 0 && s->m == 416) {
+    // k_syrk13_f64: two workgroups of 8 waves per point chunk, 91 tiles of 32 x 32 per chunk
+    // partial; target_wgs / 8 chunks (one 107 KB LDS workgroup per CU at a time).
+    pl.prog13 = 1;
+    pl.tile = 32;
+    pl.waves = 8;
+    pl.n_groups = 13;
+    pl.n_parts = 2;
+    pl.max_tiles = 6;
+    const int64_t kp13 = 16;
+    int64_t want = std::max<int64_t>(1, target_wgs / 8);
+    int64_t by_size = std::max<int64_t>(1, (s->n + min_chunk - 1) / min_chunk);
+    pl.n_chunks = std::max<int64_t>(1, std::min(want, by_size));
+    pl.chunk = (s->n + pl.n_chunks - 1) / pl.n_chunks;
+    pl.chunk = (pl.chunk + kp13 - 1) / kp13 * kp13;
+    pl.n_chunks = (s->n + pl.chunk - 1) / pl.chunk;
+    return CY3Q_OK;
+  }
+  pl.prog13 = 0;
   pl.tile = tile;
   pl.waves = waves;
   pl.n_groups = (int)(s->m / tile);
