@@ -91,6 +91,21 @@ def test_t_operator_matches_oracle(cy3q, oracle):
     assert abs(np.trace(t).real - P.nb) < 1e-11
 
 
+def test_t_operator_k6_quotient_matches_oracle(cy3q, oracle):
+    # k=6 with the quotient basis (2N = 410, padded to M = 416) runs the FP64 T pass with the
+    # compile-time wave programs (k_syrk13_f64), and the FS-Gram through the same kernel.
+    P = _oracle_problem(oracle, 6, 4000, 13, quotient=True)
+    s = cy3q.QuinticSolver.from_points(6, P.pts, quotient=True)
+    s.orthonormalise_basis_fs_gram()
+    sv, _ = s.sections()
+    assert _rel(sv, P.sv) < 1e-11
+    h1, r_ref, (tre, tim, _) = oracle.donaldson_step(P.sv, P.w, P.h, want_t=True)
+    r = s.donaldson_step()
+    assert _rel(s.last_t(), tre + 1j * tim) < 1e-12
+    assert _rel(s.h_block, h1) < 1e-11
+    assert abs(r - r_ref) <= 1e-9 * r_ref
+
+
 def test_sigma_matches_oracle(cy3q, oracle):
     P = _oracle_problem(oracle, 3, 20000, 5)
     s = cy3q.QuinticSolver.from_points(3, P.pts)
