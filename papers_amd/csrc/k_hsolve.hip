@@ -221,32 +221,28 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
     }
     if (i0 == j % 8) rowb[b][k] = rv[j / 8];
     __syncthreads();
+    // All operand reads of the step are issued together, before the pivot test, and the update
+    // is branch-free (the inactive elements keep their values through selects), so one LDS
+    // wait covers the step.
     const double piv = colb[b][j].r;
-    if (!(piv > 0.0) || !isfinite(piv)) {
-      ok = false;  // the same pivot for every thread: all leave at the same step
-      break;
-    }
-    const double inv_l = rsqrt_f64(piv), ipiv = inv_l * inv_l;
-    if (tid <= j) {
-      const Cz x = rowb[b][tid];
-      e[j][tid] = Cz{x.r * inv_l, x.i * inv_l};
-    }
-    const Cz ck = colb[b][k], rk = rowb[b][k];
+    const Cz ck = colb[b][k], rk = rowb[b][k], xe = rowb[b][tid & (TB - 1)];
+    Cz dl[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) dl[u] = colb[b][i0 + 8 * u];
+    // A bad pivot (the same for every thread) marks the factorisation failed; the remaining
+    // steps run on a unit pivot, without a branch, and their results are discarded.
+    ok = ok && piv > 0.0 && isfinite(piv);
+    const double inv_l = rsqrt_f64(ok ? piv : 1.0), ipiv = inv_l * inv_l;
+    if (tid <= j) e[j][tid] = Cz{xe.r * inv_l, xe.i * inv_l};
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int i = i0 + 8 * u;
-      if (i <= j) continue;
-      const Cz dl = colb[b][i];
-      if (k > j) {
-        if (k > i) continue;
-        const Cz p = cmulc(dl, ck);
-        dv[u].r -= p.r * ipiv;
-        dv[u].i -= p.i * ipiv;
-      } else {
-        const Cz p = cmul(dl, rk);
-        rv[u].r -= p.r * ipiv;
-        rv[u].i -= p.i * ipiv;
-      }
+      const bool left = i > j && k > j && k <= i, right = i > j && k <= j;
+      const Cz pl = cmulc(dl[u], ck), pr = cmul(dl[u], rk);
+      const double lr = dv[u].r - pl.r * ipiv, li = dv[u].i - pl.i * ipiv;
+      const double rr_ = rv[u].r - pr.r * ipiv, ri = rv[u].i - pr.i * ipiv;
+      dv[u] = Cz{left ? lr : dv[u].r, left ? li : dv[u].i};
+      rv[u] = Cz{right ? rr_ : rv[u].r, right ? ri : rv[u].i};
     }
   }
   __syncthreads();
