@@ -86,13 +86,14 @@ __device__ inline void store_tile(double *__restrict__ dst, int n, int r0, int c
 // T from xx = X^T diag(c) X:  T_ab = s * (xx[2a][2b] + xx[2a+1][2b+1] + i (xx[2a][2b+1] -
 // xx[2a+1][2b])), with s = hscale * dscale * inv_w * (N / tr) (quintic.rs:1783-1812;
 // inv_w = 1/sum_w if sum_w > 1e-12 else 0; the trace factor only if trace_normalise and
-// tr > 1e-10). One small workgroup computes s; the fill is spread over many.
+// tr > 1e-10). Every workgroup of the fill computes s itself (one launch instead of two).
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_t_scale(const double *__restrict__ xx, int64_t m, int nb,
-                                                 const double *__restrict__ sum_w, double hscale,
-                                                 const double *__restrict__ dscale,
-                                                 int trace_normalise, double *__restrict__ out) {
-  __shared__ double sh[256];
+// The scale s, computed by every workgroup of k_t_fill in the same order (so every workgroup
+// gets the same bits): a strided sum over 256 threads, then a tree in LDS.
+__device__ double t_scale_value(const double *__restrict__ xx, int64_t m, int nb,
+                                const double *__restrict__ sum_w, double hscale,
+                                const double *__restrict__ dscale, int trace_normalise,
+                                double *sh) {
   double base = hscale;
   if (dscale) base *= dscale[0];
   const double sw = *sum_w;
@@ -106,21 +107,22 @@ __global__ void __launch_bounds__(256) k_t_scale(const double *__restrict__ xx, 
     if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double trace = sh[0];
-    const double s = (trace_normalise && trace > 1e-10) ? (double)nb / trace : 1.0;
-    out[0] = base * inv_w * s;
-  }
+  const double trace = sh[0];
+  const double s = (trace_normalise && trace > 1e-10) ? (double)nb / trace : 1.0;
+  return base * inv_w * s;
 }
 
 // t (the caller-visible T), a (the elimination's left half), r (its right half, = I).
-__global__ void k_t_fill(const double *__restrict__ xx, int64_t m, int nb,
-                         const double *__restrict__ scale, double *__restrict__ t,
-                         double *__restrict__ a, double *__restrict__ r) {
+__global__ void __launch_bounds__(256) k_t_fill(const double *__restrict__ xx, int64_t m, int nb,
+                                                const double *__restrict__ sum_w, double hscale,
+                                                const double *__restrict__ dscale,
+                                                int trace_normalise, double *__restrict__ t,
+                                                double *__restrict__ a, double *__restrict__ r) {
+  __shared__ double sh[256];
+  const double s = t_scale_value(xx, m, nb, sum_w, hscale, dscale, trace_normalise, sh);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)nb * nb) return;
   const int ia = (int)(e / nb), ib = (int)(e % nb);
-  const double s = scale[0];
   const double rr = xx[(2 * ia) * m + 2 * ib], ii = xx[(2 * ia + 1) * m + 2 * ib + 1];
   const double ri = xx[(2 * ia) * m + 2 * ib + 1], ir = xx[(2 * ia + 1) * m + 2 * ib];
   const double re = (rr + ii) * s, im = (ri - ir) * s;
@@ -137,11 +139,10 @@ __global__ void k_t_fill(const double *__restrict__ xx, int64_t m, int nb,
 void launch_t_from_xx(hipStream_t st, const double *xx, int64_t m_cols, int nb,
                       const double *sum_w, double hscale, const double *dscale,
                       int trace_normalise, double *scale_buf, double *t, double *a, double *r) {
-  hipLaunchKernelGGL(k_t_scale, dim3(1), dim3(256), 0, st, xx, m_cols, nb, sum_w, hscale, dscale,
-                     trace_normalise, scale_buf);
+  (void)scale_buf;  // the scale is recomputed inside k_t_fill
   const int64_t tot = (int64_t)nb * nb;
   hipLaunchKernelGGL(k_t_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, xx, m_cols,
-                     nb, scale_buf, t, a, r);
+                     nb, sum_w, hscale, dscale, trace_normalise, t, a, r);
 }
 
 // Copies a complex matrix into the elimination buffers (a = src, r = I).

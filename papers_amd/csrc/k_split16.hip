@@ -49,9 +49,11 @@ __global__ void __launch_bounds__(1024) k_bfrag_scale(const double *__restrict__
                                                       double *__restrict__ bscale) {
   __shared__ double sh[1024];
   double mx = 0.0;
-  for (int64_t e = threadIdx.x; e < (int64_t)nb * nb; e += 1024) {
-    const int64_t row = e / nb, col = e % nb;
-    if (col <= row) mx = fmax(mx, fmax(fabs(w[2 * e]), fabs(w[2 * e + 1])));
+  // 32 rows at a time, 32 lanes along each row's lower part (no index division).
+  for (int row = (int)(threadIdx.x >> 5); row < nb; row += 32) {
+    const double *wr = w + 2 * (int64_t)row * nb;
+    for (int col = (int)(threadIdx.x & 31); col <= row; col += 32)
+      mx = fmax(mx, fmax(fabs(wr[2 * col]), fabs(wr[2 * col + 1])));
   }
   sh[threadIdx.x] = mx;
   __syncthreads();
