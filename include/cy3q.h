@@ -341,6 +341,27 @@ int cy3q_batch_autotune(cy3q_batch *b, const char *cache_path, int64_t *n_measur
 int cy3q_galerkin_stage_times(double *ms, int64_t *count, int cap, int *n_stages);
 const char *cy3q_galerkin_stage_name(int stage);
 
+/* ---- The route34 T-operator contract: Cy3DonaldsonGpu (route34/cy3_donaldson_gpu.rs:164-333),
+ * the GPU T-operator the TY / Schoen metric solvers call. A workspace without points: the
+ * caller's section values and weights are uploaded once, and each call maps h (route34
+ * convention: K_p = sum_ab h_re Re(conj(s_a) s_b) + h_im Im(conj(s_a) s_b), i.e. h = conj(G) of
+ * the quintic solver) to the unnormalised h_pair[a][b] = sum_p (w_p / max(K_p, 1e-30))
+ * conj(s_a) s_b over points with finite w_p > 0 (no division by sum w, no trace).
+ * Differences: h must be Hermitian positive definite (K is formed as |W s|^2 with W^H W =
+ * conj(h); CY3Q_ERR_NOT_PD otherwise), and the sums run in a fixed order, not the reference's
+ * 256-lane tree (results agree to rounding). */
+typedef struct cy3q_top cy3q_top;
+/* Cy3DonaldsonGpu::new (:182-209); 2 n_basis <= 1024. */
+int cy3q_top_new(int64_t n_points, int64_t n_basis, int device, cy3q_top **out);
+void cy3q_top_free(cy3q_top *t);
+/* Cy3DonaldsonGpu::upload_static (:211-247): section_values[n_points][2 n_basis] ((re, im) per
+ * basis element), weights[n_points]. */
+int cy3q_top_upload_static(cy3q_top *t, const double *section_values, const double *weights);
+/* Cy3DonaldsonGpu::t_operator_raw (:249-320): h_re, h_im, out_re, out_im are n_basis x
+ * n_basis row-major. */
+int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im, double *out_re,
+                            double *out_im);
+
 #ifdef __cplusplus
 }
 #endif

@@ -223,6 +223,27 @@ def donaldson_step(sv: np.ndarray, w: np.ndarray, h_block: np.ndarray, want_t: b
     return h_new, r
 
 
+def t_operator_raw_route34(section_values, weights, h_re, h_im):
+    """Cy3DonaldsonGpu::t_operator_raw (route34/cy3_donaldson_gpu.rs:249-320) with the kernels'
+    rules: K_p = sum_ab h_re Re(conj(s_a) s_b) + h_im Im(conj(s_a) s_b) (:62-92), raised to 1e-30
+    when not above it (a NaN included, :88-89); h_pair[a][b] = sum (w_p / K_p) conj(s_a) s_b over
+    the points with finite w_p > 0 and finite K_p > 0 (:100-160). cpu_t_operator_reference
+    (:338-386) is the same sum without the weight test. Returns (h_pair_re, h_pair_im)."""
+    sv = np.asarray(section_values, dtype=np.float64)
+    n = sv.shape[0]
+    sv = sv.reshape(n, -1)
+    s = sv[:, 0::2] + 1j * sv[:, 1::2]
+    h = np.asarray(h_re, dtype=np.float64) + 1j * np.asarray(h_im, dtype=np.float64)
+    # Re(conj(h_ab) conj(s_a) s_b) = h_re Re(conj(s_a) s_b) + h_im Im(conj(s_a) s_b)
+    k = np.einsum("pa,ab,pb->p", s.conj(), h.conj(), s).real
+    k = np.where(k > 1e-30, k, 1e-30)
+    w = np.asarray(weights, dtype=np.float64)
+    ok = np.isfinite(w) & (w > 0) & np.isfinite(k) & (k > 0)
+    f = np.where(ok, w / np.where(ok, k, 1.0), 0.0)
+    hp = (s.conj() * f[:, None]).T @ s
+    return hp.real.copy(), hp.imag.copy()
+
+
 def donaldson_solve(sv, w, h_block, max_iter: int, tol: float):
     """donaldson_solve (quintic.rs:433-443): returns (h, residual history)."""
     res = []

@@ -1678,3 +1678,134 @@ int cy3q_solver_synchronize(cy3q_solver *s) {
 }
 
 }  // extern "C"
+
+// ---- The route34 T-operator contract (Cy3DonaldsonGpu, route34/cy3_donaldson_gpu.rs:164-333) --
+// A solver workspace without points: the caller's section values (n_points x n_basis complex,
+// (re, im) interleaved) and weights are the static inputs, and each call maps a route34 h
+// (h_route34 = conj(G) of this library's G, so K_p = s^H G s is the route34
+// sum h_re Re(conj(s_a) s_b) + h_im Im(conj(s_a) s_b)) to the unnormalised
+// h_pair = sum_p (w_p / max(K_p, 1e-30)) conj(s_a) s_b over points with finite w_p > 0.
+struct cy3q_top {
+  cy3q_solver *s = nullptr;
+  bool uploaded = false;
+  std::vector<double> xx_host;
+};
+
+extern "C" {
+
+int cy3q_top_new(int64_t n_points, int64_t n_basis, int device, cy3q_top **out) {
+  if (!out) return fail(CY3Q_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_points <= 0 || n_basis <= 0)
+    return fail(CY3Q_ERR_ARG, "n_points and n_basis must be > 0");
+  if (padded_cols(n_basis) > 1024) return fail(CY3Q_ERR_ARG, "2 n_basis > 1024 is not supported");
+  cy3q_solver *s = new cy3q_solver();
+  cy3q_options_default(&s->opt);
+  s->opt.device = device;
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    s->st = nullptr;
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  auto bail = [&](int rc) {
+    delete s;
+    return rc;
+  };
+  s->nb = (int)n_basis;
+  s->m = padded_cols(n_basis);
+  s->n = n_points;
+  const int64_t n = s->n, m = s->m, nn = n_basis * n_basis;
+  const int nblk = (s->nb + 31) / 32;
+  int rc = CY3Q_OK;
+  if ((rc = s->alloc(&s->w, n)) || (rc = s->alloc(&s->wvalid, n)) ||
+      (rc = s->alloc(&s->sum_w, 1)) || (rc = s->alloc(&s->sv, n * m)) ||
+      (rc = s->alloc(&s->g, 2 * nn)) || (rc = s->alloc(&s->wmat, 2 * nn)) ||
+      (rc = s->alloc(&s->t, 2 * nn)) || (rc = s->alloc(&s->wnew, 2 * nn)) ||
+      (rc = s->alloc(&s->gnew, 2 * nn)) || (rc = s->alloc(&s->bmat, m * m)) ||
+      (rc = s->alloc(&s->xx, m * m)) || (rc = s->alloc(&s->coef, n)) ||
+      (rc = s->alloc(&s->dpart, (m / 32) * n)) || (rc = s->alloc(&s->resid, 1)) ||
+      (rc = s->alloc(&s->status, 4)) || (rc = s->alloc(&s->abuf, 2 * nn)) ||
+      (rc = s->alloc(&s->rbuf, 2 * nn)) ||
+      (rc = s->alloc(&s->einv, (size_t)nblk * 32 * 32 * 2)) ||
+      (rc = s->alloc(&s->partial, (size_t)nblk * nblk)) || (rc = s->alloc(&s->scale_buf, 2)))
+    return bail(rc);
+  if ((rc = build_syrk_plan(s, s->plan64, 16, 8, syrk_f64_max_tiles(s->m), s->syrk_target, 256,
+                            16)))
+    return bail(rc);
+  s->partials_cap = syrk_partials_count(s->plan64);
+  if ((rc = s->alloc(&s->partials, s->partials_cap))) return bail(rc);
+  cy3q_top *t = new cy3q_top();
+  t->s = s;
+  t->xx_host.resize((size_t)(m * m));
+  *out = t;
+  return CY3Q_OK;
+}
+
+void cy3q_top_free(cy3q_top *t) {
+  if (!t) return;
+  delete t->s;
+  delete t;
+}
+
+int cy3q_top_upload_static(cy3q_top *t, const double *section_values, const double *weights) {
+  if (!t || !section_values || !weights) return fail(CY3Q_ERR_ARG, "NULL argument");
+  cy3q_solver *s = t->s;
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t n = s->n, m = s->m, two_n = 2 * (int64_t)s->nb;
+  // Row p of sv: the caller's 2 n_basis values, then zero padding to m columns.
+  HIP_TRY(hipMemsetAsync(s->sv, 0, sizeof(double) * n * m, s->st));
+  HIP_TRY(hipMemcpy2DAsync(s->sv, sizeof(double) * m, section_values, sizeof(double) * two_n,
+                           sizeof(double) * two_n, n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  launch_valid_weights(s->st, s->w, n, s->wvalid);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  t->uploaded = true;
+  return CY3Q_OK;
+}
+
+int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im, double *out_re,
+                            double *out_im) {
+  if (!t || !h_re || !h_im || !out_re || !out_im) return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (!t->uploaded) return fail(CY3Q_ERR_STATE, "cy3q_top_upload_static first");
+  cy3q_solver *s = t->s;
+  HIP_TRY(hipSetDevice(s->device));
+  const int nb = s->nb;
+  const int64_t n = s->n, m = s->m, nn = (int64_t)nb * nb;
+  // G = conj(h_route34), then W with W^H W = G for the D pass.
+  std::vector<double> gh((size_t)(2 * nn));
+  for (int64_t e = 0; e < nn; e++) {
+    gh[2 * e] = h_re[e];
+    gh[2 * e + 1] = -h_im[e];
+  }
+  HIP_TRY(hipMemcpyAsync(s->g, gh.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice, s->st));
+  bool pd = false;
+  ST_TRY(w_from_g_pd(s, &pd));
+  if (!pd) return fail(CY3Q_ERR_NOT_PD, "h is not Hermitian positive definite");
+  ensure_bmat(s);
+  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
+  launch_point_coeffs_f64(s->st, s->w, s->dpart, (int)(m / 32), n, s->coef);
+  launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
+  launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(t->xx_host.data(), s->xx, sizeof(double) * m * m, hipMemcpyDeviceToHost,
+                         s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  const double *xx = t->xx_host.data();
+  for (int a = 0; a < nb; a++)
+    for (int b = 0; b < nb; b++) {
+      const int64_t i = 2 * a, j = 2 * b;
+      out_re[(int64_t)a * nb + b] = xx[i * m + j] + xx[(i + 1) * m + j + 1];
+      out_im[(int64_t)a * nb + b] = xx[i * m + j + 1] - xx[(i + 1) * m + j];
+    }
+  return CY3Q_OK;
+}
+
+}  // extern "C"

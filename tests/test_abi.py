@@ -52,6 +52,13 @@ def test_bad_arguments_fail_loudly(cy3q):
     assert L.cy3q_solver_sigma_refine_lbfgs_restarts(None, 1, 1, 4, 0.01, 1, C.byref(best),
                                                      None) == -1
     assert L.cy3q_solver_set_refine_kind(None, cy3q.REFINE_T_FP64) == -1
+    from papers_amd.donaldson_gpu import _lib as top_lib
+    L = top_lib()  # binds the cy3q_top_* signatures
+    out = C.c_void_p()
+    assert L.cy3q_top_new(0, 4, 0, C.byref(out)) == -1
+    assert L.cy3q_top_new(10, 0, 0, C.byref(out)) == -1
+    assert L.cy3q_top_upload_static(None, None, None) == -1
+    assert L.cy3q_top_t_operator_raw(None, None, None, None, None) == -1
 
 
 def test_laplacian_basis_table_bit_exact(cy3q, oracle):

@@ -1,0 +1,88 @@
+"""The route34 T-operator contract on the GPU (cy3q_top_*, Cy3DonaldsonGpu of
+route34/cy3_donaldson_gpu.rs) against the oracle's restatement, on random section values with
+an SPD h = A^H A + n I, as the reference's parity test builds them
+(route34/tests/test_cy3_donaldson_gpu_parity.rs). Tolerance: 1e-12 relative (FP64 MFMA sums
+in a fixed order against numpy's)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _case(seed, n, nb):
+    rng = np.random.default_rng(seed)
+    sv = rng.standard_normal((n, 2 * nb))
+    w = rng.uniform(0.05, 2.0, n)
+    a = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+    h = a.conj().T @ a + nb * np.eye(nb)
+    return sv, w, h
+
+
+@pytest.mark.parametrize("n,nb", [(5000, 35), (3000, 87), (2000, 205)])
+def test_t_operator_raw_matches_oracle(cy3q, oracle, n, nb):
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu
+    print(f"t_operator_raw n={n} nb={nb}", flush=True)
+    sv, w, h = _case(42 + nb, n, nb)
+    g = Cy3DonaldsonGpu(n, nb)
+    g.upload_static(sv, w)
+    re, im = g.t_operator_raw(h.real, h.imag)
+    rr, ri = oracle.t_operator_raw_route34(sv, w, h.real, h.imag)
+    got, ref = re + 1j * im, (rr + 1j * ri).ravel()
+    assert _rel(got, ref) < 1e-12
+    # Repeated calls with new h reuse the static upload.
+    re2, im2 = g.t_operator_raw(2.0 * h.real, 2.0 * h.imag)
+    assert _rel(re2 + 1j * im2, ref / 2.0) < 1e-12
+    g.close()
+
+
+def test_t_operator_raw_skips_bad_weights(cy3q, oracle):
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu
+    sv, w, h = _case(3, 1000, 20)
+    w[::7] = np.nan
+    w[3::11] = 0.0
+    w[5::13] = -1.0
+    g = Cy3DonaldsonGpu(1000, 20)
+    g.upload_static(sv, w)
+    re, im = g.t_operator_raw(h.real, h.imag)
+    rr, ri = oracle.t_operator_raw_route34(sv, w, h.real, h.imag)
+    assert np.all(np.isfinite(re)) and np.all(np.isfinite(im))
+    assert _rel(re + 1j * im, (rr + 1j * ri).ravel()) < 1e-12
+
+
+def test_t_operator_raw_errors(cy3q):
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu
+    g = Cy3DonaldsonGpu(100, 4)
+    with pytest.raises(cy3q.Cy3qError):
+        g.t_operator_raw(np.eye(4), np.zeros((4, 4)))  # before upload_static
+    rng = np.random.default_rng(1)
+    g.upload_static(rng.standard_normal((100, 8)), np.ones(100))
+    with pytest.raises(cy3q.Cy3qError):
+        g.t_operator_raw(-np.eye(4), np.zeros((4, 4)))  # not positive definite
+    with pytest.raises(ValueError):
+        g.upload_static(np.zeros((10, 8)), np.ones(10))
+    g.close()
+
+
+def test_route34_convention_matches_solver_t(cy3q):
+    # The solver's last T (quintic convention) at G equals the route34 h_pair at h = conj(G),
+    # divided by sum w and trace-normalised.
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu
+    s = cy3q.QuinticSolver.new_with_sampler(3, 6000, 17)
+    s.orthonormalise_basis_fs_gram()
+    s.donaldson_solve(2, 0.0)
+    hb = s.h_block
+    G = hb[0::2, 0::2] - 1j * hb[0::2, 1::2]
+    sv, _ = s.sections()
+    w = s.weights
+    s.donaldson_step()
+    t = s.last_t()
+    g = Cy3DonaldsonGpu(s.n_points, s.n_basis)
+    g.upload_static(sv[:, : 2 * s.n_basis], w)
+    re, im = g.t_operator_raw(G.real, -G.imag)
+    tr = (re + 1j * im).reshape(s.n_basis, s.n_basis) / np.sum(w[np.isfinite(w) & (w > 0)])
+    tr *= s.n_basis / np.trace(tr).real
+    assert _rel(tr, t) < 1e-12
