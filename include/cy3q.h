@@ -362,6 +362,23 @@ int cy3q_top_upload_static(cy3q_top *t, const double *section_values, const doub
 int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im, double *out_re,
                             double *out_im);
 
+/* ---- The route34 sigma contract: Cy3SigmaGpu (route34/cy3_sigma_gpu.rs:314-560), the sigma
+ * evaluator of the TY / Schoen solvers on 8 ambient coordinates with caller-supplied frames.
+ * section_values[n][2 n_basis], section_derivs[n][8][2 n_basis] (d/dz_i, i = 0..7), frames[n][48]
+ * (T_a[i] = (re, im) at 2 (8 a + i), a = 0..2), omega_sq[n] = |Omega|^2, weights[n]. h is used
+ * as given (K = s^H h s, :80-107) and must be Hermitian positive definite. sigma =
+ * sum w (eta / kappa - 1)^2 / sum w over points with finite eta > 0 and w > 0, NaN when no
+ * point is valid; kappa = sum w eta / sum w. Differences: the sums run in a fixed order, and a
+ * total weight below 1e-12 (the reference: 1e-30) gives NaN. */
+typedef struct cy3q_sig8 cy3q_sig8;
+int cy3q_sig8_new(int64_t n_points, int64_t n_basis, int device, cy3q_sig8 **out);  /* :336-381 */
+void cy3q_sig8_free(cy3q_sig8 *g);
+int cy3q_sig8_upload_static(cy3q_sig8 *g, const double *section_values,               /* :383-443 */
+                            const double *section_derivs, const double *frames,
+                            const double *omega_sq, const double *weights);
+int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im,   /* :445-547 */
+                            double *sigma);
+
 #ifdef __cplusplus
 }
 #endif

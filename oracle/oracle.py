@@ -244,6 +244,49 @@ def t_operator_raw_route34(section_values, weights, h_re, h_im):
     return hp.real.copy(), hp.imag.copy()
 
 
+def sigma_route34(section_values, section_derivs, frames, omega_sq, weights, h_re, h_im):
+    """cpu_sigma_reference (route34/cy3_sigma_gpu.rs:585-721): per point K = s^H H s (raised to
+    1e-30), dk_i = s^H H ds_i, g_amb_ij = (ds_j^H H ds_i) / K - dk_i conj(dk_j) / K^2,
+    g_tan = conj(T) g_amb T^T with the 3 x 8 frame T, eta = |Re det g_tan| / |Omega|^2 for points
+    with finite w > 0 and |Omega|^2 > 0 and a finite det of magnitude >= 1e-30; then
+    kappa = sum w eta / sum w and sigma = sum w (eta / kappa - 1)^2 / sum w (NaN when the weight
+    sum or kappa is below 1e-30). Returns (sigma, eta)."""
+    sv = np.asarray(section_values, dtype=np.float64)
+    n = sv.shape[0]
+    sv = sv.reshape(n, -1)
+    nb = sv.shape[1] // 2
+    s = sv[:, 0::2] + 1j * sv[:, 1::2]
+    d = np.asarray(section_derivs, dtype=np.float64).reshape(n, 8, 2 * nb)
+    ds = d[..., 0::2] + 1j * d[..., 1::2]
+    H = np.asarray(h_re, dtype=np.float64) + 1j * np.asarray(h_im, dtype=np.float64)
+    hs = s @ H.T
+    hds = ds @ H.T
+    k = np.einsum("pa,pa->p", s.conj(), hs).real
+    ks = np.maximum(k, 1e-30)
+    dk = np.einsum("pa,pia->pi", s.conj(), hds)
+    m = np.einsum("pja,pia->pij", ds.conj(), hds)
+    g = m / ks[:, None, None] - dk[:, :, None] * dk.conj()[:, None, :] / (ks * ks)[:, None, None]
+    f = np.asarray(frames, dtype=np.float64).reshape(n, 3, 8, 2)
+    t = f[..., 0] + 1j * f[..., 1]
+    gt = np.einsum("pai,pij,pbj->pab", t.conj(), g, t)
+    m1 = gt[:, 1, 1] * gt[:, 2, 2] - gt[:, 1, 2] * gt[:, 2, 1]
+    m2 = gt[:, 1, 0] * gt[:, 2, 2] - gt[:, 1, 2] * gt[:, 2, 0]
+    m3 = gt[:, 1, 0] * gt[:, 2, 1] - gt[:, 1, 1] * gt[:, 2, 0]
+    det = (gt[:, 0, 0] * m1 - gt[:, 0, 1] * m2 + gt[:, 0, 2] * m3).real
+    w = np.asarray(weights, dtype=np.float64)
+    om = np.asarray(omega_sq, dtype=np.float64)
+    ok = (np.isfinite(w) & (w > 0) & np.isfinite(om) & (om > 0) & np.isfinite(det)
+          & (np.abs(det) >= 1e-30))
+    eta = np.where(ok, np.abs(det) / np.where(ok, om, 1.0), np.nan)
+    tw = float(np.sum(w[ok]))
+    if tw < 1e-30:
+        return float("nan"), eta
+    kap = float(np.sum(w[ok] * eta[ok])) / tw
+    if abs(kap) < 1e-30:
+        return float("nan"), eta
+    return float(np.sum(w[ok] * (eta[ok] / kap - 1.0) ** 2)) / tw, eta
+
+
 def donaldson_solve(sv, w, h_block, max_iter: int, tol: float):
     """donaldson_solve (quintic.rs:433-443): returns (h, residual history)."""
     res = []

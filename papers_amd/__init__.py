@@ -22,4 +22,4 @@ from .quintic import (  # noqa: F401
     monge_ampere_residual_quintic,
     n_sections_quintic,
 )
-from .donaldson_gpu import Cy3DonaldsonGpu  # noqa: F401,E402
+from .donaldson_gpu import Cy3DonaldsonGpu, Cy3SigmaGpu  # noqa: F401,E402

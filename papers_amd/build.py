@@ -39,6 +39,7 @@ SOURCES = {
     "k_eigen.hip": [],
     "k_tridiag.hip": [],
     "k_yukawa.hip": [],
+    "k_sigma8.hip": [],
     "solver.cpp": ["-x", "hip"],
     "galerkin.cpp": ["-x", "hip"],
     "batch.cpp": ["-x", "hip"],

@@ -82,6 +82,11 @@ void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t 
                           const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum);
 void launch_coeffs_f64_from_split(hipStream_t st, const double *w, const double *dsum,
                                   const float *rs2, const double *bscale, int64_t n, double *c);
+// Per-point eta of the 8-coordinate sigma contract (k_sigma8.hip): y [n][9][m], frames [n][48].
+void launch_pack_rows9(hipStream_t st, const double *sv, const double *sd, int64_t n,
+                       int64_t two_n, int64_t m, double *x9);
+void launch_eta8(hipStream_t st, const double *y, int64_t m, int nb, const double *frames,
+                 const double *om, const double *w, int64_t n, double *eta);
 void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, const float *rs2,
                            const double *bscale, int64_t n, float *u, float *block_max,
                            float *zscale, double *tscale);

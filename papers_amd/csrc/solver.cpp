@@ -1809,3 +1809,137 @@ int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im,
 }
 
 }  // extern "C"
+
+// ---- The route34 sigma contract (Cy3SigmaGpu, route34/cy3_sigma_gpu.rs:314-560) ------------
+// Static inputs: section values [n][2N], their 8 ambient derivatives [n][8][2N], the 3 x 8
+// complex tangent frames [n][48], |Omega|^2 and weights. Per call, h (used as given:
+// K = s^H h s, route34/cy3_sigma_gpu.rs:80-107) gives W with W^H W = h, the contracted rows of
+// each point chunk (k_trigemm_f64 over 9 rows per point), eta (k_eta8) and sigma =
+// sum w (eta / kappa - 1)^2 / sum w over valid points (the two-pass reduction).
+struct cy3q_sig8 {
+  cy3q_solver *s = nullptr;
+  bool uploaded = false;
+  double *x9 = nullptr, *y9 = nullptr, *frames = nullptr, *om = nullptr, *eta = nullptr;
+  int64_t chunk = 0;
+};
+
+extern "C" {
+
+int cy3q_sig8_new(int64_t n_points, int64_t n_basis, int device, cy3q_sig8 **out) {
+  if (!out) return fail(CY3Q_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_points <= 0 || n_basis <= 0)
+    return fail(CY3Q_ERR_ARG, "n_points and n_basis must be > 0");
+  if (padded_cols(n_basis) > 1024) return fail(CY3Q_ERR_ARG, "2 n_basis > 1024 is not supported");
+  cy3q_solver *s = new cy3q_solver();
+  cy3q_options_default(&s->opt);
+  s->opt.device = device;
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    s->st = nullptr;
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  cy3q_sig8 *g = new cy3q_sig8();
+  g->s = s;
+  auto bail = [&](int rc) {
+    delete g->s;
+    delete g;
+    return rc;
+  };
+  s->nb = (int)n_basis;
+  s->m = padded_cols(n_basis);
+  s->n = n_points;
+  const int64_t n = s->n, m = s->m, nn = n_basis * n_basis;
+  const int nblk = (s->nb + 31) / 32;
+  g->chunk = std::min<int64_t>(n, 32768);
+  int rc = CY3Q_OK;
+  if ((rc = s->alloc(&s->w, n)) || (rc = s->alloc(&g->om, n)) || (rc = s->alloc(&g->eta, n)) ||
+      (rc = s->alloc(&g->frames, n * 48)) || (rc = s->alloc(&g->x9, n * 9 * m)) ||
+      (rc = s->alloc(&g->y9, g->chunk * 9 * m)) || (rc = s->alloc(&s->g, 2 * nn)) ||
+      (rc = s->alloc(&s->wmat, 2 * nn)) || (rc = s->alloc(&s->wnew, 2 * nn)) ||
+      (rc = s->alloc(&s->gnew, 2 * nn)) || (rc = s->alloc(&s->bmat, m * m)) ||
+      (rc = s->alloc(&s->status, 4)) || (rc = s->alloc(&s->abuf, 2 * nn)) ||
+      (rc = s->alloc(&s->rbuf, 2 * nn)) ||
+      (rc = s->alloc(&s->einv, (size_t)nblk * 32 * 32 * 2)) ||
+      (rc = s->alloc(&s->partial, (size_t)nblk * nblk)) || (rc = s->alloc(&s->sig_out, 8)) ||
+      (rc = s->alloc(&s->sig_scratch, sigma_reduce_scratch())))
+    return bail(rc);
+  *out = g;
+  return CY3Q_OK;
+}
+
+void cy3q_sig8_free(cy3q_sig8 *g) {
+  if (!g) return;
+  delete g->s;
+  delete g;
+}
+
+int cy3q_sig8_upload_static(cy3q_sig8 *g, const double *section_values,
+                            const double *section_derivs, const double *frames,
+                            const double *omega_sq, const double *weights) {
+  if (!g || !section_values || !section_derivs || !frames || !omega_sq || !weights)
+    return fail(CY3Q_ERR_ARG, "NULL argument");
+  cy3q_solver *s = g->s;
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t n = s->n, m = s->m, two_n = 2 * (int64_t)s->nb;
+  // x9 row 9 p: the section values of point p, rows 9 p + 1 + i: d/dz_i; zero padding to m.
+  // The host arrays go to staging buffers, then one kernel packs the rows.
+  double *sv_raw = nullptr, *sd_raw = nullptr;
+  ST_TRY(s->alloc(&sv_raw, n * two_n));
+  ST_TRY(s->alloc(&sd_raw, n * 8 * two_n));
+  HIP_TRY(hipMemcpyAsync(sv_raw, section_values, sizeof(double) * n * two_n, hipMemcpyHostToDevice,
+                         s->st));
+  HIP_TRY(hipMemcpyAsync(sd_raw, section_derivs, sizeof(double) * n * 8 * two_n,
+                         hipMemcpyHostToDevice, s->st));
+  launch_pack_rows9(s->st, sv_raw, sd_raw, n, two_n, m, g->x9);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(sv_raw);
+  s->release(sd_raw);
+  HIP_TRY(hipMemcpyAsync(g->frames, frames, sizeof(double) * n * 48, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(g->om, omega_sq, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  g->uploaded = true;
+  return CY3Q_OK;
+}
+
+int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im, double *sigma) {
+  if (!g || !h_re || !h_im || !sigma) return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (!g->uploaded) return fail(CY3Q_ERR_STATE, "cy3q_sig8_upload_static first");
+  cy3q_solver *s = g->s;
+  HIP_TRY(hipSetDevice(s->device));
+  const int nb = s->nb;
+  const int64_t n = s->n, m = s->m, nn = (int64_t)nb * nb;
+  std::vector<double> gh((size_t)(2 * nn));
+  for (int64_t e = 0; e < nn; e++) {
+    gh[2 * e] = h_re[e];
+    gh[2 * e + 1] = h_im[e];
+  }
+  HIP_TRY(hipMemcpyAsync(s->g, gh.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice, s->st));
+  bool pd = false;
+  ST_TRY(w_from_g_pd(s, &pd));
+  if (!pd) return fail(CY3Q_ERR_NOT_PD, "h is not Hermitian positive definite");
+  ensure_bmat(s);
+  for (int64_t p0 = 0; p0 < n; p0 += g->chunk) {
+    const int64_t cn = std::min(g->chunk, n - p0);
+    launch_trigemm_f64(s->st, g->x9 + p0 * 9 * m, cn * 9, m, s->bmat, g->y9, 0, 0);
+    launch_eta8(s->st, g->y9, m, nb, g->frames + p0 * 48, g->om + p0, s->w + p0, cn, g->eta + p0);
+  }
+  launch_sigma_reduce(s->st, g->eta, s->w, n, s->sig_scratch, s->sig_out);
+  HIP_TRY(hipGetLastError());
+  double o[6];
+  HIP_TRY(hipMemcpyAsync(o, s->sig_out, sizeof(o), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  *sigma = o[2];
+  return CY3Q_OK;
+}
+
+}  // extern "C"

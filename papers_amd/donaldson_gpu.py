@@ -1,4 +1,7 @@
-"""The route34 GPU T-operator contract: ``Cy3DonaldsonGpu`` of the reference
+"""The route34 GPU contracts of the 8-coordinate (Tian-Yau, Schoen) metric solvers: the
+T-operator ``Cy3DonaldsonGpu`` and the sigma evaluator ``Cy3SigmaGpu``.
+
+T-operator: ``Cy3DonaldsonGpu`` of the reference
 (``route34/cy3_donaldson_gpu.rs:164-333``), the T-operator the Tian-Yau and Schoen metric
 solvers call once per Donaldson iteration, over ``cy3q_top_*`` of ``libcy3q.so``.
 
@@ -17,7 +20,7 @@ import numpy as np
 
 from .quintic import ERR_NOT_PD, Cy3qError, _check, _load, _ptr
 
-__all__ = ["Cy3DonaldsonGpu"]
+__all__ = ["Cy3DonaldsonGpu", "Cy3SigmaGpu"]
 
 _bound = False
 
@@ -32,6 +35,11 @@ def _lib():
         L.cy3q_top_free.restype = None
         L.cy3q_top_upload_static.argtypes = [vp, vp, vp]
         L.cy3q_top_t_operator_raw.argtypes = [vp, vp, vp, vp, vp]
+        L.cy3q_sig8_new.argtypes = [i64, i64, ip, C.POINTER(vp)]
+        L.cy3q_sig8_free.argtypes = [vp]
+        L.cy3q_sig8_free.restype = None
+        L.cy3q_sig8_upload_static.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.cy3q_sig8_compute_sigma.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
         _bound = True
     return L
 
@@ -77,3 +85,50 @@ class Cy3DonaldsonGpu:
             raise Cy3qError(rc, "h is not Hermitian positive definite")
         _check(rc)
         return out_re, out_im
+
+
+class Cy3SigmaGpu:
+    """Cy3SigmaGpu::{new, upload_static, compute_sigma} (route34/cy3_sigma_gpu.rs:314-547):
+    sigma = sum w (eta / kappa - 1)^2 / sum w on 8 ambient coordinates with caller-supplied
+    3 x 8 tangent frames; h is used as given (K = s^H h s) and must be Hermitian positive
+    definite."""
+
+    def __init__(self, n_points: int, n_basis: int, device: int = 0):
+        h = C.c_void_p()
+        _check(_lib().cy3q_sig8_new(n_points, n_basis, device, C.byref(h)))
+        self._h = h
+        self.n_points, self.n_basis = int(n_points), int(n_basis)
+
+    def close(self):
+        if self._h:
+            _lib().cy3q_sig8_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_static(self, section_values, section_derivs, frames, omega_sq, weights):
+        n, two_n = self.n_points, 2 * self.n_basis
+        arrs = [np.ascontiguousarray(a, np.float64) for a in
+                (section_values, section_derivs, frames, omega_sq, weights)]
+        for a, size, name in zip(arrs, (n * two_n, n * 8 * two_n, n * 48, n, n),
+                                 ("section_values", "section_derivs", "frames", "omega_sq",
+                                  "weights")):
+            if a.size != size:
+                raise ValueError(f"{name} has {a.size} values; expected {size}")
+        _check(_lib().cy3q_sig8_upload_static(self._h, *[_ptr(a) for a in arrs]))
+
+    def compute_sigma(self, h_re, h_im) -> float:
+        nb = self.n_basis
+        hr = np.ascontiguousarray(h_re, np.float64).reshape(nb * nb)
+        hi = np.ascontiguousarray(h_im, np.float64).reshape(nb * nb)
+        out = C.c_double()
+        rc = _lib().cy3q_sig8_compute_sigma(self._h, _ptr(hr), _ptr(hi), C.byref(out))
+        if rc == ERR_NOT_PD:
+            raise Cy3qError(rc, "h is not Hermitian positive definite")
+        _check(rc)
+        return out.value
+

@@ -59,6 +59,10 @@ def test_bad_arguments_fail_loudly(cy3q):
     assert L.cy3q_top_new(10, 0, 0, C.byref(out)) == -1
     assert L.cy3q_top_upload_static(None, None, None) == -1
     assert L.cy3q_top_t_operator_raw(None, None, None, None, None) == -1
+    assert L.cy3q_sig8_new(0, 4, 0, C.byref(out)) == -1
+    assert L.cy3q_sig8_upload_static(None, None, None, None, None, None) == -1
+    sig = C.c_double()
+    assert L.cy3q_sig8_compute_sigma(None, None, None, C.byref(sig)) == -1
 
 
 def test_laplacian_basis_table_bit_exact(cy3q, oracle):

@@ -86,3 +86,45 @@ def test_route34_convention_matches_solver_t(cy3q):
     tr = (re + 1j * im).reshape(s.n_basis, s.n_basis) / np.sum(w[np.isfinite(w) & (w > 0)])
     tr *= s.n_basis / np.trace(tr).real
     assert _rel(tr, t) < 1e-12
+
+
+def _sigma_case(seed, n, nb):
+    rng = np.random.default_rng(seed)
+    sv = rng.standard_normal((n, 2 * nb))
+    sd = rng.standard_normal((n, 8, 2 * nb))
+    frames = rng.standard_normal((n, 3, 8, 2))
+    om = rng.uniform(0.2, 3.0, n)
+    w = rng.uniform(0.05, 2.0, n)
+    a = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+    h = a.conj().T @ a / nb + np.eye(nb)
+    return sv, sd, frames, om, w, h
+
+
+@pytest.mark.parametrize("n,nb", [(3000, 20), (1500, 87)])
+def test_sigma8_matches_oracle(cy3q, oracle, n, nb):
+    from papers_amd.donaldson_gpu import Cy3SigmaGpu
+    print(f"sigma8 n={n} nb={nb}", flush=True)
+    sv, sd, frames, om, w, h = _sigma_case(7 + nb, n, nb)
+    w[::17] = np.nan
+    om[5::23] = 0.0
+    g = Cy3SigmaGpu(n, nb)
+    g.upload_static(sv, sd, frames, om, w)
+    sig = g.compute_sigma(h.real, h.imag)
+    ref, _ = oracle.sigma_route34(sv, sd, frames, om, w, h.real, h.imag)
+    assert np.isfinite(ref) and abs(sig - ref) <= 1e-10 * abs(ref)
+    g.close()
+
+
+def test_sigma8_fubini_study_closed_form(cy3q):
+    # Linear sections, H = I, frames orthonormal with T_a . z = 0: eta = 1 / (|z|^6 |Omega|^2).
+    from papers_amd.donaldson_gpu import Cy3SigmaGpu
+    import test_oracle_top as T
+    rng = np.random.default_rng(9)
+    sv, sd, frames, om, w, z = T._fs_case(rng, 500)
+    g = Cy3SigmaGpu(500, 8)
+    g.upload_static(sv, sd, frames, om, w)
+    sig = g.compute_sigma(np.eye(8), np.zeros((8, 8)))
+    eta = 1.0 / (np.sum(np.abs(z) ** 2, axis=1) ** 3 * om)
+    kap = np.sum(w * eta) / np.sum(w)
+    ref = np.sum(w * (eta / kap - 1) ** 2) / np.sum(w)
+    assert abs(sig - ref) <= 1e-11 * ref

@@ -71,3 +71,40 @@ def test_route34_t_operator_scaling_and_hermitian(oracle):
     assert np.allclose(i2, i1 / 4.0, rtol=1e-14, atol=1e-16)
     assert np.allclose(r1, r1.T, rtol=1e-14, atol=0)
     assert np.allclose(i1, -i1.T, rtol=1e-14, atol=1e-16)
+
+
+def _fs_case(rng, n):
+    # Linear sections s_a = z_a (N = 8, ds_i = e_i), H = I and frames T orthonormal with
+    # T_a . z = 0: g_amb is the Fubini-Study metric delta/|z|^2 - conj(z_i) z_j / |z|^4, its
+    # restriction to the frame is I / |z|^2, so eta = 1 / (|z|^6 |Omega|^2) exactly.
+    z = rng.standard_normal((n, 8)) + 1j * rng.standard_normal((n, 8))
+    sv = np.zeros((n, 16))
+    sv[:, 0::2], sv[:, 1::2] = z.real, z.imag
+    sd = np.zeros((n, 8, 16))
+    for i in range(8):
+        sd[:, i, 2 * i] = 1.0
+    frames = np.zeros((n, 3, 8, 2))
+    for p in range(n):
+        # orthonormal complement of conj(z) (Hermitian), so that T_a . z = <conj(z), T_a> = 0
+        q, _ = np.linalg.qr(np.column_stack([z[p].conj(), rng.standard_normal((8, 7))
+                                             + 1j * rng.standard_normal((8, 7))]))
+        t = q[:, 1:4].T
+        frames[p, ..., 0], frames[p, ..., 1] = t.real, t.imag
+    om = rng.uniform(0.5, 2.0, n)
+    w = rng.uniform(0.1, 1.0, n)
+    return sv, sd, frames, om, w, z
+
+
+def test_route34_sigma_fubini_study_closed_form(oracle):
+    rng = np.random.default_rng(5)
+    sv, sd, frames, om, w, z = _fs_case(rng, 60)
+    sig, eta = oracle.sigma_route34(sv, sd, frames, om, w, np.eye(8), np.zeros((8, 8)))
+    k = np.sum(np.abs(z) ** 2, axis=1)
+    eta_ref = 1.0 / (k ** 3 * om)
+    assert np.allclose(eta, eta_ref, rtol=1e-11, atol=0)
+    kap = np.sum(w * eta_ref) / np.sum(w)
+    assert abs(sig - np.sum(w * (eta_ref / kap - 1) ** 2) / np.sum(w)) < 1e-11 * sig
+    # H -> c H scales K, dk and M by c, which leaves g_amb, eta and sigma unchanged.
+    sig2, eta2 = oracle.sigma_route34(sv, sd, frames, om, w, 3.0 * np.eye(8), np.zeros((8, 8)))
+    assert np.allclose(eta2, eta_ref, rtol=1e-11, atol=0)
+    assert abs(sig2 - sig) < 1e-12 * sig
