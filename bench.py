@@ -233,6 +233,67 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_route34(args) -> int:
+    """The route34 GPU contracts of the 8-coordinate solvers (Cy3DonaldsonGpu, Cy3SigmaGpu) on
+    synthetic inputs of a Tian-Yau k=3 size: n_basis = 87, 200,000 points (random section
+    values, derivatives and frames, uniform weights; an SPD h = A^H A / N + I). `value` is
+    t_operator_raw calls per second (each call uploads h, factors it, runs the FP64 D and T
+    passes and downloads h_pair, as the reference's call does); compute_sigma calls per second
+    are reported beside it."""
+    import numpy as np
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu, Cy3SigmaGpu
+
+    n, nb = (200_000 if args.n_points == 1_000_000 else args.n_points), 87
+    rng = np.random.default_rng(args.seed)
+    sv = rng.standard_normal((n, 2 * nb))
+    w = rng.uniform(0.1, 1.0, n)
+    a = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+    h = a.conj().T @ a / nb + np.eye(nb)
+    _log(f"route34 T-operator: n={n}, n_basis={nb}")
+    top = Cy3DonaldsonGpu(n, nb)
+    top.upload_static(sv, w)
+    for _ in range(max(args.warmup, 1)):
+        top.t_operator_raw(h.real, h.imag)
+    reps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        top.t_operator_raw(h.real, h.imag)
+    t_top = (time.perf_counter() - t0) / reps
+    top.close()
+    sd = rng.standard_normal((n, 8, 2 * nb))
+    frames = rng.standard_normal((n, 3, 8, 2))
+    om = rng.uniform(0.5, 2.0, n)
+    _log("route34 sigma")
+    sg = Cy3SigmaGpu(n, nb)
+    sg.upload_static(sv, sd, frames, om, w)
+    sig = sg.compute_sigma(h.real, h.imag)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sg.compute_sigma(h.real, h.imag)
+    t_sig = (time.perf_counter() - t0) / reps
+    sg.close()
+    out = {
+        "metric": "route34 t_operator_raw calls/s (TY-sized: n_basis 87, 200k points)",
+        "value": 1.0 / t_top,
+        "unit": "calls/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": t_top * 1e3,
+        "dtype": "f64",
+        "data": "synthetic: random section values, derivatives and frames, seed %d" % args.seed,
+        "config": {"workload": "route34 Cy3DonaldsonGpu / Cy3SigmaGpu contracts",
+                   "n_points": n, "n_basis": nb},
+        "t_operator_raw_ms": t_top * 1e3,
+        "compute_sigma_ms": t_sig * 1e3,
+        "sigma": sig,
+        "vs_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def bench_adjoint(args) -> int:
     """Differentiation workload: Fermat quintic, k = 5 quotient basis (N = 125), 400,000 SZ points
     (seed 42), FS-Gram, 10 FP64 Donaldson iterations. Then (a) sigma_L2^2 and its analytic
@@ -366,13 +427,14 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["tstep", "galerkin", "batch", "adjoint"],
+    ap.add_argument("--workload", choices=["tstep", "galerkin", "batch", "adjoint", "route34"],
                     default="tstep",
                     help="tstep: the headline T-operator benchmark; galerkin: the Laplacian and "
                          "Yukawa assembly on a balanced k=8 Dwork metric; batch: 32 Dwork "
                          "instances balanced in one captured graph vs one after another; "
                          "adjoint: sigma^2 gradient, finite-difference check and 200 Adam steps "
-                         "at k=5, 400k points")
+                         "at k=5, 400k points; route34: the 8-coordinate T-operator and sigma "
+                         "contracts at n_basis 87, 200k points")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy kernel of this many MiB (3 "
                          "launches): the known-size reference for scaling rocprofv3 "
@@ -402,6 +464,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "route34":
+        return bench_route34(args)
 
     import numpy as np
     import papers_amd as P
