@@ -544,7 +544,8 @@ def main() -> int:
         gbs = bytes_pass[name] / (avg * 1e-3) / 1e9
         stages[name] = {"avg_ms": avg, "tflops": tf, "gbs": gbs}
     dom = max(stages, key=lambda x: stages[x]["avg_ms"])
-    kern = {"d_pass": "k_dpass_split16", "t_pass": "k_tpass"}[dom]
+    kern = {"d_pass": "k_dpass_split16_b13" if m_cols == 416 else "k_dpass_split16",
+            "t_pass": "k_tpass"}[dom]
     d = stages[dom]
     mfma_frac = d["tflops"] / PEAKS_TFLOPS["fp16"]
     hbm_frac = d["gbs"] / HBM_PEAK_GBS

@@ -406,6 +406,252 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Balanced D pass for M = 416 (13 k-steps, 26 column groups). Column group cg is active in
+// k-steps ks <= cg / 2 (its level), so the per-k-step work is triangular: 26 - 2 ks groups. With
+// one barrier per k-step every wave waits for the busiest, and the round-robin kernel above
+// issues sum_ks ceil((26 - 2 ks) / 7) = 32 group-steps per wave for 26 of useful work.
+// Here k-steps j and 12 - j (j = 0..5) share one barrier (a super-step; k-step 6 is the last
+// super-step alone), and the groups are dealt so that every wave does the same work in every
+// super-step: wave m = 0..5 owns {2m, 2m+1} (level m) and {22-2m, 23-2m} (level 11-m), wave 6
+// owns {24, 25} (level 12). A level pair (a, 11-a) is active in exactly one k-step of each pair
+// (j, 12-j) and in k-step 6, so wave m runs 4 group-steps per super-step (2 in the last one),
+// as does wave 6: 26 per wave in 7 barriers, with no zero-operand products.
+// Ring position p of a block (the mover's order) holds k-step p / 2 for even p and
+// 12 - (p - 1) / 2 for odd p, so each super-step's two k-steps are adjacent in the ring.
+// Everything else (ring, mover, B prefetch, row sums) is as in k_dpass_split16.
+// ---------------------------------------------------------------------------------------------
+namespace b13 {
+constexpr int NKS = 13, NCG = 26, NSS = 7;
+__host__ __device__ constexpr int pos_ks(int p) { return (p & 1) ? 12 - (p - 1) / 2 : p / 2; }
+__host__ __device__ constexpr int ss_first(int s) { return 2 * s; }
+__host__ __device__ constexpr int ss_len(int s) { return s < 6 ? 2 : 1; }
+__host__ __device__ constexpr int n_groups(int w) { return w < 6 ? 4 : 2; }
+__host__ __device__ constexpr int group(int w, int c) {
+  return w < 6 ? (c == 0 ? 2 * w : c == 1 ? 2 * w + 1 : c == 2 ? 22 - 2 * w : 23 - 2 * w)
+               : 24 + c;
+}
+__host__ __device__ constexpr bool active(int w, int c, int ks) { return ks <= group(w, c) / 2; }
+// Unit u of wave w in super-step s: (position e within the super-step, group slot c), in the
+// order e, then c; n_units(w, s) of them.
+__host__ __device__ constexpr int n_units(int w, int s) {
+  int n = 0;
+  for (int e = 0; e < ss_len(s); e++)
+    for (int c = 0; c < n_groups(w); c++) n += active(w, c, pos_ks(ss_first(s) + e)) ? 1 : 0;
+  return n;
+}
+__host__ __device__ constexpr int unit_e(int w, int s, int u) {
+  for (int e = 0; e < ss_len(s); e++)
+    for (int c = 0; c < n_groups(w); c++)
+      if (active(w, c, pos_ks(ss_first(s) + e)) && u-- == 0) return e;
+  return -1;
+}
+__host__ __device__ constexpr int unit_c(int w, int s, int u) {
+  for (int e = 0; e < ss_len(s); e++)
+    for (int c = 0; c < n_groups(w); c++)
+      if (active(w, c, pos_ks(ss_first(s) + e)) && u-- == 0) return c;
+  return -1;
+}
+constexpr bool check_balance() {
+  int cover[NKS][NCG] = {};
+  for (int w = 0; w < DP_CW; w++) {
+    int total = 0;
+    for (int s = 0; s < NSS; s++) {
+      const int nu = n_units(w, s);
+      if (nu != (s < 6 ? 4 : 2)) return false;
+      total += nu;
+      for (int u = 0; u < nu; u++) {
+        const int ks = pos_ks(ss_first(s) + unit_e(w, s, u));
+        cover[ks][group(w, unit_c(w, s, u))]++;
+      }
+    }
+    if (total != 26) return false;
+  }
+  // Every active (k-step, group) pair exactly once, no inactive one.
+  for (int ks = 0; ks < NKS; ks++)
+    for (int cg = 0; cg < NCG; cg++)
+      if (cover[ks][cg] != (cg >= 2 * ks ? 1 : 0)) return false;
+  return true;
+}
+static_assert(DP_CW == 7 && check_balance(), "balanced M = 416 assignment");
+}  // namespace b13
+
+template <int W>
+__device__ __forceinline__ void dpass_b13_wave(half8_t (*sa)[DP_SLOT], double (*red)[DP_PTS],
+                                               int64_t n, int64_t nmine, int64_t blk0,
+                                               int64_t bstride, const half8_t *__restrict__ bfrag,
+                                               double *__restrict__ dsum) {
+  using namespace b13;
+  constexpr int NG = n_groups(W);
+  const int tid = threadIdx.x, lane = tid & 63;
+  float4_t acc[DP_TILES][NG];
+  half8_t b0h[4], b0l[4], b1h[4], b1l[4];
+  const half8_t *bfp = bfrag;
+  auto load_b = [&](int s, half8_t *hh, half8_t *ll) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (u >= n_units(W, s)) continue;
+      const int ks = pos_ks(ss_first(s) + unit_e(W, s, u));
+      const int64_t frag = (int64_t)ks * NCG + group(W, unit_c(W, s, u));
+      hh[u] = bfp[(frag * 2 + 0) * 64 + lane];
+      ll[u] = bfp[(frag * 2 + 1) * 64 + lane];
+    }
+  };
+  // Super-step s on ring slots slot0 (position ss_first(s)) and slot0 + 1. The units are
+  // ordered by position, so each position's A fragments are loaded once and used up before
+  // the next position's.
+  auto step = [&](int s, int slot0, const half8_t *bh, const half8_t *bl) {
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r += 2) {
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        if (e >= ss_len(s)) continue;
+        const int sl = (slot0 + e) & (DP_RING - 1);
+        const half8_t ah0 = sa[sl][(r * 2 + 0) * 64 + lane];
+        const half8_t al0 = sa[sl][(r * 2 + 1) * 64 + lane];
+        const half8_t ah1 = sa[sl][(r * 2 + 2) * 64 + lane];
+        const half8_t al1 = sa[sl][(r * 2 + 3) * 64 + lane];
+#pragma unroll
+        for (int pr = 0; pr < 3; pr++) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (u >= n_units(W, s) || unit_e(W, s, u) != e) continue;
+            const int c = unit_c(W, s, u);
+            const half8_t a0 = pr == 2 ? al0 : ah0, a1 = pr == 2 ? al1 : ah1;
+            const half8_t b = pr == 1 ? bl[u] : bh[u];
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a0, acc[r][c], 0, 0, 0);
+            acc[r + 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b, a1, acc[r + 1][c], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  // One block: super-steps 0..6 (even ones on set x, odd ones on set y), then the row sums.
+  auto block = [&](int64_t i, half8_t *xh, half8_t *xl, half8_t *yh, half8_t *yl) {
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r++)
+#pragma unroll
+      for (int c = 0; c < NG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    const int pbase = (int)((i * NKS) & (DP_RING - 1));
+    asm volatile("" : "+s"(bfp));
+    const bool more = i + 1 < nmine;
+#pragma unroll
+    for (int s = 0; s < NSS; s += 2) {
+      __syncthreads();
+      if (s + 1 < NSS)
+        load_b(s + 1, yh, yl);
+      else if (more)
+        load_b(0, yh, yl);
+      step(s, pbase + ss_first(s), xh, xl);
+      if (s + 1 >= NSS) break;
+      __syncthreads();
+      if (s + 2 < NSS)
+        load_b(s + 2, xh, xl);
+      else if (more)
+        load_b(0, xh, xl);
+      step(s + 1, pbase + ss_first(s + 1), yh, yl);
+    }
+    double part[DP_TILES];
+#pragma unroll
+    for (int r = 0; r < DP_TILES; r++) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int c = 0; c < NG; c++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const double y = (double)acc[r][c][e];
+          sacc += y * y;
+        }
+      part[r] = sacc;
+    }
+    const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
+    double h4[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double send = up32 ? part[q] : part[4 + q];
+      const double keep = up32 ? part[4 + q] : part[q];
+      h4[q] = keep + __shfl_xor(send, 32);
+    }
+    double h2[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const double send = up16 ? h4[q] : h4[2 + q];
+      const double keep = up16 ? h4[2 + q] : h4[q];
+      h2[q] = keep + __shfl_xor(send, 16);
+    }
+    const int rbase = 4 * (lane >> 5) + 2 * ((lane >> 4) & 1);
+    red[W][(rbase + 0) * 16 + (lane & 15)] = h2[0];
+    red[W][(rbase + 1) * 16 + (lane & 15)] = h2[1];
+    __syncthreads();
+    if (tid < DP_PTS) {
+      double sacc = 0.0;
+      for (int w = 0; w < DP_CW; w++) sacc += red[w][tid];
+      const int64_t p = (blk0 + i * bstride) * DP_PTS + tid;
+      if (p < n) dsum[p] = sacc;
+    }
+  };
+  load_b(0, b0h, b0l);
+  // NSS is odd, so the sets swap roles from one block to the next.
+  for (int64_t i = 0; i < nmine;) {
+    block(i, b0h, b0l, b1h, b1l);
+    if (++i >= nmine) break;
+    block(i, b1h, b1l, b0h, b0l);
+    ++i;
+  }
+}
+
+__global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16_b13(const half8_t *__restrict__ xd,
+                                                                     int64_t n, int64_t nblk,
+                                                                     const half8_t *__restrict__ bfrag,
+                                                                     double *__restrict__ dsum) {
+  using namespace b13;
+  __shared__ half8_t sa[DP_RING][DP_SLOT];
+  __shared__ double red[DP_CW][DP_PTS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t blk0 = blockIdx.x, bstride = gridDim.x;
+  const int64_t nmine = blk0 < nblk ? (nblk - 1 - blk0) / bstride + 1 : 0;
+  switch (wave) {
+    case 0: dpass_b13_wave<0>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 1: dpass_b13_wave<1>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 2: dpass_b13_wave<2>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 3: dpass_b13_wave<3>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 4: dpass_b13_wave<4>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 5: dpass_b13_wave<5>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    case 6: dpass_b13_wave<6>(sa, red, n, nmine, blk0, bstride, bfrag, dsum); return;
+    default: break;
+  }
+  // Data mover: global position g is ring position g % 13 of this workgroup's block g / 13.
+  // The barrier that opens a super-step comes after its positions' copies have landed; after
+  // it, the slots of the previous super-step are free, so the ring is refilled up to the
+  // super-step's first position + DP_RING - 1.
+  const int64_t total = nmine * NKS;
+  auto issue = [&](int64_t g) {
+    const int64_t blk = blk0 + (g / NKS) * bstride;
+    const int ks = pos_ks((int)(g % NKS));
+    half8_t *slot = sa[(int)(g & (DP_RING - 1))];
+    const half8_t *src = xd + (blk * DP_TILES * NKS * 2) * 64 + lane;
+#pragma unroll
+    for (int b = 0; b < DP_LOADS; b++) {
+      const int rg = b >> 1, pl = b & 1;
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(src + (((int64_t)rg * NKS + ks) * 2 + pl) * 64),
+          (__attribute__((address_space(3))) void *)(slot + b * 64), 16, 0, 0);
+    }
+  };
+  int64_t issued = 0;
+  for (; issued < DP_RING && issued < total; issued++) issue(issued);
+  for (int64_t i = 0; i < nmine; i++) {
+    for (int s = 0; s < NSS; s++) {
+      const int64_t g0 = i * NKS + ss_first(s), g1 = g0 + ss_len(s) - 1;
+      dp_mover_wait(issued - 1 - g1);
+      __builtin_amdgcn_s_barrier();
+      for (; issued < total && issued <= g0 + DP_RING - 1; issued++) issue(issued);
+    }
+    __builtin_amdgcn_s_barrier();  // the block's row reduction
+  }
+}
+
 // n_cu: the device's compute units; the grid is one workgroup per CU (at most one block each).
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
                           const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum) {
@@ -419,7 +665,10 @@ void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t 
     hipLaunchKernelGGL(k_dpass_split16<k>, grid, block, 0, st, xd, n, nblk, bfrag, cglist, dsum); \
     break;
     DP_CASE(1) DP_CASE(2) DP_CASE(3) DP_CASE(4) DP_CASE(5) DP_CASE(6) DP_CASE(7)
-    DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(13) DP_CASE(14)
+    DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(14)
+    case 13:
+      hipLaunchKernelGGL(k_dpass_split16_b13, grid, block, 0, st, xd, n, nblk, bfrag, dsum);
+      break;
 #undef DP_CASE
     default: break;  // rejected by the host (M <= 448)
   }
