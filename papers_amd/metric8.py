@@ -13,9 +13,12 @@ contracts of :mod:`papers_amd.donaldson_gpu`, restating ``route34/ty_metric.rs``
   (restore after 5 consecutive residuals above 100x a minimum below 1e-2, or on a non-finite
   residual), and the ``residual < tol`` stop.
 
+* :class:`TikhonovShift`, :class:`TikhonovGating` and :class:`GatingState` are the
+  residual-dependent Tikhonov schedule and its stall-band gate (``schoen_metric.rs:200-500``),
+  with the auto-default helpers of both callers (``ty_metric.rs:262-310``).
+
 The T-operator and sigma are callables, normally ``Cy3DonaldsonGpu.t_operator_raw`` and
-``Cy3SigmaGpu.compute_sigma``. Only a constant Tikhonov shift is supported (the reference's
-residual-dependent ``TikhonovShift`` schedules and their stall-band gating are not restated).
+``Cy3SigmaGpu.compute_sigma``. The Tikhonov shift is a constant or a :class:`TikhonovShift`.
 """
 from __future__ import annotations
 
@@ -24,12 +27,129 @@ import math
 import numpy as np
 import scipy.linalg as sla
 
-__all__ = ["AdaptiveDamping", "donaldson_iteration_route34", "solve_route34"]
+__all__ = ["AdaptiveDamping", "GatingState", "TikhonovGating", "TikhonovShift",
+           "auto_schoen_gated_tikhonov", "auto_schoen_tikhonov", "auto_ty_gated_tikhonov",
+           "auto_ty_tikhonov", "donaldson_iteration_route34", "resolve_tikhonov", "solve_route34"]
 
 CATASTROPHE_FLOOR = 1.0e-2
 CATASTROPHE_MULTIPLIER = 100.0
 CATASTROPHE_STREAK = 5
 
+
+# Largest lambda as a fraction of the current residual (schoen_metric.rs:420-441).
+TIKHONOV_STEP_FRACTION = 1.0e-2
+
+
+class TikhonovGating:
+    """When the schedule engages (``TikhonovGating``, schoen_metric.rs:245-300): on every
+    iteration (``always_on``), or only once the residual has sat in [residual_lo, residual_hi]
+    with per-iteration ratios in [ratio_lo, ratio_hi] for min_stuck_iters consecutive
+    iterations (``stall_band``)."""
+
+    def __init__(self, stall_band: bool, residual_lo: float = 0.0, residual_hi: float = 0.0,
+                 min_stuck_iters: int = 0, ratio_lo: float = 0.95, ratio_hi: float = 1.05):
+        self.stall_band = bool(stall_band)
+        self.residual_lo, self.residual_hi = float(residual_lo), float(residual_hi)
+        self.min_stuck_iters = int(min_stuck_iters)
+        self.ratio_lo, self.ratio_hi = float(ratio_lo), float(ratio_hi)
+
+    @classmethod
+    def always_on(cls) -> "TikhonovGating":
+        return cls(False)
+
+    @classmethod
+    def schoen_stall_band_default(cls) -> "TikhonovGating":
+        """Band [1e-7, 1e-3], ratios [0.95, 1.05], streak 3 (schoen_metric.rs:288-299)."""
+        return cls(True, 1.0e-7, 1.0e-3, 3, 0.95, 1.05)
+
+
+class GatingState:
+    """The streak counter of the stall-band gate (``GatingState``, schoen_metric.rs:300-375).
+    ``update(r)`` runs after each iteration; ``is_open(r_prev, gating)`` before each inversion."""
+
+    def __init__(self):
+        self.last_residual = None
+        self.in_band_streak = 0
+
+    def update(self, residual: float, ratio_lo: float, ratio_hi: float):
+        if not math.isfinite(residual) or residual <= 0.0:
+            self.in_band_streak = 0
+            self.last_residual = None
+            return
+        prev = self.last_residual
+        if prev is not None and prev > 0.0 and math.isfinite(prev):
+            ratio = residual / prev
+            self.in_band_streak = self.in_band_streak + 1 if ratio_lo <= ratio <= ratio_hi else 0
+        else:
+            self.in_band_streak = 0
+        self.last_residual = residual
+
+    def is_open(self, r_curr: float, gating: TikhonovGating) -> bool:
+        if not gating.stall_band:
+            return True
+        if not math.isfinite(r_curr) or r_curr <= 0.0:
+            return False
+        if r_curr < gating.residual_lo or r_curr > gating.residual_hi:
+            return False
+        return self.in_band_streak >= gating.min_stuck_iters
+
+
+class TikhonovShift:
+    """T(G) -> T(G) + lambda I before the inversion, with lambda from a residual-dependent
+    schedule (``TikhonovShift``, schoen_metric.rs:200-245, 375-445)."""
+
+    def __init__(self, lambda_max: float, lambda_min: float, schedule_exponent: float,
+                 gating: TikhonovGating | None = None):
+        self.lambda_max, self.lambda_min = float(lambda_max), float(lambda_min)
+        self.schedule_exponent = float(schedule_exponent)
+        self.gating = gating if gating is not None else TikhonovGating.always_on()
+
+    @classmethod
+    def k4_default(cls) -> "TikhonovShift":
+        return cls(1.0e-3, 1.0e-9, 1.0, TikhonovGating.always_on())
+
+    @classmethod
+    def k4_gated_default(cls) -> "TikhonovShift":
+        return cls(1.0e-3, 1.0e-9, 1.0, TikhonovGating.schoen_stall_band_default())
+
+    def lambda_at(self, residual_curr: float, residual_init: float) -> float:
+        """max(lambda_max (r / r_init)^p, lambda_min), capped at 1% of r and kept within
+        [lambda_min, lambda_max]; lambda_max while either residual is missing (NaN, <= 0)."""
+        lo = max(self.lambda_min, 0.0)
+        hi = max(self.lambda_max, lo)
+        if (not math.isfinite(residual_init) or residual_init <= 0.0
+                or not math.isfinite(residual_curr) or residual_curr <= 0.0):
+            return hi
+        ratio = min(max(residual_curr / residual_init, 0.0), 1.0)
+        p = max(self.schedule_exponent, 0.0)
+        geom = hi * math.pow(ratio, p)
+        lam = min(geom, residual_curr * TIKHONOV_STEP_FRACTION)
+        return min(max(lam, lo), hi)
+
+
+def resolve_tikhonov(override):
+    """The solvers never engage a schedule on their own: None stays None, a given schedule is
+    used as is (``resolve_ty_tikhonov``, ``resolve_schoen_tikhonov``)."""
+    return override
+
+
+def auto_ty_tikhonov(k_degree: int):
+    """k4_default() for Tian-Yau k >= 4, else None (ty_metric.rs:284-293)."""
+    return TikhonovShift.k4_default() if k_degree >= 4 else None
+
+
+def auto_ty_gated_tikhonov(k_degree: int):
+    return TikhonovShift.k4_gated_default() if k_degree >= 4 else None
+
+
+def auto_schoen_tikhonov(d_x: int, d_y: int, d_t: int):
+    """k4_default() for Schoen bidegrees with d_x + d_y + d_t >= 10, else None
+    (schoen_metric.rs:475-481)."""
+    return TikhonovShift.k4_default() if d_x + d_y + d_t >= 10 else None
+
+
+def auto_schoen_gated_tikhonov(d_x: int, d_y: int, d_t: int):
+    return TikhonovShift.k4_gated_default() if d_x + d_y + d_t >= 10 else None
 
 class AdaptiveDamping:
     """AdaptiveDampingState (schoen_metric.rs:576-625): alpha x 1.5 (capped) after 3 residuals
@@ -109,11 +229,14 @@ def donaldson_iteration_route34(t_operator, h_re, h_im, weights, damping: float 
 
 
 def solve_route34(t_operator, sigma, h_re, h_im, weights, max_iter: int, tol: float,
-                  damping=None, tikhonov: float = 0.0) -> dict:
+                  damping=None, tikhonov=0.0) -> dict:
     """The Donaldson loop of solve_ty_metric. damping: None or a float (static alpha, clamped to
-    [1e-6, 1]) or an AdaptiveDamping. sigma: callable(h_re, h_im) or None. Returns a dict with
-    h_re, h_im, iterations_run, final_donaldson_residual, donaldson_history, sigma_history,
-    sigma_fs_identity."""
+    [1e-6, 1]) or an AdaptiveDamping. tikhonov: a constant shift (0 disables it) or a
+    TikhonovShift; a schedule's lambda for an iteration is lambda_at(previous residual, first
+    finite residual) while its gate is open, else 0 (ty_metric.rs:620-668). sigma:
+    callable(h_re, h_im) or None. Returns a dict with h_re, h_im, iterations_run,
+    final_donaldson_residual, donaldson_history, sigma_history, sigma_fs_identity and
+    tikhonov_history (the lambda of each iteration)."""
     h_re = np.array(h_re, dtype=np.float64)
     h_im = np.array(h_im, dtype=np.float64)
     adaptive = damping if isinstance(damping, AdaptiveDamping) else None
@@ -127,12 +250,31 @@ def solve_route34(t_operator, sigma, h_re, h_im, weights, max_iter: int, tol: fl
     out = {"sigma_fs_identity": sig_fn(h_re, h_im), "donaldson_history": [], "sigma_history": []}
     min_res, min_h, min_it, bad = math.inf, (h_re.copy(), h_im.copy()), 0, 0
     final_res, iters = math.inf, 0
+    shift = tikhonov if isinstance(tikhonov, TikhonovShift) else None
+    gate = GatingState()
+    residual_init, prev_residual = math.nan, math.nan
+    out["tikhonov_history"] = []
     for it in range(eff_max):
         alpha = adaptive.alpha() if adaptive is not None else static
+        if shift is None:
+            lam = float(tikhonov) if tikhonov is not None else 0.0
+        elif gate.is_open(prev_residual, shift.gating):
+            lam = shift.lambda_at(prev_residual, residual_init)
+        else:
+            lam = 0.0
+        out["tikhonov_history"].append(lam)
         h_re_n, h_im_n, r = donaldson_iteration_route34(t_operator, h_re, h_im, weights, alpha,
-                                                        tikhonov)
+                                                        lam)
         if adaptive is not None:
             adaptive.update(r)
+        if shift is not None and shift.gating.stall_band:
+            gate.update(r, shift.gating.ratio_lo, shift.gating.ratio_hi)
+        else:
+            gate.update(r, 0.95, 1.05)
+        if not math.isfinite(residual_init) and math.isfinite(r) and r > 0.0:
+            residual_init = r
+        if math.isfinite(r):
+            prev_residual = r
         if not math.isfinite(r):
             if math.isfinite(min_res):
                 h_re, h_im = min_h[0].copy(), min_h[1].copy()
