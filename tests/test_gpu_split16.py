@@ -29,6 +29,28 @@ def test_split16_step_matches_fp64(cy3q, k, quotient, n):
     assert abs(sb / sa - 1) < 1e-6
 
 
+
+@pytest.mark.parametrize("n", [3001, 70001])
+def test_split16_k6_block_counts(cy3q, n):
+    # M = 416 runs the balanced D pass (k-steps j and 12 - j per barrier). 3001 points give 24
+    # ragged blocks (one per workgroup); 70001 give 547, two or three per workgroup, so the
+    # prefetch across blocks and the register-set swap of an odd block count both run.
+    # The split error of T is a sum of independent per-point rounding errors, so its relative
+    # size grows as 1/sqrt(n): the 2e-6 bound at 40k points becomes 2e-6 sqrt(40000 / n) below
+    # that (7.3e-6 at 3001). A (k-step, column group) unit dealt twice or not at all would move
+    # D_p by a share of one of 26 groups, percents, and T by orders of magnitude more.
+    tol = 2e-6 * max(1.0, (40000.0 / n) ** 0.5)
+    print(f"k=6 quotient split16 vs fp64, n={n}, tol={tol:.2e}", flush=True)
+    a = cy3q.QuinticSolver.new_with_sampler(6, n, 11, quotient=True)
+    b = cy3q.QuinticSolver.new_with_sampler(6, n, 11, quotient=True, precision=cy3q.PREC_SPLIT16)
+    for s in (a, b):
+        s.orthonormalise_basis_fs_gram()
+    for it in range(2):
+        a.donaldson_step()
+        b.donaldson_step()
+        assert _rel(b.last_t(), a.last_t()) < tol, it
+        assert _rel(b.h_block, a.h_block) < tol, it
+
 def test_split16_matches_oracle_one_step(cy3q, oracle):
     P = oracle.Problem(4, 30000, 13)
     s = cy3q.QuinticSolver.from_points(4, P.pts, precision=cy3q.PREC_SPLIT16)
