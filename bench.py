@@ -294,6 +294,97 @@ def bench_route34(args) -> int:
     return 0
 
 
+def bench_config1(args) -> int:
+    """Plain-method workload: Fermat quintic, degree k = 3 (N = 35), 100,000 SZ points (20,000
+    lines x 5 roots, seed 42), FS-Gram, 10 FP64 Donaldson T-operator iterations from h = I, then
+    sigma of the balanced metric on a separate 50,000-point SZ sample (seed 43), as
+    sigma_at_metric_closure (quintic.rs:1520-1616) evaluates it. `value` is the whole protocol
+    per second on the GPU (sampling through the held-out sigma), timed after one untimed run.
+    The cpu_baseline runs the same protocol at full size through the oracle (the FP64
+    restatement of the reference's CPU path) on the host cores."""
+    import numpy as np
+    import papers_amd as P
+
+    n_pts, n_eval, iters, k = 100_000, 50_000, 10, 3
+    seed, eval_seed = args.seed, args.seed + 1
+
+    def protocol():
+        s = P.QuinticSolver.new_with_sampler(k, n_pts, seed)
+        s.orthonormalise_basis_fs_gram()
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.donaldson_solve(iters, 0.0)
+        s.synchronize()
+        t_iter = time.perf_counter() - t0
+        sig_in = s.sigma()
+        s1, kap, l2, n_ev, _, _ = s.sigma_at_sample(n_eval, eval_seed)
+        return s, t_iter, sig_in, s1, kap, n_ev
+
+    protocol()  # warm-up (module load, first allocations)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        s, t_iter, sig_in, sig_out, kap, n_ev = protocol()
+        times.append((time.perf_counter() - t0, t_iter))
+    times.sort()
+    t_all, t_iter = times[1]
+    h_gpu = s.h_block
+    _log(f"protocol {t_all * 1e3:.1f} ms, 10 iterations {t_iter * 1e3:.2f} ms, sigma in-sample "
+         f"{sig_in:.8f}, held-out {sig_out:.8f} on {n_ev} points")
+    cpu = None
+    sig_cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        O.build()
+        _log("cpu baseline: the whole protocol through the oracle")
+        t0 = time.perf_counter()
+        prob = O.Problem(k, n_pts, seed)
+        t_setup = time.perf_counter() - t0
+        h = prob.h
+        t1 = time.perf_counter()
+        for _ in range(iters):
+            h, _ = O.donaldson_step(prob.sv, prob.w, h)
+        t_iters_cpu = time.perf_counter() - t1
+        t2 = time.perf_counter()
+        sig_cpu = O.sigma_at_sample(prob, h, n_eval, eval_seed)[0]
+        t_sig_cpu = time.perf_counter() - t2
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": 1.0 / t_cpu, "unit": "protocol runs/s", "cores": O.lib().orc_num_threads(),
+               "kind": "port",
+               "sample": f"the full protocol (k={k}, {n_pts} points, {iters} iterations, sigma on "
+                         f"{n_eval} fresh points): set-up {t_setup:.2f} s, iterations "
+                         f"{t_iters_cpu:.2f} s ({iters / t_iters_cpu:.2f} T-op iters/s), "
+                         f"held-out sigma {t_sig_cpu:.2f} s",
+               "t_op_iters_per_s": iters / t_iters_cpu,
+               "h_rel_vs_gpu": float(np.linalg.norm(h - h_gpu) / np.linalg.norm(h))}
+        _log(f"cpu: {t_cpu:.2f} s, held-out sigma {sig_cpu:.8f}")
+    out = {
+        "metric": "config-1 protocol runs/s: k=3, 100k pts, 10 FP64 T-op iters, sigma on 50k fresh pts",
+        "value": 1.0 / t_all,
+        "unit": "protocol runs/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "f64",
+        "data": "synthetic: SZ line-intersection sample of the Fermat quintic, LCG seed 42; "
+                "held-out sample seed 43",
+        "config": {"workload": f"plain FP64 method, k={k} N=35, {n_pts} SZ points, FS-Gram, "
+                               f"{iters} T-iterations from h=I, sigma on {n_eval} fresh points",
+                   "k": k, "n_basis": s.n_basis, "n_points": s.n_points},
+        "protocol_ms": t_all * 1e3,
+        "t_op_iters_per_s": iters / t_iter,
+        "sigma_in_sample": sig_in,
+        "sigma_held_out": sig_out,
+        "kappa_held_out": kap,
+        "n_held_out": n_ev,
+        "sigma_held_out_oracle": sig_cpu,
+        "sigma_held_out_rel_vs_oracle": None if sig_cpu is None else abs(sig_out / sig_cpu - 1.0),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def bench_adjoint(args) -> int:
     """Differentiation workload: Fermat quintic, k = 5 quotient basis (N = 125), 400,000 SZ points
     (seed 42), FS-Gram, 10 FP64 Donaldson iterations. Then (a) sigma_L2^2 and its analytic
@@ -427,14 +518,16 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["tstep", "galerkin", "batch", "adjoint", "route34"],
+    ap.add_argument("--workload",
+                    choices=["tstep", "galerkin", "batch", "adjoint", "route34", "config1"],
                     default="tstep",
                     help="tstep: the headline T-operator benchmark; galerkin: the Laplacian and "
                          "Yukawa assembly on a balanced k=8 Dwork metric; batch: 32 Dwork "
                          "instances balanced in one captured graph vs one after another; "
                          "adjoint: sigma^2 gradient, finite-difference check and 200 Adam steps "
                          "at k=5, 400k points; route34: the 8-coordinate T-operator and sigma "
-                         "contracts at n_basis 87, 200k points")
+                         "contracts at n_basis 87, 200k points; config1: the plain k=3 protocol "
+                         "with sigma on a separate 50k sample, and its full-size CPU run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy kernel of this many MiB (3 "
                          "launches): the known-size reference for scaling rocprofv3 "
@@ -466,6 +559,8 @@ def main() -> int:
         return bench_adjoint(args)
     if args.workload == "route34":
         return bench_route34(args)
+    if args.workload == "config1":
+        return bench_config1(args)
 
     import numpy as np
     import papers_amd as P

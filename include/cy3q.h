@@ -137,6 +137,19 @@ int cy3q_solver_sigma(cy3q_solver *s, double *sigma);
 int cy3q_solver_sigma_ext(cy3q_solver *s, double *sigma_l1, double *kappa, double *sigma_l2sq,
                           double *eta);
 
+/* sigma_at_metric_closure and sigma_and_etas_at_metric_closure (quintic.rs:1520-1700) with the
+ * solver's Bergman metric g = ddbar log(s^H G s) of its current G as the metric callback: a fresh
+ * sample of n_points points (sampler and seed as the reference's sample_quintic_points_with,
+ * quintic.rs:4006-4016; the Dwork samplers use the solver's psi, the Fermat samplers need
+ * psi = 0), the CY weights, the sections of the solver's basis at those points carried through
+ * the FS-Gram basis changes already applied to the solver, then sigma as
+ * compute_sigma_from_workspace (quintic.rs:1301-1499). The solver's own points and state are
+ * not touched. *n_actual: points sampled; sigma_l1, kappa and sigma_l2sq are NaN for an empty
+ * sample. eta ([n_points], NaN where skipped) and points ([n_points][10]) may be NULL. */
+int cy3q_solver_sigma_at_sample(cy3q_solver *s, int64_t n_points, uint64_t seed, int sampler,
+                                double *sigma_l1, double *kappa, double *sigma_l2sq,
+                                int64_t *n_actual, double *eta, double *points);
+
 /* sigma_squared_and_gradient (quintic.rs:2714-2860): sigma_L2^2 = sum w (eta/kappa-1)^2 / sum w,
  * its gradient with respect to h and sigma_L1. grad: 2 N^2 doubles or NULL; grad[a N + b] =
  * d/dh_re(a, b) (the diagonal halved, as the reference), grad[N^2 + a N + b] = d/dh_im(a, b) for
@@ -368,8 +381,8 @@ int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im,
  * (T_a[i] = (re, im) at 2 (8 a + i), a = 0..2), omega_sq[n] = |Omega|^2, weights[n]. h is used
  * as given (K = s^H h s, :80-107) and must be Hermitian positive definite. sigma =
  * sum w (eta / kappa - 1)^2 / sum w over points with finite eta > 0 and w > 0, NaN when no
- * point is valid; kappa = sum w eta / sum w. Differences: the sums run in a fixed order, and a
- * total weight below 1e-12 (the reference: 1e-30) gives NaN. */
+ * point is valid or the total weight is below 1e-30 (cy3_sigma_gpu.rs:514-519); kappa =
+ * sum w eta / sum w. Difference: the sums run in a fixed order. */
 typedef struct cy3q_sig8 cy3q_sig8;
 int cy3q_sig8_new(int64_t n_points, int64_t n_basis, int device, cy3q_sig8 **out);  /* :336-381 */
 void cy3q_sig8_free(cy3q_sig8 *g);
@@ -378,6 +391,42 @@ int cy3q_sig8_upload_static(cy3q_sig8 *g, const double *section_values,         
                             const double *omega_sq, const double *weights);
 int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im,   /* :445-547 */
                             double *sigma);
+
+/* ---- The sigma-gradient kernel contract: AdamGradientKernel (gpu_adam.rs:1533-2194), the
+ * object the reference's callers build once per (n_points, n_basis) and pass to
+ * sigma_refine_analytic_gpu / sigma_refine_lbfgs_gpu (quintic.rs:6514, 6650, 7637). Static
+ * inputs are the solver's arrays in the reference layouts: section_values[n][2 n_basis],
+ * section_derivs[n][5][2 n_basis], points[n][10], weights[n], log_omega_sq[n] (Fermat quintic,
+ * as the reference kernel). h_block is the 2N x 2N real block; its Hermitian part is used and
+ * must be positive definite (CY3Q_ERR_NOT_PD otherwise; the reference forms s^T H s
+ * directly). Nothing per point is stored for the gradient: the per-point metric is recomputed. */
+typedef struct cy3q_adamgrad cy3q_adamgrad;
+/* AdamGradientKernel::new (gpu_adam.rs:1563-1614). */
+int cy3q_adamgrad_new(int64_t n_points, int64_t n_basis, int device, cy3q_adamgrad **out);
+void cy3q_adamgrad_free(cy3q_adamgrad *g);
+/* AdamGradientKernel::upload_static_inputs (gpu_adam.rs:2062-2113). */
+int cy3q_adamgrad_upload_static_inputs(cy3q_adamgrad *g, const double *section_values,
+                                       const double *section_derivs, const double *points,
+                                       const double *weights, const double *log_omega_sq);
+/* AdamGradientKernel::compute_sigma_grad_h_only (gpu_adam.rs:1769-1938): sigma_L2^2, its
+ * gradient (2 n_basis^2 doubles, the layout of cy3q_solver_sigma_sq_grad) and sigma_L1. */
+int cy3q_adamgrad_compute_sigma_grad_h_only(cy3q_adamgrad *g, const double *h_block,
+                                            double *sigma_l2sq, double *grad, double *sigma_l1);
+/* AdamGradientKernel::compute_per_point_h_only (gpu_adam.rs:2119-2176): per_point_buf
+ * [n][2 + 2 n_basis^2], row p = (eta_p, w_p, d log eta_p / dh) in the reference's layout
+ * (cpu_compute_per_point_reference, gpu_adam.rs:2677-2739): (NaN, 0, 0, ...) for a point with an
+ * invalid weight or eta, a zero gradient where the tangent metric does not invert.
+ * n_basis <= 340. */
+int cy3q_adamgrad_compute_per_point_h_only(cy3q_adamgrad *g, const double *h_block,
+                                           double *per_point_buf);
+/* AdamGradientKernel::compute_per_point (gpu_adam.rs:1940-2060): upload_static_inputs, then
+ * compute_per_point_h_only. */
+int cy3q_adamgrad_compute_per_point(cy3q_adamgrad *g, const double *section_values,
+                                    const double *section_derivs, const double *h_block,
+                                    const double *points, const double *weights,
+                                    const double *log_omega_sq, double *per_point_buf);
+/* AdamGradientKernel::total_device_bytes (gpu_adam.rs:2180-2194). */
+int cy3q_adamgrad_total_device_bytes(const cy3q_adamgrad *g, int64_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -196,8 +196,8 @@ ORC_API void orc_quintic_eval(const double *pts, int64_t n, double psi, double *
 /* One line: draws a, b, builds the monic quintic in t, runs Durand–Kerner and writes the 5
  * candidate z (normalised to |z| = 1) with an accept flag each. Returns 0 when the line is
  * degenerate (|c5|^2 < 1e-300) and the reference 'continue's before emitting anything. */
-static int sz_line(orc_lcg *rng, int dwork, double psi, int dk_iters, double zs[5][10],
-                   int acc[5]) {
+static int sz_line(orc_lcg *rng, int dwork, double psi, int dk_iters, int polish,
+                   double zs[5][10], int acc[5]) {
     double a[5][2], b[5][2];
     for (int i = 0; i < 5; i++) {
         a[i][0] = lcg_next_normal(rng);
@@ -313,6 +313,28 @@ static int sz_line(orc_lcg *rng, int dwork, double psi, int dk_iters, double zs[
             }
         }
     }
+    /* FP64 Newton polish of each root on the monic line polynomial (t <- t - P(t)/P'(t),
+     * Horner for both), `polish` passes. Not in the reference (polish = 0 there); the
+     * north-star root polish after the Durand-Kerner sweeps. */
+    for (int pass = 0; pass < polish; pass++) {
+        for (int k = 0; k < 5; k++) {
+            double tr = roots[k][0], ti = roots[k][1];
+            double vr = 1.0, vi = 0.0, dvr = 0.0, dvi = 0.0;
+            for (int j = 4; j >= 0; j--) {
+                double ndr = dvr * tr - dvi * ti + vr, ndi = dvr * ti + dvi * tr + vi;
+                double nr = vr * tr - vi * ti + p[j][0], ni = vr * ti + vi * tr + p[j][1];
+                dvr = ndr;
+                dvi = ndi;
+                vr = nr;
+                vi = ni;
+            }
+            double dn = dvr * dvr + dvi * dvi;
+            if (dn > 1e-300) {
+                roots[k][0] = tr - (vr * dvr + vi * dvi) / dn;
+                roots[k][1] = ti - (vi * dvr - vr * dvi) / dn;
+            }
+        }
+    }
     for (int k = 0; k < 5; k++) {
         acc[k] = 0;
         double tr = roots[k][0], ti = roots[k][1];
@@ -342,8 +364,18 @@ static int sz_line(orc_lcg *rng, int dwork, double psi, int dk_iters, double zs[
 /* dwork = 0: sample_quintic_points_sz (Fermat, accept |f| <= 1e-3, max_tries = 4 * lines).
  * dwork = 1: sample_deformed_quintic_points_sz (accept |f| <= 1e-2, max_tries = 8 * lines).
  * out holds n * 10 doubles; returns the number of points written. */
+ORC_API int64_t orc_sample_sz_polish(int64_t n, uint64_t seed, int dwork, double psi,
+                                     int dk_iters, int polish, double *out,
+                                     int64_t *lines_tried);
 ORC_API int64_t orc_sample_sz(int64_t n, uint64_t seed, int dwork, double psi, int dk_iters,
                               double *out, int64_t *lines_tried) {
+    return orc_sample_sz_polish(n, seed, dwork, psi, dk_iters, 0, out, lines_tried);
+}
+
+/* The same sampler with `polish` FP64 Newton passes per root after the DK sweeps. */
+ORC_API int64_t orc_sample_sz_polish(int64_t n, uint64_t seed, int dwork, double psi,
+                                     int dk_iters, int polish, double *out,
+                                     int64_t *lines_tried) {
     orc_lcg rng;
     lcg_init(&rng, seed);
     int64_t n_lines_target = (n + 4) / 5;
@@ -353,7 +385,7 @@ ORC_API int64_t orc_sample_sz(int64_t n, uint64_t seed, int dwork, double psi, i
         tried++;
         double zs[5][10];
         int acc[5];
-        if (!sz_line(&rng, dwork, psi, dk_iters, zs, acc)) continue;
+        if (!sz_line(&rng, dwork, psi, dk_iters, polish, zs, acc)) continue;
         for (int k = 0; k < 5; k++) {
             if (np >= n) break;
             if (!acc[k]) continue;
@@ -449,7 +481,7 @@ ORC_API void orc_sz_lines(int64_t n_lines, uint64_t seed, int dwork, double psi,
         double zs[5][10];
         int acc[5];
         memset(zs, 0, sizeof zs);
-        if (!sz_line(&rng, dwork, psi, dk_iters, zs, acc))
+        if (!sz_line(&rng, dwork, psi, dk_iters, 0, zs, acc))
             for (int k = 0; k < 5; k++) acc[k] = 0;
         memcpy(zs_out + 50 * l, zs, sizeof zs);
         for (int k = 0; k < 5; k++) acc_out[5 * l + k] = acc[k];
@@ -1365,6 +1397,41 @@ ORC_API double orc_sigma_sq_grad(const double *pts, const double *sv, const doub
     free(swg);
     free(buf);
     return l2 / tw;
+}
+
+/* cpu_compute_per_point_reference (gpu_adam.rs:2677-2739): buf [n][2 + 2 nb^2], row p =
+ * (eta_p, w_p, d log eta_p / dh) from per_point_log_det_gradient; (NaN, 0, 0...) where the
+ * weight or eta is invalid. */
+ORC_API void orc_per_point(const double *pts, const double *sv, const double *sd,
+                           const double *log_om, const double *w, const double *h, int64_t n,
+                           int64_t nb, double psi, double *buf) {
+    const int64_t n_dof = 2 * nb * nb, stride = 2 + n_dof;
+#pragma omp parallel
+    {
+        double *scr = (double *)malloc(sizeof(double) * (size_t)(26 * nb + 2 * nb * nb + 16));
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; p++) {
+            double *slot = buf + p * stride;
+            double wp = w[p];
+            for (int64_t k = 2; k < stride; k++) slot[k] = 0.0;
+            if (!isfinite(wp) || wp <= 0.0) {
+                slot[0] = NAN;
+                slot[1] = 0.0;
+                continue;
+            }
+            int ok = 0;
+            double eta = grad_point(sv + p * 2 * nb, sd + p * 10 * nb, h, pts + 10 * p, log_om[p],
+                                    nb, psi, slot + 2, scr, &ok);
+            if (!isfinite(eta) || eta <= 0.0) {
+                slot[0] = NAN;
+                slot[1] = 0.0;
+            } else {
+                slot[0] = eta;
+                slot[1] = wp;
+            }
+        }
+        free(scr);
+    }
 }
 
 /* One step of sigma_refine_analytic (quintic.rs:453-514) after the gradient: Adam on

@@ -107,14 +107,26 @@ def quintic_eval(pts: np.ndarray, psi: float = 0.0):
     return f, g
 
 
-def sample_sz(n: int, seed: int, psi: float = 0.0, dwork: bool | None = None, dk_iters: int = 50):
+def sample_sz(n: int, seed: int, psi: float = 0.0, dwork: bool | None = None, dk_iters: int = 50,
+              polish: int = 0):
     """dwork=False: sample_quintic_points_sz; dwork=True: sample_deformed_quintic_points_sz.
-    Default: Fermat form at psi == 0, Dwork form otherwise."""
+    Default: Fermat form at psi == 0, Dwork form otherwise. polish > 0 adds that many FP64
+    Newton passes per root after the Durand-Kerner sweeps (not in the reference)."""
     if dwork is None:
         dwork = psi != 0.0
     out = np.zeros((n, 10))
     tried = C.c_int64(0)
-    got = lib().orc_sample_sz(n, seed, int(dwork), psi, dk_iters, out, C.byref(tried))
+    L = lib()
+    if polish:
+        if not getattr(L, "_polish_bound", False):
+            L.orc_sample_sz_polish.argtypes = [C.c_int64, C.c_uint64, C.c_int, C.c_double,
+                                               C.c_int, C.c_int, _dp, C.POINTER(C.c_int64)]
+            L.orc_sample_sz_polish.restype = C.c_int64
+            L._polish_bound = True
+        got = L.orc_sample_sz_polish(n, seed, int(dwork), psi, dk_iters, polish, out,
+                                     C.byref(tried))
+    else:
+        got = L.orc_sample_sz(n, seed, int(dwork), psi, dk_iters, out, C.byref(tried))
     return out[:got].copy(), int(tried.value)
 
 
@@ -334,6 +346,23 @@ def sigma_sq_grad(pts, sv, sd, log_om, w, h_block, psi: float = 0.0):
     return s2, g, l1.value
 
 
+def per_point(pts, sv, sd, log_om, w, h_block, psi: float = 0.0):
+    """cpu_compute_per_point_reference (gpu_adam.rs:2677-2739): [n, 2 + 2 N^2] rows
+    (eta_p, w_p, d log eta_p / dh)."""
+    L = lib()
+    if not getattr(L, "_pp_bound", False):
+        L.orc_per_point.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, C.c_int64, C.c_int64,
+                                    C.c_double, _dp]
+        L._pp_bound = True
+    n, two_n = sv.shape
+    nb = two_n // 2
+    out = np.zeros((n, 2 + 2 * nb * nb))
+    L.orc_per_point(np.ascontiguousarray(pts), np.ascontiguousarray(sv), np.ascontiguousarray(sd),
+                    np.ascontiguousarray(log_om), np.ascontiguousarray(w),
+                    np.ascontiguousarray(h_block), n, nb, psi, out)
+    return out
+
+
 def adam_step(h_block, grad, m, v, t: int, lr: float, beta1: float = 0.9, beta2: float = 0.999,
               eps: float = 1e-8):
     """One Adam step of sigma_refine_analytic (quintic.rs:453-514, 2071-2161, 3169-3182) on a
@@ -360,6 +389,36 @@ def sigma_refine_analytic(prob, h_block, n_iter: int, lr: float):
         hist.append(s1)
         h, t = adam_step(h, g, m, v, t, lr)
     return hist, h
+
+
+def sigma_at_sample(prob, h_block, n_pts: int, seed: int, sampler: str = "sz",
+                    want_eta: bool = False):
+    """sigma_at_metric_closure (quintic.rs:1520-1616) with the Bergman metric of h_block in the
+    basis of `prob` as the callback: a fresh sample (sample_quintic_points_with, :4006-4016;
+    sampler "sz" or "newton", Fermat at psi = 0, the Dwork forms at prob.psi != 0), its CY
+    weights and log|Omega|^2, the raw sections carried through prob's FS-Gram basis change
+    (apply_inverse_l_to_*, :3095-3133), then eta and the two-pass reduction of
+    compute_sigma_from_workspace (:1301-1499). Returns (sigma_L1, kappa, sigma_L2^2, n_actual)
+    and, with want_eta, also eta and the points."""
+    if sampler == "sz":
+        pts, _ = sample_sz(n_pts, seed, prob.psi)
+    elif sampler == "newton":
+        pts, _, _ = sample_newton(n_pts, seed, prob.psi)
+    else:
+        raise ValueError(sampler)
+    n = pts.shape[0]
+    if n == 0:
+        out = (float("nan"), float("nan"), float("nan"), 0)
+        return out + (np.zeros(0), pts) if want_eta else out
+    w, lo, _ = weights_logomega(pts, prob.psi)
+    sv, sd = sections(pts, prob.mono, True)
+    if prob.L is not None:
+        lre, lim = prob.L
+        lib().orc_apply_inv_l(lre, lim, n, prob.nb, sv, _ptr(sd))
+    e = eta(pts, sv, sd, lo, h_block, prob.psi)
+    s1, kap, l2 = sigma_reduce(e, w)
+    out = (s1, kap, l2, n)
+    return out + (e, pts) if want_eta else out
 
 
 class Problem:

@@ -23,3 +23,4 @@ from .quintic import (  # noqa: F401
     n_sections_quintic,
 )
 from .donaldson_gpu import Cy3DonaldsonGpu, Cy3SigmaGpu  # noqa: F401,E402
+from .adam_gpu import AdamGradientKernel  # noqa: F401,E402

@@ -130,12 +130,18 @@ void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const
 // sigma reduction into out6 (see k_sigma.hip); scratch holds sigma_reduce_scratch() doubles.
 int sigma_reduce_scratch();
 void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
-                         double *scratch, double *out6);
+                         double *scratch, double *out6, double min_total_w = 1e-12);
 // Adjoint of sigma_L2^2 and Adam (k_adjoint.hip).
 void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const double *yd,
                       const double *log_om, const double *w, const double *sv, const double *sd,
                       int64_t n, int64_t m_cols, double psi, const double *sig, double *phi,
-                      double *vs);
+                      double *vs, double *eta_out = nullptr);
+// Rows of the reference's per-point buffer ([n][2 + 2 nb^2]: eta, w, d log eta / dh) from the
+// sig == nullptr rows of launch_grad_rows and their eta. 12 * 2 nb doubles of LDS per point.
+int per_point_grad_max_nb();
+void launch_per_point_grad(hipStream_t st, const double *sv, const double *sd, const double *phi,
+                           const double *vs, const double *eta, const double *w, int nb,
+                           int64_t m_cols, int64_t n, double *out);
 int64_t gemm_tn_groups(int64_t rows, int64_t m_cols);
 void launch_gemm_tn_acc(hipStream_t st, const double *u, const double *v, int64_t rows,
                         int64_t m_cols, int64_t n_groups, double *part, double *acc);

@@ -39,7 +39,8 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
                                                    int64_t m, double psi,
                                                    const double *__restrict__ sig,
                                                    double *__restrict__ phi,
-                                                   double *__restrict__ vs) {
+                                                   double *__restrict__ vs,
+                                                   double *__restrict__ eta_out) {
   // Per point: [0] gamma/K, [1] gamma/K^2, [2] beta/2, [3..52] Y (Y[j][i] at 3 + 2 (5 j + i)),
   // [53..62] v_i, [63..72] w_j.
   __shared__ double cf[4 * GR_PPW][73];
@@ -52,8 +53,11 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
   if (sl == 0) {
     double *c = cf[slot];
     for (int q = 0; q < 73; q++) c[q] = 0.0;
-    const double kappa = sig[1], tw = sig[3], sa2 = sig[5];
+    // sig == nullptr: the per-point gradient of log eta alone (gamma = 1, no kappa term).
+    const bool unit = sig == nullptr;
+    const double kappa = unit ? 1.0 : sig[1], tw = unit ? 1.0 : sig[3], sa2 = unit ? 0.0 : sig[5];
     const double wp = in ? w[p] : 0.0, lo = in ? log_om[p] : NAN;
+    double eta_p = NAN;
     if (in && isfinite(wp) && wp > 0.0 && isfinite(lo) && isfinite(kappa) && isfinite(sa2) &&
         tw >= 1e-12) {
       double dk[5][2], ddk[5][5][2];
@@ -66,9 +70,11 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
       double gi[3][3][2];
       if (isfinite(pm.det) && fabs(pm.det) >= 1e-30) {
         const double eta = exp(log(fabs(pm.det)) - lo);
+        if (isfinite(eta) && eta > 0.0) eta_p = eta;
         if (isfinite(eta) && eta > 0.0 && inv3_herm(pm.gt, pm.det, gi)) {
           const double en = eta / kappa;
-          const double gamma = wp * 2.0 * (en - 1.0) * en - sa2 / (kappa * tw) * (wp * eta);
+          const double gamma =
+              unit ? 1.0 : wp * 2.0 * (en - 1.0) * en - sa2 / (kappa * tw) * (wp * eta);
           double y[5][5][2];
           y5_from_frame(pm.fr, gi, y);
           double tr_ym = 0.0, tr_yd = 0.0;
@@ -109,6 +115,7 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
         }
       }
     }
+    if (eta_out && in) eta_out[p] = eta_p;
   }
   __syncthreads();
   if (!in) return;
@@ -154,11 +161,83 @@ __global__ void __launch_bounds__(256) k_grad_rows(const double *__restrict__ pt
 void launch_grad_rows(hipStream_t st, const double *pts, const double *y0, const double *yd,
                       const double *log_om, const double *w, const double *sv, const double *sd,
                       int64_t n, int64_t m_cols, double psi, const double *sig, double *phi,
-                      double *vs) {
+                      double *vs, double *eta_out) {
   if (n <= 0) return;
   const int64_t per_wg = 4 * GR_PPW;
   hipLaunchKernelGGL(k_grad_rows, dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256), 0, st,
-                     pts, y0, yd, log_om, w, sv, sd, n, m_cols, psi, sig, phi, vs);
+                     pts, y0, yd, log_om, w, sv, sd, n, m_cols, psi, sig, phi, vs, eta_out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The reference's per-point buffer (cpu_compute_per_point_reference, gpu_adam.rs:2677-2739):
+// row p of out ([n][2 + 2 nb^2]) is (eta_p, w_p, grad L_p) with grad L_p = d log eta_p / dh in
+// the gradient layout (d/dh_re with the diagonal halved, then d/dh_im for a < b). From the
+// gamma = 1 rows of k_grad_rows: A_ab = sum_j conj(ds_j)_a phi'_jb + conj(s_a) v_b over the six
+// row pairs of the point; grad_re = Re(A_ab + A_ba), halved on the diagonal, and grad_im =
+// -Im(A_ab - A_ba) for a < b, as k_grad_assemble. A point with an invalid weight or eta gets (NaN, 0, 0...); a point whose
+// tangent metric does not invert keeps eta and w with a zero gradient, as the reference.
+// One workgroup per point; its 12 rows (2 nb doubles each) are staged in LDS.
+// ---------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_per_point_grad(const double *__restrict__ sv,
+                                                        const double *__restrict__ sd,
+                                                        const double *__restrict__ phi,
+                                                        const double *__restrict__ vs,
+                                                        const double *__restrict__ eta,
+                                                        const double *__restrict__ w, int nb,
+                                                        int64_t m, double *__restrict__ out) {
+  extern __shared__ double rows[];  // u[6][2 nb], v[6][2 nb]: (ds_0..ds_4, s), (phi'_0..4, v)
+  const int64_t p = blockIdx.x;
+  const int two_n = 2 * nb;
+  const int64_t nn = (int64_t)nb * nb, stride = 2 + 2 * nn;
+  double *o = out + p * stride;
+  const double e = eta[p], wp = w[p];
+  const bool valid = isfinite(e) && e > 0.0 && isfinite(wp) && wp > 0.0;
+  for (int q = threadIdx.x; q < 6 * two_n; q += blockDim.x) {
+    const int r = q / two_n, c = q % two_n;
+    rows[q] = r < 5 ? sd[(p * 5 + r) * m + c] : sv[p * m + c];
+    rows[6 * two_n + q] = r < 5 ? phi[(p * 5 + r) * m + c] : vs[p * m + c];
+  }
+  if (threadIdx.x == 0) {
+    o[0] = valid ? e : NAN;
+    o[1] = valid ? wp : 0.0;
+  }
+  __syncthreads();
+  const double *u = rows, *v = rows + 6 * two_n;
+  for (int64_t q = threadIdx.x; q < nn; q += blockDim.x) {
+    const int a = (int)(q / nb), b = (int)(q % nb);
+    double gre = 0.0, gim = 0.0;
+    if (valid) {
+      // A_xy = sum_r conj(u_r[x]) v_r[y]
+      double abr = 0.0, abi = 0.0, bar = 0.0, bai = 0.0;
+#pragma unroll
+      for (int r = 0; r < 6; r++) {
+        const double *ur = u + r * two_n, *vr = v + r * two_n;
+        const double uar = ur[2 * a], uai = ur[2 * a + 1], ubr = ur[2 * b], ubi = ur[2 * b + 1];
+        const double var_ = vr[2 * a], vai = vr[2 * a + 1], vbr = vr[2 * b], vbi = vr[2 * b + 1];
+        abr += uar * vbr + uai * vbi;
+        abi += uar * vbi - uai * vbr;
+        bar += ubr * var_ + ubi * vai;
+        bai += ubr * vai - ubi * var_;
+      }
+      gre = abr + bar;
+      if (a == b) gre *= 0.5;
+      gim = a < b ? -(abi - bai) : 0.0;
+    }
+    o[2 + q] = gre;
+    o[2 + nn + q] = gim;
+  }
+}
+
+int per_point_grad_max_nb() { return 340; }  // 12 * 2 * 340 * 8 bytes < 64 KB of LDS
+
+void launch_per_point_grad(hipStream_t st, const double *sv, const double *sd, const double *phi,
+                           const double *vs, const double *eta, const double *w, int nb,
+                           int64_t m_cols, int64_t n, double *out) {
+  if (n <= 0) return;
+  const size_t lds = sizeof(double) * 12 * 2 * (size_t)nb;
+  hipLaunchKernelGGL(k_per_point_grad, dim3((unsigned)n), dim3(256), lds, st, sv, sd, phi, vs, eta,
+                     w, nb, m_cols, out);
 }
 
 // ---------------------------------------------------------------------------------------------

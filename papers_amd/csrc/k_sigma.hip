@@ -95,7 +95,7 @@ __device__ inline bool sr_valid(double e, double wp) {
 template <int PASS>
 __global__ void __launch_bounds__(SR_THREADS) k_sigma_part(const double *__restrict__ eta,
                                                            const double *__restrict__ w, int64_t n,
-                                                           int64_t chunk,
+                                                           int64_t chunk, double min_tw,
                                                            const double *__restrict__ stats,
                                                            double *__restrict__ part) {
   __shared__ double sh[3][SR_THREADS];
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(SR_THREADS) k_sigma_part(const double *__restr
     }
   } else {
     const double total_w = stats[0], count = stats[2];
-    const bool ok = count > 0.0 && total_w >= 1e-12;
+    const bool ok = count > 0.0 && total_w >= min_tw;
     const double kappa = ok ? stats[1] / total_w : NAN;
     if (ok && fabs(kappa) >= 1e-30) {
       for (int64_t p = p0 + threadIdx.x; p < p1; p += SR_THREADS) {
@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(SR_THREADS) k_sigma_part(const double *__restr
 // Block partials in block order. Pass 1 writes stats; pass 2 writes out (with stats).
 template <int PASS>
 __global__ void __launch_bounds__(SR_THREADS) k_sigma_final(const double *__restrict__ part,
-                                                            int nparts, double *__restrict__ stats,
+                                                            int nparts, double min_tw,
+                                                            double *__restrict__ stats,
                                                             double *__restrict__ out) {
   __shared__ double sh[3][SR_THREADS];
   double a = 0.0, b = 0.0, c = 0.0;
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(SR_THREADS) k_sigma_final(const double *__rest
     return;
   }
   const double total_w = stats[0], count = stats[2];
-  const bool ok = count > 0.0 && total_w >= 1e-12;
+  const bool ok = count > 0.0 && total_w >= min_tw;
   const double kappa = ok ? stats[1] / total_w : NAN;
   const bool ok2 = ok && fabs(kappa) >= 1e-30;
   out[0] = ok2 ? a / total_w : NAN;
@@ -167,19 +168,20 @@ __global__ void __launch_bounds__(SR_THREADS) k_sigma_final(const double *__rest
 
 int sigma_reduce_scratch() { return 3 * SR_BLOCKS + 3; }
 
-// scratch: sigma_reduce_scratch() doubles.
+// scratch: sigma_reduce_scratch() doubles. min_total_w: the weight sum below which sigma is
+// NaN (1e-12 in the quintic sigma, quintic.rs:1479; 1e-30 in Cy3SigmaGpu, cy3_sigma_gpu.rs:514).
 void launch_sigma_reduce(hipStream_t st, const double *eta, const double *w, int64_t n,
-                         double *scratch, double *out6) {
+                         double *scratch, double *out6, double min_total_w) {
   const int64_t chunk = (n + SR_BLOCKS - 1) / SR_BLOCKS > 0 ? (n + SR_BLOCKS - 1) / SR_BLOCKS : 1;
   double *part = scratch, *stats = scratch + 3 * SR_BLOCKS;
   hipLaunchKernelGGL(k_sigma_part<1>, dim3(SR_BLOCKS), dim3(SR_THREADS), 0, st, eta, w, n, chunk,
-                     stats, part);
-  hipLaunchKernelGGL(k_sigma_final<1>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS, stats,
-                     out6);
+                     min_total_w, stats, part);
+  hipLaunchKernelGGL(k_sigma_final<1>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS,
+                     min_total_w, stats, out6);
   hipLaunchKernelGGL(k_sigma_part<2>, dim3(SR_BLOCKS), dim3(SR_THREADS), 0, st, eta, w, n, chunk,
-                     stats, part);
-  hipLaunchKernelGGL(k_sigma_final<2>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS, stats,
-                     out6);
+                     min_total_w, stats, part);
+  hipLaunchKernelGGL(k_sigma_final<2>, dim3(1), dim3(SR_THREADS), 0, st, part, SR_BLOCKS,
+                     min_total_w, stats, out6);
 }
 
 }  // namespace cy3q

@@ -10,6 +10,8 @@
 // w or |Omega|^2 not finite and positive, or with a non-finite or |det| < 1e-30.
 // One wave per point: the lanes split the N complex entries of the 45 inner products, a
 // butterfly reduction gives every lane the sums, and lane 0 does the 8 x 8 -> 3 x 3 algebra.
+#include <algorithm>
+
 #include "cy3q_internal.h"
 
 namespace cy3q {
@@ -146,22 +148,25 @@ __global__ void __launch_bounds__(256) k_eta8(const double *__restrict__ y, int6
 }
 
 // x9[p][0] = sv[p], x9[p][1 + i] = sd[p][i] (i = 0..7), each row two_n values then zeros to m.
+// Grid-stride over the n * 9 * m entries, so any size fits a capped grid.
 __global__ void k_pack_rows9(const double *__restrict__ sv, const double *__restrict__ sd,
                              int64_t n, int64_t two_n, int64_t m, double *__restrict__ x9) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * 9 * m) return;
-  const int64_t c = e % m, r = e / m, v = r % 9, p = r / 9;
-  double x = 0.0;
-  if (c < two_n) x = v == 0 ? sv[p * two_n + c] : sd[(p * 8 + (v - 1)) * two_n + c];
-  x9[e] = x;
+  const int64_t tot = n * 9 * m, step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += step) {
+    const int64_t c = e % m, r = e / m, v = r % 9, p = r / 9;
+    double x = 0.0;
+    if (c < two_n) x = v == 0 ? sv[p * two_n + c] : sd[(p * 8 + (v - 1)) * two_n + c];
+    x9[e] = x;
+  }
 }
 
 void launch_pack_rows9(hipStream_t st, const double *sv, const double *sd, int64_t n,
                        int64_t two_n, int64_t m, double *x9) {
   const int64_t tot = n * 9 * m;
   if (tot <= 0) return;
-  hipLaunchKernelGGL(k_pack_rows9, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, sv, sd,
-                     n, two_n, m, x9);
+  const int64_t blocks = std::min<int64_t>((tot + 255) / 256, 1 << 20);
+  hipLaunchKernelGGL(k_pack_rows9, dim3((unsigned)blocks), dim3(256), 0, st, sv, sd, n, two_n, m,
+                     x9);
 }
 
 void launch_eta8(hipStream_t st, const double *y, int64_t m, int nb, const double *frames,

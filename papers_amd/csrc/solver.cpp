@@ -106,6 +106,7 @@ struct cy3q_solver {
   int refine_kind = CY3Q_REFINE_T_FP64;  // what one refinement sweep is
   std::vector<uint32_t> mono;
   std::vector<void *> allocs;
+  std::vector<size_t> alloc_bytes;  // bytes of each entry of allocs
 
   // static workspace
   double *pts = nullptr, *w = nullptr, *log_om = nullptr, *sv = nullptr, *sd = nullptr;
@@ -115,6 +116,9 @@ struct cy3q_solver {
   // h state: G = W^H W, W lower triangular
   double *g = nullptr, *wmat = nullptr, *t = nullptr, *wnew = nullptr, *gnew = nullptr;
   double *bmat = nullptr, *dpart = nullptr, *coef = nullptr, *xx = nullptr, *partials = nullptr;
+  // Accumulated FS-Gram basis change (complex N x N, lower triangular): the solver's sections
+  // are conj(wfs) times the raw monomials. nullptr while no FS-Gram has been applied.
+  double *wfs = nullptr;
   double *resid = nullptr, *sig_out = nullptr, *sig_scratch = nullptr;
   // elimination workspace of the N x N solve
   double *abuf = nullptr, *rbuf = nullptr, *einv = nullptr, *partial = nullptr,
@@ -187,6 +191,7 @@ struct cy3q_solver {
       if (e != hipSuccess) return fail(CY3Q_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     }
     allocs.push_back(q);
+    alloc_bytes.push_back(bytes);
     *p = (T *)q;
     return CY3Q_OK;
   }
@@ -194,8 +199,14 @@ struct cy3q_solver {
     auto it = std::find(allocs.begin(), allocs.end(), p);
     if (it != allocs.end()) {
       hipFree(p);
+      alloc_bytes.erase(alloc_bytes.begin() + (it - allocs.begin()));
       allocs.erase(it);
     }
+  }
+  size_t device_bytes() const {
+    size_t b = 0;
+    for (size_t x : alloc_bytes) b += x;
+    return b;
   }
 
   // --- profiling helpers: events bracket a stage on the solver's stream ---
@@ -466,6 +477,28 @@ int sample_points_newton(cy3q_solver *s, int64_t n_target, uint64_t seed, double
   s->release(bsum);
   s->release(total);
   s->n = have;
+  return CY3Q_OK;
+}
+
+// Section rows and derivative rows of s replaced by conj(V) times them, V lower triangular
+// (complex N x N on the device): the basis change of the FS-Gram orthonormalisation.
+int rebase_sections(cy3q_solver *s, const double *v_cplx) {
+  const int64_t n = s->n, m = s->m;
+  launch_build_bmat(s->st, v_cplx, s->nb, m, 1, s->bmat);
+  s->bmat_valid = false;
+  double *sv2 = nullptr, *sd2 = nullptr;
+  ST_TRY(s->alloc(&sv2, n * m));
+  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, sv2, 0, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(s->sv);
+  s->sv = sv2;
+  ST_TRY(s->alloc(&sd2, n * 5 * m));
+  launch_trigemm_f64(s->st, s->sd, n * 5, m, s->bmat, sd2, 0, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(s->sd);
+  s->sd = sd2;
   return CY3Q_OK;
 }
 
@@ -1010,20 +1043,20 @@ int cy3q_solver_orthonormalise_fs_gram(cy3q_solver *s) {
   if (stt != 0) return fail(CY3Q_ERR_NOT_PD, "FS Gram matrix not positive-definite");
   // s' = conj(L)^-1 s = conj(W) s for the section rows and all derivative rows
   // (apply_inverse_l_to_sections/derivs, quintic.rs:3095-3133), as triangular contractions.
-  launch_build_bmat(s->st, s->wnew, nb, m, 1, s->bmat);
-  double *sv2 = nullptr, *sd2 = nullptr;
-  ST_TRY(s->alloc(&sv2, n * m));
-  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, sv2, 0, 0);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(s->st));
-  s->release(s->sv);
-  s->sv = sv2;
-  ST_TRY(s->alloc(&sd2, n * 5 * m));
-  launch_trigemm_f64(s->st, s->sd, n * 5, m, s->bmat, sd2, 0, 0);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(s->st));
-  s->release(s->sd);
-  s->sd = sd2;
+  ST_TRY(rebase_sections(s, s->wnew));
+  // Keep the accumulated basis change for evaluations at other points: a second FS-Gram
+  // composes, conj(W2) conj(W1) = conj(W2 W1).
+  const int64_t nn = (int64_t)nb * nb;
+  if (!s->wfs) {
+    ST_TRY(s->alloc(&s->wfs, 2 * nn));
+    HIP_TRY(hipMemcpyAsync(s->wfs, s->wnew, sizeof(double) * 2 * nn, hipMemcpyDeviceToDevice,
+                           s->st));
+  } else {
+    launch_zgemm(s->st, (const double2 *)s->wnew, (const double2 *)s->wfs, (double2 *)s->gnew, nb,
+                 nb, nb, nb, nb, nb, 0, 0);
+    HIP_TRY(hipMemcpyAsync(s->wfs, s->gnew, sizeof(double) * 2 * nn, hipMemcpyDeviceToDevice,
+                           s->st));
+  }
   // h <- I in the new basis.
   launch_identity_cplx(s->st, nb, s->g);
   launch_identity_cplx(s->st, nb, s->wmat);
@@ -1537,6 +1570,59 @@ int cy3q_solver_sigma(cy3q_solver *s, double *sigma) {
   return cy3q_solver_sigma_ext(s, sigma, nullptr, nullptr, nullptr);
 }
 
+int cy3q_solver_sigma_at_sample(cy3q_solver *s, int64_t n_points, uint64_t seed, int sampler,
+                                double *sigma_l1, double *kappa, double *sigma_l2sq,
+                                int64_t *n_actual, double *eta, double *points) {
+  if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
+  if (n_points <= 0) return fail(CY3Q_ERR_ARG, "n_points must be > 0");
+  if (sampler != CY3Q_SAMPLER_SZ && sampler != CY3Q_SAMPLER_SZ_DWORK &&
+      sampler != CY3Q_SAMPLER_NEWTON && sampler != CY3Q_SAMPLER_NEWTON_DWORK)
+    return fail(CY3Q_ERR_ARG, "unknown sampler");
+  const bool dwork = sampler == CY3Q_SAMPLER_SZ_DWORK || sampler == CY3Q_SAMPLER_NEWTON_DWORK;
+  if (!dwork && s->psi != 0.0)
+    return fail(CY3Q_ERR_ARG, "the solver's metric lives on a Dwork quintic; use a Dwork sampler");
+  const bool newton = sampler == CY3Q_SAMPLER_NEWTON || sampler == CY3Q_SAMPLER_NEWTON_DWORK;
+  HIP_TRY(hipSetDevice(s->device));
+  // The metric's state must be complete before the second workspace reads it on its stream.
+  HIP_TRY(hipStreamSynchronize(s->st));
+  cy3q_solver *t = nullptr, *tout = nullptr;
+  ST_TRY(make_solver(s->k, dwork ? s->psi : 0.0, &s->opt, &tout, &t));
+  auto done = [&](int rc) {
+    delete t;
+    return rc;
+  };
+  int rc = newton ? sample_points_newton(t, n_points, seed, s->opt.newton_tol, dwork ? 1 : 0)
+                  : sample_points(t, n_points, seed, dwork ? 1 : 0);
+  if (rc != CY3Q_OK) return done(rc);
+  if (n_actual) *n_actual = t->n;
+  if (t->n == 0) {
+    // The reference returns NaN for an empty sample (quintic.rs:1531-1533).
+    if (sigma_l1) *sigma_l1 = NAN;
+    if (kappa) *kappa = NAN;
+    if (sigma_l2sq) *sigma_l2sq = NAN;
+    return done(CY3Q_OK);
+  }
+  if ((rc = finish_workspace(t)) != CY3Q_OK) return done(rc);
+  if (s->wfs && (rc = rebase_sections(t, s->wfs)) != CY3Q_OK) return done(rc);
+  const int64_t nn = (int64_t)s->nb * s->nb;
+  hipError_t e = hipMemcpyAsync(t->wmat, s->wmat, sizeof(double) * 2 * nn,
+                                hipMemcpyDeviceToDevice, t->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(t->g, s->g, sizeof(double) * 2 * nn, hipMemcpyDeviceToDevice, t->st);
+  if (e != hipSuccess) return done(fail(CY3Q_ERR_HIP, hipGetErrorString(e)));
+  t->bmat_valid = false;
+  double o[5];
+  if ((rc = sigma_impl(t, o, eta)) != CY3Q_OK) return done(rc);
+  if (points) {
+    e = hipMemcpy(points, t->pts, sizeof(double) * 10 * t->n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return done(fail(CY3Q_ERR_HIP, hipGetErrorString(e)));
+  }
+  if (sigma_l1) *sigma_l1 = o[0];
+  if (kappa) *kappa = o[1];
+  if (sigma_l2sq) *sigma_l2sq = o[2];
+  return done(CY3Q_OK);
+}
+
 int cy3q_solver_get_h_block(cy3q_solver *s, double *h_block) {
   if (!s || !h_block) return fail(CY3Q_ERR_ARG, "NULL argument");
   HIP_TRY(hipSetDevice(s->device));
@@ -1933,12 +2019,221 @@ int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im
     launch_trigemm_f64(s->st, g->x9 + p0 * 9 * m, cn * 9, m, s->bmat, g->y9, 0, 0);
     launch_eta8(s->st, g->y9, m, nb, g->frames + p0 * 48, g->om + p0, s->w + p0, cn, g->eta + p0);
   }
-  launch_sigma_reduce(s->st, g->eta, s->w, n, s->sig_scratch, s->sig_out);
+  launch_sigma_reduce(s->st, g->eta, s->w, n, s->sig_scratch, s->sig_out, 1e-30);
   HIP_TRY(hipGetLastError());
   double o[6];
   HIP_TRY(hipMemcpyAsync(o, s->sig_out, sizeof(o), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
   *sigma = o[2];
+  return CY3Q_OK;
+}
+
+}  // extern "C"
+
+// ---- The sigma-gradient kernel contract: AdamGradientKernel (gpu_adam.rs:1533-2194) ----------
+// A points-free solver workspace holds the caller's static inputs (sections, derivatives,
+// points, weights, log|Omega|^2; the Fermat quintic, as the reference kernel). Per call, h gives
+// W with W^H W = G (its Hermitian part, as cy3q_solver_set_h_block), then either the sigma^2
+// gradient of the solver (sigma_grad_impl) or the reference's per-point buffer in chunks.
+struct cy3q_adamgrad {
+  cy3q_solver *s = nullptr;
+  bool uploaded = false;
+  double *pp = nullptr, *eta = nullptr;  // per-point output chunk and eta of the chunk
+  int64_t pp_chunk = 0;
+  std::vector<double> grad_host;
+};
+
+namespace {
+constexpr double kPerPointChunkBytes = 1 << 30;
+
+int adamgrad_set_h(cy3q_adamgrad *g, const double *h_block) {
+  cy3q_solver *s = g->s;
+  const int64_t tn = 2 * (int64_t)s->nb;
+  double *d = nullptr;
+  ST_TRY(s->alloc(&d, tn * tn));
+  HIP_TRY(hipMemcpyAsync(d, h_block, sizeof(double) * tn * tn, hipMemcpyHostToDevice, s->st));
+  launch_block_to_cplx(s->st, d, s->nb, s->g);
+  launch_hermitian_part(s->st, s->g, s->nb);
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(d);
+  bool pd = false;
+  ST_TRY(w_from_g_pd(s, &pd));
+  if (!pd) return fail(CY3Q_ERR_NOT_PD, "h_block is not Hermitian positive definite");
+  return CY3Q_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cy3q_adamgrad_new(int64_t n_points, int64_t n_basis, int device, cy3q_adamgrad **out) {
+  if (!out) return fail(CY3Q_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_points <= 0 || n_basis <= 0)
+    return fail(CY3Q_ERR_ARG, "n_points and n_basis must be > 0");
+  if (padded_cols(n_basis) > 1024) return fail(CY3Q_ERR_ARG, "2 n_basis > 1024 is not supported");
+  cy3q_solver *s = new cy3q_solver();
+  cy3q_options_default(&s->opt);
+  s->opt.device = device;
+  s->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    s->st = nullptr;
+    delete s;
+    return fail(CY3Q_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  s->nb = (int)n_basis;
+  s->m = padded_cols(n_basis);
+  s->n = n_points;
+  const int64_t n = s->n, m = s->m, nn = n_basis * n_basis;
+  const int nblk = (s->nb + 31) / 32;
+  cy3q_adamgrad *g = new cy3q_adamgrad();
+  g->s = s;
+  auto bail = [&](int rc) {
+    delete s;
+    delete g;
+    return rc;
+  };
+  int rc = CY3Q_OK;
+  if ((rc = s->alloc(&s->pts, n * 10)) || (rc = s->alloc(&s->w, n)) ||
+      (rc = s->alloc(&s->log_om, n)) || (rc = s->alloc(&s->wvalid, n)) ||
+      (rc = s->alloc(&s->sum_w, 1)) || (rc = s->alloc(&s->sv, n * m)) ||
+      (rc = s->alloc(&s->sd, n * 5 * m)) || (rc = s->alloc(&s->g, 2 * nn)) ||
+      (rc = s->alloc(&s->wmat, 2 * nn)) || (rc = s->alloc(&s->wnew, 2 * nn)) ||
+      (rc = s->alloc(&s->gnew, 2 * nn)) || (rc = s->alloc(&s->bmat, m * m)) ||
+      (rc = s->alloc(&s->status, 4)) || (rc = s->alloc(&s->abuf, 2 * nn)) ||
+      (rc = s->alloc(&s->rbuf, 2 * nn)) ||
+      (rc = s->alloc(&s->einv, (size_t)nblk * 32 * 32 * 2)) ||
+      (rc = s->alloc(&s->partial, (size_t)nblk * nblk)) || (rc = s->alloc(&s->sig_out, 8)) ||
+      (rc = s->alloc(&s->sig_scratch, sigma_reduce_scratch())))
+    return bail(rc);
+  launch_identity_cplx(s->st, s->nb, s->g);
+  launch_identity_cplx(s->st, s->nb, s->wmat);
+  e = hipStreamSynchronize(s->st);
+  if (e != hipSuccess) return bail(fail(CY3Q_ERR_HIP, hipGetErrorString(e)));
+  *out = g;
+  return CY3Q_OK;
+}
+
+void cy3q_adamgrad_free(cy3q_adamgrad *g) {
+  if (!g) return;
+  delete g->s;
+  delete g;
+}
+
+int cy3q_adamgrad_upload_static_inputs(cy3q_adamgrad *g, const double *section_values,
+                                       const double *section_derivs, const double *points,
+                                       const double *weights, const double *log_omega_sq) {
+  if (!g || !section_values || !section_derivs || !points || !weights || !log_omega_sq)
+    return fail(CY3Q_ERR_ARG, "NULL argument");
+  cy3q_solver *s = g->s;
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t n = s->n, m = s->m, two_n = 2 * (int64_t)s->nb;
+  // Rows padded from 2 N to m columns with zeros.
+  HIP_TRY(hipMemsetAsync(s->sv, 0, sizeof(double) * n * m, s->st));
+  HIP_TRY(hipMemsetAsync(s->sd, 0, sizeof(double) * n * 5 * m, s->st));
+  HIP_TRY(hipMemcpy2DAsync(s->sv, sizeof(double) * m, section_values, sizeof(double) * two_n,
+                           sizeof(double) * two_n, n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpy2DAsync(s->sd, sizeof(double) * m, section_derivs, sizeof(double) * two_n,
+                           sizeof(double) * two_n, n * 5, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->pts, points, sizeof(double) * 10 * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->log_om, log_omega_sq, sizeof(double) * n, hipMemcpyHostToDevice,
+                         s->st));
+  launch_valid_weights(s->st, s->w, n, s->wvalid);
+  launch_sum_f64(s->st, s->wvalid, n, s->sum_w);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s->st));
+  // Contracted rows computed for the previous inputs are stale.
+  s->bmat_valid = false;
+  g->uploaded = true;
+  return CY3Q_OK;
+}
+
+int cy3q_adamgrad_compute_sigma_grad_h_only(cy3q_adamgrad *g, const double *h_block,
+                                            double *sigma_l2sq, double *grad, double *sigma_l1) {
+  if (!g || !h_block) return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (!g->uploaded) return fail(CY3Q_ERR_STATE, "cy3q_adamgrad_upload_static_inputs first");
+  cy3q_solver *s = g->s;
+  HIP_TRY(hipSetDevice(s->device));
+  ST_TRY(adamgrad_set_h(g, h_block));
+  double o[6];
+  ST_TRY(sigma_grad_impl(s, o));
+  if (grad)
+    HIP_TRY(hipMemcpy(grad, s->grad_dev, sizeof(double) * 2 * (int64_t)s->nb * s->nb,
+                      hipMemcpyDeviceToHost));
+  if (sigma_l2sq) *sigma_l2sq = o[2];
+  if (sigma_l1) *sigma_l1 = o[0];
+  return CY3Q_OK;
+}
+
+int cy3q_adamgrad_compute_per_point_h_only(cy3q_adamgrad *g, const double *h_block,
+                                           double *per_point_buf) {
+  if (!g || !h_block || !per_point_buf) return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (!g->uploaded) return fail(CY3Q_ERR_STATE, "cy3q_adamgrad_upload_static_inputs first");
+  cy3q_solver *s = g->s;
+  if (s->nb > per_point_grad_max_nb())
+    return fail(CY3Q_ERR_ARG, "the per-point buffer supports n_basis <= " +
+                                  std::to_string(per_point_grad_max_nb()));
+  HIP_TRY(hipSetDevice(s->device));
+  ST_TRY(adamgrad_set_h(g, h_block));
+  const int64_t n = s->n, m = s->m, nn = (int64_t)s->nb * s->nb, stride = 2 + 2 * nn;
+  if (!g->pp) {
+    int64_t chunk = (int64_t)(kPerPointChunkBytes / (8.0 * (double)stride));
+    chunk = std::max<int64_t>(1, std::min<int64_t>({chunk, n, kRowChunk}));
+    g->pp_chunk = chunk;
+    ST_TRY(s->alloc(&g->pp, chunk * stride));
+    ST_TRY(s->alloc(&g->eta, chunk));
+    if (!s->g_phi) {
+      s->g_chunk = std::min<int64_t>(n, kRowChunk);
+      s->g_groups = gemm_tn_groups(5 * s->g_chunk, m);
+      ST_TRY(s->alloc(&s->g_phi, s->g_chunk * 5 * m));
+      ST_TRY(s->alloc(&s->g_vs, s->g_chunk * m));
+      ST_TRY(s->alloc(&s->g_part, s->g_groups * m * m));
+      ST_TRY(s->alloc(&s->g_cm, m * m));
+    }
+    if (!s->y0all) {
+      ST_TRY(s->alloc(&s->y0all, s->g_chunk * m));
+      ST_TRY(s->alloc(&s->ydall, s->g_chunk * 5 * m));
+    }
+  }
+  ensure_bmat(s);
+  const int64_t chunk = std::min(g->pp_chunk, s->g_chunk);
+  for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+    const int64_t cn = std::min(chunk, n - p0);
+    // With rows resident from a gradient call, the chunk's rows are recomputed here all the same
+    // (they may belong to an older h).
+    launch_trigemm_f64(s->st, s->sv + p0 * m, cn, m, s->bmat, s->y0all, 0, 0);
+    launch_trigemm_f64(s->st, s->sd + p0 * 5 * m, cn * 5, m, s->bmat, s->ydall, 0, 0);
+    launch_grad_rows(s->st, s->pts + 10 * p0, s->y0all, s->ydall, s->log_om + p0, s->w + p0,
+                     s->sv + p0 * m, s->sd + p0 * 5 * m, cn, m, s->psi, nullptr, s->g_phi,
+                     s->g_vs, g->eta);
+    launch_per_point_grad(s->st, s->sv + p0 * m, s->sd + p0 * 5 * m, s->g_phi, s->g_vs, g->eta,
+                          s->w + p0, s->nb, m, cn, g->pp);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(per_point_buf + p0 * stride, g->pp, sizeof(double) * cn * stride,
+                           hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+  }
+  return CY3Q_OK;
+}
+
+int cy3q_adamgrad_compute_per_point(cy3q_adamgrad *g, const double *section_values,
+                                    const double *section_derivs, const double *h_block,
+                                    const double *points, const double *weights,
+                                    const double *log_omega_sq, double *per_point_buf) {
+  ST_TRY(cy3q_adamgrad_upload_static_inputs(g, section_values, section_derivs, points, weights,
+                                            log_omega_sq));
+  return cy3q_adamgrad_compute_per_point_h_only(g, h_block, per_point_buf);
+}
+
+int cy3q_adamgrad_total_device_bytes(const cy3q_adamgrad *g, int64_t *bytes) {
+  if (!g || !bytes) return fail(CY3Q_ERR_ARG, "NULL argument");
+  *bytes = (int64_t)g->s->device_bytes();
   return CY3Q_OK;
 }
 

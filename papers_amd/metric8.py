@@ -27,6 +27,8 @@ import math
 import numpy as np
 import scipy.linalg as sla
 
+from .quintic import ERR_NOT_PD, Cy3qError
+
 __all__ = ["AdaptiveDamping", "GatingState", "TikhonovGating", "TikhonovShift",
            "auto_schoen_gated_tikhonov", "auto_schoen_tikhonov", "auto_ty_gated_tikhonov",
            "auto_ty_tikhonov", "donaldson_iteration_route34", "resolve_tikhonov", "solve_route34"]
@@ -190,7 +192,16 @@ def donaldson_iteration_route34(t_operator, h_re, h_im, weights, damping: float 
     h_re = np.asarray(h_re, dtype=np.float64)
     h_im = np.asarray(h_im, dtype=np.float64)
     n = h_re.shape[0]
-    pr, pi = t_operator(h_re, h_im)
+    try:
+        pr, pi = t_operator(h_re, h_im)
+    except Cy3qError as e:
+        # The GPU T-operator forms K_p = |W s_p|^2 with W^H W = conj(h), so it rejects an h that
+        # is not positive definite, where the reference clamps K at 1e-30 and goes on
+        # (cy3_donaldson_gpu.rs:84). A NaN residual hands the iterate to the caller's
+        # minimum-residual snapshot and catastrophe guard.
+        if e.code != ERR_NOT_PD:
+            raise
+        return h_re, h_im, float("nan")
     pr = np.asarray(pr, dtype=np.float64).reshape(n, n)
     pi = np.asarray(pi, dtype=np.float64).reshape(n, n)
     wsum = float(np.sum(np.asarray(weights, dtype=np.float64)))

@@ -78,6 +78,8 @@ def _load():
     L.cy3q_solver_donaldson_solve.argtypes = [vp, i64, dbl, C.POINTER(i64), vp]
     L.cy3q_solver_sigma.argtypes = [vp, C.POINTER(dbl)]
     L.cy3q_solver_sigma_ext.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl), vp]
+    L.cy3q_solver_sigma_at_sample.argtypes = [vp, i64, u64, ip, C.POINTER(dbl), C.POINTER(dbl),
+                                              C.POINTER(dbl), C.POINTER(i64), vp, vp]
     L.cy3q_solver_sigma_sq_grad.argtypes = [vp, C.POINTER(dbl), vp, C.POINTER(dbl)]
     L.cy3q_solver_sigma_refine_analytic.argtypes = [vp, i64, dbl, vp, C.POINTER(i64)]
     L.cy3q_solver_set_adam.argtypes = [vp, dbl, dbl, dbl]
@@ -266,6 +268,23 @@ class QuinticSolver:
         eta = np.zeros(self.n_points) if want_eta else None
         _check(_load().cy3q_solver_sigma_ext(self._h, C.byref(s), C.byref(k), C.byref(l2), _ptr(eta)))
         return s.value, k.value, l2.value, eta
+
+    def sigma_at_sample(self, n_points: int, seed: int, sampler: int = SAMPLER_SZ,
+                        want_eta: bool = False):
+        """sigma_at_metric_closure / sigma_and_etas_at_metric_closure (quintic.rs:1520-1700)
+        with this solver's Bergman metric as the callback: sigma on a fresh sample of n_points
+        points drawn with `sampler` and `seed`. Returns (sigma_L1, kappa, sigma_L2^2, n_actual,
+        eta, points); eta and points are None unless want_eta."""
+        s, k, l2 = C.c_double(), C.c_double(), C.c_double()
+        na = C.c_int64()
+        eta = np.zeros(n_points) if want_eta else None
+        pts = np.zeros((n_points, 10)) if want_eta else None
+        _check(_load().cy3q_solver_sigma_at_sample(self._h, n_points, seed, sampler, C.byref(s),
+                                                   C.byref(k), C.byref(l2), C.byref(na),
+                                                   _ptr(eta), _ptr(pts)))
+        m = na.value
+        return (s.value, k.value, l2.value, m, None if eta is None else eta[:m],
+                None if pts is None else pts[:m])
 
     def sigma_squared_and_gradient(self):
         """sigma_squared_and_gradient (quintic.rs:2714-2860): (sigma_L2^2, grad [2 N^2], sigma_L1).

@@ -204,3 +204,36 @@ def test_legacy_iteration_matches_oracle(cy3q, oracle, weighted):
     assert len(res) == 6
     assert np.abs(np.array(res) - np.array(res_ref)).max() <= 1e-10 * max(res_ref)
     assert np.linalg.norm(s.h_block - h_ref) / np.linalg.norm(h_ref) < 1e-11
+
+
+@pytest.mark.parametrize("psi", [0.0, 0.5])
+def test_sampler_newton_polish_matches_oracle(cy3q, oracle, psi):
+    # cy3q_options.newton_polish: FP64 Newton passes on each DK root (the north star's root
+    # polish; the reference has none). After 50 DK sweeps two passes move no point beyond
+    # rounding, so the accepted set and its order are the reference's.
+    print(f"case: SZ sampler psi={psi}, 5000 points seed 42, newton_polish=2", flush=True)
+    if psi == 0.0:
+        s = cy3q.QuinticSolver.new_with_sampler(3, 5000, 42, newton_polish=2)
+    else:
+        s = cy3q.QuinticSolver.new_with_psi_sz(3, 5000, 42, psi, newton_polish=2)
+    ref, _ = oracle.sample_sz(5000, 42, psi, polish=2)
+    got = s.points
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) < 1e-12
+    unpolished, _ = oracle.sample_sz(5000, 42, psi)
+    assert np.max(np.abs(got - unpolished)) < 1e-12
+
+
+def test_sampler_newton_polish_after_short_dk(cy3q, oracle):
+    # With 12 DK sweeps, 4 polish passes converge most of the unconverged roots on the GPU as in
+    # the oracle (the few near-collisions DK leaves stay as they are).
+    print("case: SZ sampler, dk_iters=12, newton_polish=4, 10000 points", flush=True)
+    a = lambda f: np.abs(f[:, 0] + 1j * f[:, 1])  # noqa: E731
+    s = cy3q.QuinticSolver.new_with_sampler(3, 10000, 42, dk_iters=12, newton_polish=4)
+    got = s.points
+    ref, _ = oracle.sample_sz(10000, 42, dk_iters=12, polish=4)
+    fg, _ = oracle.quintic_eval(got)
+    fr, _ = oracle.quintic_eval(ref)
+    bad_g, bad_r = np.sum(a(fg) > 1e-10), np.sum(a(fr) > 1e-10)
+    assert abs(int(bad_g) - int(bad_r)) <= max(5, bad_r // 10)
+    assert np.quantile(a(fg), 0.9) < 1e-14

@@ -115,6 +115,26 @@ def test_sigma8_matches_oracle(cy3q, oracle, n, nb):
     g.close()
 
 
+def test_sigma8_tiny_weights(cy3q, oracle):
+    # Weights scaled by 1e-16 (a sum near 1e-13): the reference still returns sigma down to a
+    # weight sum of 1e-30 (cy3_sigma_gpu.rs:514-519), and so does the contract. sigma is
+    # invariant under a common weight scale.
+    from papers_amd.donaldson_gpu import Cy3SigmaGpu
+    print("sigma8 tiny weights n=900 nb=20", flush=True)
+    sv, sd, frames, om, w, h = _sigma_case(41, 900, 20)
+    g = Cy3SigmaGpu(900, 20)
+    g.upload_static(sv, sd, frames, om, w * 1e-16)
+    sig = g.compute_sigma(h.real, h.imag)
+    ref, _ = oracle.sigma_route34(sv, sd, frames, om, w * 1e-16, h.real, h.imag)
+    ref1, _ = oracle.sigma_route34(sv, sd, frames, om, w, h.real, h.imag)
+    assert np.sum(w) * 1e-16 < 1e-12
+    assert np.isfinite(sig) and abs(sig - ref) <= 1e-10 * abs(ref)
+    assert abs(ref / ref1 - 1) < 1e-12
+    g.upload_static(sv, sd, frames, om, w * 1e-40)
+    assert np.isnan(g.compute_sigma(h.real, h.imag))
+    g.close()
+
+
 def test_sigma8_fubini_study_closed_form(cy3q):
     # Linear sections, H = I, frames orthonormal with T_a . z = 0: eta = 1 / (|z|^6 |Omega|^2).
     from papers_amd.donaldson_gpu import Cy3SigmaGpu

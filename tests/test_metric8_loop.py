@@ -246,3 +246,33 @@ def test_gated_schedule_is_bitwise_inert_on_healthy_descent(oracle):
                          tikhonov=M8.TikhonovShift.k4_default())
     assert c["tikhonov_history"][0] == 1e-3 and c["donaldson_history"][0] != \
         a["donaldson_history"][0]
+
+
+def test_not_positive_definite_h_restores_snapshot(oracle):
+    # The GPU T-operator raises CY3Q_ERR_NOT_PD for an h that is not positive definite; the
+    # loop turns it into a NaN residual, so the minimum-residual snapshot comes back (the
+    # reference keeps iterating on K clamped at 1e-30, cy3_donaldson_gpu.rs:84). Other errors
+    # propagate.
+    from papers_amd.quintic import ERR_HIP, ERR_NOT_PD, Cy3qError
+
+    sv, w, nb = _problem(seed=9)
+    base = _top(oracle, sv, w)
+    calls = {"n": 0}
+
+    def not_pd(hr, hi):
+        calls["n"] += 1
+        if calls["n"] == 3:
+            raise Cy3qError(ERR_NOT_PD, "h is not Hermitian positive definite")
+        return base(hr, hi)
+
+    res = M8.solve_route34(not_pd, None, np.eye(nb), np.zeros((nb, nb)), w, max_iter=10, tol=0.0)
+    good = M8.solve_route34(base, None, np.eye(nb), np.zeros((nb, nb)), w, max_iter=2, tol=0.0)
+    assert res["iterations_run"] == 2
+    assert np.array_equal(res["h_re"], good["h_re"])
+    assert res["donaldson_history"][-1] == min(res["donaldson_history"][:2])
+
+    def hip_fails(hr, hi):
+        raise Cy3qError(ERR_HIP, "launch failed")
+
+    with pytest.raises(Cy3qError):
+        M8.solve_route34(hip_fails, None, np.eye(nb), np.zeros((nb, nb)), w, max_iter=3, tol=0.0)

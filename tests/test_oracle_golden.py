@@ -379,3 +379,24 @@ def test_monge_ampere_pipeline_in_published_bounds(oracle):
     assert res[-1] < res[0]
     s = oracle.sigma(pts, sv, sd, log_om, w, h)
     assert np.isfinite(s) and 0.05 <= s <= 1.0, s
+
+
+def test_newton_polish_of_the_dk_roots(oracle):
+    """The FP64 Newton polish after the Durand-Kerner sweeps (not in the reference, which stops
+    after 50 sweeps). Converged roots are Newton fixed points, so after 50 sweeps the polish moves
+    no point by more than rounding; after 12 sweeps it converges the roots DK left unconverged,
+    which a wrong step (sign, derivative) would not do."""
+    a = lambda f: np.abs(f[:, 0] + 1j * f[:, 1])  # noqa: E731
+    for psi in (0.0, 0.5):
+        p0, t0 = oracle.sample_sz(4000, 42, psi)
+        p2, t2 = oracle.sample_sz(4000, 42, psi, polish=2)
+        assert t0 == t2 and p0.shape == p2.shape
+        assert np.max(np.abs(p2 - p0)) < 1e-12
+        f2, _ = oracle.quintic_eval(p2, psi)
+        assert np.max(a(f2)) < 1e-12
+    p, _ = oracle.sample_sz(10000, 42, dk_iters=12)
+    pp, _ = oracle.sample_sz(10000, 42, dk_iters=12, polish=4)
+    f, _ = oracle.quintic_eval(p)
+    fp, _ = oracle.quintic_eval(pp)
+    assert np.sum(a(fp) > 1e-10) * 5 < np.sum(a(f) > 1e-10)
+    assert np.quantile(a(fp), 0.9) < 1e-14 < np.quantile(a(f), 0.9)
