@@ -281,6 +281,17 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
 int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double tol, int device,
                          double *eig, double *vecs, int64_t *sweeps);
 
+/* No counterpart in the reference. cy3q_hermitian_eigen without any device allocation, for
+ * callers that cannot allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): d_a ([n][n] complex, read only), d_eig (n values, ascending) and d_vecs
+ * ([n][n] complex or NULL) are device pointers on `device`; *sweeps is host memory. The working
+ * matrices live in a fixed 16.8 MB device scratch array of the library (one per device, kept
+ * until the process ends), so n <= 512 (CY3Q_ERR_ARG beyond). Work runs on the null stream and
+ * the call returns when the results are written. Calls that overlap on one device share the
+ * scratch array and race; serialise them. */
+int cy3q_hermitian_eigen_noalloc(const double *d_a, int64_t n, int64_t max_sweeps, double tol,
+                                 int device, double *d_eig, double *d_vecs, int64_t *sweeps);
+
 /* sz_weights (route34/yukawa_overlap_real.rs:222-268): w / |Omega|^2 renormalised to sum one,
  * zero for an unusable point; *uniformity = 1 / (1 + sd/mean). */
 int cy3q_yukawa_sz_weights(const double *weights, const double *omega, int64_t n, int device,

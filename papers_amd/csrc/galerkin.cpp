@@ -12,6 +12,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -154,61 +155,122 @@ struct StageTimer {
 // [n][n] complex; shift is added to the diagonal first. eig: ascending; vecs (device
 // [n][n], column j for eig[j]) when not null.
 // ------------------------------------------------------------------------------------------
-int heev(Ctx &c, const double2 *a, int n, int64_t max_sweeps, double tol, double shift,
-         std::vector<double> &eig, double2 *vecs, int64_t *sweeps_out) {
+// Device buffers of one Jacobi eigensolve of an n x n matrix (np = n rounded up to even):
+// A0, A1 (and V0, V1 for vectors) np x np, t0, t1 np tags, part 2 offnorm_parts(), nrm 2,
+// diag np, cols np.
+struct HeevBufs {
+  double2 *A0 = nullptr, *A1 = nullptr, *V0 = nullptr, *V1 = nullptr;
+  int *t0 = nullptr, *t1 = nullptr, *cols = nullptr;
+  double *part = nullptr, *nrm = nullptr, *diag = nullptr;
+};
+
+// The parallel Jacobi sweeps on b (hermitian_jacobi_n, metric_laplacian.rs:616-693) until the
+// relative off-diagonal norm is below tol or max_sweeps have run; eig (host, n values) ascending,
+// vecs (device, [n][n], column j for eig[j]) when b holds V0/V1. d and tag are host arrays of
+// np entries. No allocation.
+int heev_core(hipStream_t st, const double2 *a, int n, int64_t max_sweeps, double tol,
+              double shift, HeevBufs b, double *eig, double2 *vecs, int64_t *sweeps_out,
+              double *d, int *tag) {
   const int np = n + (n & 1);
-  double2 *A0, *A1, *V0 = nullptr, *V1 = nullptr;
-  int *t0, *t1;
-  double *part, *nrm, *diag;
-  G_TRY(c.alloc(&A0, (size_t)np * np));
-  G_TRY(c.alloc(&A1, (size_t)np * np));
-  if (vecs) {
-    G_TRY(c.alloc(&V0, (size_t)np * np));
-    G_TRY(c.alloc(&V1, (size_t)np * np));
-  }
-  G_TRY(c.alloc(&t0, np));
-  G_TRY(c.alloc(&t1, np));
-  G_TRY(c.alloc(&part, 2 * offnorm_parts()));
-  G_TRY(c.alloc(&nrm, 2));
-  G_TRY(c.alloc(&diag, np));
-  launch_jacobi_init(c.st, a, n, np, shift, A0, V0, t0);
+  launch_jacobi_init(st, a, n, np, shift, b.A0, vecs ? b.V0 : nullptr, b.t0);
   int64_t sw = 0;
   for (;;) {
     double h2[2];
-    launch_offnorm(c.st, A0, np, part, nrm);
-    G_HIP(hipMemcpyAsync(h2, nrm, sizeof(h2), hipMemcpyDeviceToHost, c.st));
-    G_HIP(hipStreamSynchronize(c.st));
+    launch_offnorm(st, b.A0, np, b.part, b.nrm);
+    G_HIP(hipMemcpyAsync(h2, b.nrm, sizeof(h2), hipMemcpyDeviceToHost, st));
+    G_HIP(hipStreamSynchronize(st));
     if (!std::isfinite(h2[0]) || !std::isfinite(h2[1]))
       return set_error(CY3Q_ERR_ARG, "non-finite entries in the eigenproblem");
     if (h2[0] == 0.0 || std::sqrt(h2[0]) <= tol * std::sqrt(h2[1]) || sw >= max_sweeps) break;
     for (int r = 0; r < np - 1; r++) {
-      launch_jacobi_round(c.st, A0, A1, V0, V1, t0, t1, np, vecs ? 1 : 0);
-      std::swap(A0, A1);
-      std::swap(V0, V1);
-      std::swap(t0, t1);
+      launch_jacobi_round(st, b.A0, b.A1, b.V0, b.V1, b.t0, b.t1, np, vecs ? 1 : 0);
+      std::swap(b.A0, b.A1);
+      std::swap(b.V0, b.V1);
+      std::swap(b.t0, b.t1);
     }
     G_HIP(hipGetLastError());
     sw++;
   }
-  std::vector<double> d(np);
-  std::vector<int> tag(np);
-  launch_diag_re(c.st, A0, np, diag);
-  G_HIP(hipMemcpyAsync(d.data(), diag, sizeof(double) * np, hipMemcpyDeviceToHost, c.st));
-  G_HIP(hipMemcpyAsync(tag.data(), t0, sizeof(int) * np, hipMemcpyDeviceToHost, c.st));
-  G_HIP(hipStreamSynchronize(c.st));
-  std::vector<int> slots;
+  launch_diag_re(st, b.A0, np, b.diag);
+  G_HIP(hipMemcpyAsync(d, b.diag, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+  G_HIP(hipMemcpyAsync(tag, b.t0, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+  G_HIP(hipStreamSynchronize(st));
+  // The slots holding the n eigenvalues (the padding slot of an odd n has tag n), ascending by
+  // value; equal values keep slot order (an in-place sort with the slot as the tie-break).
+  int m = 0;
   for (int x = 0; x < np; x++)
-    if (tag[x] < n) slots.push_back(x);
-  std::stable_sort(slots.begin(), slots.end(), [&](int a_, int b_) { return d[a_] < d[b_]; });
-  eig.resize(n);
-  for (int j = 0; j < n; j++) eig[j] = d[slots[j]];
+    if (tag[x] < n) tag[m++] = x;
+  std::sort(tag, tag + m, [&](int a_, int b_) { return d[a_] < d[b_] || (d[a_] == d[b_] && a_ < b_); });
+  for (int j = 0; j < n; j++) eig[j] = d[tag[j]];
   if (vecs) {
-    int *cols;
-    G_TRY(c.upload(&cols, slots.data(), slots.size()));
-    launch_gather_cols(c.st, V0, np, n, cols, nullptr, n, vecs);
+    G_HIP(hipMemcpyAsync(b.cols, tag, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    launch_gather_cols(st, b.V0, np, n, b.cols, nullptr, n, vecs);
+    G_HIP(hipStreamSynchronize(st));
   }
   if (sweeps_out) *sweeps_out = sw;
   G_HIP(hipGetLastError());
+  return CY3Q_OK;
+}
+
+int heev(Ctx &c, const double2 *a, int n, int64_t max_sweeps, double tol, double shift,
+         std::vector<double> &eig, double2 *vecs, int64_t *sweeps_out) {
+  const int np = n + (n & 1);
+  HeevBufs b;
+  G_TRY(c.alloc(&b.A0, (size_t)np * np));
+  G_TRY(c.alloc(&b.A1, (size_t)np * np));
+  if (vecs) {
+    G_TRY(c.alloc(&b.V0, (size_t)np * np));
+    G_TRY(c.alloc(&b.V1, (size_t)np * np));
+  }
+  G_TRY(c.alloc(&b.t0, np));
+  G_TRY(c.alloc(&b.t1, np));
+  G_TRY(c.alloc(&b.cols, np));
+  G_TRY(c.alloc(&b.part, 2 * offnorm_parts()));
+  G_TRY(c.alloc(&b.nrm, 2));
+  G_TRY(c.alloc(&b.diag, np));
+  std::vector<double> d(np);
+  std::vector<int> tag(np);
+  eig.resize(n);
+  return heev_core(c.st, a, n, max_sweeps, tol, shift, b, eig.data(), vecs, sweeps_out, d.data(),
+                   tag.data());
+}
+
+}  // namespace
+
+// Scratch of cy3q_hermitian_eigen_noalloc: the eigensolve buffers of HeevBufs for matrices up to
+// kNoAllocMaxN, carved in order A0, A1, V0, V1 (kNoAllocMaxN^2 complex each), part
+// (2 kNoAllocParts), nrm (2), diag (kNoAllocMaxN doubles), t0, t1, cols (kNoAllocMaxN ints
+// each): 16.8 MB per device, held until the process ends. Every call on a device shares it.
+constexpr int kNoAllocMaxN = 512;
+constexpr int kNoAllocParts = 256;
+constexpr size_t kNoAllocBytes =
+    4 * sizeof(double2) * (size_t)kNoAllocMaxN * kNoAllocMaxN +
+    sizeof(double) * (2 * kNoAllocParts + 2 + kNoAllocMaxN) + sizeof(int) * 3 * kNoAllocMaxN;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// The device address of `scratch` on `device` (each device holds its own copy), looked up once
+// per device with that device current. The host-side name of the array is not a device
+// address and is never passed to a kernel or a copy.
+int noalloc_scratch(int device, unsigned char **out) {
+  static std::mutex mu;
+  static unsigned char *addr[64] = {};
+  int count = 0;
+  G_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count || device >= 64)
+    return set_error(CY3Q_ERR_ARG, "bad device ordinal");
+  G_HIP(hipSetDevice(device));
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(CY3Q_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+    addr[device] = static_cast<unsigned char *>(p);
+  }
+  *out = addr[device];
   return CY3Q_OK;
 }
 
@@ -522,6 +584,40 @@ int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double 
   if (vecs)
     G_HIP(hipMemcpyAsync(vecs, d_v, sizeof(double2) * n * n, hipMemcpyDeviceToHost, c.st));
   G_HIP(hipStreamSynchronize(c.st));
+  return CY3Q_OK;
+}
+
+int cy3q_hermitian_eigen_noalloc(const double *d_a, int64_t n, int64_t max_sweeps, double tol,
+                                 int device, double *d_eig, double *d_vecs, int64_t *sweeps) {
+  if (!d_a || !d_eig || n <= 0) return set_error(CY3Q_ERR_ARG, "bad matrix arguments");
+  if (n > kNoAllocMaxN)
+    return set_error(CY3Q_ERR_ARG, "cy3q_hermitian_eigen_noalloc: n = " + std::to_string(n) +
+                                       " exceeds its scratch limit of " +
+                                       std::to_string(kNoAllocMaxN));
+  if (offnorm_parts() > kNoAllocParts)
+    return set_error(CY3Q_ERR_STATE, "offnorm partials exceed the scratch layout");
+  unsigned char *base = nullptr;
+  G_TRY(noalloc_scratch(device, &base));
+  const size_t mat = (size_t)kNoAllocMaxN * kNoAllocMaxN;
+  HeevBufs b;
+  b.A0 = reinterpret_cast<double2 *>(base);
+  b.A1 = b.A0 + mat;
+  b.V0 = b.A1 + mat;
+  b.V1 = b.V0 + mat;
+  b.part = reinterpret_cast<double *>(b.V1 + mat);
+  b.nrm = b.part + 2 * kNoAllocParts;
+  b.diag = b.nrm + 2;
+  b.t0 = reinterpret_cast<int *>(b.diag + kNoAllocMaxN);
+  b.t1 = b.t0 + kNoAllocMaxN;
+  b.cols = b.t1 + kNoAllocMaxN;
+  double d[kNoAllocMaxN], eig[kNoAllocMaxN];
+  int tag[kNoAllocMaxN];
+  // The null stream: no stream is created.
+  const hipStream_t st = nullptr;
+  G_TRY(heev_core(st, reinterpret_cast<const double2 *>(d_a), (int)n, max_sweeps, tol, 0.0, b, eig,
+                  reinterpret_cast<double2 *>(d_vecs), sweeps, d, tag));
+  G_HIP(hipMemcpyAsync(d_eig, eig, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  G_HIP(hipStreamSynchronize(st));
   return CY3Q_OK;
 }
 
