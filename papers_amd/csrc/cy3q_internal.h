@@ -63,6 +63,11 @@ void launch_sections(hipStream_t st, const double *pts, int64_t n, const uint32_
 // mode 0: store y (ld = m_cols); mode 1: rowsumsq partial per 32-col part into dpart.
 void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m_cols,
                         const double *bmat, double *y, int mode, int full);
+// FP64 D pass: D parts dpart[q][p] (q < dpass_f64_ranges(M)) of D_p = |x_p B|^2 and the
+// coefficients coef[p] = w_p / max(D_p, 1e-30) (0 for an invalid weight).
+int dpass_f64_ranges(int64_t m_cols);
+void launch_dpass_f64(hipStream_t st, const double *x, int64_t rows, int64_t m_cols,
+                      const double *bmat, const double *w, double *dpart, double *coef);
 // Weighted symmetric rank-k: partial lower tiles of X^T diag(c) X over point chunks.
 struct SyrkPlan;
 int syrk_f64_max_tiles(int64_t m_cols);  // accumulator tiles per wave of k_syrk_f64

@@ -138,6 +138,225 @@ void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m
 }
 
 // ----------------------------------------------------------------------------------------
+// D pass in FP64: D_p = |x_p B|^2 = sum over the columns of (X B)_pc^2, B the real-ified W^T
+// (block upper triangular: column c needs k < 2 (c / 2) + 2). A workgroup takes a block of
+// DG_P = 96 points and a range of at most 448 columns (all of them for M <= 448, so X is read
+// from HBM once and D_p is complete in the workgroup). Its 8 waves are 2 point groups x 4
+// column owners: wave (pg, cm) holds the 16 x 16 accumulators of point tiles 3 pg .. 3 pg + 2
+// and column tiles cg = cm, cm + 4, ... of the range. At stage s the active tiles are those
+// with cg >= s, so the owners differ by at most one active tile. Each 16-deep k stage of X (64 rows) and of B
+// (16 rows of the range) is copied from global memory into one of two LDS slots with 16-byte
+// global_load_lds copies, one stage ahead; copies holding only columns of tiles that are
+// already finished are skipped, so the B bytes per stage shrink with the triangle. The
+// operands of a 4-deep k step are read from LDS before the previous step's MFMAs issue.
+// Epilogue: row sums of squares, reduced over the 16 column lanes and the 2 owners in a fixed
+// order; with one column range the coefficient c_p = w_p / max(D_p, 1e-30) (0 for an invalid
+// weight) is written too.
+// MFMA f64 16x16x4: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[i][j] at lane
+// j + 16 (i % 4), element i / 4.
+// ----------------------------------------------------------------------------------------
+static constexpr int DG_OWN = 4;        // column owners (the other factor of 8 waves: point groups)
+static constexpr int DG_PT = 3;         // point tiles of 16 per wave
+static constexpr int DG_P = 16 * DG_PT * (8 / DG_OWN);  // points per workgroup
+static constexpr int DG_MAXW = 416;     // columns per range
+static constexpr int DG_NT = (DG_MAXW / 16 + DG_OWN - 1) / DG_OWN;  // column tiles per wave
+static constexpr int DG_KS = 16;        // k per stage
+static constexpr int DG_ALD = DG_KS + 2;  // LDS row stride of the X stage (doubles)
+static constexpr int DG_ACOPY = 2;      // X-stage copies per wave: 16 KB per stage
+static constexpr int DG_ABYTES = 8 * DG_ACOPY * 1024;
+static constexpr int DG_ASLOTS = 3;     // X stages in flight: from HBM, two stages ahead
+static constexpr int DG_BCPW = 7;       // B-stage copy slots per wave
+static_assert(DG_P * DG_ALD * 8 <= DG_ABYTES, "X stage size");
+static_assert(8 % DG_OWN == 0, "wave layout");
+
+__host__ __device__ constexpr int dg_bld(int width) { return width + 16; }
+__host__ __device__ constexpr int dg_bbytes(int width) {
+  return (DG_KS * dg_bld(width) * 8 + 1023) / 1024 * 1024;
+}
+static_assert(dg_bbytes(DG_MAXW) / 1024 <= 8 * DG_BCPW, "copy slots per wave");
+static_assert(DG_ASLOTS * DG_ABYTES + 2 * dg_bbytes(DG_MAXW) <= 160 * 1024, "LDS per CU");
+
+__global__ void __launch_bounds__(512) k_dpass_f64(const double *__restrict__ x, int64_t rows,
+                                                   int64_t m, const double *__restrict__ bm,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ dpart,
+                                                   double *__restrict__ coef, int n_ranges,
+                                                   int range_tiles) {
+  extern __shared__ __attribute__((aligned(1024))) char dlds[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // Waves w and w + 4 share a SIMD; giving them owners cm and cm + 2 (mod 4) pairs an owner with
+  // more active tiles at a stage with one with fewer, so the SIMDs stay balanced.
+  const int pg = wave / DG_OWN, cm = (wave + 2 * pg) % DG_OWN;
+  const int q = (int)(blockIdx.x % n_ranges);
+  const int64_t p0 = (int64_t)(blockIdx.x / n_ranges) * DG_P;
+  const int ncg = (int)(m / 16);
+  const int cg_lo = q * range_tiles;
+  const int cg_hi = cg_lo + range_tiles < ncg ? cg_lo + range_tiles : ncg;
+  const int ntile_r = cg_hi - cg_lo;
+  const int width = 16 * ntile_r;
+  const int bld = dg_bld(width);
+  const int bbytes = dg_bbytes(width);
+  const int bcopies = bbytes / 1024;
+  char *aring = dlds, *bring = dlds + DG_ASLOTS * DG_ABYTES;
+  const int n_stage = 16 * cg_hi / DG_KS;  // k < 16 cg_hi
+  // Copy g of a stage writes LDS bytes [1024 g + 16 lane, +16) of its slot. X copy u of this
+  // wave (g = wave + 8 u) reads xb + aofs[u] + KS s, B copy u reads bb + bofs[u] + KS m s
+  // (doubles); padding pieces fetch a valid element that is never read. bmax[u]: the last range
+  // column a B copy holds (-1: none), so it is skipped once every tile it feeds is finished.
+  const double *xb = x + p0 * m;
+  const double *bb = bm + 16 * cg_lo;
+  const int64_t rows_here = rows - p0 < DG_P ? rows - p0 : DG_P;
+  int aofs[DG_ACOPY], bofs[DG_BCPW], bmax[DG_BCPW];
+#pragma unroll
+  for (int u = 0; u < DG_ACOPY; u++) {
+    const int o = (wave + 8 * u) * 1024 + lane * 16;
+    const int r = o / (DG_ALD * 8), b = o % (DG_ALD * 8);
+    const int row = r < rows_here ? r : (int)rows_here - 1;
+    aofs[u] = (r < DG_P && b < DG_KS * 8) ? row * (int)m + b / 8 : 0;
+  }
+#pragma unroll
+  for (int u = 0; u < DG_BCPW; u++) {
+    const int g = wave + 8 * u;
+    const int o2 = g * 1024 + lane * 16;
+    const int kr = o2 / (bld * 8), b = o2 % (bld * 8);
+    bofs[u] = (kr < DG_KS && b < width * 8) ? kr * (int)m + b / 8 : 0;
+    const int rowb = bld * 8, s0 = g * 1024, e0 = s0 + 1023;
+    const int rs = s0 / rowb, re = e0 / rowb;
+    const int last = (e0 % rowb) / 8 < width - 1 ? (e0 % rowb) / 8 : width - 1;
+    bmax[u] = g >= bcopies || rs >= DG_KS ? -1 : (re > rs ? width - 1 : last);
+  }
+  auto issue_a = [&](int st) {
+    char *dst = aring + (st % DG_ASLOTS) * DG_ABYTES;
+#pragma unroll
+    for (int u = 0; u < DG_ACOPY; u++)
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(xb + aofs[u] + (int64_t)st * DG_KS),
+          (__attribute__((address_space(3))) void *)(dst + (wave + 8 * u) * 1024), 16, 0, 0);
+  };
+  auto issue_b = [&](int st) {
+    char *dst = bring + (st & 1) * bbytes;
+    const int need = st > cg_lo ? 16 * (st - cg_lo) : 0;  // first range column still needed
+#pragma unroll
+    for (int u = 0; u < DG_BCPW; u++)
+      if (bmax[u] >= need)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(bb + bofs[u] + (int64_t)st * DG_KS * m),
+            (__attribute__((address_space(3))) void *)(dst + (wave + 8 * u) * 1024), 16, 0, 0);
+  };
+  double4_t acc[DG_PT][DG_NT];
+#pragma unroll
+  for (int t = 0; t < DG_PT; t++)
+#pragma unroll
+    for (int v = 0; v < DG_NT; v++) acc[t][v] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  // Issue order: X(0), B(0), X(1), then at stage s: B(s + 1), X(s + 2). At the top of stage s
+  // the only copies of this wave that may still be in flight are the DG_ACOPY of X(s + 1).
+  issue_a(0);
+  issue_b(0);
+  if (n_stage > 1) issue_a(1);
+  for (int st = 0; st < n_stage; st++) {
+    asm volatile("" ::: "memory");
+    if (st + 1 < n_stage)
+      __builtin_amdgcn_s_waitcnt(0x72);  // vmcnt(2): X(st) and B(st) have landed
+    else
+      __builtin_amdgcn_s_waitcnt(0x70);  // vmcnt(0)
+    __syncthreads();                     // ... for every wave; B slot (st + 1) & 1 and X slot
+    asm volatile("" ::: "memory");       // (st + 2) % 3 are free
+    if (st + 1 < n_stage) issue_b(st + 1);
+    if (st + 2 < n_stage) issue_a(st + 2);
+    const double *sa = reinterpret_cast<const double *>(aring + (st % DG_ASLOTS) * DG_ABYTES);
+    const double *sb = reinterpret_cast<const double *>(bring + (st & 1) * bbytes);
+    // The active column tiles of this wave (cg >= st, cg < cg_hi) are the slots v with
+    // v_lo <= v < v_hi. All operands of a 4-deep k step are read before its MFMAs, and the next
+    // step's operands are read before the current step's MFMAs are issued.
+    const int v_lo = st > cg_lo + cm ? (st - cg_lo - cm + DG_OWN - 1) / DG_OWN : 0;
+    const int v_hi = ntile_r - cm > 0 ? (ntile_r - cm + DG_OWN - 1) / DG_OWN : 0;
+    auto load = [&](int kk, double (&a)[DG_PT], double (&b)[DG_NT]) {
+      const int kl = 4 * kk + (lane >> 4);
+#pragma unroll
+      for (int t = 0; t < DG_PT; t++)
+        a[t] = sa[(16 * (DG_PT * pg + t) + (lane & 15)) * DG_ALD + kl];
+#pragma unroll
+      for (int v = 0; v < DG_NT; v++) {
+        const int c = cm + DG_OWN * v < ntile_r ? cm + DG_OWN * v : ntile_r - 1;  // unused if idle
+        b[v] = sb[kl * bld + 16 * c + (lane & 15)];
+      }
+    };
+    double a0[DG_PT], b0[DG_NT], a1[DG_PT], b1[DG_NT];
+    load(0, a0, b0);
+#pragma unroll
+    for (int kk = 0; kk < DG_KS / 4; kk++) {
+      double(&a)[DG_PT] = (kk & 1) ? a1 : a0;
+      double(&b)[DG_NT] = (kk & 1) ? b1 : b0;
+      if (kk + 1 < DG_KS / 4) {
+        if (kk & 1)
+          load(kk + 1, a0, b0);
+        else
+          load(kk + 1, a1, b1);
+      }
+#pragma unroll
+      for (int v = 0; v < DG_NT; v++)
+        if (v >= v_lo && v < v_hi)
+#pragma unroll
+          for (int t = 0; t < DG_PT; t++)
+            acc[t][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[v], acc[t][v], 0, 0, 0);
+    }
+  }
+  // Row sums of squares: element e of lane l is point 16 (PT pg + t) + 4 e + l / 16.
+  __syncthreads();  // the LDS slots are reused for the owner partials
+  double *part = reinterpret_cast<double *>(dlds);  // [DG_OWN][DG_P]
+#pragma unroll
+  for (int t = 0; t < DG_PT; t++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      double sq = 0.0;
+#pragma unroll
+      for (int v = 0; v < DG_NT; v++) sq += acc[t][v][e] * acc[t][v][e];
+      sq += __shfl_xor(sq, 1);
+      sq += __shfl_xor(sq, 2);
+      sq += __shfl_xor(sq, 4);
+      sq += __shfl_xor(sq, 8);
+      if ((lane & 15) == 0) part[cm * DG_P + 16 * (DG_PT * pg + t) + 4 * e + (lane >> 4)] = sq;
+    }
+  __syncthreads();
+  if (tid < DG_P) {
+    const int64_t p = p0 + tid;
+    if (p < rows) {
+      double d = 0.0;
+#pragma unroll
+      for (int o = 0; o < DG_OWN; o++) d += part[o * DG_P + tid];
+      dpart[(int64_t)q * rows + p] = d;
+      if (coef) {
+        const double wp = w[p];
+        coef[p] = (isfinite(wp) && wp > 0.0) ? (1.0 / fmax(d, 1e-30)) * wp : 0.0;
+      }
+    }
+  }
+}
+
+// Column ranges of the FP64 D pass for M real columns.
+int dpass_f64_ranges(int64_t m_cols) {
+  const int64_t ncg = m_cols / 16, per = DG_MAXW / 16;
+  return (int)((ncg + per - 1) / per);
+}
+
+// D parts (dpart[q][p], q < dpass_f64_ranges(M)) and, with one range, the coefficients c_p;
+// with several ranges the coefficients come from launch_point_coeffs_f64 over the parts.
+void launch_dpass_f64(hipStream_t st, const double *x, int64_t rows, int64_t m_cols,
+                      const double *bmat, const double *w, double *dpart, double *coef) {
+  if (rows <= 0) return;
+  const int nr = dpass_f64_ranges(m_cols);
+  const int ncg = (int)(m_cols / 16);
+  const int per = (ncg + nr - 1) / nr;  // tiles per range, balanced
+  const int width = 16 * per;
+  const size_t lds = (size_t)DG_ASLOTS * DG_ABYTES + 2 * (size_t)dg_bbytes(width);
+  const int64_t blocks = (rows + DG_P - 1) / DG_P;
+  hipLaunchKernelGGL(k_dpass_f64, dim3((unsigned)(blocks * nr)), dim3(512), lds, st, x, rows,
+                     m_cols, bmat, w, dpart, nr == 1 ? coef : nullptr, nr, per);
+  if (nr > 1) launch_point_coeffs_f64(st, w, dpart, nr, rows, coef);
+}
+
+// ----------------------------------------------------------------------------------------
 // Weighted SYRK partials. Each workgroup (8 waves) handles one point chunk and one "part" (a
 // set of lower 16x16 sub-tiles of the M x M result). Wave w of part q takes its own list of
 // up to max_tiles sub-tiles, ordered by row group so the A fragment is reused.

@@ -515,11 +515,8 @@ int t_step_fp64(cy3q_solver *s) {
   const int64_t n = s->n, m = s->m;
   ensure_bmat(s);
   s->stage_begin(STG_DPASS);
-  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
+  launch_dpass_f64(s->st, s->sv, n, m, s->bmat, s->w, s->dpart, s->coef);
   s->stage_end(STG_DPASS);
-  s->stage_begin(STG_COEFF);
-  launch_point_coeffs_f64(s->st, s->w, s->dpart, (int)(m / 32), n, s->coef);
-  s->stage_end(STG_COEFF);
   s->stage_begin(STG_TPASS);
   launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
   s->stage_end(STG_TPASS);
@@ -828,8 +825,7 @@ int legacy_step(cy3q_solver *s, int weighted, double *residual) {
   }
   const double *wv = weighted ? s->w : s->ones;
   ensure_bmat(s);
-  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
-  launch_point_coeffs_f64(s->st, wv, s->dpart, (int)(m / 32), n, s->coef);
+  launch_dpass_f64(s->st, s->sv, n, m, s->bmat, wv, s->dpart, s->coef);
   launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
   launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
   launch_t_from_xx(s->st, s->xx, m, s->nb, weighted ? s->sum_w : s->n_as_w, 1.0, nullptr, 1,
@@ -1876,8 +1872,7 @@ int cy3q_top_t_operator_raw(cy3q_top *t, const double *h_re, const double *h_im,
   ST_TRY(w_from_g_pd(s, &pd));
   if (!pd) return fail(CY3Q_ERR_NOT_PD, "h is not Hermitian positive definite");
   ensure_bmat(s);
-  launch_trigemm_f64(s->st, s->sv, n, m, s->bmat, s->dpart, 1, 0);
-  launch_point_coeffs_f64(s->st, s->w, s->dpart, (int)(m / 32), n, s->coef);
+  launch_dpass_f64(s->st, s->sv, n, m, s->bmat, s->w, s->dpart, s->coef);
   launch_syrk_f64(s->st, s->plan64, s->sv, s->coef, n, s->partials);
   launch_syrk_reduce(s->st, s->plan64, s->partials, s->xx);
   HIP_TRY(hipGetLastError());
