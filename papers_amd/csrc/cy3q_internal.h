@@ -125,8 +125,10 @@ void launch_t_from_xx(hipStream_t st, const double *xx, int64_t m_cols, int nb,
 void launch_elim_init(hipStream_t st, const double *src, int nb, double *a, double *r);
 void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv, double *w, int nb,
                              int32_t *status);
+// With counter (a zero-initialised unsigned that the kernel leaves at zero), the last tile
+// workgroup forms the residual; otherwise a second launch does.
 void launch_gram_residual(hipStream_t st, double *w, int nb, double *g, const int32_t *status,
-                          double *partial, double *residual);
+                          double *partial, double *residual, unsigned *counter = nullptr);
 
 // sigma (k_sigma.hip).
 void launch_sigma_eta(hipStream_t st, const double *pts, const double *y0, const double *yd,

@@ -175,34 +175,21 @@ __device__ inline double rsqrt_f64(double x) {
   return fma(0.5 * y, t, y);
 }
 
-__global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a,
-                                                    const double *__restrict__ r, int n, int jblk,
-                                                    double *__restrict__ einv,
-                                                    double *__restrict__ w,
-                                                    int32_t *__restrict__ status) {
-  __shared__ Cz colb[2][TB];     // column j of the left tile at step j (double-buffered)
-  __shared__ Cz rowb[2][TB];     // row j of the right tile at step j
-  __shared__ Cz e[TB][TB + 1];   // L_JJ^-1
+// Factors the diagonal tile of block column jblk (in LDS, identity outside the matrix) and
+// inverts it: W_JJ = E = L_JJ^-1 into einv and w. 256 threads; colb, rowb and e are LDS
+// scratch. A bad pivot sets status = ST_ERR_NOT_PD (later kernels return early).
+__device__ void panel_core(const Cz (*tile)[TB + 1], int n, int jblk, double *__restrict__ einv,
+                           double *__restrict__ w, int32_t *__restrict__ status,
+                           Cz (*colb)[TB], Cz (*rowb)[TB], Cz (*e)[TB + 1]) {
   const int tid = threadIdx.x;
-  const int j0 = jblk * TB;
-  if (jblk == 0) {
-    if (tid == 0) *status = ST_OK;
-  } else if (*status != ST_OK) {
-    return;
-  }
   // Thread tid owns elements (i, k) = (tid / 32 + 8 u, tid % 32), u = 0..3, of the left tile
-  // A_JJ (identity outside the matrix) and of the right tile (starts as I), in registers.
+  // A_JJ and of the right tile (starts as I), in registers.
   const int k = tid % TB, i0 = tid / TB;
   Cz dv[4], rv[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int i = i0 + 8 * u;
-    dv[u] = Cz{i == k ? 1.0 : 0.0, 0.0};
-    if (j0 + i < n && j0 + k < n) {
-      const double2 v =
-          *reinterpret_cast<const double2 *>(a + 2 * ((int64_t)(j0 + i) * n + j0 + k));
-      dv[u] = Cz{v.x, v.y};
-    }
+    dv[u] = tile[i][k];
     rv[u] = Cz{i == k ? 1.0 : 0.0, 0.0};
   }
   for (int q = tid; q < TB * TB; q += 256) e[q / TB][q % TB] = Cz{0.0, 0.0};
@@ -253,13 +240,26 @@ __global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a
   }
   // L_JJ^-1 for the update kernel (a 32 x 32 tile per block column), and W_JJ = L_JJ^-1.
   for (int q = tid; q < TB * TB; q += 256) {
-    const int i = q / TB, k = q % TB;
+    const int i = q / TB, kk = q % TB;
     double2 v;
-    v.x = e[i][k].r;
-    v.y = e[i][k].i;
+    v.x = e[i][kk].r;
+    v.y = e[i][kk].i;
     *reinterpret_cast<double2 *>(einv + 2 * ((int64_t)jblk * TB * TB + q)) = v;
   }
-  store_tile(w, n, j0, j0, e, tid, 256);
+  store_tile(w, n, jblk * TB, jblk * TB, e, tid, 256);
+}
+
+// The first block column's panel (the update kernel of block column J factors panel J + 1).
+__global__ void __launch_bounds__(256) k_elim_panel(const double *__restrict__ a, int n,
+                                                    double *__restrict__ einv,
+                                                    double *__restrict__ w,
+                                                    int32_t *__restrict__ status) {
+  __shared__ Cz colb[2][TB], rowb[2][TB];
+  __shared__ Cz tile[TB][TB + 1], e[TB][TB + 1];
+  if (threadIdx.x == 0) *status = ST_OK;
+  load_tile(tile, a, n, 0, 0, true, threadIdx.x, 256);
+  __syncthreads();
+  panel_core(tile, n, 0, einv, w, status, colb, rowb, e);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -357,10 +357,11 @@ __device__ inline void quad_store(double *__restrict__ dst, int n, int r0, int c
 // All tiles a workgroup needs are fetched at once, so the kernel pays one memory latency.
 __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
                                                      double *__restrict__ r,
-                                                     const double *__restrict__ einv,
+                                                     double *__restrict__ einv,
                                                      double *__restrict__ w, int n, int jblk,
-                                                     const int32_t *__restrict__ status) {
+                                                     int32_t *__restrict__ status) {
   __shared__ Cz e[TB][TB + 1], t1[TB][TB + 1], t2[TB][TB + 1], t3[TB][TB + 1];
+  __shared__ Cz colb[2][TB], rowb[2][TB];
   if (*status != ST_OK) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int qr = wave >> 1, qc = wave & 1;
@@ -428,6 +429,21 @@ __global__ void __launch_bounds__(256) k_elim_update(double *__restrict__ a,
     cgemm32(t1, 0, need_b ? t2 : e, 0, qr, qc, lane, xr, xi);  // L_IJ W_JK
     quad_sub_store(r, n, I * TB, K * TB, t3, xr, xi, qr, qc, lane);
   }
+  // The next diagonal tile (J + 1, J + 1) is final once this workgroup has updated it: factor
+  // it here (the next block column's panel) instead of in a launch of its own. Its elements
+  // outside the matrix are the identity.
+  if (type1 && I == jblk + 1 && K == jblk + 1) {
+    __syncthreads();
+#pragma unroll
+    for (int e4 = 0; e4 < 4; e4++) {
+      const int rr = 16 * qr + 4 * e4 + (lane >> 4), cc = 16 * qc + (lane & 15);
+      const bool in = I * TB + rr < n && K * TB + cc < n;
+      t2[rr][cc] = in ? Cz{t3[rr][cc].r - xr[e4], t3[rr][cc].i - xi[e4]}
+                      : Cz{rr == cc ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    panel_core(t2, n, jblk + 1, einv, w, status, colb, rowb, t1);
+  }
 }
 
 // Last block row of W: W_JK = L_JJ^-1 R_JK for K < J = nblk - 1 (no update kernel follows it).
@@ -455,13 +471,12 @@ __global__ void __launch_bounds__(256) k_elim_lastrow(const double *__restrict__
 void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv, double *w, int nb,
                              int32_t *status) {
   const int nblk = (nb + TB - 1) / TB;
-  for (int j = 0; j < nblk; j++) {
-    hipLaunchKernelGGL(k_elim_panel, dim3(1), dim3(256), 0, st, a, r, nb, j, einv, w, status);
+  hipLaunchKernelGGL(k_elim_panel, dim3(1), dim3(256), 0, st, a, nb, einv, w, status);
+  for (int j = 0; j + 1 < nblk; j++) {
     const int mrem = nblk - j - 1;
     const int tiles = mrem * (mrem + 1) / 2 + mrem * (j + 1);
-    if (tiles > 0)
-      hipLaunchKernelGGL(k_elim_update, dim3(tiles), dim3(256), 0, st, a, r, einv, w, nb, j,
-                         status);
+    hipLaunchKernelGGL(k_elim_update, dim3(tiles), dim3(256), 0, st, a, r, einv, w, nb, j,
+                       status);
   }
   if (nblk > 1)
     hipLaunchKernelGGL(k_elim_lastrow, dim3(nblk - 1), dim3(256), 0, st, r, einv, w, nb, status);
@@ -479,7 +494,9 @@ void launch_inverse_cholesky(hipStream_t st, double *a, double *r, double *einv,
 __global__ void __launch_bounds__(256) k_gram_tiles(double *__restrict__ w, int n,
                                                     double *__restrict__ g,
                                                     const int32_t *__restrict__ status,
-                                                    double *__restrict__ partial) {
+                                                    double *__restrict__ partial,
+                                                    unsigned *__restrict__ counter,
+                                                    double *__restrict__ residual) {
   __shared__ Cz wi[2][TB][TB + 1], wk[2][TB][TB + 1];
   __shared__ double red[256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -533,7 +550,25 @@ __global__ void __launch_bounds__(256) k_gram_tiles(double *__restrict__ w, int 
       if (tid < off) red[tid] += red[tid + off];
       __syncthreads();
     }
-    if (tid == 0) partial[blockIdx.x] = red[0];
+    // With a counter, the last tile to finish sums the per-tile partials in tile order (the
+    // counter wraps back to zero for the next launch); otherwise k_residual_final does.
+    __shared__ bool last;
+    if (tid == 0) {
+      partial[blockIdx.x] = red[0];
+      last = false;
+      if (counter) {
+        __threadfence();
+        last = atomicInc(counter, gridDim.x - 1) == gridDim.x - 1;
+      }
+    }
+    __syncthreads();
+    if (last && tid == 0) {
+      __threadfence();
+      double acc = 0.0;
+      for (unsigned i = 0; i < gridDim.x; i++)
+        acc += __hip_atomic_load(partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *residual = sqrt(acc);
+    }
   }
 }
 
@@ -547,10 +582,12 @@ __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict_
 }
 
 void launch_gram_residual(hipStream_t st, double *w, int nb, double *g, const int32_t *status,
-                          double *partial, double *residual) {
+                          double *partial, double *residual, unsigned *counter) {
   const int nblk = (nb + TB - 1) / TB;
-  hipLaunchKernelGGL(k_gram_tiles, dim3(nblk * nblk), dim3(256), 0, st, w, nb, g, status, partial);
-  if (partial && residual)
+  const bool fused = partial && residual && counter;
+  hipLaunchKernelGGL(k_gram_tiles, dim3(nblk * nblk), dim3(256), 0, st, w, nb, g, status, partial,
+                     fused ? counter : nullptr, fused ? residual : nullptr);
+  if (partial && residual && !fused)
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, st, partial, nblk * nblk, residual);
 }
 

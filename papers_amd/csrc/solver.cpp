@@ -348,6 +348,9 @@ int finish_workspace(cy3q_solver *s) {
   ST_TRY(s->alloc(&s->sig_out, 8));
   ST_TRY(s->alloc(&s->sig_scratch, sigma_reduce_scratch()));
   ST_TRY(s->alloc(&s->status, 4));
+  // status[3] is the completion counter of the Gram kernel's fused residual: it starts at zero
+  // and every launch leaves it at zero.
+  HIP_TRY(hipMemsetAsync(s->status, 0, 4 * sizeof(int32_t), s->st));
   const int nblk = (nb + 31) / 32;
   ST_TRY(s->alloc(&s->abuf, 2 * nn));
   ST_TRY(s->alloc(&s->rbuf, 2 * nn));
@@ -533,7 +536,8 @@ int finish_step(cy3q_solver *s, double scale, const double *dscale) {
   launch_t_from_xx(s->st, s->xx, s->m, nb, s->sum_w, scale, dscale, 1, s->scale_buf, s->t,
                    s->abuf, s->rbuf);
   launch_inverse_cholesky(s->st, s->abuf, s->rbuf, s->einv, s->wnew, nb, s->status);
-  launch_gram_residual(s->st, s->wnew, nb, s->g, s->status, s->partial, s->resid);
+  launch_gram_residual(s->st, s->wnew, nb, s->g, s->status, s->partial, s->resid,
+                       reinterpret_cast<unsigned *>(s->status + 3));
   std::swap(s->wmat, s->wnew);
   s->bmat_valid = false;
   s->stage_end(STG_NXN);
