@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <string>
+#include <vector>
 
 namespace cy3q {
 
@@ -225,20 +226,41 @@ void launch_add_diag(hipStream_t st, double2 *a, int n, double shift);
 void launch_zgemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C, int M, int N,
                   int K, int lda, int ldb, int ldc, int opa, int opb);
 
-// Yukawa triple overlaps (k_yukawa.hip).
+// Yukawa triple overlaps (k_yukawa.hip): a plan of pair products x mode values on the FP64
+// MFMA for nm modes over n points.
+struct YukDims {
+  int nm = 0, npair = 0, vld = 0, vldp = 0, uld = 0, ntiles = 0, nparts = 0, n_trip = 0;
+  int cfull_rows = 0;
+  int64_t chunk = 0, n_chunks = 0;
+  size_t lds_bytes = 0;
+};
+struct YukPlanShape : YukDims {
+  std::vector<short2> pairs, tiles;
+  std::vector<int4> parts;
+  std::vector<short4> trip;
+};
+struct YukPlan : YukDims {
+  double *psi_v = nullptr, *partials = nullptr, *cfull = nullptr;
+  double2 *g = nullptr;
+  const short2 *pairs = nullptr, *tiles = nullptr;
+  const int4 *parts = nullptr;
+  const short4 *trip = nullptr;
+};
 int yukawa_max_modes();
-int64_t yukawa_chunks(int64_t n, int n_trip);
-void launch_yukawa(hipStream_t st, const double2 *modes, int64_t ld, const double *w,
-                   const double *omega, int nm, int64_t n, int check_psi, const short4 *trip,
-                   int n_trip, const double *hdiag, int64_t n_chunks, double2 *fac,
-                   unsigned long long *bad, double2 *partials, double2 *y_tri, double2 *y_full);
+YukPlanShape yukawa_plan_shape(int nm, int64_t n);
+void launch_yuk_prep(hipStream_t st, const double2 *modes, int64_t mode_stride,
+                     int64_t point_stride, const double *w, const double *omega, int64_t n,
+                     int check_psi, int nm, int vld, double *psi_v, double2 *g);
+void launch_yuk_contract(hipStream_t st, const YukPlan &pl, int64_t n, const double *hdiag,
+                         double2 *y_tri, double2 *y_full);
+void launch_modes_bmat(hipStream_t st, const double2 *coef, int nsec, int nm, int64_t krows,
+                       int64_t ncols, double *b);
+void launch_gemm_nn_f64(hipStream_t st, const double *x, int64_t rows, int64_t k_cols,
+                        const double *bmat, int64_t ldb, int64_t n_cols, double *y, int64_t ldy);
 void launch_sz_weights(hipStream_t st, const double *w, const double *omega, int64_t n, double *sz,
                        double *out);
 void launch_scale_weights(hipStream_t st, const double *w, const int32_t *count,
                           const double *total, int64_t n, double *out);
-void launch_modes_from_sections(hipStream_t st, const double *sv, int64_t n, int64_t m_cols,
-                                int nsec, const double2 *coef, int nm, double2 *modes,
-                                int64_t ld);
 void launch_measure_w_omega2(hipStream_t st, const double *w, const double *log_om, int64_t n,
                              double *c);
 void launch_bootstrap_counts(hipStream_t st, uint64_t seed_b, int64_t n, int32_t *count);

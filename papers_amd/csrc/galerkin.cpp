@@ -505,29 +505,27 @@ std::vector<uint8_t> raw_monomials(uint32_t d) {
   return out;
 }
 
-std::vector<short4> triples(int nm) {
-  std::vector<short4> t;
-  for (int i = 0; i < nm; i++)
-    for (int j = i; j < nm; j++)
-      for (int k = j; k < nm; k++) t.push_back(make_short4((short)i, (short)j, (short)k, 0));
-  return t;
-}
-
-// Device Yukawa workspace for n points and nm modes.
+// Device Yukawa plan for n points and nm modes.
 struct YukWs {
-  short4 *trip = nullptr;
-  int n_trip = 0;
-  int64_t n_chunks = 0;
-  double2 *fac = nullptr, *partials = nullptr;
-  unsigned long long *bad = nullptr;
+  YukPlan pl;
   int init(Ctx &c, int nm, int64_t n) {
-    std::vector<short4> t = triples(nm);
-    n_trip = (int)t.size();
-    G_TRY(c.upload(&trip, t.data(), t.size()));
-    n_chunks = yukawa_chunks(n, n_trip);
-    G_TRY(c.alloc(&fac, n));
-    G_TRY(c.alloc(&bad, n));
-    G_TRY(c.alloc(&partials, (size_t)n_chunks * n_trip));
+    YukPlanShape sh = yukawa_plan_shape(nm, n);
+    static_cast<YukDims &>(pl) = sh;
+    short2 *pairs, *tiles;
+    int4 *parts;
+    short4 *trip;
+    G_TRY(c.upload(&pairs, sh.pairs.data(), sh.pairs.size()));
+    G_TRY(c.upload(&tiles, sh.tiles.data(), sh.tiles.size()));
+    G_TRY(c.upload(&parts, sh.parts.data(), sh.parts.size()));
+    G_TRY(c.upload(&trip, sh.trip.data(), sh.trip.size()));
+    pl.pairs = pairs;
+    pl.tiles = tiles;
+    pl.parts = parts;
+    pl.trip = trip;
+    G_TRY(c.alloc(&pl.psi_v, (size_t)n * sh.vldp));
+    G_TRY(c.alloc(&pl.g, (size_t)n));
+    G_TRY(c.alloc(&pl.partials, (size_t)sh.n_chunks * sh.ntiles * 256));
+    G_TRY(c.alloc(&pl.cfull, (size_t)sh.cfull_rows * sh.vld));
     return CY3Q_OK;
   }
 };
@@ -850,8 +848,8 @@ int cy3q_yukawa_triple_overlap(const double *modes, const double *omega, const d
   G_TRY(c.alloc(&d_y, (size_t)nm * nm * nm));
   StageTimer tm(c.st);
   tm.begin();
-  launch_yukawa(c.st, d_modes, n, d_w, d_om, nm, n, 1, ws.trip, ws.n_trip, d_h, ws.n_chunks,
-                ws.fac, ws.bad, ws.partials, nullptr, d_y);
+  launch_yuk_prep(c.st, d_modes, n, 1, d_w, d_om, n, 1, nm, ws.pl.vldp, ws.pl.psi_v, ws.pl.g);
+  launch_yuk_contract(c.st, ws.pl, n, d_h, nullptr, d_y);
   tm.end(GS_YUK_TRIPLES);
   G_HIP(hipGetLastError());
   G_HIP(hipMemcpyAsync(y, d_y, sizeof(double2) * nm * nm * nm, hipMemcpyDeviceToHost, c.st));
@@ -890,15 +888,15 @@ int cy3q_yukawa_couplings(const double *modes, const double *omega, const double
   YukWs ws;
   G_TRY(ws.init(c, nm, n));
   launch_sz_weights(c.st, d_w, d_om, n, d_sz, d_score);
-  launch_yukawa(c.st, d_modes, n, d_sz, d_om, nm, n, 1, ws.trip, ws.n_trip, d_h, ws.n_chunks,
-                ws.fac, ws.bad, ws.partials, nullptr, d_y);
+  launch_yuk_prep(c.st, d_modes, n, 1, d_sz, d_om, n, 1, nm, ws.pl.vldp, ws.pl.psi_v, ws.pl.g);
+  launch_yuk_contract(c.st, ws.pl, n, d_h, nullptr, d_y);
   const int64_t nbs = std::max<int64_t>(n_bootstrap, 1);
   for (int64_t b = 0; b < nbs; b++) {
     G_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * n, c.st));
     launch_bootstrap_counts(c.st, seed + (uint64_t)b * 2654435761ull, n, d_cnt);
     launch_scale_weights(c.st, d_sz, d_cnt, nullptr, n, d_wb);
-    launch_yukawa(c.st, d_modes, n, d_wb, d_om, nm, n, 0, ws.trip, ws.n_trip, d_h, ws.n_chunks,
-                  ws.fac, ws.bad, ws.partials, nullptr, d_yb);
+    launch_yuk_prep(c.st, d_modes, n, 1, d_wb, d_om, n, 0, nm, ws.pl.vldp, ws.pl.psi_v, ws.pl.g);
+    launch_yuk_contract(c.st, ws.pl, n, d_h, nullptr, d_yb);
     launch_bootstrap_acc(c.st, d_yb, d_y, n3, d_acc);
   }
   launch_bootstrap_final(c.st, d_acc, n3, (double)nbs, d_unc);
@@ -907,8 +905,9 @@ int cy3q_yukawa_couplings(const double *modes, const double *omega, const double
   if (half >= 8) {
     launch_sum_f64(c.st, d_sz, half, d_tot);
     launch_scale_weights(c.st, d_sz, nullptr, d_tot, half, d_wb);
-    launch_yukawa(c.st, d_modes, n, d_wb, d_om, nm, half, 1, ws.trip, ws.n_trip, d_h,
-                  yukawa_chunks(half, ws.n_trip), ws.fac, ws.bad, ws.partials, nullptr, d_yb);
+    launch_yuk_prep(c.st, d_modes, n, 1, d_wb, d_om, half, 1, nm, ws.pl.vldp, ws.pl.psi_v,
+                    ws.pl.g);
+    launch_yuk_contract(c.st, ws.pl, half, d_h, nullptr, d_yb);
     launch_conv_ratio(c.st, d_y, d_yb, n3, d_conv);
     G_HIP(hipMemcpyAsync(&conv, d_conv, sizeof(double), hipMemcpyDeviceToHost, c.st));
   }
@@ -940,35 +939,41 @@ int cy3q_solver_yukawa_random_modes(cy3q_solver *s, uint32_t degree, int64_t n_m
   const int64_t n = v.n;
   const int64_t m5 = padded_cols(nsec);
   const int64_t chunk = std::min<int64_t>(n, 65536);
+  // psi = S C as one real GEMM per point chunk: the section rows [Re s, Im s] times the
+  // real-ified coefficients give the rows [Re psi_a, Im psi_a] of the modes.
+  const int64_t ncols = (2 * nm + 31) / 32 * 32;
   StageTimer tm(c.st);
   uint32_t *d_mono;
-  double *sv5, *cw;
-  double2 *coef, *modes, *d_y;
+  double *sv5, *cw, *bmat, *mrows;
+  double2 *coef, *d_y;
   G_TRY(c.upload(&d_mono, sm.data(), sm.size()));
   G_TRY(c.alloc(&sv5, (size_t)(chunk * m5)));
+  G_TRY(c.alloc(&mrows, (size_t)(chunk * ncols)));
   G_TRY(c.alloc(&cw, (size_t)n));
   G_TRY(c.alloc(&coef, (size_t)nsec * nm));
-  G_TRY(c.alloc(&modes, (size_t)nm * n));
+  G_TRY(c.alloc(&bmat, (size_t)(m5 * ncols)));
+  YukWs ws;
+  G_TRY(ws.init(c, nm, n));
+  G_TRY(c.alloc(&d_y, (size_t)ws.pl.n_trip));
   tm.begin();
   launch_random_coeffs(c.st, coeff_seed, (int64_t)nsec * nm, coef);
+  launch_modes_bmat(c.st, coef, nsec, nm, m5, ncols, bmat);
+  launch_measure_w_omega2(c.st, v.w, v.log_om, n, cw);
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
     launch_sections(c.st, v.pts + 10 * p0, cn, d_mono, nsec, (int)degree, m5, sv5, nullptr);
-    launch_modes_from_sections(c.st, sv5, cn, m5, nsec, coef, nm, modes + p0, n);
+    launch_gemm_nn_f64(c.st, sv5, cn, m5, bmat, ncols, ncols, mrows, ncols);
+    launch_yuk_prep(c.st, reinterpret_cast<const double2 *>(mrows), 1, ncols / 2, cw + p0,
+                    nullptr, cn, 0, nm, ws.pl.vldp, ws.pl.psi_v + p0 * ws.pl.vldp, ws.pl.g + p0);
   }
-  launch_measure_w_omega2(c.st, v.w, v.log_om, n, cw);
   tm.end(GS_YUK_MODES);
-  YukWs ws;
-  G_TRY(ws.init(c, nm, n));
-  G_TRY(c.alloc(&d_y, (size_t)ws.n_trip));
   tm.begin();
-  launch_yukawa(c.st, modes, n, cw, nullptr, nm, n, 0, ws.trip, ws.n_trip, nullptr, ws.n_chunks,
-                ws.fac, ws.bad, ws.partials, d_y, nullptr);
+  launch_yuk_contract(c.st, ws.pl, n, nullptr, d_y, nullptr);
   tm.end(GS_YUK_TRIPLES);
   G_HIP(hipGetLastError());
-  if (n_triples) *n_triples = ws.n_trip;
+  if (n_triples) *n_triples = ws.pl.n_trip;
   if (y_triples)
-    G_HIP(hipMemcpyAsync(y_triples, d_y, sizeof(double2) * ws.n_trip, hipMemcpyDeviceToHost, c.st));
+    G_HIP(hipMemcpyAsync(y_triples, d_y, sizeof(double2) * ws.pl.n_trip, hipMemcpyDeviceToHost, c.st));
   if (coeffs)
     G_HIP(hipMemcpyAsync(coeffs, coef, sizeof(double2) * nsec * nm, hipMemcpyDeviceToHost, c.st));
   G_HIP(hipStreamSynchronize(c.st));

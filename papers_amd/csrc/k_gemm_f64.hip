@@ -31,10 +31,10 @@ static constexpr int TG_KT = 16;
 __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ x, int64_t rows,
                                                      int64_t m_cols, const double *__restrict__ bm,
                                                      double *__restrict__ y, int mode, int full,
-                                                     int64_t n_rowtiles) {
+                                                     int64_t n_rowtiles, int n_cb, int64_t ldb,
+                                                     int64_t ldy) {
   __shared__ double sa[2][TG_ROWS][TG_KT + 1];
   __shared__ double sb[2][TG_KT][TG_COLS];
-  const int n_cb = (int)(m_cols / TG_COLS);
   // XCD-aware order: consecutive workgroups are dealt round-robin over the 8 XCDs, so the
   // column blocks of one row tile are grouped on one XCD and share its L2.
   const int64_t g = blockIdx.x;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
     }
     {
       const int kr = tid >> 4, c2 = (tid & 15) * 2;
-      rb = *reinterpret_cast<const double2 *>(bm + (k0 + kr) * m_cols + c0 + c2);
+      rb = *reinterpret_cast<const double2 *>(bm + (k0 + kr) * ldb + c0 + c2);
     }
   };
   auto put = [&](int buf) {
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) k_trigemm_f64(const double *__restrict__ 
         for (int e = 0; e < 4; e++) {
           int64_t gr = r0 + wave * 32 + i * 16 + 4 * e + (lane >> 4);
           int gc = c0 + j * 16 + (lane & 15);
-          if (gr < rows) y[gr * m_cols + gc] = acc[i][j][e];
+          if (gr < rows) y[gr * ldy + gc] = acc[i][j][e];
         }
   } else {
     // Row sum of squares over this block's 32 columns -> y[cb * rows + row].
@@ -134,7 +134,19 @@ void launch_trigemm_f64(hipStream_t st, const double *x, int64_t rows, int64_t m
   int64_t n_cb = m_cols / TG_COLS;
   int64_t grid = n_rt8 * n_cb;
   hipLaunchKernelGGL(k_trigemm_f64, dim3((unsigned)grid), dim3(256), 0, st, x, rows, m_cols, bmat,
-                     y, mode, full, n_rt);
+                     y, mode, full, n_rt, (int)n_cb, m_cols, m_cols);
+}
+
+// Y = X B for a dense B (k_cols x n_cols, leading dimension ldb; n_cols a multiple of 32 and
+// k_cols of 16): y [rows][n_cols] with leading dimension ldy.
+void launch_gemm_nn_f64(hipStream_t st, const double *x, int64_t rows, int64_t k_cols,
+                        const double *bmat, int64_t ldb, int64_t n_cols, double *y, int64_t ldy) {
+  if (rows <= 0) return;
+  int64_t n_rt = (rows + TG_ROWS - 1) / TG_ROWS;
+  int64_t n_rt8 = (n_rt + 7) / 8 * 8;
+  int64_t n_cb = n_cols / TG_COLS;
+  hipLaunchKernelGGL(k_trigemm_f64, dim3((unsigned)(n_rt8 * n_cb)), dim3(256), 0, st, x, rows,
+                     k_cols, bmat, y, 0, 1, n_rt, (int)n_cb, ldb, ldy);
 }
 
 // ----------------------------------------------------------------------------------------
