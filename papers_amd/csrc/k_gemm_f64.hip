@@ -750,15 +750,34 @@ __global__ void __launch_bounds__(512) k_syrk13_f64(const double *__restrict__ x
   }
 }
 
-// Chunk sums of the 91 tile partials, in chunk order, into the full 416 x 416 matrix (a diagonal
-// tile keeps its lower elements and mirrors them, so every element has one writer).
-__global__ void k_syrk13_reduce(const double *__restrict__ partials, int64_t n_chunks,
-                                double *__restrict__ xx) {
-  const int64_t per_chunk = 91 * 1024;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= per_chunk) return;
+// Chunk sums of the 91 tile partials into the full 416 x 416 matrix (a diagonal tile keeps its
+// lower elements and mirrors them, so every element has one writer). A workgroup takes 64
+// consecutive partial entries; its 16 groups of 64 threads sum the chunks grp, grp + 16, ...
+// and the group sums are added in group order, so the result is the same on every run.
+static constexpr int RED_GROUPS = 16;
+
+__device__ inline double grouped_chunk_sum(const double *__restrict__ partials, int64_t per_chunk,
+                                           int64_t n_chunks, int64_t idx, double (*red)[64]) {
+  const int tid = threadIdx.x, grp = tid >> 6;
   double s = 0.0;
-  for (int64_t c = 0; c < n_chunks; c++) s += partials[c * per_chunk + idx];
+  if (idx < per_chunk)
+    for (int64_t c = grp; c < n_chunks; c += RED_GROUPS) s += partials[c * per_chunk + idx];
+  red[grp][tid & 63] = s;
+  __syncthreads();
+  double r = 0.0;
+  if (tid < 64)
+    for (int q = 0; q < RED_GROUPS; q++) r += red[q][tid];
+  return r;
+}
+
+__global__ void __launch_bounds__(1024) k_syrk13_reduce(const double *__restrict__ partials,
+                                                        int64_t n_chunks,
+                                                        double *__restrict__ xx) {
+  __shared__ double red[RED_GROUPS][64];
+  const int64_t per_chunk = 91 * 1024;
+  const int64_t idx = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const double s = grouped_chunk_sum(partials, per_chunk, n_chunks, idx, red);
+  if (threadIdx.x >= 64 || idx >= per_chunk) return;
   const int t = (int)(idx / 1024), el = (int)(idx % 1024);
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= t) I++;
@@ -793,21 +812,26 @@ void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, cons
   }
 }
 
-// Sum of the chunk partials, in chunk order, scattered into the full M x M matrix (lower tiles
-// and their mirror). One thread per (part, wave, tile slot, element).
+// Sum of the chunk partials (grouped as in k_syrk13_reduce), scattered into the full M x M
+// matrix (lower tiles and their mirror). One result per (part, wave, tile slot, element).
 // Element (e, lane) of a 16x16 f64 tile is (row 4e + lane/16, col lane%16), the MFMA output layout.
-__global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t *__restrict__ tiles,
-                              int n_parts, int waves, int max_tiles, int64_t n_chunks, int tile,
-                              int64_t m_cols, double *__restrict__ xx) {
+__global__ void __launch_bounds__(1024) k_syrk_reduce(const double *__restrict__ partials,
+                                                      const int16_t *__restrict__ tiles,
+                                                      int n_parts, int waves, int max_tiles,
+                                                      int64_t n_chunks, int tile, int64_t m_cols,
+                                                      double *__restrict__ xx) {
+  __shared__ double red[RED_GROUPS][64];
   const int64_t per_chunk = (int64_t)n_parts * waves * max_tiles * tile * tile;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= per_chunk) return;
-  int64_t slot = idx / (tile * tile);
-  int el = (int)(idx % (tile * tile));
-  int rg = tiles[2 * slot], cg = tiles[2 * slot + 1];
-  if (rg < 0) return;
-  double s = 0.0;
-  for (int64_t c = 0; c < n_chunks; c++) s += partials[c * per_chunk + idx];
+  const int64_t idx = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  // Slots padded with -1 are skipped by every thread of the workgroup before the reduction
+  // (a slot spans whole workgroups: tile * tile is a multiple of 64).
+  const int64_t slot0 = ((int64_t)blockIdx.x * 64) / (tile * tile);
+  if ((int64_t)blockIdx.x * 64 >= per_chunk || tiles[2 * slot0] < 0) return;
+  const double s = grouped_chunk_sum(partials, per_chunk, n_chunks, idx, red);
+  if (threadIdx.x >= 64 || idx >= per_chunk) return;
+  const int64_t slot = idx / (tile * tile);
+  const int el = (int)(idx % (tile * tile));
+  const int rg = tiles[2 * slot], cg = tiles[2 * slot + 1];
   int r, col;
   if (tile == 16) {
     int e = el / 64, ln = el % 64;
@@ -829,12 +853,12 @@ __global__ void k_syrk_reduce(const double *__restrict__ partials, const int16_t
 void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *partials,
                         double *xx_full) {
   if (plan.prog13) {
-    hipLaunchKernelGGL(k_syrk13_reduce, dim3((unsigned)((91 * 1024 + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_syrk13_reduce, dim3((unsigned)((91 * 1024 + 63) / 64)), dim3(1024), 0,
                        st, partials, plan.n_chunks, xx_full);
     return;
   }
   int64_t per_chunk = (int64_t)plan.n_parts * plan.waves * plan.max_tiles * plan.tile * plan.tile;
-  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)((per_chunk + 255) / 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)((per_chunk + 63) / 64)), dim3(1024), 0, st,
                      partials, plan.d_tiles, plan.n_parts, plan.waves, plan.max_tiles,
                      plan.n_chunks, plan.tile, plan.m_cols, xx_full);
 }
