@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace cy3q {
@@ -84,8 +85,11 @@ void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t 
                           half8_t *bfrag, double *b_scale);
 int dpass_split16_waves();
 int dpass_split16_max_cg();
+std::vector<std::pair<int, int>> dpass_split16_ranges(int64_t m_cols);
+bool dpass_split16_supported(int64_t m_cols);
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
-                          const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum);
+                          const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum,
+                          double *dsum_parts);
 void launch_coeffs_f64_from_split(hipStream_t st, const double *w, const double *dsum,
                                   const float *rs2, const double *bscale, int64_t n, double *c);
 // Per-point eta of the 8-coordinate sigma contract (k_sigma8.hip): y [n][9][m], frames [n][48].
@@ -98,10 +102,11 @@ void launch_coeffs_split16(hipStream_t st, const double *w, const double *dsum, 
                            float *zscale, double *tscale);
 // Split-fp16 T pass (k_tpass.hip).
 int tpass_max_cols();
+int64_t tpass_segment_cols(int64_t m_cols);
 size_t tpass_partials(int64_t m_cols, int64_t n_chunks);
 size_t tpass_seg_count(int64_t m_cols, int64_t n_chunks);
-void launch_tpass(hipStream_t st, const float *xt, const float *u, int64_t n, int64_t m_cols,
-                  int64_t chunk, int64_t n_chunks, float *partials);
+void launch_tpass(hipStream_t st, const float *xt, const float *u, int64_t n, int64_t n_pad,
+                  int64_t m_cols, int64_t chunk, int64_t n_chunks, float *partials);
 void launch_tpass_reduce(hipStream_t st, const float *partials, int64_t m_cols, int64_t n_chunks,
                          double *seg, double *xx);
 

@@ -14,6 +14,10 @@
 // D pass  (k_dpass_split16): v_mfma_f32_16x16x32_f16, 128 points x all M columns per workgroup,
 //                            so D_p = sum_j Y_pj^2 completes inside the workgroup.
 // T pass: k_tpass.hip.
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "cy3q_internal.h"
 
 namespace cy3q {
@@ -112,9 +116,11 @@ void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t 
 //  xd: D-pass A fragments of v_mfma_f32_16x16x32_f16. For point tile t (16 points), k-step ks
 //      (32 columns), plane h (hi, lo), lane l: half8 of X[16t + l%16][32ks + 8(l/16) + j].
 //      Index ((t * n_ks + ks) * 2 + h) * 64 + l.
-//  xt: T-pass stage records (FP32). For stage s (16 points), quad q (4 points), column c: the
-//      4 floats X[16s + 4q + j][c], j = 0..3. Index ((s * 4 + q) * m + c) * 4 + j, so that one
-//      16-byte load per lane over consecutive columns reads a contiguous run.
+//  xt: T-pass stage records (FP32), per column segment of sw columns (sw = m up to 416, else
+//      192; the last segment holds the rest, ms columns). For stage s (16 points), quad q
+//      (4 points), column c of segment a: the 4 floats X[16s + 4q + j][a sw + c], j = 0..3, at
+//      n_pad a sw + ((s * 4 + q) * ms + c) * 4 + j, so that one 16-byte load per lane over
+//      consecutive columns reads a contiguous run.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__ sv, int64_t n,
                                                       int64_t m, int *__restrict__ ex,
@@ -134,7 +140,7 @@ __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__
 }
 
 __global__ void k_split_planes(const double *__restrict__ sv, const int *__restrict__ ex,
-                               int64_t n, int64_t n_pad, int64_t m,
+                               int64_t n, int64_t n_pad, int64_t m, int64_t sw,
                                _Float16 *__restrict__ xd, float *__restrict__ xt) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_pad * m) return;
@@ -148,7 +154,8 @@ __global__ void k_split_planes(const double *__restrict__ sv, const int *__restr
   const int lane = (int)(p % 16) + 16 * (int)((c % 32) / 8), j = (int)(c % 8);
   xd[((((t * n_ks + ks) * 2 + 0) * 64 + lane) * 8) + j] = h;
   xd[((((t * n_ks + ks) * 2 + 1) * 64 + lane) * 8) + j] = l;
-  xt[(((p / 16) * 4 + (p % 16) / 4) * m + c) * 4 + p % 4] = x;
+  const int64_t a = c / sw, cl = c - a * sw, ms = m - a * sw < sw ? m - a * sw : sw;
+  xt[n_pad * a * sw + (((p / 16) * 4 + (p % 16) / 4) * ms + cl) * 4 + p % 4] = x;
 }
 
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
@@ -157,7 +164,7 @@ void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_
                      rs2);
   int64_t tot = n_pad * m;
   hipLaunchKernelGGL(k_split_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, sv, ex,
-                     n, n_pad, m, xd, xt);
+                     n, n_pad, m, tpass_segment_cols(m), xd, xt);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -215,30 +222,36 @@ __device__ __forceinline__ void dp_mover_wait(int64_t ahead) {
 }
 
 // Largest number of active column groups (cg >= 2 ks) held by any compute wave at k-step ks,
-// for the round-robin assignment cg -> wave cg % DP_CW.
-__host__ __device__ constexpr int dp_slots(int n_cg, int ks) {
+// for the groups cg0 .. cg0 + n_cg - 1 dealt round-robin, cg -> wave (cg - cg0) % DP_CW.
+__host__ __device__ constexpr int dp_slots(int cg0, int n_cg, int ks) {
   int best = 0;
   for (int w = 0; w < DP_CW; w++) {
     int cnt = 0;
-    for (int cg = w; cg < n_cg; cg += DP_CW) cnt += cg >= 2 * ks ? 1 : 0;
+    for (int cg = cg0 + w; cg < cg0 + n_cg; cg += DP_CW) cnt += cg >= 2 * ks ? 1 : 0;
     best = cnt > best ? cnt : best;
   }
   return best;
 }
 
-// NKS = M / 32 k-steps, a compile-time count: the k-step loops unroll completely, so the
+// Column range: the NCG column groups CG0 .. CG0 + NCG - 1 (at most DP_CW * DP_MAXCG) of an
+// M = 32 NKS_ALL column contraction, which need the k-steps 0 .. KS_END - 1 (group cg is active
+// in k-steps ks <= cg / 2). With one range (CG0 = 0, NCG = M / 16, KS_END = NKS_ALL) dsum is
+// D_p itself; with several, each range writes its own partial sum and the caller adds them.
+// The k-step count is a compile-time constant: the k-step loops unroll completely, so the
 // compiler's memory-counter waits are exact (no loop-carried merging of pending loads).
-template <int NKS>
+template <int NKS_ALL, int KS_END, int CG0, int NCG>
 __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *__restrict__ xd,
                                                                  int64_t n, int64_t nblk,
                                                                  const half8_t *__restrict__ bfrag,
                                                                  const int16_t *__restrict__ cglist,
                                                                  double *__restrict__ dsum) {
+  static_assert(NCG <= DP_CW * DP_MAXCG && CG0 + NCG <= 2 * NKS_ALL, "column range");
+  static_assert(KS_END == (CG0 + NCG - 1) / 2 + 1, "k-steps of the range");
   __shared__ half8_t sa[DP_RING][DP_SLOT];  // slot: [(tile * 2 + plane) * 64 + lane]
   __shared__ double red[DP_CW][DP_PTS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  constexpr int n_ks = NKS, n_cg = 2 * NKS;
+  constexpr int n_ks = KS_END, ks_all = NKS_ALL, n_cg_all = 2 * NKS_ALL;
   const int64_t blk0 = blockIdx.x, bstride = gridDim.x;
   const int64_t nmine = blk0 < nblk ? (nblk - 1 - blk0) / bstride + 1 : 0;
 
@@ -251,12 +264,12 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
       const int64_t blk = blk0 + (g / n_ks) * bstride;
       const int ks = (int)(g % n_ks);
       half8_t *slot = sa[(int)(g & (DP_RING - 1))];
-      const half8_t *src = xd + (blk * DP_TILES * n_ks * 2) * 64 + lane;
+      const half8_t *src = xd + (blk * DP_TILES * ks_all * 2) * 64 + lane;
 #pragma unroll
       for (int b = 0; b < DP_LOADS; b++) {
         const int rg = b >> 1, pl = b & 1;
         __builtin_amdgcn_global_load_lds(
-            (const void *)(src + (((int64_t)rg * n_ks + ks) * 2 + pl) * 64),
+            (const void *)(src + (((int64_t)rg * ks_all + ks) * 2 + pl) * 64),
             (__attribute__((address_space(3))) void *)(slot + b * 64), 16, 0, 0);
       }
     };
@@ -273,7 +286,7 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
 
   // The wave's column groups: round-robin (cg = wave + DP_CW j), ascending and right-aligned in
   // the DP_MAXCG slots (unused slots first, -1). At k-step ks the groups cg >= 2 ks are active,
-  // always the wave's last slots. Every wave runs the same dp_slots(n_cg, ks) slots (the largest
+  // always the wave's last slots. Every wave runs the same dp_slots(CG0, NCG, ks) slots (the largest
   // active count of any wave, a compile-time number), so the k-step body has no branches; a slot
   // that is idle for this wave multiplies by the all-zero fragment and leaves its sums unchanged.
   // The list is also passed in (cglist) by the host, which builds the same table.
@@ -286,16 +299,17 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
   // wait for the set in use leaves the other set's loads in flight). The last k-step of a block
   // prefetches k-step 0 of the next one into the free set.
   half8_t b0h[DP_MAXCG], b0l[DP_MAXCG], b1h[DP_MAXCG], b1l[DP_MAXCG];
-  constexpr int64_t zero_frag = (int64_t)n_ks * n_cg;
+  constexpr int64_t zero_frag = (int64_t)ks_all * n_cg_all;
   // The B fragments are the same for every block. An opaque copy of their base, renewed per
   // block, keeps the compiler from hoisting all n_ks * DP_MAXCG fragment addresses out of the
-  // block loop (they would not fit in registers alongside the accumulators).
+  // block loop (they would not fit in registers alongside the accumulators). The lambdas are
+  // forced inline: their register arrays, passed by pointer, stay in registers only when inlined.
   const half8_t *bfp = bfrag;
-  auto load_b = [&](int ks, half8_t *hh, half8_t *ll) {
+  auto load_b = [&](int ks, half8_t *hh, half8_t *ll) __attribute__((always_inline)) {
 #pragma unroll
     for (int c = 0; c < DP_MAXCG; c++) {
-      if (c < DP_MAXCG - dp_slots(n_cg, ks)) continue;
-      const int64_t frag = cgs[c] >= 2 * ks ? (int64_t)ks * n_cg + cgs[c] : zero_frag;
+      if (c < DP_MAXCG - dp_slots(CG0, NCG, ks)) continue;
+      const int64_t frag = cgs[c] >= 2 * ks ? (int64_t)ks * n_cg_all + cgs[c] : zero_frag;
       hh[c] = bfp[(frag * 2 + 0) * 64 + lane];
       ll[c] = bfp[(frag * 2 + 1) * 64 + lane];
     }
@@ -303,8 +317,9 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
   // MFMAs with swapped operands: B as the row operand and X as the column operand give the
   // transpose Y^T, so that element e of lane l is column 16 cg + 4 (l / 16) + e of point
   // 16 r + l % 16 (the fragment layouts of both operands serve either role).
-  auto step = [&](int slot, int ks, const half8_t *bh, const half8_t *bl) {
-    const int c0 = DP_MAXCG - dp_slots(n_cg, ks);
+  auto step = [&](int slot, int ks, const half8_t *bh,
+                  const half8_t *bl) __attribute__((always_inline)) {
+    const int c0 = DP_MAXCG - dp_slots(CG0, NCG, ks);
 #pragma unroll
     for (int r = 0; r < DP_TILES; r += 2) {
       if (c0 >= DP_MAXCG) break;
@@ -326,16 +341,22 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
     }
   };
   // One block: the k-steps (even ones on set x, odd ones on set y), then the row sums.
-  auto block = [&](int64_t i, half8_t *xh, half8_t *xl, half8_t *yh, half8_t *yl) {
+  auto block = [&](int64_t i, half8_t *xh, half8_t *xl, half8_t *yh,
+                   half8_t *yl) __attribute__((always_inline)) {
 #pragma unroll
     for (int r = 0; r < DP_TILES; r++)
 #pragma unroll
       for (int c = 0; c < DP_MAXCG; c++) acc[r][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
     const int slot0 = (int)((i * n_ks) & (DP_RING - 1));
-    asm volatile("" : "+s"(bfp));
+    if constexpr (KS_END <= 20) asm volatile("" : "+s"(bfp));
     const bool more = i + 1 < nmine;
 #pragma unroll
     for (int ks = 0; ks < n_ks; ks += 2) {
+      // Past 20 k-steps (the top column ranges of k = 8) the base is renewed per k-step pair
+      // instead, in vector registers: the compiler would otherwise start the B loads of many
+      // later k-steps at once, more than the registers left beside the accumulators (and a
+      // scalar-register base fails to compile at this size).
+      if constexpr (KS_END > 20) asm volatile("" : "+v"(bfp));
       __syncthreads();
       if (ks + 1 < n_ks)
         load_b(ks + 1, yh, yl);
@@ -400,7 +421,7 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16(const half8_t *
   for (int64_t i = 0; i < nmine;) {
     block(i, b0h, b0l, b1h, b1l);
     if (++i >= nmine) break;
-    if constexpr (NKS % 2 == 0) continue;
+    if constexpr (KS_END % 2 == 0) continue;
     block(i, b1h, b1l, b0h, b0l);
     ++i;
   }
@@ -652,26 +673,85 @@ __global__ void __launch_bounds__(64 * DP_WAVES) k_dpass_split16_b13(const half8
   }
 }
 
+// Column ranges of the D pass: all M / 16 groups in one range up to M = 448; beyond, ranges of
+// DP_CW * DP_MAXCG groups taken from the last group down (the top range needs every k-step, the
+// lower ones fewer), so that the k-steps read over all ranges stay few.
+std::vector<std::pair<int, int>> dpass_split16_ranges(int64_t m_cols) {
+  const int n_cg = (int)(m_cols / 16), cap = DP_CW * DP_MAXCG;
+  std::vector<std::pair<int, int>> r;
+  if (n_cg <= cap) {
+    r.push_back({0, n_cg});
+    return r;
+  }
+  for (int hi = n_cg; hi > 0; hi -= cap) r.push_back({std::max(0, hi - cap), std::min(cap, hi)});
+  return r;
+}
+
+// dsum[p] = sum over the ranges, in range order, of the range partials (parts[r * n + p]).
+__global__ void k_dsum_ranges(const double *__restrict__ parts, int n_ranges, int64_t n,
+                              double *__restrict__ dsum) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double s = 0.0;
+  for (int r = 0; r < n_ranges; r++) s += parts[r * n + p];
+  dsum[p] = s;
+}
+
 // n_cu: the device's compute units; the grid is one workgroup per CU (at most one block each).
+// cglist: DP_CW x DP_MAXCG group slots per range; dsum_parts: n doubles per range (several
+// ranges only).
 void launch_dpass_split16(hipStream_t st, const half8_t *xd, int64_t n, int64_t m_cols,
-                          const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum) {
+                          const half8_t *bfrag, const int16_t *cglist, int n_cu, double *dsum,
+                          double *dsum_parts) {
   if (n <= 0) return;
   const int64_t nblk = (n + DP_PTS - 1) / DP_PTS;
   const int64_t wgs = std::min<int64_t>(nblk, std::max(n_cu, 1));
   const dim3 grid((unsigned)wgs), block(64 * DP_WAVES);
-  switch (m_cols / 32) {
-#define DP_CASE(k)                                                                             \
-  case k:                                                                                      \
-    hipLaunchKernelGGL(k_dpass_split16<k>, grid, block, 0, st, xd, n, nblk, bfrag, cglist, dsum); \
+  const int nks = (int)(m_cols / 32);
+  if (m_cols <= 32 * 14) {
+    switch (nks) {
+#define DP_CASE(k)                                                                      \
+  case k:                                                                               \
+    hipLaunchKernelGGL((k_dpass_split16<k, k, 0, 2 * k>), grid, block, 0, st, xd, n, nblk, \
+                       bfrag, cglist, dsum);                                            \
     break;
-    DP_CASE(1) DP_CASE(2) DP_CASE(3) DP_CASE(4) DP_CASE(5) DP_CASE(6) DP_CASE(7)
-    DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(14)
-    case 13:
-      hipLaunchKernelGGL(k_dpass_split16_b13, grid, block, 0, st, xd, n, nblk, bfrag, dsum);
-      break;
+      DP_CASE(1) DP_CASE(2) DP_CASE(3) DP_CASE(4) DP_CASE(5) DP_CASE(6) DP_CASE(7)
+      DP_CASE(8) DP_CASE(9) DP_CASE(10) DP_CASE(11) DP_CASE(12) DP_CASE(14)
+      case 13:
+        hipLaunchKernelGGL(k_dpass_split16_b13, grid, block, 0, st, xd, n, nblk, bfrag, dsum);
+        break;
 #undef DP_CASE
-    default: break;  // rejected by the host (M <= 448)
+      default: break;
+    }
+    return;
   }
+  // Several ranges: the quotient bases of k = 7 and 8 (2N = 630, 920) and the raw k = 8 basis
+  // (2N = 990).
+  const std::vector<std::pair<int, int>> rr = dpass_split16_ranges(m_cols);
+  for (size_t r = 0; r < rr.size(); r++) {
+    const int16_t *cl = cglist + r * DP_CW * DP_MAXCG;
+    double *out = dsum_parts + r * n;
+    const int key = nks * 10000 + rr[r].first * 100 + rr[r].second;
+    switch (key) {
+#define DP_RANGE(k, c0, nc)                                                                   \
+  case k * 10000 + c0 * 100 + nc:                                                             \
+    hipLaunchKernelGGL((k_dpass_split16<k, (c0 + nc - 1) / 2 + 1, c0, nc>), grid, block, 0, st, \
+                       xd, n, nblk, bfrag, cl, out);                                          \
+    break;
+      DP_RANGE(20, 12, 28) DP_RANGE(20, 0, 12)
+      DP_RANGE(29, 30, 28) DP_RANGE(29, 2, 28) DP_RANGE(29, 0, 2)
+      DP_RANGE(31, 34, 28) DP_RANGE(31, 6, 28) DP_RANGE(31, 0, 6)
+#undef DP_RANGE
+      default: break;  // rejected by the host (dpass_split16_supported)
+    }
+  }
+  hipLaunchKernelGGL(k_dsum_ranges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     dsum_parts, (int)rr.size(), n, dsum);
+}
+
+// The column counts the D pass is built for: every M up to 448, and 640, 928, 992.
+bool dpass_split16_supported(int64_t m_cols) {
+  return m_cols <= 448 || m_cols == 640 || m_cols == 928 || m_cols == 992;
 }
 
 // ---------------------------------------------------------------------------------------------

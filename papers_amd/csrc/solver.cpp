@@ -143,7 +143,7 @@ struct cy3q_solver {
   int *row_ex = nullptr;
   _Float16 *xd = nullptr;
   float *xt = nullptr;
-  double *dsum = nullptr, *bscale = nullptr, *tscale = nullptr;
+  double *dsum = nullptr, *dsum_parts = nullptr, *bscale = nullptr, *tscale = nullptr;
   half8_t *bfrag = nullptr;
   int16_t *cglist = nullptr;
 
@@ -551,8 +551,10 @@ int ensure_split16(cy3q_solver *s) {
   const int64_t n = s->n, m = s->m;
   const int n_cg = (int)(m / 16);
   const int dw = dpass_split16_waves(), dmc = dpass_split16_max_cg();
-  if (n_cg > dw * dmc || m > tpass_max_cols())
-    return fail(CY3Q_ERR_ARG, "split16 precision supports 2N <= 416 (N <= 208); use FP64");
+  if (!dpass_split16_supported(m) || m > tpass_max_cols())
+    return fail(CY3Q_ERR_ARG,
+                "split16 precision supports 2N <= 448, the quotient bases of k = 7, 8 (2N = 630, "
+                "920) and the raw k = 8 basis (2N = 990); use FP64");
   if (!s->xd) {
     const int64_t n_pad = (n + 127) / 128 * 128;
     ST_TRY(s->alloc(&s->xd, n_pad * m * 2));
@@ -573,11 +575,17 @@ int ensure_split16(cy3q_solver *s) {
     // contiguous range, which round-robin spreads evenly over the waves (each k-step ends at a
     // barrier, so the per-step balance is what counts). Each wave's list is ascending and
     // right-aligned, unused slots first (-1), so its active groups are always its last slots.
-    std::vector<int16_t> cl((size_t)dw * dmc, (int16_t)-1);
-    for (int w = 0; w < dw; w++) {
-      const int cnt = (n_cg - w + dw - 1) / dw;
-      for (int j = 0; j < cnt; j++) cl[(size_t)w * dmc + (dmc - cnt) + j] = (int16_t)(w + j * dw);
-    }
+    // Past 448 columns the groups are split into column ranges (one D-pass launch each, whose
+    // partial sums are then added): one such table per range, over the range's groups.
+    const std::vector<std::pair<int, int>> ranges = dpass_split16_ranges(m);
+    std::vector<int16_t> cl(ranges.size() * dw * dmc, (int16_t)-1);
+    for (size_t r = 0; r < ranges.size(); r++)
+      for (int w = 0; w < dw; w++) {
+        const int cnt = (ranges[r].second - w + dw - 1) / dw;
+        for (int j = 0; j < cnt; j++)
+          cl[(r * dw + w) * dmc + (dmc - cnt) + j] = (int16_t)(ranges[r].first + w + j * dw);
+      }
+    if (ranges.size() > 1) ST_TRY(s->alloc(&s->dsum_parts, (int64_t)ranges.size() * n));
     ST_TRY(s->alloc(&s->cglist, cl.size()));
     HIP_TRY(hipMemcpyAsync(s->cglist, cl.data(), cl.size() * sizeof(int16_t),
                            hipMemcpyHostToDevice, s->st));
@@ -610,14 +618,15 @@ int t_step_split16(cy3q_solver *s) {
   }
   s->stage_begin(STG_DPASS);
   launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->n_cu,
-                       s->dsum);
+                       s->dsum, s->dsum_parts);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
   launch_coeffs_split16(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->u, s->block_max,
                         s->zscale, s->tscale);
   s->stage_end(STG_COEFF);
   s->stage_begin(STG_TPASS);
-  launch_tpass(s->st, s->xt, s->u, n, m, s->tp_chunk, s->tp_chunks, s->partials32);
+  launch_tpass(s->st, s->xt, s->u, n, (n + 127) / 128 * 128, m, s->tp_chunk, s->tp_chunks,
+               s->partials32);
   s->stage_end(STG_TPASS);
   s->stage_begin(STG_TREDUCE);
   launch_tpass_reduce(s->st, s->partials32, m, s->tp_chunks, s->tseg, s->xx);
@@ -637,7 +646,7 @@ int t_step_refine_t64(cy3q_solver *s) {
   }
   s->stage_begin(STG_DPASS);
   launch_dpass_split16(s->st, (const half8_t *)s->xd, n, m, s->bfrag, s->cglist, s->n_cu,
-                       s->dsum);
+                       s->dsum, s->dsum_parts);
   s->stage_end(STG_DPASS);
   s->stage_begin(STG_COEFF);
   launch_coeffs_f64_from_split(s->st, s->w, s->dsum, s->rs2, s->bscale, n, s->coef);

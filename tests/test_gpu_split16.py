@@ -156,3 +156,45 @@ def test_refinement_kinds(cy3q):
     with pytest.raises(cy3q.Cy3qError):
         b.set_refine_kind(7)
 
+
+
+@pytest.mark.parametrize("k,quotient,n", [(7, True, 40000), (8, True, 40000), (8, True, 3001)])
+def test_split16_past_416_columns(cy3q, k, quotient, n):
+    # 2N = 630 (M = 640) and 920 (M = 928): the D pass in column ranges whose partial sums are
+    # added, the T pass in parts (the lower triangle of each 192-column segment and the
+    # rectangle of each segment pair); 3001 points leave a ragged last block and chunk. A tile
+    # or range dealt twice or not at all would move T by whole column blocks, far above the
+    # split rounding (about 1e-6 relative at 40k points). (The raw k = 8 basis, M = 992, has
+    # linearly dependent sections on the quintic, so its FS-Gram matrix is singular.)
+    tol = 3e-6 * max(1.0, (40000.0 / n) ** 0.5)
+    print(f"k={k} quotient={quotient} split16 vs fp64, n={n}, tol={tol:.2e}", flush=True)
+    a = cy3q.QuinticSolver.new_with_sampler(k, n, 5, quotient=quotient)
+    b = cy3q.QuinticSolver.new_with_sampler(k, n, 5, quotient=quotient, precision=cy3q.PREC_SPLIT16)
+    for s in (a, b):
+        s.orthonormalise_basis_fs_gram()
+    for it in range(2):
+        ra = a.donaldson_step()
+        rb = b.donaldson_step()
+        et, eh = _rel(b.last_t(), a.last_t()), _rel(b.h_block, a.h_block)
+        print(f"  step {it}: T rel {et:.2e}, h rel {eh:.2e}, residual {ra:.6e} / {rb:.6e}",
+              flush=True)
+        assert et < tol, it
+        assert eh < tol, it
+        assert abs(rb - ra) <= 1e-5 * ra + 1e-12
+
+
+def test_split16_k8_solve_with_refinement_sigma(cy3q):
+    # The config-5 metric (Dwork psi = 0.5, k = 8 quotient) balanced in split precision with the
+    # FP64 refinement sweep, against the FP64 solve: sigma within the 1e-6 target.
+    n = 60000
+    a = cy3q.QuinticSolver.new_with_psi_sz(8, n, 42, 0.5, quotient=True)
+    b = cy3q.QuinticSolver.new_with_psi_sz(8, n, 42, 0.5, quotient=True,
+                                           precision=cy3q.PREC_SPLIT16)
+    for s in (a, b):
+        s.orthonormalise_basis_fs_gram()
+        s.donaldson_solve(6, 0.0)
+    sa, sb = a.sigma(), b.sigma()
+    print(f"k=8 psi=0.5: sigma fp64 {sa:.12e}, split+refine {sb:.12e}, rel {abs(sb / sa - 1):.2e}",
+          flush=True)
+    assert abs(sb / sa - 1) < 1e-6
+    assert _rel(b.h_block, a.h_block) < 1e-5
