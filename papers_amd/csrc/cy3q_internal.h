@@ -296,6 +296,11 @@ struct SyrkPlan {
   // Layout: [n_parts][waves][max_tiles][2] int16.
   int16_t *d_tiles = nullptr;
   // Row-group order of the per-wave lists lets a wave reuse the row fragment.
+  // Segment plans (M > 448): per part, (first column group, groups) of the two staged column
+  // segments, [n_parts][4] int32; seg_cols = the most columns a part stages. Otherwise null / 0
+  // (every part stages all M columns).
+  int32_t *d_pcols = nullptr;
+  int seg_cols = 0;
 };
 
 // ---------------------------------------------------------------------------------------

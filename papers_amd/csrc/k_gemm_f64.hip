@@ -385,16 +385,29 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
                                                   int64_t m_cols, int64_t chunk, int n_parts,
                                                   int max_tiles, int64_t n_chunks,
                                                   const int16_t *__restrict__ tiles,
+                                                  const int32_t *__restrict__ pcols,
+                                                  int seg_cols,
                                                   double *__restrict__ partials) {
-  extern __shared__ double sx[];  // 2 buffers of [SY_KP][m_cols + 1] then c[2][SY_KP]
-  const int64_t ld = m_cols + 1;
-  const int64_t bufsz = SY_KP * ld;
-  double *sc = sx + 2 * bufsz;
+  extern __shared__ double sx[];  // 2 buffers of [SY_KP][staged columns + 1] then c[2][SY_KP]
   // XCD-aware: the parts of one chunk share an XCD (and its L2).
   const int64_t g = blockIdx.x, xcd = g & 7, qq = g >> 3;
   const int part = (int)(qq % n_parts);
   const int64_t chunk_id = (qq / n_parts) * 8 + xcd;
   if (chunk_id >= n_chunks) return;
+  // Staged columns: all M, or (segment plans) the part's segment a then segment b, with tile
+  // groups renumbered to their staged position.
+  int a0 = 0, wa = (int)(m_cols / 16), b0 = 0, wb = 0;
+  if (pcols) {
+    a0 = pcols[4 * part];
+    wa = pcols[4 * part + 1];
+    b0 = pcols[4 * part + 2];
+    wb = pcols[4 * part + 3];
+  }
+  const int64_t lcols = 16 * (int64_t)(wa + wb);
+  const int64_t ld = (pcols ? seg_cols : m_cols) + 1;
+  const int64_t bufsz = SY_KP * ld;
+  double *sc = sx + 2 * bufsz;
+  auto local_group = [&](int grp) { return grp >= a0 && grp < a0 + wa ? grp - a0 : wa + grp - b0; };
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int16_t *tl = tiles + ((int64_t)(part * SY_WAVES + wave) * max_tiles) * 2;
@@ -405,6 +418,10 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
     trg[t] = t < max_tiles ? tl[2 * t] : -1;
     tcg[t] = t < max_tiles ? tl[2 * t + 1] : -1;
     if (trg[t] >= 0) ntile = t + 1;
+    if (trg[t] >= 0) {
+      trg[t] = local_group(trg[t]);
+      tcg[t] = local_group(tcg[t]);
+    }
   }
 
   double4_t acc[SY_MAXT];
@@ -414,7 +431,7 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
   const int64_t p_begin = chunk_id * chunk;
   const int64_t p_end = p_begin + chunk < n ? p_begin + chunk : n;
   const int n_stage = (int)((p_end - p_begin + SY_KP - 1) / SY_KP);
-  const int64_t half = m_cols / 2;
+  const int64_t half = lcols / 2;
   const int64_t per = SY_KP * half;  // double2 per stage
   double2 rx[SY_STG];
   double rc = 0.0;
@@ -427,7 +444,8 @@ __global__ void __launch_bounds__(512) k_syrk_f64(const double *__restrict__ x,
       if (e < per) {
         const int r = (int)(e / half);
         const int64_t cc = (e % half) * 2;
-        if (p0 + r < p_end) v = *reinterpret_cast<const double2 *>(x + (p0 + r) * m_cols + cc);
+        const int64_t gc = cc < 16 * wa ? 16 * a0 + cc : 16 * b0 + (cc - 16 * wa);
+        if (p0 + r < p_end) v = *reinterpret_cast<const double2 *>(x + (p0 + r) * m_cols + gc);
       }
       rx[u] = v;
     }
@@ -799,16 +817,18 @@ void launch_syrk_f64(hipStream_t st, const SyrkPlan &plan, const double *x, cons
   if (plan.prog13) {
     hipLaunchKernelGGL(k_syrk13_f64, dim3((unsigned)(n_ch8 * 2)), dim3(512), 0, st, x, c, n,
                        plan.chunk, plan.n_chunks, partials);
-  } else if (plan.m_cols <= 448) {
-    size_t lds = sizeof(double) * 2 * (16 * (plan.m_cols + 1) + 16);
+  } else if (plan.m_cols <= 448 || plan.d_pcols) {
+    // All columns (M <= 448) or a segment pair (at most 416 columns) per 16-point stage.
+    const int64_t cols = plan.d_pcols ? plan.seg_cols : plan.m_cols;
+    size_t lds = sizeof(double) * 2 * (16 * (cols + 1) + 16);
     hipLaunchKernelGGL((k_syrk_f64<22, 7, 16>), dim3((unsigned)grid), dim3(512), lds, st, x, c, n,
                        plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks,
-                       plan.d_tiles, partials);
+                       plan.d_tiles, plan.d_pcols, plan.seg_cols, partials);
   } else {  // m_cols <= 1024: 8-point stages keep the two LDS buffers under 160 KB
     size_t lds = sizeof(double) * 2 * (8 * (plan.m_cols + 1) + 8);
     hipLaunchKernelGGL((k_syrk_f64<12, 8, 8>), dim3((unsigned)grid), dim3(512), lds, st, x, c, n,
                        plan.m_cols, plan.chunk, plan.n_parts, plan.max_tiles, plan.n_chunks,
-                       plan.d_tiles, partials);
+                       plan.d_tiles, plan.d_pcols, plan.seg_cols, partials);
   }
 }
 

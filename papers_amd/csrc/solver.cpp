@@ -173,6 +173,7 @@ struct cy3q_solver {
     }
     for (void *p : allocs) hipFree(p);
     if (plan64.d_tiles) hipFree(plan64.d_tiles);
+    if (plan64.d_pcols) hipFree(plan64.d_pcols);
     if (st) hipStreamDestroy(st);
   }
 
@@ -242,6 +243,68 @@ struct cy3q_solver {
 
 namespace {
 
+// Plan of the FP64 rank-k pass past 448 columns: the column groups are cut into segments of
+// SY_SEGG groups (208 columns), and each part is a segment pair a <= b: the lower triangle of
+// segment a's tiles (a = b) or all of segment b's rows by segment a's columns. A part's
+// workgroups stage only its two segments (at most 416 columns) per 16-point stage, where the
+// all-column plan re-read every point's whole row once per part and staged 8 points at a time.
+// (At k = 8, 500k points: 22.4 ms per pass against 23.8 ms; 144-column segments, at two
+// workgroups per CU, took 29 ms.) Per part, pcols holds (first group, groups) of segment a and
+// of segment b; the tiles keep their global (row group, column group) for the reduction.
+constexpr int SY_SEGG = 13;
+
+int build_syrk_segment_plan(cy3q_solver *s, SyrkPlan &pl, int64_t target_wgs, int64_t min_chunk,
+                            int64_t kp) {
+  const int ng = pl.n_groups, waves = pl.waves;
+  const int n_seg = (ng + SY_SEGG - 1) / SY_SEGG;
+  std::vector<std::vector<std::pair<int, int>>> part_tiles;
+  std::vector<int32_t> pc;
+  for (int b = 0; b < n_seg; b++)
+    for (int a = 0; a <= b; a++) {
+      const int a0 = a * SY_SEGG, wa = std::min(SY_SEGG, ng - a0);
+      const int b0 = b * SY_SEGG, wb = std::min(SY_SEGG, ng - b0);
+      std::vector<std::pair<int, int>> tl;
+      for (int r = 0; r < (a == b ? wa : wb); r++)
+        for (int c = 0; c < (a == b ? r + 1 : wa); c++)
+          tl.push_back({(a == b ? a0 : b0) + r, a0 + c});
+      part_tiles.push_back(tl);
+      pc.insert(pc.end(), {a0, wa, a == b ? 0 : b0, a == b ? 0 : wb});
+    }
+  pl.n_parts = (int)part_tiles.size();
+  int max_t = 0;
+  for (const auto &tl : part_tiles) max_t = std::max<int>(max_t, ((int)tl.size() + waves - 1) / waves);
+  pl.max_tiles = max_t;
+  pl.seg_cols = 16 * 2 * SY_SEGG;
+  std::vector<int16_t> h((size_t)pl.n_parts * waves * pl.max_tiles * 2, (int16_t)-1);
+  for (int q = 0; q < pl.n_parts; q++) {
+    const auto &tl = part_tiles[q];
+    const int64_t total = (int64_t)tl.size(), base = total / waves, extra = total % waves;
+    int64_t idx = 0;
+    for (int w = 0; w < waves; w++) {
+      const int64_t take = base + (w < extra ? 1 : 0);
+      for (int64_t t = 0; t < take; t++, idx++) {
+        const size_t o = (((size_t)q * waves + w) * pl.max_tiles + t) * 2;
+        h[o] = (int16_t)tl[idx].first;
+        h[o + 1] = (int16_t)tl[idx].second;
+      }
+    }
+  }
+  int64_t want = std::max<int64_t>(1, target_wgs / pl.n_parts);
+  int64_t by_size = std::max<int64_t>(1, (s->n + min_chunk - 1) / min_chunk);
+  pl.n_chunks = std::max<int64_t>(1, std::min(want, by_size));
+  pl.chunk = (s->n + pl.n_chunks - 1) / pl.n_chunks;
+  pl.chunk = (pl.chunk + kp - 1) / kp * kp;
+  pl.n_chunks = (s->n + pl.chunk - 1) / pl.chunk;
+  if (pl.d_tiles) hipFree(pl.d_tiles);
+  if (pl.d_pcols) hipFree(pl.d_pcols);
+  pl.d_pcols = nullptr;
+  HIP_TRY(hipMalloc(&pl.d_tiles, h.size() * sizeof(int16_t)));
+  HIP_TRY(hipMemcpy(pl.d_tiles, h.data(), h.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&pl.d_pcols, pc.size() * sizeof(int32_t)));
+  HIP_TRY(hipMemcpy(pl.d_pcols, pc.data(), pc.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return CY3Q_OK;
+}
+
 // Lower sub-tiles of the M x M symmetric product, in row-group-major order, dealt evenly in
 // contiguous runs to (part, wave) slots of at most max_tiles each. With pad_repeat, a short
 // list is padded with repeats of its first tile (for kernels that read a static number of
@@ -272,6 +335,9 @@ int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_t
   pl.tile = tile;
   pl.waves = waves;
   pl.n_groups = (int)(s->m / tile);
+  if (tile == 16 && !pad_repeat && fixed_tiles == 0 && s->m > 448)
+    return build_syrk_segment_plan(s, pl, target_wgs, min_chunk, kp);
+  pl.seg_cols = 0;
   std::vector<std::pair<int, int>> tl;
   for (int r = 0; r < pl.n_groups; r++)
     for (int c = 0; c <= r; c++) tl.push_back({r, c});
