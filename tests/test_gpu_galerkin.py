@@ -161,8 +161,11 @@ def test_sz_weights_match_oracle(cy3q, oracle):
     assert abs(sc - sco) <= 1e-12
 
 
-@pytest.mark.parametrize("nm,n", [(1, 100), (7, 3000), (20, 4096)])
+@pytest.mark.parametrize("nm,n", [(1, 100), (7, 3000), (20, 4096), (40, 2500), (64, 1500)])
 def test_triple_overlap_matches_oracle(cy3q, oracle, nm, n):
+    # nm = 20 fits one part of 64 tiles of the pair-product GEMM; 40 and 64 modes split into
+    # about 5 and 17 parts, and every n here leaves a ragged last stage of 16 points.
+    print(f"case: triple overlap, {nm} modes, {n} points", flush=True)
     from papers_amd import galerkin as G
     rng = np.random.default_rng(nm)
     modes = _rand_c(rng, nm, n)
