@@ -220,7 +220,7 @@ void launch_diag_re(hipStream_t st, const double2 *A, int np, double *d);
 void launch_gather_cols(hipStream_t st, const double2 *V, int np, int n, const int *col,
                         const double *scale, int r, double2 *out);
 void launch_herm_project(hipStream_t st, double2 *A, int n);
-// Householder tridiagonalisation (in place; d: n, e: n - 1, v and p: n double2, scal: 3) and
+// Householder tridiagonalisation (in place; d: n, e: n - 1, v: n double2, p: 2 n double2, scal: 3) and
 // bisection of the lowest n_want eigenvalues of the tridiagonal (k_tridiag.hip).
 int tridiag_smem_ok(int n);
 void launch_hh_tridiag(hipStream_t st, double2 *A, int n, double2 *v, double2 *p, double *scal,

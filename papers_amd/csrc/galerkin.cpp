@@ -382,7 +382,7 @@ int tridiag_eigvals(Ctx &c, const double2 *a, int n, int n_want, std::vector<dou
   double *scal, *d, *e, *dev;
   G_TRY(c.alloc(&A, (size_t)n * n));
   G_TRY(c.alloc(&v, (size_t)n));
-  G_TRY(c.alloc(&p, (size_t)n));
+  G_TRY(c.alloc(&p, (size_t)(2 * n)));  // p, then the contiguous Householder column
   G_TRY(c.alloc(&scal, 3));
   G_TRY(c.alloc(&d, (size_t)n));
   G_TRY(c.alloc(&e, (size_t)n));

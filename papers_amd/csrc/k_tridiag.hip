@@ -13,7 +13,7 @@
 namespace cy3q {
 
 static constexpr int HH_THREADS = 256;
-static constexpr int HH_ROWS = 16;  // rows of p per matvec workgroup (4 per wave)
+static constexpr int HH_ROWS = 4;   // rows of p per matvec workgroup (1 per wave)
 static constexpr int HH_TILE = 64;  // update tile edge
 static constexpr double kEps = 2.220446049250313e-16;  // 2^-52
 
@@ -35,10 +35,12 @@ __device__ inline double2 cmul2(double2 a, double2 b) {
 }
 
 // Reflector of step k and p = tau A_trail v. Every workgroup forms v (length m = n - k - 1) from
-// column k in LDS (dynamic, m double2); workgroup 0 also publishes v, (beta, tau) and d[k], e[k].
-// With tau = 0 (x = 0 and alpha real) H = I, p = 0 and the update kernel does nothing.
+// column k below the diagonal, which the previous step's update (or k_hh_col0 for k = 0) left
+// contiguous in col, in LDS (dynamic, m double2); workgroup 0 also publishes v, (beta, tau) and
+// d[k], e[k]. With tau = 0 (x = 0 and alpha real) H = I, p = 0 and the update kernel does nothing.
 __global__ void __launch_bounds__(HH_THREADS) k_hh_matvec(const double2 *__restrict__ A, int n,
-                                                          int k, double2 *__restrict__ v_out,
+                                                          int k, const double2 *__restrict__ col,
+                                                          double2 *__restrict__ v_out,
                                                           double2 *__restrict__ p,
                                                           double *__restrict__ scal,
                                                           double *__restrict__ d,
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(HH_THREADS) k_hh_matvec(const double2 *__restr
   const int m = n - k - 1;
   double xn = 0.0;
   for (int i = tid; i < m; i += HH_THREADS) {
-    const double2 a = A[(int64_t)(k + 1 + i) * n + k];
+    const double2 a = col[i];
     vs[i] = a;
     if (i > 0) xn += a.x * a.x + a.y * a.y;
   }
@@ -103,17 +105,27 @@ __global__ void __launch_bounds__(HH_THREADS) k_hh_matvec(const double2 *__restr
 }
 
 // A_trail -= v w^H + w v^H with w = p + alpha v, alpha = -(tau/2)(p^H v); one 64 x 64 tile of the
-// trailing block per workgroup (every workgroup forms alpha itself).
+// trailing block per workgroup (every workgroup forms alpha itself). The workgroups of tile
+// column 0 also copy the updated column k + 1 below the diagonal into col (contiguous) for the
+// next step's matvec.
 __global__ void __launch_bounds__(HH_THREADS) k_hh_update(double2 *__restrict__ A, int n, int k,
                                                           const double2 *__restrict__ v,
                                                           const double2 *__restrict__ p,
-                                                          const double *__restrict__ scal) {
+                                                          const double *__restrict__ scal,
+                                                          double2 *__restrict__ col) {
   __shared__ double red[HH_THREADS];
   __shared__ double2 vi[HH_TILE], wi[HH_TILE], vj[HH_TILE], wj[HH_TILE];
   const double tr = scal[1], ti = scal[2];
-  if (tr == 0.0 && ti == 0.0) return;
   const int tid = threadIdx.x;
   const int m = n - k - 1;
+  if (tr == 0.0 && ti == 0.0) {  // H = I: the trailing block is unchanged
+    if (blockIdx.x == 0)
+      for (int q = tid; q < HH_TILE; q += HH_THREADS) {
+        const int i = blockIdx.y * HH_TILE + q;
+        if (i >= 1 && i < m) col[i - 1] = A[(int64_t)(k + 1 + i) * n + (k + 1)];
+      }
+    return;
+  }
   double dr = 0.0, di = 0.0;  // p^H v
   for (int i = tid; i < m; i += HH_THREADS) {
     const double2 a = p[i], b = v[i];
@@ -151,62 +163,93 @@ __global__ void __launch_bounds__(HH_THREADS) k_hh_update(double2 *__restrict__ 
     const double ui = a.y * b.x - a.x * b.y + c.y * f.x - c.x * f.y;
     double2 *x = A + (int64_t)(k + 1 + i) * n + (k + 1 + j);
     const double2 old = *x;
-    *x = make_double2(old.x - ur, old.y - ui);
+    const double2 nv = make_double2(old.x - ur, old.y - ui);
+    *x = nv;
+    if (j == 0 && i >= 1) col[i - 1] = nv;
   }
+}
+
+// Column 0 below the diagonal, contiguous, for step 0.
+__global__ void k_hh_col0(const double2 *__restrict__ A, int n, double2 *__restrict__ col) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n - 1) col[i] = A[(int64_t)(i + 1) * n];
 }
 
 __global__ void k_hh_last(const double2 *__restrict__ A, int n, double *__restrict__ d) {
   d[n - 1] = A[(int64_t)(n - 1) * n + (n - 1)].x;
 }
 
-// The j-th smallest eigenvalue (j < n_want) of the symmetric tridiagonal (d, e) by bisection on
-// the Sturm count, from the Gershgorin interval down to a few ulps.
-__global__ void k_tridiag_bisect(const double *__restrict__ d, const double *__restrict__ e, int n,
-                                 int n_want, double *__restrict__ ev) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// The j-th smallest eigenvalue (j < n_want) of the symmetric tridiagonal (d, e) by multisection on
+// the Sturm count, from the Gershgorin interval down to a few ulps: one wave per eigenvalue, whose
+// 64 lanes count at 64 interior points of the current bracket, so each round narrows it 65-fold
+// (about 10 rounds instead of about 60 halvings, each a serial pass over the n pivots).
+__device__ inline int sturm_count(const double *__restrict__ d, const double *__restrict__ e, int n,
+                                  double x, double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (q < 0.0) cnt++;
+  for (int i = 1; i < n; i++) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+    if (q < 0.0) cnt++;
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(64) k_tridiag_bisect(const double *__restrict__ d,
+                                                       const double *__restrict__ e, int n,
+                                                       int n_want, double *__restrict__ ev) {
+  const int j = blockIdx.x, lane = threadIdx.x;
   if (j >= n_want) return;
   double lo = INFINITY, hi = -INFINITY, emax2 = 0.0;
-  for (int i = 0; i < n; i++) {
+  for (int i = lane; i < n; i += 64) {
     const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
     lo = fmin(lo, d[i] - r);
     hi = fmax(hi, d[i] + r);
     if (i < n - 1) emax2 = fmax(emax2, e[i] * e[i]);
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off));
+    hi = fmax(hi, __shfl_xor(hi, off));
+    emax2 = fmax(emax2, __shfl_xor(emax2, off));
+  }
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
   const double span = fmax(fabs(lo), fabs(hi));
   lo -= 2.0 * kEps * span + pivmin;
   hi += 2.0 * kEps * span + pivmin;
-  for (int it = 0; it < 256; it++) {
-    const double mid = 0.5 * (lo + hi);
-    if (hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi)
-      break;
-    int cnt = 0;
-    double q = d[0] - mid;
-    if (q < 0.0) cnt++;
-    for (int i = 1; i < n; i++) {
-      if (fabs(q) < pivmin) q = -pivmin;
-      q = (d[i] - mid) - e[i - 1] * e[i - 1] / q;
-      if (q < 0.0) cnt++;
-    }
-    if (cnt > j)
-      hi = mid;
-    else
-      lo = mid;
+  // Invariant: count(lo) <= j < count(hi). Every lane holds the same bracket.
+  for (int it = 0; it < 64; it++) {
+    if (hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double x = lo + (hi - lo) * ((double)(lane + 1) / 65.0);
+    const bool above = sturm_count(d, e, n, x, pivmin) > j;
+    const unsigned long long mask = __ballot(above);
+    const int first = mask ? __ffsll((long long)mask) - 1 : 64;  // first point with count > j
+    const double xlo = __shfl(x, first > 0 ? first - 1 : 0);
+    const double xhi = __shfl(x, first < 64 ? first : 63);
+    const double nlo = first > 0 ? xlo : lo, nhi = first < 64 ? xhi : hi;
+    if (nlo == lo && nhi == hi) break;  // the points no longer split the bracket
+    lo = nlo;
+    hi = nhi;
   }
-  ev[j] = 0.5 * (lo + hi);
+  if (lane == 0) ev[j] = 0.5 * (lo + hi);
 }
 
 int tridiag_smem_ok(int n) { return (size_t)n * sizeof(double2) <= 160 * 1024 - 4096; }
 
+// v and p hold n complex values each; p[n .. 2n) is the contiguous column buffer.
 void launch_hh_tridiag(hipStream_t st, double2 *A, int n, double2 *v, double2 *p, double *scal,
                        double *d, double *e) {
+  double2 *col = p + n;
+  if (n > 1)
+    hipLaunchKernelGGL(k_hh_col0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n, col);
   for (int k = 0; k + 1 < n; k++) {
     const int m = n - k - 1;
     hipLaunchKernelGGL(k_hh_matvec, dim3((unsigned)((m + HH_ROWS - 1) / HH_ROWS)),
-                       dim3(HH_THREADS), (size_t)m * sizeof(double2), st, A, n, k, v, p, scal, d,
-                       e);
+                       dim3(HH_THREADS), (size_t)m * sizeof(double2), st, A, n, k, col, v, p,
+                       scal, d, e);
     const unsigned nt = (unsigned)((m + HH_TILE - 1) / HH_TILE);
-    hipLaunchKernelGGL(k_hh_update, dim3(nt, nt), dim3(HH_THREADS), 0, st, A, n, k, v, p, scal);
+    hipLaunchKernelGGL(k_hh_update, dim3(nt, nt), dim3(HH_THREADS), 0, st, A, n, k, v, p, scal,
+                       col);
   }
   hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(1), 0, st, A, n, d);
 }
@@ -214,7 +257,7 @@ void launch_hh_tridiag(hipStream_t st, double2 *A, int n, double2 *v, double2 *p
 void launch_tridiag_bisect(hipStream_t st, const double *d, const double *e, int n, int n_want,
                            double *ev) {
   if (n_want <= 0) return;
-  hipLaunchKernelGGL(k_tridiag_bisect, dim3((unsigned)((n_want + 63) / 64)), dim3(64), 0, st, d, e,
+  hipLaunchKernelGGL(k_tridiag_bisect, dim3((unsigned)n_want), dim3(64), 0, st, d, e,
                      n, n_want, ev);
 }
 
