@@ -8,8 +8,9 @@
 // g_amb_ij = M_ij / K - dk_i conj(dk_j) / K^2 (K raised to 1e-30), g_tan = conj(T) g_amb T^T
 // with the point's 3 x 8 frame T, and eta = |Re det g_tan| / |Omega|^2; NaN for a point with
 // w or |Omega|^2 not finite and positive, or with a non-finite or |det| < 1e-30.
-// One wave per point: the lanes split the N complex entries of the 45 inner products, a
-// butterfly reduction gives every lane the sums, and lane 0 does the 8 x 8 -> 3 x 3 algebra.
+// Eight points per wave: each point's 8 lanes split the N complex entries of the 45 inner
+// products, a 3-step butterfly within the 8 lanes gives them the sums (a fixed order), and the
+// group's first lane does the 8 x 8 -> 3 x 3 algebra.
 #include <algorithm>
 
 #include "cy3q_internal.h"
@@ -20,9 +21,12 @@ static constexpr int E8_NC = 8;  // ambient coordinates
 static constexpr int E8_NF = 3;  // tangent frame size
 static constexpr int E8_NM = E8_NC * (E8_NC + 1) / 2;  // M_ij, i <= j
 
-__device__ inline double wave_sum(double v) {
+static constexpr int E8_L = 8;              // lanes per point
+static constexpr int E8_PPW = 64 / E8_L;    // points per wave
+
+__device__ inline double group_sum(double v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  for (int off = E8_L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
 }
 
@@ -33,16 +37,16 @@ __global__ void __launch_bounds__(256) k_eta8(const double *__restrict__ y, int6
                                               const double *__restrict__ om,
                                               const double *__restrict__ w, int64_t n,
                                               double *__restrict__ eta) {
-  const int lane = threadIdx.x & 63;
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= n) return;
-  const double *yp = y + p * 9 * m;
+  const int lane = threadIdx.x & 63, sl = lane % E8_L;
+  const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * E8_PPW + lane / E8_L;
+  const bool in = p < n;
+  const double *yp = y + (in ? p : 0) * 9 * m;
   double k = 0.0, dkr[E8_NC], dki[E8_NC], mr[E8_NM], mi[E8_NM];
 #pragma unroll
   for (int i = 0; i < E8_NC; i++) dkr[i] = dki[i] = 0.0;
 #pragma unroll
   for (int t = 0; t < E8_NM; t++) mr[t] = mi[t] = 0.0;
-  for (int c = lane; c < nb; c += 64) {
+  for (int c = sl; c < (in ? nb : 0); c += E8_L) {
     double vr[9], vi[9];
 #pragma unroll
     for (int v = 0; v < 9; v++) {
@@ -65,18 +69,18 @@ __global__ void __launch_bounds__(256) k_eta8(const double *__restrict__ y, int6
         mi[t] += vr[1 + j] * vi[1 + i] - vi[1 + j] * vr[1 + i];
       }
   }
-  k = wave_sum(k);
+  k = group_sum(k);
 #pragma unroll
   for (int i = 0; i < E8_NC; i++) {
-    dkr[i] = wave_sum(dkr[i]);
-    dki[i] = wave_sum(dki[i]);
+    dkr[i] = group_sum(dkr[i]);
+    dki[i] = group_sum(dki[i]);
   }
 #pragma unroll
   for (int t = 0; t < E8_NM; t++) {
-    mr[t] = wave_sum(mr[t]);
-    mi[t] = wave_sum(mi[t]);
+    mr[t] = group_sum(mr[t]);
+    mi[t] = group_sum(mi[t]);
   }
-  if (lane != 0) return;
+  if (sl != 0 || !in) return;
   const double wp = w[p], omp = om[p];
   if (!isfinite(wp) || wp <= 0.0 || !isfinite(omp) || omp <= 0.0) {
     eta[p] = NAN;
@@ -172,8 +176,9 @@ void launch_pack_rows9(hipStream_t st, const double *sv, const double *sd, int64
 void launch_eta8(hipStream_t st, const double *y, int64_t m, int nb, const double *frames,
                  const double *om, const double *w, int64_t n, double *eta) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_eta8, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, y, m, nb, frames,
-                     om, w, n, eta);
+  const int64_t per_wg = 4 * E8_PPW;
+  hipLaunchKernelGGL(k_eta8, dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256), 0, st, y, m,
+                     nb, frames, om, w, n, eta);
 }
 
 }  // namespace cy3q
