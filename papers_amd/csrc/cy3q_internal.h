@@ -188,6 +188,13 @@ void launch_lap_metric(hipStream_t st, const double *pts, const double *y0, cons
 void launch_lap_rows_quintic(hipStream_t st, const double *pts, const double *pr, int64_t n,
                              const uint8_t *smono, int ns, int sdeg, int64_t mc, double *x0,
                              double *x1);
+// The intrinsic rows of the split assembly straight into the fp16 hi/lo planes of its four
+// buffers (16 points per stage), after a pass for the chunk's largest |entry| (one per stage).
+void launch_lap_rows_absmax(hipStream_t st, const double *pts, const double *pr, int64_t n,
+                            const uint8_t *smono, int ns, int sdeg, double *part);
+void launch_lap_rows_planes(hipStream_t st, const double *pts, const double *pr, int64_t n,
+                            const uint8_t *smono, int ns, int sdeg, int64_t mc,
+                            const double *scale, half8_t *const *h, half8_t *const *l);
 void launch_lap_rows_ambient(hipStream_t st, const double *pts, const double *w,
                              const double *omega, int64_t n, const uint8_t *exps, int nb,
                              int64_t mc, double *x0, double *x1, double *wv);
@@ -197,6 +204,7 @@ int herk_max_buffers();
 void launch_rows_scale_pair(hipStream_t st, const double *a, int64_t na, const double *b,
                             int64_t nb, double *part, double *scale);
 int rows_scale_parts();
+void launch_rows_scale_from_parts(hipStream_t st, const double *part, int nparts, double *scale);
 void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t stages, int64_t mc,
                        const double *scale, half8_t *xh, half8_t *xl);
 void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,

@@ -736,12 +736,15 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   G_TRY(c.alloc(&pr, (size_t)(32 * chunk)));
   G_TRY(c.alloc(&wv, (size_t)n));
   G_TRY(c.alloc(&tw, 1));
-  G_TRY(c.zeros(&x0, (size_t)(chunk * mc)));
-  G_TRY(c.zeros(&x1, (size_t)(3 * chunk * mc)));
+  x0 = x1 = nullptr;
+  if (!split) {  // the split path writes its fp16 planes straight from the row kernel
+    G_TRY(c.zeros(&x0, (size_t)(chunk * mc)));
+    G_TRY(c.zeros(&x1, (size_t)(3 * chunk * mc)));
+  }
   G_TRY(c.zeros(&cmass, (size_t)(nparts * mc * mc)));
   for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(nparts * mc * mc)));
   if (split) {
-    G_TRY(c.alloc(&part, (size_t)rows_scale_parts()));
+    G_TRY(c.alloc(&part, (size_t)std::max<int64_t>(rows_scale_parts(), stages)));
     G_TRY(c.alloc(&scale, 2));
     G_TRY(c.alloc(&planes, (size_t)(4 * 2 * stages * mc * 2)));  // 4 buffers x hi/lo planes
   }
@@ -752,28 +755,32 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
     launch_lap_metric(c.st, v.pts + 10 * p0, y0, yd, v.w + p0, v.log_om + p0, cn, m, v.psi, pr,
                       wv + p0);
-    launch_lap_rows_quintic(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, mc, x0,
-                            x1);
+    if (!split)
+      launch_lap_rows_quintic(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, mc,
+                              x0, x1);
     tm.end(GS_ROWS);
     const double *xs[4] = {x0, x1, x1 + cn * mc, x1 + 2 * cn * mc};
     int64_t rows[4] = {cn, cn, cn, cn};
     double *cs[4] = {cmass, ck[0], ck[1], ck[2]};
     if (split) {
-      // One power-of-two range for the chunk's rows (x0 and x1 are adjacent buffers of the
-      // same layout, so both are covered by scanning each), then the fp16 hi/lo planes.
+      // One power-of-two range for the chunk's rows (the largest |entry| over the mass and
+      // stiffness rows, from a pass that generates them without storing them), then the rows
+      // straight into the fp16 hi/lo planes: buffer 0 the mass rows, buffer 1 + c the rows f_c.
       const int64_t cst = (cn + 15) / 16;
       tm.begin();
-      launch_rows_scale_pair(c.st, x0, cn * mc, x1, 3 * cn * mc, part, scale);
-      const half8_t *ph[4], *pl[4];
+      launch_lap_rows_absmax(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, part);
+      launch_rows_scale_from_parts(c.st, part, (int)cst, scale);
+      half8_t *ph[4], *pl[4];
       for (int q = 0; q < 4; q++) {
-        half8_t *h = planes + (int64_t)q * 2 * stages * mc * 2, *l = h + stages * mc * 2;
-        launch_split_rows(c.st, xs[q], cn, cst, mc, scale, h, l);
-        ph[q] = h;
-        pl[q] = l;
+        ph[q] = planes + (int64_t)q * 2 * stages * mc * 2;
+        pl[q] = ph[q] + stages * mc * 2;
       }
+      launch_lap_rows_planes(c.st, v.pts + 10 * p0, pr, cn, d_sm, ns, (int)section_degree, mc,
+                             scale, ph, pl);
       tm.end(GS_SPLIT);
       tm.begin();
-      launch_herk_split16(c.st, ph, pl, cs, 4, cst, mc, nparts, scale);
+      const half8_t *cph[4] = {ph[0], ph[1], ph[2], ph[3]}, *cpl[4] = {pl[0], pl[1], pl[2], pl[3]};
+      launch_herk_split16(c.st, cph, cpl, cs, 4, cst, mc, nparts, scale);
       tm.end(GS_HERK);
     } else {
       tm.begin();

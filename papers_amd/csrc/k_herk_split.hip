@@ -73,6 +73,11 @@ void launch_rows_scale_pair(hipStream_t st, const double *a, int64_t na, const d
 
 int rows_scale_parts() { return HS_PART_BLOCKS; }
 
+// The same scale from per-block maxima computed elsewhere (nparts of them).
+void launch_rows_scale_from_parts(hipStream_t st, const double *part, int nparts, double *scale) {
+  hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, st, part, nparts, scale);
+}
+
 // x [rows][mc] FP64 -> hi, lo [stages][mc][16] fp16 of fp32(x 2^s); zero past `rows`. One thread
 // per (stage, column): 16 loads down the column (coalesced across threads), 32 bytes per plane.
 __global__ void __launch_bounds__(256) k_split_rows(const double *__restrict__ x, int64_t rows,

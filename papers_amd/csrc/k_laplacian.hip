@@ -208,6 +208,186 @@ void launch_lap_rows_quintic(hipStream_t st, const double *pts, const double *pr
 }
 
 // ---------------------------------------------------------------------------------------------
+// Split-precision form of the intrinsic rows: the same entries, generated 16 points (one stage
+// of the split rank-k update) per workgroup and written straight into its fp16 hi/lo planes
+// ([stage][column][16 points], the layout of k_split_rows), so the FP64 rows are never stored.
+// Buffer 0 holds the mass rows, buffer 1 + c the stiffness rows of f_c. A first pass over the
+// same entries finds the chunk's largest |entry| (k_lap_rows_absmax); k_absmax_scale turns it
+// into the power-of-two scale that both the planes and the update use.
+// LDS (dynamic): u, s and t of the 16 points, 5 ns double2 each, and the 16 factors.
+// ---------------------------------------------------------------------------------------------
+static constexpr int LS_PTS = 16;
+
+__device__ inline void rows_stage_setup(const double *__restrict__ pts,
+                                        const double *__restrict__ pr, int64_t n,
+                                        const uint8_t *__restrict__ smono, int ns, int sdeg,
+                                        int64_t p0, double2 *su, double2 *ss, double2 *st_,
+                                        double *sfac) {
+  const int q = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int j = q; j < LS_PTS; j += 4) {
+    const int64_t p = p0 + j;
+    double2 *uj = su + j * ns, *sj = ss + j * ns, *tj = st_ + j * 3 * ns;
+    if (p >= n) {
+      if (l == 0) sfac[j] = 0.0;
+      for (int b = l; b < ns; b += 64) {
+        uj[b] = sj[b] = make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; c++) tj[c * ns + b] = make_double2(0.0, 0.0);
+      }
+      continue;
+    }
+    double z[10];
+    for (int k = 0; k < 10; k++) z[k] = pts[10 * p + k];
+    double r2 = 0.0;
+    for (int k = 0; k < 5; k++) r2 += z[2 * k] * z[2 * k] + z[2 * k + 1] * z[2 * k + 1];
+    double rd = 1.0;
+    for (int k = 0; k < sdeg; k++) rd *= r2;
+    const double *R = pr + 32 * p + 2;
+    if (l == 0) sfac[j] = pr[32 * p];
+    for (int b = l; b < ns; b += 64) {
+      const uint8_t *e = smono + 5 * b;
+      double vr = 1.0, vi = 0.0;
+      for (int k = 0; k < 5; k++)
+        for (int r = 0; r < e[k]; r++) {
+          const double nr = vr * z[2 * k] - vi * z[2 * k + 1];
+          vi = vr * z[2 * k + 1] + vi * z[2 * k];
+          vr = nr;
+        }
+      sj[b] = make_double2(vr, vi);
+      uj[b] = make_double2(vr / rd, vi / rd);
+      double gr[5], gim[5];
+      for (int i = 0; i < 5; i++) {
+        double dr = 0.0, di = 0.0;
+        if (e[i] > 0) {
+          dr = (double)e[i];
+          for (int k = 0; k < 5; k++) {
+            const int ek = e[k] - (k == i ? 1 : 0);
+            for (int r = 0; r < ek; r++) {
+              const double nr = dr * z[2 * k] - di * z[2 * k + 1];
+              di = dr * z[2 * k + 1] + di * z[2 * k];
+              dr = nr;
+            }
+          }
+        }
+        const double cr = vr * z[2 * i] + vi * z[2 * i + 1];
+        const double ci = vi * z[2 * i] - vr * z[2 * i + 1];
+        gr[i] = dr - (double)sdeg * cr / r2;
+        gim[i] = di - (double)sdeg * ci / r2;
+      }
+      for (int c = 0; c < 3; c++) {
+        double tr = 0.0, ti = 0.0;
+        for (int i = 0; i < 5; i++) {
+          const double rr = R[2 * (5 * c + i)], ri = -R[2 * (5 * c + i) + 1];
+          tr += rr * gr[i] - ri * gim[i];
+          ti += rr * gim[i] + ri * gr[i];
+        }
+        tj[c * ns + b] = make_double2(tr, ti);
+      }
+    }
+  }
+}
+
+// Entry (buffer bf, real column col < 2B) of point j: fac u_a conj(v_b) with v = s (bf = 0) or
+// t_{bf-1}, alpha = a ns + b = col / 2, the real (col even) or imaginary part, as the FP64 rows.
+__device__ inline double rows_entry(const double2 *su, const double2 *ss, const double2 *st_,
+                                    const double *sfac, int ns, int bf, int col, int j) {
+  const int al = col >> 1, a = al / ns, b = al - (al / ns) * ns;
+  const double f = sfac[j];
+  const double2 u = su[j * ns + a];
+  const double ur = u.x * f, ui = u.y * f;
+  const double2 v = bf == 0 ? ss[j * ns + b] : st_[(j * 3 + (bf - 1)) * ns + b];
+  return (col & 1) ? ui * v.x - ur * v.y : ur * v.x + ui * v.y;
+}
+
+__global__ void __launch_bounds__(256) k_lap_rows_absmax(const double *__restrict__ pts,
+                                                         const double *__restrict__ pr, int64_t n,
+                                                         const uint8_t *__restrict__ smono, int ns,
+                                                         int sdeg, double *__restrict__ part) {
+  extern __shared__ double2 ls[];
+  double2 *su = ls, *ss = su + LS_PTS * ns, *st_ = ss + LS_PTS * ns;
+  double *sfac = reinterpret_cast<double *>(st_ + 3 * LS_PTS * ns);
+  __shared__ double red[256];
+  const int64_t p0 = (int64_t)blockIdx.x * LS_PTS;
+  rows_stage_setup(pts, pr, n, smono, ns, sdeg, p0, su, ss, st_, sfac);
+  __syncthreads();
+  const int cols = 2 * ns * ns;
+  double mx = 0.0;
+  for (int it = threadIdx.x; it < 4 * cols; it += 256) {
+    const int bf = it / cols, col = it - bf * cols;
+    for (int j = 0; j < LS_PTS; j++) mx = fmax(mx, fabs(rows_entry(su, ss, st_, sfac, ns, bf, col, j)));
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+struct RowPlanes {
+  half8_t *h[4], *l[4];
+};
+
+__global__ void __launch_bounds__(256) k_lap_rows_planes(const double *__restrict__ pts,
+                                                         const double *__restrict__ pr, int64_t n,
+                                                         const uint8_t *__restrict__ smono, int ns,
+                                                         int sdeg, int64_t mc,
+                                                         const double *__restrict__ scale,
+                                                         RowPlanes out) {
+  extern __shared__ double2 ls[];
+  double2 *su = ls, *ss = su + LS_PTS * ns, *st_ = ss + LS_PTS * ns;
+  double *sfac = reinterpret_cast<double *>(st_ + 3 * LS_PTS * ns);
+  const int64_t s0 = blockIdx.x;
+  rows_stage_setup(pts, pr, n, smono, ns, sdeg, s0 * LS_PTS, su, ss, st_, sfac);
+  __syncthreads();
+  const double sc = scale[0];
+  const int cols = 2 * ns * ns;
+  for (int64_t it = threadIdx.x; it < 4 * mc; it += 256) {
+    const int bf = (int)(it / mc), col = (int)(it - bf * mc);
+    half8_t h[2], l[2];
+#pragma unroll
+    for (int j = 0; j < LS_PTS; j++) {
+      const float v =
+          col < cols ? (float)(rows_entry(su, ss, st_, sfac, ns, bf, col, j) * sc) : 0.f;
+      const _Float16 hv = (_Float16)v;
+      h[j >> 3][j & 7] = hv;
+      l[j >> 3][j & 7] = (_Float16)(v - (float)hv);
+    }
+    const int64_t q = s0 * mc + col;
+    out.h[bf][q * 2] = h[0];
+    out.h[bf][q * 2 + 1] = h[1];
+    out.l[bf][q * 2] = l[0];
+    out.l[bf][q * 2 + 1] = l[1];
+  }
+}
+
+static size_t rows_stage_lds(int ns) {
+  return sizeof(double2) * 5 * LS_PTS * (size_t)ns + sizeof(double) * LS_PTS;
+}
+
+void launch_lap_rows_absmax(hipStream_t st, const double *pts, const double *pr, int64_t n,
+                            const uint8_t *smono, int ns, int sdeg, double *part) {
+  if (n <= 0) return;
+  const int64_t stages = (n + LS_PTS - 1) / LS_PTS;
+  hipLaunchKernelGGL(k_lap_rows_absmax, dim3((unsigned)stages), dim3(256), rows_stage_lds(ns), st,
+                     pts, pr, n, smono, ns, sdeg, part);
+}
+
+void launch_lap_rows_planes(hipStream_t st, const double *pts, const double *pr, int64_t n,
+                            const uint8_t *smono, int ns, int sdeg, int64_t mc,
+                            const double *scale, half8_t *const *h, half8_t *const *l) {
+  if (n <= 0) return;
+  RowPlanes o;
+  for (int q = 0; q < 4; q++) {
+    o.h[q] = h[q];
+    o.l[q] = l[q];
+  }
+  const int64_t stages = (n + LS_PTS - 1) / LS_PTS;
+  hipLaunchKernelGGL(k_lap_rows_planes, dim3((unsigned)stages), dim3(256), rows_stage_lds(ns), st,
+                     pts, pr, n, smono, ns, sdeg, mc, scale, o);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Ambient rows (the reference's form): fac = sqrt(w |Omega|^2) for a usable point
 // (w finite > 0, Omega finite), 0 otherwise; X0[p] = fac phi, X1[8p + i] = fac d phi / dz_i.
 // wv[p] = w for a usable point, 0 otherwise (for the sum of weights).
