@@ -231,6 +231,35 @@ def bench_batch(args) -> int:
     same = all(np.array_equal(x, s.h_block) for x, s in zip(h_seq, b.solvers))
     sig = [s.sigma() for s in b.solvers]
     t_seq, t_graph = min(times_seq), min(times_graph)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        # One FP64 donaldson_step per degree on a bounded Dwork sample gives the per-point cost
+        # of an iteration at that k; the 32 instances' 15 iterations are priced from it.
+        O.build()
+        sample = max(1000, args.cpu_sample // 2)
+        per_pt, notes = {}, []
+        for kk in sorted({sp[0] for sp in specs}):
+            prob = O.Problem(kk, sample, 7, psi=0.45, dwork=True, quotient=True, derivs=False)
+            h = prob.h
+            h, _ = O.donaldson_step(prob.sv, prob.w, h)  # warm-up
+            tc = []
+            for _ in range(args.cpu_iters):
+                t0 = time.perf_counter()
+                h, _ = O.donaldson_step(prob.sv, prob.w, h)
+                tc.append(time.perf_counter() - t0)
+            med = sorted(tc)[len(tc) // 2]
+            per_pt[kk] = med / prob.n
+            notes.append(f"k={kk} N={prob.nb}: {med * 1e3:.1f} ms")
+        t_cpu = sum(iters * per_pt[sp[0]] * sp[1] for sp in specs)
+        cpu = {"value": 32 * iters / t_cpu, "unit": "instance T-op iters/s",
+               "cores": int(O.lib().orc_num_threads()), "kind": "port",
+               "sample": (f"oracle donaldson_step (FP64, OpenMP K loop, sequential T sum) on a "
+                          f"{sample}-point Dwork psi=0.45 sample per degree, median of "
+                          f"{args.cpu_iters} ({', '.join(notes)}), priced per point over the 32 "
+                          f"instances x {iters} iterations = {t_cpu:.1f} s")}
+        _log(f"cpu: {t_cpu:.1f} s for the whole batch (projected)")
     out = {
         "metric": "batched balancing: 32 Dwork instances x 15 T-op iterations, one graph launch",
         "value": 32 * iters / t_graph,
@@ -248,6 +277,7 @@ def bench_batch(args) -> int:
         "graph_equals_sequential_bitwise": bool(same),
         "build_s": t_build, "autotune_s": t_tune, "autotuned_shapes": n_tuned,
         "tuning": b.tuning(), "sigma": sig,
+        "cpu_baseline": cpu,
     }
     b.close()
     print(json.dumps(out))
@@ -500,6 +530,29 @@ def bench_adjoint(args) -> int:
     _log(f"sigma_refine_lbfgs_gpu: {t_lbfgs:.2f} s, {len(hist_lb) - 1} accepted steps, "
          f"sigma {hist_lb[0]:.5f} -> {min(hist_lb):.5f}")
     n = s.n_points
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        O.build()
+        sample = min(2 * args.cpu_sample, n)
+        _log(f"cpu baseline: the oracle's sigma^2 gradient on a {sample}-point sample")
+        prob = O.Problem(k, sample, args.seed, quotient=True)
+        prob.sigma_sq_grad(h0)  # warm-up
+        tc = []
+        for _ in range(args.cpu_iters):
+            t0 = time.perf_counter()
+            prob.sigma_sq_grad(h0)
+            tc.append(time.perf_counter() - t0)
+        med = sorted(tc)[len(tc) // 2]
+        per_eval = med * n / prob.n
+        cpu = {"value": 1.0 / per_eval, "unit": "grad evals/s",
+               "cores": int(O.lib().orc_num_threads()), "kind": "port",
+               "sample": (f"oracle sigma_sq_grad (per-point eta and its adjoint, OpenMP over "
+                          f"points) on a {prob.n}-point SZ sample (k={k}, N={prob.nb}) at the "
+                          f"GPU-balanced h, median of {len(tc)} = {med:.3f} s, scaled "
+                          f"x{n / prob.n:g} to {n} points")}
+        _log(f"cpu: {med:.3f} s per gradient at {prob.n} points -> {per_eval:.2f} s at {n}")
     out = {
         "metric": "sigma_L2^2 gradient evaluations per second, k=5 quotient (N=125), 400k pts",
         "value": 1.0 / t_grad,
@@ -523,6 +576,7 @@ def bench_adjoint(args) -> int:
         "lbfgs_30_iters_m10_s": t_lbfgs,
         "lbfgs_accepted_steps": len(hist_lb) - 1,
         "lbfgs_sigma_first_best": [hist_lb[0], min(hist_lb)] if hist_lb else None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     return 0

@@ -81,6 +81,9 @@ void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *part
 // Split-fp16 MFMA contractions (k_split16.hip).
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
                          int *ex, float *rs2, _Float16 *xd, float *xt);
+// ex[r] = e with max_j |x[r][j]| in [2^(e-1), 2^e) (0 for a zero row); rs2[r] = 4^-e if rs2.
+void launch_row_exponent(hipStream_t st, const double *x, int64_t rows, int64_t m, int *ex,
+                         float *rs2);
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale);
 int dpass_split16_waves();
@@ -207,6 +210,12 @@ int rows_scale_parts();
 void launch_rows_scale_from_parts(hipStream_t st, const double *part, int nparts, double *scale);
 void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t stages, int64_t mc,
                        const double *scale, half8_t *xh, half8_t *xl);
+// Y = X B on the fp16 split MFMA (FP32 accumulation, FP64 output): X [rows][m] FP64 with row
+// exponents ex (launch_row_exponent), B the hi/lo planes of the real-ified W^T from
+// launch_split_rows(B, m, m / 16, m, bscale); y [rows][ldy]. m a multiple of 32.
+void launch_rowgemm_split16(hipStream_t st, const double *x, int64_t rows, int64_t m,
+                            const int *ex, const half8_t *bh, const half8_t *bl,
+                            const double *bscale, double *y, int64_t ldy);
 void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,
                          double *const *c, int nbuf, int64_t stages, int64_t mc, int nparts,
                          const double *scale);

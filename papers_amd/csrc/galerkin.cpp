@@ -743,16 +743,38 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   }
   G_TRY(c.zeros(&cmass, (size_t)(nparts * mc * mc)));
   for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(nparts * mc * mc)));
+  // Split path: the metric contraction y = W x of the section and derivative rows also runs on
+  // the fp16 split MFMA, with B = W^T as hi/lo planes (one power-of-two scale) and one exponent
+  // per row.
+  double *bscale = nullptr;
+  half8_t *wph = nullptr, *wpl = nullptr;
+  int *rex = nullptr;
   if (split) {
     G_TRY(c.alloc(&part, (size_t)std::max<int64_t>(rows_scale_parts(), stages)));
     G_TRY(c.alloc(&scale, 2));
     G_TRY(c.alloc(&planes, (size_t)(4 * 2 * stages * mc * 2)));  // 4 buffers x hi/lo planes
+    G_TRY(c.alloc(&bscale, 2));
+    G_TRY(c.alloc(&wph, (size_t)(m * m / 8)));
+    G_TRY(c.alloc(&wpl, (size_t)(m * m / 8)));
+    G_TRY(c.alloc(&rex, (size_t)(6 * chunk)));
+    tm.begin();
+    launch_rows_scale_pair(c.st, v.bmat, m * m, nullptr, 0, part, bscale);
+    launch_split_rows(c.st, v.bmat, m, m / 16, m, bscale, wph, wpl);
+    tm.end(GS_ROWS);
   }
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
     tm.begin();
-    launch_trigemm_f64(c.st, v.sv + p0 * m, cn, m, v.bmat, y0, 0, 0);
-    launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
+    if (split) {
+      launch_row_exponent(c.st, v.sv + p0 * m, cn, m, rex, nullptr);
+      launch_row_exponent(c.st, v.sd + p0 * 5 * m, cn * 5, m, rex + cn, nullptr);
+      launch_rowgemm_split16(c.st, v.sv + p0 * m, cn, m, rex, wph, wpl, bscale, y0, m);
+      launch_rowgemm_split16(c.st, v.sd + p0 * 5 * m, cn * 5, m, rex + cn, wph, wpl, bscale, yd,
+                             m);
+    } else {
+      launch_trigemm_f64(c.st, v.sv + p0 * m, cn, m, v.bmat, y0, 0, 0);
+      launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
+    }
     launch_lap_metric(c.st, v.pts + 10 * p0, y0, yd, v.w + p0, v.log_om + p0, cn, m, v.psi, pr,
                       wv + p0);
     if (!split)
@@ -793,7 +815,9 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     launch_sum_parts(c.st, cmass, mc * mc, nparts);
     for (int q = 0; q < 3; q++) launch_sum_parts(c.st, ck[q], mc * mc, nparts);
     tm.end(GS_HERK);
-    for (void *p : {(void *)part, (void *)scale, (void *)planes}) c.release(p);
+    for (void *p : {(void *)part, (void *)scale, (void *)planes, (void *)bscale, (void *)wph,
+                    (void *)wpl, (void *)rex})
+      c.release(p);
   }
   launch_sum_f64(c.st, wv, n, tw);
   double2 *M, *K;

@@ -261,4 +261,148 @@ void launch_sum_parts(hipStream_t st, double *c, int64_t len, int nparts) {
 
 int herk_split_tile() { return HS_TB; }
 
+// ---------------------------------------------------------------------------------------------
+// Split metric contraction of the intrinsic rows: Y = X B, X [rows][m] the FP64 section (or
+// derivative) rows, B [m][m] the real-ified W^T (block upper triangular: column c needs only
+// k < 2 (c / 2) + 2, so column block J stops at k = min(256 (J + 1), m)). Y_r. is then W x_r,
+// whose inner products give the point's K, dK and ddK (the metric). The FP64 form is
+// k_trigemm_f64. Here:
+//   * row r is scaled by 2^(14 - ex[r]) (max |x| < 2^14, read from launch_row_exponent), split
+//     into fp16 hi/lo as the stage is staged into LDS;
+//   * B comes as fp16 hi/lo planes [m / 16][m][16] of B 2^sB (launch_split_rows with one
+//     power-of-two scale, bscale[0] = 2^sB);
+//   * a product is hi hi' + hi lo' + lo hi' on v_mfma_f32_32x32x16_f16 with FP32 accumulation
+//     over the k range, and Y = acc 2^(ex[r] - 14) / bscale[0] is stored in FP64.
+// Workgroup: a 256 x 256 output tile, 16 waves as in k_herk_split16 (wave (qr, qc) the 64 x 64
+// quadrant as 2 x 2 tiles of 32 x 32); threads 0..511 stage the A panel (row tid / 2, 8 k
+// values), threads 512..1023 the B panel (two 16-byte units each), one stage ahead in
+// registers. Workgroups are dealt to XCDs so that the column blocks of one row tile share an
+// L2 (X is read from HBM about once).
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
+    const double *__restrict__ x, int64_t rows, int64_t m, const int *__restrict__ ex,
+    const half8_t *__restrict__ bh, const half8_t *__restrict__ bl,
+    const double *__restrict__ bscale, double *__restrict__ y, int64_t ldy, int64_t n_rt,
+    int n_cb) {
+  __shared__ half8_t lds[2][2][2][HS_TB * 2];  // [buffer][panel A/B][plane hi/lo][unit]
+  const int64_t g = blockIdx.x;
+  const int64_t xcd = g & 7, qq = g >> 3;
+  const int cb = (int)(qq % n_cb);
+  const int64_t rt = (qq / n_cb) * 8 + xcd;
+  if (rt >= n_rt) return;
+  const int64_t r0 = rt * HS_TB;
+  const int64_t c0 = (int64_t)cb * HS_TB;
+  const int64_t kend = c0 + HS_TB < m ? c0 + HS_TB : m;
+  const int64_t nst = kend / HS_PTS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int qr = wave >> 2, qc = wave & 3;
+
+  float16_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[a][c][e] = 0.f;
+
+  // Every thread stages a quarter stage-row of A (row tid / 4, k values 4 (tid % 4) .. + 3,
+  // loaded as FP64 and split in put, after the stage's MFMAs, so the loads' latency is hidden)
+  // and one 16-byte unit of B (plane tid / 512, unit tid % 512 = column * 2 + k half).
+  const int ar = tid >> 2, aq = tid & 3;
+  const int64_t arow = r0 + ar;
+  const bool a_in = arow < rows;
+  const double a_sc = a_in ? ldexp(1.0, 14 - ex[arow]) : 0.0;
+  const double *__restrict__ xrow = x + (a_in ? arow : 0) * m + 4 * aq;
+  const int bpl = tid >> 9, bunit = tid & 511;
+  const int64_t bcol = c0 + (bunit >> 1);
+  const half8_t *__restrict__ bsrc = (bpl ? bl : bh) + (bcol < m ? bcol : 0) * 2 + (bunit & 1);
+  double2 ra[2];
+  half8_t rb;
+  auto fetch = [&](int64_t s) {
+    ra[0] = ra[1] = double2{0.0, 0.0};
+    if (a_in) {
+      ra[0] = *reinterpret_cast<const double2 *>(xrow + s * HS_PTS);
+      ra[1] = *reinterpret_cast<const double2 *>(xrow + s * HS_PTS + 2);
+    }
+    rb = half8_t{};
+    if (bcol < m) rb = bsrc[s * m * 2];
+  };
+  auto put = [&](int buf) {
+    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+    half4_t rh, rl;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float f = (float)((j & 1 ? ra[j >> 1].y : ra[j >> 1].x) * a_sc);
+      const _Float16 h = (_Float16)f;
+      rh[j] = h;
+      rl[j] = (_Float16)(f - (float)h);
+    }
+    half4_t *uh = reinterpret_cast<half4_t *>(&lds[buf][0][0][ar * 2 + (aq >> 1)]);
+    half4_t *ul = reinterpret_cast<half4_t *>(&lds[buf][0][1][ar * 2 + (aq >> 1)]);
+    uh[aq & 1] = rh;
+    ul[aq & 1] = rl;
+    lds[buf][1][bpl][bunit] = rb;
+  };
+
+  if (nst > 0) {
+    fetch(0);
+    put(0);
+  }
+  __syncthreads();
+  const int fl = lane & 31, fh = lane >> 5;
+  for (int64_t s = 0; s < nst; s++) {
+    const int buf = (int)(s & 1);
+    if (s + 1 < nst) fetch(s + 1);
+    half8_t ah[2], al[2], bhv[2], blv[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
+      ah[a] = lds[buf][0][0][ca];
+      al[a] = lds[buf][0][1][ca];
+      const int cb2 = (qc * 64 + a * 32 + fl) * 2 + fh;
+      bhv[a] = lds[buf][1][0][cb2];
+      blv[a] = lds[buf][1][1][cb2];
+    }
+#pragma unroll
+    for (int pr = 0; pr < 3; pr++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? al[a] : ah[a],
+                                                             pr == 1 ? blv[c] : bhv[c], acc[a][c],
+                                                             0, 0, 0);
+    if (s + 1 < nst) put(buf ^ 1);
+    __syncthreads();
+  }
+
+  const double bre = 1.0 / bscale[0];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int e4 = 0; e4 < 4; e4++)
+#pragma unroll
+      for (int e1 = 0; e1 < 4; e1++) {
+        const int64_t gr = r0 + qr * 64 + a * 32 + 8 * e4 + 4 * fh + e1;
+        if (gr >= rows) continue;
+        const double rs = ldexp(bre, ex[gr] - 14);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const int64_t gc = c0 + qc * 64 + c * 32 + fl;
+          if (gc < m) y[gr * ldy + gc] = (double)acc[a][c][4 * e4 + e1] * rs;
+        }
+      }
+}
+
+void launch_rowgemm_split16(hipStream_t st, const double *x, int64_t rows, int64_t m,
+                            const int *ex, const half8_t *bh, const half8_t *bl,
+                            const double *bscale, double *y, int64_t ldy) {
+  if (rows <= 0) return;
+  const int64_t n_rt = (rows + HS_TB - 1) / HS_TB;
+  const int64_t n_rt8 = (n_rt + 7) / 8 * 8;
+  const int n_cb = (int)((m + HS_TB - 1) / HS_TB);
+  hipLaunchKernelGGL(k_rowgemm_split16, dim3((unsigned)(n_rt8 * n_cb)), dim3(64 * HS_WAVES), 0,
+                     st, x, rows, m, ex, bh, bl, bscale, y, ldy, n_rt, n_cb);
+}
+
 }  // namespace cy3q

@@ -178,7 +178,7 @@ __device__ inline double rsqrt_f64(double x) {
 // Factors the diagonal tile of block column jblk (in LDS, identity outside the matrix) and
 // inverts it: W_JJ = E = L_JJ^-1 into einv and w. 256 threads; colb, rowb and e are LDS
 // scratch. A bad pivot sets status = ST_ERR_NOT_PD (later kernels return early).
-__device__ void panel_core(const Cz (*tile)[TB + 1], int n, int jblk, double *__restrict__ einv,
+__device__ __attribute__((always_inline)) inline void panel_core(const Cz (*tile)[TB + 1], int n, int jblk, double *__restrict__ einv,
                            double *__restrict__ w, int32_t *__restrict__ status,
                            Cz (*colb)[TB], Cz (*rowb)[TB], Cz (*e)[TB + 1]) {
   const int tid = threadIdx.x;
@@ -199,15 +199,29 @@ __device__ void panel_core(const Cz (*tile)[TB + 1], int n, int jblk, double *__
   // reads the pivot and its operands from those two short vectors and updates its registers.
   // Buffer j & 1 is rewritten at step j + 2, after the barrier of step j + 1, so its readers
   // at step j are done.
+  // The pivot loop runs as 4 passes of 8 unrolled steps (j = 8 jo + ji): fully unrolled, its
+  // straight-line code is far larger than the instruction cache it runs from once per launch.
   bool ok = true;
+#pragma unroll 1
+  for (int jo = 0; jo < TB / 8; jo++)
 #pragma unroll
-  for (int j = 0; j < TB; j++) {
-    const int b = j & 1;
+  for (int ji = 0; ji < 8; ji++) {
+    const int j = 8 * jo + ji;
+    const int b = ji & 1;
     if (k == j) {
 #pragma unroll
       for (int u = 0; u < 4; u++) colb[b][i0 + 8 * u] = dv[u];
     }
-    if (i0 == j % 8) rowb[b][k] = rv[j / 8];
+    if (i0 == ji) {  // rv[jo], by a uniform branch (an indexed read would go through scratch)
+      if (jo == 0)
+        rowb[b][k] = rv[0];
+      else if (jo == 1)
+        rowb[b][k] = rv[1];
+      else if (jo == 2)
+        rowb[b][k] = rv[2];
+      else
+        rowb[b][k] = rv[3];
+    }
     __syncthreads();
     // All operand reads of the step are issued together, before the pivot test, and the update
     // is branch-free (the inactive elements keep their values through selects), so one LDS

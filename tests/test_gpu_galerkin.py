@@ -93,8 +93,10 @@ def test_ambient_laplacian_empty_inputs(cy3q):
 # Intrinsic Laplacian of the quintic metric
 # ------------------------------------------------------------------------------------------
 def _quintic(cy3q, oracle, k, n, seed, psi=0.0, steps=2):
-    P = oracle.Problem(k, n, seed, psi=psi)
-    s = cy3q.QuinticSolver.from_points(k, P.pts, psi=psi)
+    # From k = 5 the raw monomials are dependent on the quintic: the quotient basis is used.
+    quotient = k >= 5
+    P = oracle.Problem(k, n, seed, psi=psi, quotient=quotient)
+    s = cy3q.QuinticSolver.from_points(k, P.pts, psi=psi, quotient=quotient)
     s.orthonormalise_basis_fs_gram()
     h = P.h
     for _ in range(steps):
@@ -120,11 +122,14 @@ def test_quintic_laplacian_matches_oracle(cy3q, oracle, sdeg, k, n, psi):
     assert abs(low[0]) <= 1e-10 * ref.max()
 
 
-@pytest.mark.parametrize("sdeg,k,n,psi", [(1, 2, 1500, 0.0), (3, 3, 20000, 0.5)])
+@pytest.mark.parametrize("sdeg,k,n,psi", [(1, 2, 1500, 0.0), (3, 3, 20000, 0.5), (3, 6, 8000, 0.5),
+                                          (3, 7, 6000, 0.0)])
 def test_quintic_laplacian_split_assembly(cy3q, oracle, sdeg, k, n, psi):
-    # The fp16 split rank-k assembly against the FP64 one on the same solver: M and K to 1e-6 of
-    # their largest entry (about 22-bit products, FP32 partials over at most 4096 points), the
-    # eigenvalues to 1e-5 relative, and bit-identical repeats.
+    # The fp16 split assembly (the metric contraction y = W x and the rank-k update) against the
+    # FP64 one on the same solver: M and K to 1e-6 of their largest entry (about 22-bit
+    # products, FP32 partials over at most 4096 points or one column block of W), the
+    # eigenvalues to 1e-5 relative, and bit-identical repeats. k = 6 and 7 (2N = 420 and 660
+    # real columns) take two and three 256-column blocks of the split contraction.
     from papers_amd import galerkin as G
     from papers_amd.quintic import PREC_FP64, PREC_SPLIT16
     P, s, h = _quintic(cy3q, oracle, k, n, 42, psi)
