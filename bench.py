@@ -86,6 +86,54 @@ def cpu_baseline(points, k: int, quotient: bool, sample: int, iters: int, n_full
     }
 
 
+def _galerkin_cpu_baseline(args, k: int, psi: float, n_full: int):
+    """The Galerkin workload through the oracle, priced from bounded samples: the intrinsic M, K
+    assembly (B = 1225) on a 2000-point sample and the 20-mode triple overlaps on a 20,000-point
+    sample, each scaled by the point count; the Jacobi spectrum of the reference
+    (metric_laplacian.rs:342-693, default 128 sweeps, absolute tolerance 1e-12) at the full
+    B = 1225, timed for its set-up (regularised inverse, M^-1 K) and its first sweep only. At
+    this size the absolute tolerance is not reached and all 128 sweeps run; the later ones skip
+    the rotations of elements already below the tolerance and cost less than the first, so the
+    first sweep alone makes the CPU figure a lower bound."""
+    import numpy as np
+    from oracle import oracle as O
+
+    O.build()
+    n_lap = 2000
+    prob = O.Problem(k, n_lap, args.seed, psi=psi, dwork=True, quotient=True)
+    h = O.identity_block(prob.nb)
+    t0 = time.perf_counter()
+    M, K, _ = O.lap_quintic_mk(prob, h, 3)
+    t_mk = (time.perf_counter() - t0) * n_full / prob.n
+    t0 = time.perf_counter()
+    O.lap_spectrum_from_mk(M, K, max_sweeps=0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.lap_spectrum_from_mk(M, K, max_sweeps=1)
+    t_sweep = max(time.perf_counter() - t0 - t_setup, 0.0)
+    t_eig = t_setup + t_sweep
+    n_y = 20000
+    py = O.Problem(k, n_y, args.seed + 1, psi=psi, dwork=True, quotient=True, derivs=False,
+                   fs_gram=False)
+    rng = np.random.default_rng(42)
+    t0 = time.perf_counter()
+    s5, _ = O.sections(py.pts, O.monomials(5, False), derivs=False)
+    coef = rng.standard_normal((s5.shape[1], 20)) + 1j * rng.standard_normal((s5.shape[1], 20))
+    modes = np.ascontiguousarray((s5 @ coef).T)
+    om = np.exp(0.5 * py.log_om).astype(np.complex128)
+    O.yuk_triple(modes, om, py.w)
+    t_yuk = (time.perf_counter() - t0) * n_full / py.n
+    total = t_mk + t_eig + t_yuk
+    _log(f"cpu: assembly {t_mk:.1f} s, spectrum {t_eig:.1f} s, yukawa {t_yuk:.1f} s (projected)")
+    return {"value": total, "unit": "s", "cores": int(O.lib().orc_num_threads()), "kind": "port",
+            "sample": (f"oracle lap_quintic_mk (B=1225, OpenMP over points) on {prob.n} points "
+                       f"scaled x{n_full / prob.n:g} = {t_mk:.1f} s; Jacobi spectrum at B=1225: "
+                       f"set-up {t_setup:.1f} s + the first of its 128 sweeps {t_sweep:.1f} s = "
+                       f"{t_eig:.1f} s (a lower bound); degree-5 sections, 20 modes and "
+                       f"yuk_triple on {py.n} points scaled x{n_full / py.n:g} = "
+                       f"{t_yuk:.1f} s")}
+
+
 def bench_galerkin(args) -> int:
     """Spectral/overlap workload: Dwork quintic psi = 0.5, degree k = 8 quotient basis (N = 460),
     500,000 SZ points (seed 42), FS-Gram, 10 Donaldson iterations (timed in FP64 and in split
@@ -152,6 +200,9 @@ def bench_galerkin(args) -> int:
     t_yuk = time.perf_counter() - t0
     st_yuk = G.last_stage_times()
     _log(f"yukawa: {t_yuk:.3f} s, {y.shape[0]} triples")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _galerkin_cpu_baseline(args, k, psi, n)
     B = 35 * 35
     herk_ms = st_lap["herk"][0]
     herk64_ms = st_lap64["herk"][0]
@@ -186,6 +237,7 @@ def bench_galerkin(args) -> int:
         "roofline": {"bound": "mfma", "kernel": "k_herk_split16", "achieved": round(tf, 2),
                      "peak": PEAKS_TFLOPS["fp16"], "unit": "TFLOP/s",
                      "frac": round(tf / PEAKS_TFLOPS["fp16"], 4), "traffic": None},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     return 0

@@ -81,9 +81,6 @@ void launch_syrk_reduce(hipStream_t st, const SyrkPlan &plan, const double *part
 // Split-fp16 MFMA contractions (k_split16.hip).
 void launch_split_planes(hipStream_t st, const double *sv, int64_t n, int64_t n_pad, int64_t m,
                          int *ex, float *rs2, _Float16 *xd, float *xt);
-// ex[r] = e with max_j |x[r][j]| in [2^(e-1), 2^e) (0 for a zero row); rs2[r] = 4^-e if rs2.
-void launch_row_exponent(hipStream_t st, const double *x, int64_t rows, int64_t m, int *ex,
-                         float *rs2);
 void launch_build_bfrag16(hipStream_t st, const double *v_cplx, int nb, int64_t m_cols,
                           half8_t *bfrag, double *b_scale);
 int dpass_split16_waves();
@@ -208,13 +205,21 @@ void launch_rows_scale_pair(hipStream_t st, const double *a, int64_t na, const d
                             int64_t nb, double *part, double *scale);
 int rows_scale_parts();
 void launch_rows_scale_from_parts(hipStream_t st, const double *part, int nparts, double *scale);
+// ldx: the leading dimension of x (columns past it are zero); 0 means mc.
 void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t stages, int64_t mc,
-                       const double *scale, half8_t *xh, half8_t *xl);
-// Y = X B on the fp16 split MFMA (FP32 accumulation, FP64 output): X [rows][m] FP64 with row
-// exponents ex (launch_row_exponent), B the hi/lo planes of the real-ified W^T from
-// launch_split_rows(B, m, m / 16, m, bscale); y [rows][ldy]. m a multiple of 32.
-void launch_rowgemm_split16(hipStream_t st, const double *x, int64_t rows, int64_t m,
-                            const int *ex, const half8_t *bh, const half8_t *bl,
+                       const double *scale, half8_t *xh, half8_t *xl, int64_t ldx = 0);
+// Y = X B on the fp16 split MFMA (FP32 accumulation, FP64 output). X [rows][m] FP64 is first
+// written as row-scaled hi/lo planes of (m / 16) * rowgemm_rows_pad(rows) * 2 units each, with
+// the row exponents ex; B is the real-ified W^T as hi/lo planes of (m / 16) *
+// rowgemm_cols_pad(m) * 2 units from launch_split_rows(B, m, m / 16, cols_pad, bscale, .., m).
+// y [rows][ldy]. Requires rowgemm_supported(m) (m a multiple of 32, at most 1024).
+int64_t rowgemm_rows_pad(int64_t rows);
+int64_t rowgemm_cols_pad(int64_t m);
+inline bool rowgemm_supported(int64_t m) { return m > 0 && m % 32 == 0 && m <= 1024; }
+void launch_rows_split_planes(hipStream_t st, const double *x, int64_t rows, int64_t m, int *ex,
+                              half8_t *xh, half8_t *xl);
+void launch_rowgemm_split16(hipStream_t st, const half8_t *xh, const half8_t *xl, int64_t rows,
+                            int64_t m, const int *ex, const half8_t *bh, const half8_t *bl,
                             const double *bscale, double *y, int64_t ldy);
 void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,
                          double *const *c, int nbuf, int64_t stages, int64_t mc, int nparts,

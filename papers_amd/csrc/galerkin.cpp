@@ -744,33 +744,38 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   G_TRY(c.zeros(&cmass, (size_t)(nparts * mc * mc)));
   for (int q = 0; q < 3; q++) G_TRY(c.zeros(&ck[q], (size_t)(nparts * mc * mc)));
   // Split path: the metric contraction y = W x of the section and derivative rows also runs on
-  // the fp16 split MFMA, with B = W^T as hi/lo planes (one power-of-two scale) and one exponent
-  // per row.
+  // the fp16 split MFMA, with B = W^T as hi/lo planes (one power-of-two scale) and the rows as
+  // hi/lo planes with one exponent per row.
+  const bool split_rows = split && rowgemm_supported(m);
   double *bscale = nullptr;
-  half8_t *wph = nullptr, *wpl = nullptr;
+  half8_t *wph = nullptr, *wpl = nullptr, *xph = nullptr, *xpl = nullptr;
   int *rex = nullptr;
   if (split) {
     G_TRY(c.alloc(&part, (size_t)std::max<int64_t>(rows_scale_parts(), stages)));
     G_TRY(c.alloc(&scale, 2));
     G_TRY(c.alloc(&planes, (size_t)(4 * 2 * stages * mc * 2)));  // 4 buffers x hi/lo planes
+  }
+  if (split_rows) {
+    const int64_t mcp = rowgemm_cols_pad(m), xunits = (m / 16) * rowgemm_rows_pad(5 * chunk) * 2;
     G_TRY(c.alloc(&bscale, 2));
-    G_TRY(c.alloc(&wph, (size_t)(m * m / 8)));
-    G_TRY(c.alloc(&wpl, (size_t)(m * m / 8)));
-    G_TRY(c.alloc(&rex, (size_t)(6 * chunk)));
+    G_TRY(c.alloc(&wph, (size_t)((m / 16) * mcp * 2)));
+    G_TRY(c.alloc(&wpl, (size_t)((m / 16) * mcp * 2)));
+    G_TRY(c.alloc(&xph, (size_t)xunits));
+    G_TRY(c.alloc(&xpl, (size_t)xunits));
+    G_TRY(c.alloc(&rex, (size_t)(5 * chunk)));
     tm.begin();
     launch_rows_scale_pair(c.st, v.bmat, m * m, nullptr, 0, part, bscale);
-    launch_split_rows(c.st, v.bmat, m, m / 16, m, bscale, wph, wpl);
+    launch_split_rows(c.st, v.bmat, m, m / 16, mcp, bscale, wph, wpl, m);
     tm.end(GS_ROWS);
   }
   for (int64_t p0 = 0; p0 < n; p0 += chunk) {
     const int64_t cn = std::min(chunk, n - p0);
     tm.begin();
-    if (split) {
-      launch_row_exponent(c.st, v.sv + p0 * m, cn, m, rex, nullptr);
-      launch_row_exponent(c.st, v.sd + p0 * 5 * m, cn * 5, m, rex + cn, nullptr);
-      launch_rowgemm_split16(c.st, v.sv + p0 * m, cn, m, rex, wph, wpl, bscale, y0, m);
-      launch_rowgemm_split16(c.st, v.sd + p0 * 5 * m, cn * 5, m, rex + cn, wph, wpl, bscale, yd,
-                             m);
+    if (split_rows) {
+      launch_rows_split_planes(c.st, v.sv + p0 * m, cn, m, rex, xph, xpl);
+      launch_rowgemm_split16(c.st, xph, xpl, cn, m, rex, wph, wpl, bscale, y0, m);
+      launch_rows_split_planes(c.st, v.sd + p0 * 5 * m, cn * 5, m, rex, xph, xpl);
+      launch_rowgemm_split16(c.st, xph, xpl, cn * 5, m, rex, wph, wpl, bscale, yd, m);
     } else {
       launch_trigemm_f64(c.st, v.sv + p0 * m, cn, m, v.bmat, y0, 0, 0);
       launch_trigemm_f64(c.st, v.sd + p0 * 5 * m, cn * 5, m, v.bmat, yd, 0, 0);
@@ -816,7 +821,7 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
     for (int q = 0; q < 3; q++) launch_sum_parts(c.st, ck[q], mc * mc, nparts);
     tm.end(GS_HERK);
     for (void *p : {(void *)part, (void *)scale, (void *)planes, (void *)bscale, (void *)wph,
-                    (void *)wpl, (void *)rex})
+                    (void *)wpl, (void *)xph, (void *)xpl, (void *)rex})
       c.release(p);
   }
   launch_sum_f64(c.st, wv, n, tw);

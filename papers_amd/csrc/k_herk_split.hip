@@ -78,10 +78,11 @@ void launch_rows_scale_from_parts(hipStream_t st, const double *part, int nparts
   hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, st, part, nparts, scale);
 }
 
-// x [rows][mc] FP64 -> hi, lo [stages][mc][16] fp16 of fp32(x 2^s); zero past `rows`. One thread
-// per (stage, column): 16 loads down the column (coalesced across threads), 32 bytes per plane.
+// x [rows][ldx] FP64 -> hi, lo [stages][mc][16] fp16 of fp32(x 2^s); zero past `rows` and past
+// column ldx. One thread per (stage, column): 16 loads down the column (coalesced across
+// threads), 32 bytes per plane.
 __global__ void __launch_bounds__(256) k_split_rows(const double *__restrict__ x, int64_t rows,
-                                                    int64_t stages, int64_t mc,
+                                                    int64_t ldx, int64_t stages, int64_t mc,
                                                     const double *__restrict__ scale,
                                                     half8_t *__restrict__ xh,
                                                     half8_t *__restrict__ xl) {
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(256) k_split_rows(const double *__restrict__ x
 #pragma unroll
   for (int j = 0; j < HS_PTS; j++) {
     const int64_t p = s * HS_PTS + j;
-    const float v = p < rows ? (float)(x[p * mc + c] * sc) : 0.f;
+    const float v = p < rows && c < ldx ? (float)(x[p * ldx + c] * sc) : 0.f;
     const _Float16 hv = (_Float16)v;
     h[j >> 3][j & 7] = hv;
     l[j >> 3][j & 7] = (_Float16)(v - (float)hv);
@@ -105,20 +106,27 @@ __global__ void __launch_bounds__(256) k_split_rows(const double *__restrict__ x
 }
 
 void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t stages, int64_t mc,
-                       const double *scale, half8_t *xh, half8_t *xl) {
+                       const double *scale, half8_t *xh, half8_t *xl, int64_t ldx) {
   const int64_t tot = stages * mc;
   if (tot <= 0) return;
+  if (ldx <= 0) ldx = mc;
   hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, rows,
-                     stages, mc, scale, xh, xl);
+                     ldx, stages, mc, scale, xh, xl);
 }
 
 // ---------------------------------------------------------------------------------------------
 // C_b,k[lower 256 x 256 tiles] += sum over the stages of part k of X^T X, b = blockIdx.z (buffer),
 // k = blockIdx.y (point part), tile = blockIdx.x. 16 waves; wave (qr, qc) owns the 64 x 64
 // quadrant of the tile as 2 x 2 MFMA tiles of 32 x 32 (the quadrants above the diagonal of a
-// diagonal tile are skipped: only the lower triangle is read later). Each stage's two 256-column
-// panels (hi and lo, 32 KB) are loaded into registers one stage ahead and written to an LDS
-// double buffer; a diagonal tile loads one panel.
+// diagonal tile are skipped: only the lower triangle is read later).
+// A stage is the two 256-column panels I and J of 16 points, hi and lo planes (32 KB; a
+// diagonal tile copies its one panel twice). Stages stream into a ring of HS_RING LDS slots,
+// HS_RING - 1 stages ahead, by 16-byte global_load_lds copies: each wave copies two 1 KB runs
+// (64 consecutive units of one panel plane) per stage, so no staging registers are used and
+// three stages (96 KB) are in flight per CU. The stage loop is unrolled by HS_RING and every
+// slot is its own __shared__ array, so the copies into the other slots are told apart from the
+// reads of the current one and the loop keeps them in flight. The barrier is a bare s_barrier
+// after an explicit wait for this stage's copies (a workgroup fence would drain all copies).
 // 32x32 MFMA output: element e of lane l is (row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32).
 // ---------------------------------------------------------------------------------------------
 struct HerkSplitArgs {
@@ -127,12 +135,52 @@ struct HerkSplitArgs {
   double *c[4];
 };
 
+static constexpr int HS_RING = 4;
+static constexpr int HS_SLOT = 4 * HS_TB * 2;  // half8 units per slot: [panel][plane][512]
+
+// s_waitcnt with vmcnt = v (0..63) and the other counters at their maxima (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0, vmcnt[5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
+#define HS_WAIT_VM(v) \
+  __builtin_amdgcn_s_waitcnt(((v)&15) | (((v) >> 4) << 14) | (7 << 4) | (15 << 8))
+
+// Wait for the copies of the current stage (two per wave per stage; `ahead` later stages, at
+// most HS_RING - 2, may stay in flight; copies complete in issue order), then the barrier.
+__device__ __forceinline__ void hs_stage_sync(int ahead) {
+  static_assert(HS_RING == 4, "wait immediates assume at most two stages ahead");
+  if (ahead >= 2)
+    HS_WAIT_VM(4);
+  else if (ahead == 1)
+    HS_WAIT_VM(2);
+  else
+    HS_WAIT_VM(0);
+  __builtin_amdgcn_s_barrier();
+}
+
+// Copy c = w, w + 16 (of 32) of a stage: panel c >> 4, plane (c >> 3) & 1, units 64 (c & 7) +
+// lane. Panel 0 is read from planes (ih, il) at unit off_i, panel 1 from (jh, jl) at off_j.
+__device__ __forceinline__ void hs_issue(half8_t *slot, const half8_t *__restrict__ ih,
+                                         const half8_t *__restrict__ il, int64_t off_i,
+                                         const half8_t *__restrict__ jh,
+                                         const half8_t *__restrict__ jl, int64_t off_j, int wave,
+                                         int lane) {
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int cp = wave + 16 * u;
+    const int panel = cp >> 4, plane = (cp >> 3) & 1;
+    const half8_t *src = (panel ? (plane ? jl : jh) : (plane ? il : ih)) +
+                         (panel ? off_j : off_i) + 64 * (cp & 7) + lane;
+    __builtin_amdgcn_global_load_lds((const void *)src,
+                                     (__attribute__((address_space(3))) void *)(slot + cp * 64),
+                                     16, 0, 0);
+  }
+}
+
 __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs args, int64_t stages,
                                                                int64_t mc, int nparts,
                                                                const double *__restrict__ scale) {
-  // [buffer][panel I/J][plane hi/lo][column * 2 + half]
-  __shared__ half8_t lds[2][2][2][HS_TB * 2];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ half8_t ring0[HS_SLOT], ring1[HS_SLOT], ring2[HS_SLOT], ring3[HS_SLOT];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int qr = wave >> 2, qc = wave & 3;
   const int b = blockIdx.z, kp = blockIdx.y;
   const int t = blockIdx.x;
@@ -143,6 +191,7 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs ar
   const bool diag = I == J;
   const bool active = !(diag && qr < qc);
   const int64_t s0 = stages * kp / nparts, s1 = stages * (kp + 1) / nparts;
+  const int nst = (int)(s1 - s0);
   const half8_t *__restrict__ xh = args.xh[b];
   const half8_t *__restrict__ xl = args.xl[b];
 
@@ -154,26 +203,6 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs ar
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[a][c][e] = 0.f;
 
-  // Thread tid moves the 16-byte units q = tid and tid + 1024 of the stage: panel q >> 10,
-  // plane (q >> 9) & 1, unit q & 511 (column unit >> 1, half unit & 1).
-  half8_t reg[2];
-  auto fetch = [&](int64_t s) {
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int q = tid + 1024 * u, panel = q >> 10, plane = (q >> 9) & 1, unit = q & 511;
-      if (diag && panel == 1) continue;
-      const int64_t col = (int64_t)(panel ? J : I) * HS_TB + (unit >> 1);
-      reg[u] = (plane ? xl : xh)[(s * mc + col) * 2 + (unit & 1)];
-    }
-  };
-  auto put = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int q = tid + 1024 * u, panel = q >> 10, plane = (q >> 9) & 1, unit = q & 511;
-      if (diag && panel == 1) continue;
-      lds[buf][panel][plane][unit] = reg[u];
-    }
-  };
   const double restore = scale[1];
   auto flush = [&]() {
     if (!active) return;
@@ -191,42 +220,55 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs ar
           acc[a][c][e] = 0.f;
         }
   };
-
-  if (s1 > s0) {
-    fetch(s0);
-    put(0);
-  }
-  __syncthreads();
-  const int jp = diag ? 0 : 1;
+  // Unit 0 of panel I / J at stage s: ((s0 + s) * mc + column base) * 2.
+  auto issue = [&](int s, half8_t *slot) {
+    const int64_t st = (s0 + s) * mc;
+    hs_issue(slot, xh, xl, (st + (int64_t)I * HS_TB) * 2, xh, xl, (st + (int64_t)J * HS_TB) * 2,
+             wave, lane);
+  };
   const int fl = lane & 31, fh = lane >> 5;
-  for (int64_t s = s0; s < s1; s++) {
-    const int buf = (int)((s - s0) & 1);
-    if (s + 1 < s1) fetch(s + 1);
-    if (active) {
-      half8_t ah[2], al[2], bh[2], bl[2];
+  auto compute = [&](const half8_t *slot) {
+    if (!active) return;
+    half8_t ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
-        ah[a] = lds[buf][0][0][ca];
-        al[a] = lds[buf][0][1][ca];
-        const int cb = (qc * 64 + a * 32 + fl) * 2 + fh;
-        bh[a] = lds[buf][jp][0][cb];
-        bl[a] = lds[buf][jp][1][cb];
-      }
-#pragma unroll
-      for (int pr = 0; pr < 3; pr++)
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-          for (int c = 0; c < 2; c++)
-            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? al[a] : ah[a],
-                                                               pr == 1 ? bl[c] : bh[c], acc[a][c],
-                                                               0, 0, 0);
+    for (int a = 0; a < 2; a++) {
+      const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
+      ah[a] = slot[ca];
+      al[a] = slot[HS_TB * 2 + ca];
+      const int cb = (qc * 64 + a * 32 + fl) * 2 + fh;
+      bh[a] = slot[2 * HS_TB * 2 + cb];
+      bl[a] = slot[3 * HS_TB * 2 + cb];
     }
-    if ((s - s0 + 1) % HS_FLUSH == 0 || s + 1 == s1) flush();
-    if (s + 1 < s1) put(buf ^ 1);
-    __syncthreads();
+#pragma unroll
+    for (int pr = 0; pr < 3; pr++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? al[a] : ah[a],
+                                                             pr == 1 ? bl[c] : bh[c], acc[a][c],
+                                                             0, 0, 0);
+  };
+  // Stage s: wait for its slot, refill the slot of stage s - 1 with stage s + 3, compute.
+  auto step = [&](int s, const half8_t *cur, half8_t *refill) {
+    if (s >= nst) return;
+    hs_stage_sync(min(nst - 1 - s, HS_RING - 2));
+    if (s + HS_RING - 1 < nst) issue(s + HS_RING - 1, refill);
+    compute(cur);
+  };
+
+  static_assert(HS_FLUSH % HS_RING == 0, "flushes fall on ring boundaries");
+  if (nst > 0) issue(0, ring0);
+  if (nst > 1) issue(1, ring1);
+  if (nst > 2) issue(2, ring2);
+  for (int sb = 0; sb < nst; sb += HS_RING) {
+    step(sb, ring0, ring3);
+    step(sb + 1, ring1, ring0);
+    step(sb + 2, ring2, ring1);
+    step(sb + 3, ring3, ring2);
+    if ((sb + HS_RING) % HS_FLUSH == 0 && sb + HS_RING < nst) flush();
   }
+  flush();
 }
 
 void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,
@@ -267,24 +309,71 @@ int herk_split_tile() { return HS_TB; }
 // k < 2 (c / 2) + 2, so column block J stops at k = min(256 (J + 1), m)). Y_r. is then W x_r,
 // whose inner products give the point's K, dK and ddK (the metric). The FP64 form is
 // k_trigemm_f64. Here:
-//   * row r is scaled by 2^(14 - ex[r]) (max |x| < 2^14, read from launch_row_exponent), split
-//     into fp16 hi/lo as the stage is staged into LDS;
-//   * B comes as fp16 hi/lo planes [m / 16][m][16] of B 2^sB (launch_split_rows with one
-//     power-of-two scale, bscale[0] = 2^sB);
-//   * a product is hi hi' + hi lo' + lo hi' on v_mfma_f32_32x32x16_f16 with FP32 accumulation
-//     over the k range, and Y = acc 2^(ex[r] - 14) / bscale[0] is stored in FP64.
-// Workgroup: a 256 x 256 output tile, 16 waves as in k_herk_split16 (wave (qr, qc) the 64 x 64
-// quadrant as 2 x 2 tiles of 32 x 32); threads 0..511 stage the A panel (row tid / 2, 8 k
-// values), threads 512..1023 the B panel (two 16-byte units each), one stage ahead in
-// registers. Workgroups are dealt to XCDs so that the column blocks of one row tile share an
-// L2 (X is read from HBM about once).
+//   * k_rows_split_planes scales row r by 2^(14 - ex[r]) (its largest |x| below 2^14) and
+//     writes the fp16 hi/lo planes [m / 16][rows_pad][16] (zero rows up to rows_pad, a multiple
+//     of 256), and ex[r];
+//   * B comes as fp16 hi/lo planes [m / 16][mcp][16] of B 2^sB (launch_split_rows with one
+//     power-of-two scale, bscale[0] = 2^sB, mcp = m rounded up to 256, zero columns past m);
+//   * k_rowgemm_split16 forms each 256 x 256 tile of Y as k_herk_split16 forms its tiles (the
+//     same 16 waves, stage ring and MFMA order; panel I from the row planes, panel J from B's),
+//     with FP32 accumulation over the tile's k range, and stores Y = acc 2^(ex[r] - 14) /
+//     bscale[0] in FP64. Workgroups are dealt to XCDs so that the column blocks of one row
+//     tile share an L2.
 // ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_rows_split_planes(const double *__restrict__ x,
+                                                            int64_t rows, int64_t rows_pad,
+                                                            int64_t m, int *__restrict__ ex,
+                                                            half8_t *__restrict__ xh,
+                                                            half8_t *__restrict__ xl) {
+  // One wave per row; lane l owns the 16-byte units u = l, l + 64, ... (k = 8u .. 8u + 7).
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 6);
+  if (r >= rows_pad) return;
+  const int nu = (int)(m / 8);
+  constexpr int MAXU = 2;  // m <= 1024
+  double v[MAXU][8];
+  double mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < MAXU; i++) {
+    const int u = lane + 64 * i;
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) {
+      double2 d = double2{0.0, 0.0};
+      if (r < rows && u < nu) d = *reinterpret_cast<const double2 *>(x + r * m + 8 * u + j);
+      v[i][j] = d.x;
+      v[i][j + 1] = d.y;
+      mx = fmax(mx, fmax(fabs(d.x), fabs(d.y)));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  if (lane == 0 && r < rows) ex[r] = e;
+  const double sc = ldexp(1.0, 14 - e);
+#pragma unroll
+  for (int i = 0; i < MAXU; i++) {
+    const int u = lane + 64 * i;
+    if (u >= nu) continue;
+    half8_t h, l;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float f = (float)(v[i][j] * sc);
+      h[j] = (_Float16)f;
+      l[j] = (_Float16)(f - (float)h[j]);
+    }
+    const int64_t q = ((int64_t)(u >> 1) * rows_pad + r) * 2 + (u & 1);
+    xh[q] = h;
+    xl[q] = l;
+  }
+}
+
 __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
-    const double *__restrict__ x, int64_t rows, int64_t m, const int *__restrict__ ex,
+    const half8_t *__restrict__ ah, const half8_t *__restrict__ al, int64_t rows,
+    int64_t rows_pad, int64_t m, int64_t mcp, const int *__restrict__ ex,
     const half8_t *__restrict__ bh, const half8_t *__restrict__ bl,
     const double *__restrict__ bscale, double *__restrict__ y, int64_t ldy, int64_t n_rt,
     int n_cb) {
-  __shared__ half8_t lds[2][2][2][HS_TB * 2];  // [buffer][panel A/B][plane hi/lo][unit]
+  __shared__ half8_t ring0[HS_SLOT], ring1[HS_SLOT], ring2[HS_SLOT], ring3[HS_SLOT];
   const int64_t g = blockIdx.x;
   const int64_t xcd = g & 7, qq = g >> 3;
   const int cb = (int)(qq % n_cb);
@@ -293,8 +382,9 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
   const int64_t r0 = rt * HS_TB;
   const int64_t c0 = (int64_t)cb * HS_TB;
   const int64_t kend = c0 + HS_TB < m ? c0 + HS_TB : m;
-  const int64_t nst = kend / HS_PTS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nst = (int)(kend / HS_PTS);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int qr = wave >> 2, qc = wave & 3;
 
   float16_t acc[2][2];
@@ -305,63 +395,21 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[a][c][e] = 0.f;
 
-  // Every thread stages a quarter stage-row of A (row tid / 4, k values 4 (tid % 4) .. + 3,
-  // loaded as FP64 and split in put, after the stage's MFMAs, so the loads' latency is hidden)
-  // and one 16-byte unit of B (plane tid / 512, unit tid % 512 = column * 2 + k half).
-  const int ar = tid >> 2, aq = tid & 3;
-  const int64_t arow = r0 + ar;
-  const bool a_in = arow < rows;
-  const double a_sc = a_in ? ldexp(1.0, 14 - ex[arow]) : 0.0;
-  const double *__restrict__ xrow = x + (a_in ? arow : 0) * m + 4 * aq;
-  const int bpl = tid >> 9, bunit = tid & 511;
-  const int64_t bcol = c0 + (bunit >> 1);
-  const half8_t *__restrict__ bsrc = (bpl ? bl : bh) + (bcol < m ? bcol : 0) * 2 + (bunit & 1);
-  double2 ra[2];
-  half8_t rb;
-  auto fetch = [&](int64_t s) {
-    ra[0] = ra[1] = double2{0.0, 0.0};
-    if (a_in) {
-      ra[0] = *reinterpret_cast<const double2 *>(xrow + s * HS_PTS);
-      ra[1] = *reinterpret_cast<const double2 *>(xrow + s * HS_PTS + 2);
-    }
-    rb = half8_t{};
-    if (bcol < m) rb = bsrc[s * m * 2];
+  auto issue = [&](int s, half8_t *slot) {
+    hs_issue(slot, ah, al, ((int64_t)s * rows_pad + r0) * 2, bh, bl, ((int64_t)s * mcp + c0) * 2,
+             wave, lane);
   };
-  auto put = [&](int buf) {
-    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-    half4_t rh, rl;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const float f = (float)((j & 1 ? ra[j >> 1].y : ra[j >> 1].x) * a_sc);
-      const _Float16 h = (_Float16)f;
-      rh[j] = h;
-      rl[j] = (_Float16)(f - (float)h);
-    }
-    half4_t *uh = reinterpret_cast<half4_t *>(&lds[buf][0][0][ar * 2 + (aq >> 1)]);
-    half4_t *ul = reinterpret_cast<half4_t *>(&lds[buf][0][1][ar * 2 + (aq >> 1)]);
-    uh[aq & 1] = rh;
-    ul[aq & 1] = rl;
-    lds[buf][1][bpl][bunit] = rb;
-  };
-
-  if (nst > 0) {
-    fetch(0);
-    put(0);
-  }
-  __syncthreads();
   const int fl = lane & 31, fh = lane >> 5;
-  for (int64_t s = 0; s < nst; s++) {
-    const int buf = (int)(s & 1);
-    if (s + 1 < nst) fetch(s + 1);
-    half8_t ah[2], al[2], bhv[2], blv[2];
+  auto compute = [&](const half8_t *slot) {
+    half8_t xa[2], xb[2], ya[2], yb[2];
 #pragma unroll
     for (int a = 0; a < 2; a++) {
       const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
-      ah[a] = lds[buf][0][0][ca];
-      al[a] = lds[buf][0][1][ca];
+      xa[a] = slot[ca];
+      xb[a] = slot[HS_TB * 2 + ca];
       const int cb2 = (qc * 64 + a * 32 + fl) * 2 + fh;
-      bhv[a] = lds[buf][1][0][cb2];
-      blv[a] = lds[buf][1][1][cb2];
+      ya[a] = slot[2 * HS_TB * 2 + cb2];
+      yb[a] = slot[3 * HS_TB * 2 + cb2];
     }
 #pragma unroll
     for (int pr = 0; pr < 3; pr++)
@@ -369,11 +417,24 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
-          acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? al[a] : ah[a],
-                                                             pr == 1 ? blv[c] : bhv[c], acc[a][c],
+          acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? xb[a] : xa[a],
+                                                             pr == 1 ? yb[c] : ya[c], acc[a][c],
                                                              0, 0, 0);
-    if (s + 1 < nst) put(buf ^ 1);
-    __syncthreads();
+  };
+  auto step = [&](int s, const half8_t *cur, half8_t *refill) {
+    if (s >= nst) return;
+    hs_stage_sync(min(nst - 1 - s, HS_RING - 2));
+    if (s + HS_RING - 1 < nst) issue(s + HS_RING - 1, refill);
+    compute(cur);
+  };
+  if (nst > 0) issue(0, ring0);
+  if (nst > 1) issue(1, ring1);
+  if (nst > 2) issue(2, ring2);
+  for (int sb = 0; sb < nst; sb += HS_RING) {
+    step(sb, ring0, ring3);
+    step(sb + 1, ring1, ring0);
+    step(sb + 2, ring2, ring1);
+    step(sb + 3, ring3, ring2);
   }
 
   const double bre = 1.0 / bscale[0];
@@ -394,15 +455,27 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
       }
 }
 
-void launch_rowgemm_split16(hipStream_t st, const double *x, int64_t rows, int64_t m,
-                            const int *ex, const half8_t *bh, const half8_t *bl,
+int64_t rowgemm_rows_pad(int64_t rows) { return (rows + HS_TB - 1) / HS_TB * HS_TB; }
+int64_t rowgemm_cols_pad(int64_t m) { return (m + HS_TB - 1) / HS_TB * HS_TB; }
+
+void launch_rows_split_planes(hipStream_t st, const double *x, int64_t rows, int64_t m, int *ex,
+                              half8_t *xh, half8_t *xl) {
+  if (rows <= 0) return;
+  const int64_t rp = rowgemm_rows_pad(rows);
+  hipLaunchKernelGGL(k_rows_split_planes, dim3((unsigned)(rp / 16)), dim3(1024), 0, st, x, rows,
+                     rp, m, ex, xh, xl);
+}
+
+void launch_rowgemm_split16(hipStream_t st, const half8_t *xh, const half8_t *xl, int64_t rows,
+                            int64_t m, const int *ex, const half8_t *bh, const half8_t *bl,
                             const double *bscale, double *y, int64_t ldy) {
   if (rows <= 0) return;
-  const int64_t n_rt = (rows + HS_TB - 1) / HS_TB;
+  const int64_t rp = rowgemm_rows_pad(rows), mcp = rowgemm_cols_pad(m);
+  const int64_t n_rt = rp / HS_TB;
   const int64_t n_rt8 = (n_rt + 7) / 8 * 8;
-  const int n_cb = (int)((m + HS_TB - 1) / HS_TB);
+  const int n_cb = (int)(mcp / HS_TB);
   hipLaunchKernelGGL(k_rowgemm_split16, dim3((unsigned)(n_rt8 * n_cb)), dim3(64 * HS_WAVES), 0,
-                     st, x, rows, m, ex, bh, bl, bscale, y, ldy, n_rt, n_cb);
+                     st, xh, xl, rows, rp, m, mcp, ex, bh, bl, bscale, y, ldy, n_rt, n_cb);
 }
 
 }  // namespace cy3q

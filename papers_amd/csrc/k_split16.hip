@@ -135,15 +135,8 @@ __global__ void __launch_bounds__(256) k_row_exponent(const double *__restrict__
   if (mx > 0.0) frexp(mx, &e);
   if (lane == 0) {
     ex[p] = e;
-    if (rs2) rs2[p] = (float)ldexp(1.0, -2 * e);
+    rs2[p] = (float)ldexp(1.0, -2 * e);
   }
-}
-
-void launch_row_exponent(hipStream_t st, const double *x, int64_t rows, int64_t m, int *ex,
-                         float *rs2) {
-  if (rows <= 0) return;
-  hipLaunchKernelGGL(k_row_exponent, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, x, rows,
-                     m, ex, rs2);
 }
 
 __global__ void k_split_planes(const double *__restrict__ sv, const int *__restrict__ ex,
