@@ -221,6 +221,7 @@ void launch_rows_split_planes(hipStream_t st, const double *x, int64_t rows, int
 void launch_rowgemm_split16(hipStream_t st, const half8_t *xh, const half8_t *xl, int64_t rows,
                             int64_t m, const int *ex, const half8_t *bh, const half8_t *bl,
                             const double *bscale, double *y, int64_t ldy);
+int herk_split_parts(int nbuf, int n_tiles, int n_cu);
 void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t *const *xl,
                          double *const *c, int nbuf, int64_t stages, int64_t mc, int nparts,
                          const double *scale);

@@ -716,14 +716,14 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
   // The split update tiles the columns by 256; the FP64 one by 128.
   const int64_t mc = round_up(2 * (int64_t)B, split ? herk_split_tile() : 128);
   const int64_t chunk = std::min<int64_t>(n, std::max<int64_t>(256, (int64_t)(6e8 / (4.0 * mc))));
-  // Split path: the point parts of each tile's sum (one accumulator copy each), enough
-  // workgroups for about three per CU.
+  // Split path: the point parts of each tile's sum (one accumulator copy each), as many as fill
+  // the XCDs' rounds of workgroups best.
   int nparts = 1;
   if (split) {
     int n_cu = 0;
     G_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
     const int64_t nt = mc / herk_split_tile(), tiles = 4 * nt * (nt + 1) / 2;
-    nparts = (int)std::min<int64_t>(8, std::max<int64_t>(1, (3 * (int64_t)n_cu + tiles - 1) / tiles));
+    nparts = herk_split_parts(4, (int)(tiles / 4), n_cu);
   }
   const int64_t stages = (chunk + 15) / 16;
   uint8_t *d_sm;

@@ -18,7 +18,7 @@ namespace cy3q {
 
 static constexpr int HS_TB = 256;     // output tile edge (real columns)
 static constexpr int HS_PTS = 16;     // points per stage, the MFMA k dimension
-static constexpr int HS_WAVES = 16;   // 4 x 4 waves, each a 64 x 64 quadrant
+static constexpr int HS_WAVES = 8;    // 2 x 4 waves, each a 128 x 64 block
 static constexpr int HS_FLUSH = 256;  // stages per FP32 partial (4096 points)
 
 // ---------------------------------------------------------------------------------------------
@@ -116,12 +116,13 @@ void launch_split_rows(hipStream_t st, const double *x, int64_t rows, int64_t st
 
 // ---------------------------------------------------------------------------------------------
 // C_b,k[lower 256 x 256 tiles] += sum over the stages of part k of X^T X, b = blockIdx.z (buffer),
-// k = blockIdx.y (point part), tile = blockIdx.x. 16 waves; wave (qr, qc) owns the 64 x 64
-// quadrant of the tile as 2 x 2 MFMA tiles of 32 x 32 (the quadrants above the diagonal of a
-// diagonal tile are skipped: only the lower triangle is read later).
+// k = blockIdx.y (point part), tile = blockIdx.x. 8 waves; wave (qr, qc) owns the 128 x 64
+// block (rows 128 qr.., columns 64 qc..) of the tile as 4 x 2 MFMA tiles of 32 x 32, so a stage
+// reads 12 fragments from LDS for 24 MFMAs (the two blocks above the diagonal of a diagonal tile
+// are skipped: only the lower triangle is read later).
 // A stage is the two 256-column panels I and J of 16 points, hi and lo planes (32 KB; a
 // diagonal tile copies its one panel twice). Stages stream into a ring of HS_RING LDS slots,
-// HS_RING - 1 stages ahead, by 16-byte global_load_lds copies: each wave copies two 1 KB runs
+// HS_RING - 1 stages ahead, by 16-byte global_load_lds copies: each wave copies four 1 KB runs
 // (64 consecutive units of one panel plane) per stage, so no staging registers are used and
 // three stages (96 KB) are in flight per CU. The stage loop is unrolled by HS_RING and every
 // slot is its own __shared__ array, so the copies into the other slots are told apart from the
@@ -143,29 +144,30 @@ static constexpr int HS_SLOT = 4 * HS_TB * 2;  // half8 units per slot: [panel][
 #define HS_WAIT_VM(v) \
   __builtin_amdgcn_s_waitcnt(((v)&15) | (((v) >> 4) << 14) | (7 << 4) | (15 << 8))
 
-// Wait for the copies of the current stage (two per wave per stage; `ahead` later stages, at
+// Wait for the copies of the current stage (four per wave per stage; `ahead` later stages, at
 // most HS_RING - 2, may stay in flight; copies complete in issue order), then the barrier.
 __device__ __forceinline__ void hs_stage_sync(int ahead) {
-  static_assert(HS_RING == 4, "wait immediates assume at most two stages ahead");
+  static_assert(HS_RING == 4 && HS_WAVES == 8, "wait immediates: two stages of 4 copies ahead");
   if (ahead >= 2)
-    HS_WAIT_VM(4);
+    HS_WAIT_VM(8);
   else if (ahead == 1)
-    HS_WAIT_VM(2);
+    HS_WAIT_VM(4);
   else
     HS_WAIT_VM(0);
   __builtin_amdgcn_s_barrier();
 }
 
-// Copy c = w, w + 16 (of 32) of a stage: panel c >> 4, plane (c >> 3) & 1, units 64 (c & 7) +
-// lane. Panel 0 is read from planes (ih, il) at unit off_i, panel 1 from (jh, jl) at off_j.
+// Copy c = w, w + 8, w + 16, w + 24 (of 32) of a stage: panel c >> 4, plane (c >> 3) & 1, units
+// 64 (c & 7) + lane. Panel 0 is read from planes (ih, il) at unit off_i, panel 1 from (jh, jl)
+// at off_j.
 __device__ __forceinline__ void hs_issue(half8_t *slot, const half8_t *__restrict__ ih,
                                          const half8_t *__restrict__ il, int64_t off_i,
                                          const half8_t *__restrict__ jh,
                                          const half8_t *__restrict__ jl, int64_t off_j, int wave,
                                          int lane) {
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
-    const int cp = wave + 16 * u;
+  for (int u = 0; u < 4; u++) {
+    const int cp = wave + 8 * u;
     const int panel = cp >> 4, plane = (cp >> 3) & 1;
     const half8_t *src = (panel ? (plane ? jl : jh) : (plane ? il : ih)) +
                          (panel ? off_j : off_i) + 64 * (cp & 7) + lane;
@@ -176,47 +178,59 @@ __device__ __forceinline__ void hs_issue(half8_t *slot, const half8_t *__restric
 }
 
 __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs args, int64_t stages,
-                                                               int64_t mc, int nparts,
+                                                               int64_t mc, int nparts, int nbuf,
+                                                               int n_tiles,
                                                                const double *__restrict__ scale) {
   __shared__ half8_t ring0[HS_SLOT], ring1[HS_SLOT], ring2[HS_SLOT], ring3[HS_SLOT];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int qr = wave >> 2, qc = wave & 3;
-  const int b = blockIdx.z, kp = blockIdx.y;
-  const int t = blockIdx.x;
+  // Workgroup L is dealt to XCD L % 8. Each XCD takes whole (buffer, part) groups, g = xcd,
+  // xcd + 8, ..., one after another, and a group's tiles in order, so the workgroups that run
+  // together on one XCD share the panels of their stage range in its L2 (about 8 tiles per panel
+  // instead of almost none when a group's tiles are spread over the XCDs).
+  const int L = blockIdx.x, xcd = L & 7, jx = L >> 3;
+  const int grp = (jx / n_tiles) * 8 + xcd;
+  if (grp >= nbuf * nparts) return;
+  const int t = jx % n_tiles;
+  const int b = grp / nparts, kp = grp % nparts;
   int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) I++;
   while (I * (I + 1) / 2 > t) I--;
   const int J = t - I * (I + 1) / 2;
   const bool diag = I == J;
-  const bool active = !(diag && qr < qc);
+  const bool active = !(diag && qr == 0 && qc >= 2);
   const int64_t s0 = stages * kp / nparts, s1 = stages * (kp + 1) / nparts;
   const int nst = (int)(s1 - s0);
   const half8_t *__restrict__ xh = args.xh[b];
   const half8_t *__restrict__ xl = args.xl[b];
 
-  float16_t acc[2][2];
+  float16_t acc[4][2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[a][c][e] = 0.f;
 
   const double restore = scale[1];
+  const int fl = lane & 31, fh = lane >> 5;
   auto flush = [&]() {
     if (!active) return;
-    double *__restrict__ cm = args.c[b] + (int64_t)kp * mc * mc;
+    // The lane's element (a, c, e) is at cm + (32 a + 8 (e / 4) + e % 4) mc + 32 c. The asm
+    // makes cm opaque here, so its 64-bit addresses are formed at the flush instead of being
+    // hoisted out of the stage loop (where they would not fit in registers).
+    double *cm = args.c[b] + (int64_t)kp * mc * mc +
+                 ((int64_t)I * HS_TB + qr * 128 + 4 * fh) * mc + (int64_t)J * HS_TB + qc * 64 + fl;
+    asm volatile("" : "+v"(cm));
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-          const int64_t gi = (int64_t)I * HS_TB + qr * 64 + a * 32 + 8 * (e / 4) + 4 * (lane >> 5) +
-                             (e % 4);
-          const int64_t gj = (int64_t)J * HS_TB + qc * 64 + c * 32 + (lane & 31);
-          cm[gi * mc + gj] += (double)acc[a][c][e] * restore;
+          double *q = cm + (int64_t)(32 * a + 8 * (e / 4) + e % 4) * mc + 32 * c;
+          *q += (double)acc[a][c][e] * restore;
           acc[a][c][e] = 0.f;
         }
   };
@@ -226,23 +240,25 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_herk_split16(HerkSplitArgs ar
     hs_issue(slot, xh, xl, (st + (int64_t)I * HS_TB) * 2, xh, xl, (st + (int64_t)J * HS_TB) * 2,
              wave, lane);
   };
-  const int fl = lane & 31, fh = lane >> 5;
   auto compute = [&](const half8_t *slot) {
     if (!active) return;
-    half8_t ah[2], al[2], bh[2], bl[2];
+    half8_t ah[4], al[4], bh[2], bl[2];
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
+    for (int a = 0; a < 4; a++) {
+      const int ca = (qr * 128 + a * 32 + fl) * 2 + fh;
       ah[a] = slot[ca];
       al[a] = slot[HS_TB * 2 + ca];
-      const int cb = (qc * 64 + a * 32 + fl) * 2 + fh;
-      bh[a] = slot[2 * HS_TB * 2 + cb];
-      bl[a] = slot[3 * HS_TB * 2 + cb];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int cb = (qc * 64 + c * 32 + fl) * 2 + fh;
+      bh[c] = slot[2 * HS_TB * 2 + cb];
+      bl[c] = slot[3 * HS_TB * 2 + cb];
     }
 #pragma unroll
     for (int pr = 0; pr < 3; pr++)
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
           acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? al[a] : ah[a],
@@ -282,8 +298,28 @@ void launch_herk_split16(hipStream_t st, const half8_t *const *xh, const half8_t
   }
   const int nt = (int)(mc / HS_TB);
   const int n_tiles = nt * (nt + 1) / 2;
-  hipLaunchKernelGGL(k_herk_split16, dim3((unsigned)n_tiles, (unsigned)nparts, (unsigned)nbuf),
-                     dim3(64 * HS_WAVES), 0, st, a, stages, mc, nparts, scale);
+  const int groups8 = (nbuf * nparts + 7) / 8;  // groups per XCD
+  hipLaunchKernelGGL(k_herk_split16, dim3((unsigned)(8 * groups8 * n_tiles)), dim3(64 * HS_WAVES),
+                     0, st, a, stages, mc, nparts, nbuf, n_tiles, scale);
+}
+
+// Point parts for nbuf buffers of n_tiles tiles on n_cu CUs (one workgroup per CU): the count in
+// 1..8 whose workgroups fill the XCDs' rounds best (each XCD runs whole groups of n_tiles).
+int herk_split_parts(int nbuf, int n_tiles, int n_cu) {
+  const int per_xcd = n_cu / 8 > 0 ? n_cu / 8 : 1;
+  int best = 1;
+  double best_eff = -1.0;
+  for (int np = 1; np <= 8; np++) {
+    const int groups8 = (nbuf * np + 7) / 8;
+    const int64_t wg = (int64_t)groups8 * n_tiles;  // workgroups of the busiest XCD
+    const int64_t rounds = (wg + per_xcd - 1) / per_xcd;
+    const double eff = (double)nbuf * np * n_tiles / (8.0 * rounds * per_xcd);
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = np;
+    }
+  }
+  return best;
 }
 
 // The accumulator copies of the point parts summed into copy 0, in part order.
@@ -315,7 +351,7 @@ int herk_split_tile() { return HS_TB; }
 //   * B comes as fp16 hi/lo planes [m / 16][mcp][16] of B 2^sB (launch_split_rows with one
 //     power-of-two scale, bscale[0] = 2^sB, mcp = m rounded up to 256, zero columns past m);
 //   * k_rowgemm_split16 forms each 256 x 256 tile of Y as k_herk_split16 forms its tiles (the
-//     same 16 waves, stage ring and MFMA order; panel I from the row planes, panel J from B's),
+//     same 8 waves, stage ring and MFMA order; panel I from the row planes, panel J from B's),
 //     with FP32 accumulation over the tile's k range, and stores Y = acc 2^(ex[r] - 14) /
 //     bscale[0] in FP64. Workgroups are dealt to XCDs so that the column blocks of one row
 //     tile share an L2.
@@ -387,9 +423,9 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int qr = wave >> 2, qc = wave & 3;
 
-  float16_t acc[2][2];
+  float16_t acc[4][2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -401,20 +437,23 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
   };
   const int fl = lane & 31, fh = lane >> 5;
   auto compute = [&](const half8_t *slot) {
-    half8_t xa[2], xb[2], ya[2], yb[2];
+    half8_t xa[4], xb[4], ya[2], yb[2];
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-      const int ca = (qr * 64 + a * 32 + fl) * 2 + fh;
+    for (int a = 0; a < 4; a++) {
+      const int ca = (qr * 128 + a * 32 + fl) * 2 + fh;
       xa[a] = slot[ca];
       xb[a] = slot[HS_TB * 2 + ca];
-      const int cb2 = (qc * 64 + a * 32 + fl) * 2 + fh;
-      ya[a] = slot[2 * HS_TB * 2 + cb2];
-      yb[a] = slot[3 * HS_TB * 2 + cb2];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int cb2 = (qc * 64 + c * 32 + fl) * 2 + fh;
+      ya[c] = slot[2 * HS_TB * 2 + cb2];
+      yb[c] = slot[3 * HS_TB * 2 + cb2];
     }
 #pragma unroll
     for (int pr = 0; pr < 3; pr++)
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
           acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr == 2 ? xb[a] : xa[a],
@@ -439,12 +478,12 @@ __global__ void __launch_bounds__(64 * HS_WAVES) k_rowgemm_split16(
 
   const double bre = 1.0 / bscale[0];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int e4 = 0; e4 < 4; e4++)
 #pragma unroll
       for (int e1 = 0; e1 < 4; e1++) {
-        const int64_t gr = r0 + qr * 64 + a * 32 + 8 * e4 + 4 * fh + e1;
+        const int64_t gr = r0 + qr * 128 + a * 32 + 8 * e4 + 4 * fh + e1;
         if (gr >= rows) continue;
         const double rs = ldexp(bre, ex[gr] - 14);
 #pragma unroll
