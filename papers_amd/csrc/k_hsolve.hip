@@ -34,8 +34,6 @@ struct Cz {
 };
 
 __device__ inline Cz cmul(Cz a, Cz b) { return Cz{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-// a * conj(b)
-__device__ inline Cz cmulc(Cz a, Cz b) { return Cz{a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }
 
 // Loads tile (r0, c0) of an n x n complex matrix into LDS, zero outside the matrix (or the
 // identity on the diagonal when pad_identity). With 256 threads, all four global loads of a
@@ -223,8 +221,7 @@ __device__ __attribute__((always_inline)) inline void panel_core(const Cz (*tile
         rowb[b][k] = rv[3];
     }
     __syncthreads();
-    // All operand reads of the step are issued together, before the pivot test, and the update
-    // is branch-free (the inactive elements keep their values through selects), so one LDS
+    // All operand reads of the step are issued together, before the pivot test, so one LDS
     // wait covers the step.
     const double piv = colb[b][j].r;
     const Cz ck = colb[b][k], rk = rowb[b][k], xe = rowb[b][tid & (TB - 1)];
@@ -236,15 +233,24 @@ __device__ __attribute__((always_inline)) inline void panel_core(const Cz (*tile
     ok = ok && piv > 0.0 && isfinite(piv);
     const double inv_l = rsqrt_f64(ok ? piv : 1.0), ipiv = inv_l * inv_l;
     if (tid <= j) e[j][tid] = Cz{xe.r * inv_l, xe.i * inv_l};
+    // Row i = i0 + 8 u changes only while i > j, and then in one of its two elements: the left
+    // one (i, k) if j < k <= i (it loses d_ij conj(d_kj) / d_jj), else the right one if k <= j
+    // (it loses d_ij rr_jk / d_jj). So each active element forms one complex product, and a
+    // wave skips a row group by a branch once both its rows (i0 = 2 w, 2 w + 1) are final: the
+    // step's issue shrinks with j instead of updating all eight elements through selects.
+    const bool lf = k > j;
+    const Cz bop = lf ? Cz{ck.r, -ck.i} : rk;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int i = i0 + 8 * u;
-      const bool left = i > j && k > j && k <= i, right = i > j && k <= j;
-      const Cz pl = cmulc(dl[u], ck), pr = cmul(dl[u], rk);
-      const double lr = dv[u].r - pl.r * ipiv, li = dv[u].i - pl.i * ipiv;
-      const double rr_ = rv[u].r - pr.r * ipiv, ri = rv[u].i - pr.i * ipiv;
-      dv[u] = Cz{left ? lr : dv[u].r, left ? li : dv[u].i};
-      rv[u] = Cz{right ? rr_ : rv[u].r, right ? ri : rv[u].i};
+      if (i > j) {
+        const Cz pp = cmul(dl[u], bop);
+        if (lf) {
+          if (k <= i) dv[u] = Cz{dv[u].r - pp.r * ipiv, dv[u].i - pp.i * ipiv};
+        } else {
+          rv[u] = Cz{rv[u].r - pp.r * ipiv, rv[u].i - pp.i * ipiv};
+        }
+      }
     }
   }
   __syncthreads();
