@@ -403,6 +403,47 @@ int cy3q_sig8_upload_static(cy3q_sig8 *g, const double *section_values,         
 int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im,   /* :445-547 */
                             double *sigma);
 
+/* ---- Tian-Yau and Schoen set-up for the 8-coordinate callers of the two contracts above:
+ * samples, Groebner-reduced section bases, tangent frames and section tables
+ * (solve_ty_metric, route34/ty_metric.rs:461-548; solve_schoen_metric, route34/schoen_metric.rs:
+ * 782-830). Coordinates are [n][8] complex as (re, im) pairs: TY z0..z3 w0..w3, Schoen
+ * x0..x2 y0..y2 t0 t1. */
+#define CY3Q_CICY_TY 0     /* BicubicPair::z3_invariant_default (cicy_sampler.rs:244-275) */
+#define CY3Q_CICY_SCHOEN 1 /* SchoenPoly::z3xz3_invariant_default (schoen_sampler.rs:188-217) */
+#define CY3Q_CICY_RESTARTS 0 /* the sequential samplers: up to 4 Newton restarts per line */
+#define CY3Q_CICY_LANES 1    /* TY only: the batch sampler of the reference's GPU build */
+
+/* Sample n_target points with the reference's ChaCha8 + StandardNormal stream for `seed`.
+ *   CY3Q_CICY_RESTARTS: CicySampler::sample_batch (cicy_sampler.rs:591-617, the CPU build) or
+ *     SchoenSampler::sample_points (route34/schoen_sampler.rs:482-498);
+ *   CY3Q_CICY_LANES: CicySampler::sample_batch_gpu (cicy_sampler.rs:637-752, the GPU build):
+ *     max(4 n_target, 64) lines drawn up front, one damped Newton attempt each
+ *     (gpu_sampler.rs:265-412), the first n_target accepted kept.
+ * Outputs: rescaled coordinates [n][16], |Omega|^2, weights normalised to sum one (no quotient
+ * factor: apply_z3_quotient divides by 3, apply_z3xz3_quotient by 9), the metric point's patch
+ * indices [n][3] (TY: z_idx in 0..3, w_idx in 0..3, -1; Schoen: x_idx, y_idx, t_idx as
+ * coordinate indices, ty_metric.rs:479-496 and schoen_metric.rs:1132-1151), *n_out accepted
+ * points (fewer than n_target when the line budget runs out) and *n_lines lines used. The
+ * output arrays hold n_target rows. */
+int cy3q_cicy_sample(int variety, int mode, uint64_t seed, int64_t n_target, int device,
+                     double *coords, double *omega_sq, double *weight, int32_t *patch,
+                     int64_t *n_out, int64_t *n_lines);
+/* build_ty_invariant_reduced_basis (ty_metric.rs:873-895; degrees[0] = k) and
+ * build_schoen_invariant_reduced_basis (schoen_metric.rs:1185-1212; degrees = d_x, d_y, d_t):
+ * *n monomials, written as [n][8] exponents to out when out is not NULL (cap rows). */
+int cy3q_cicy_basis(int variety, const uint32_t *degrees, uint32_t *out, int64_t cap, int64_t *n);
+/* ty_affine_chart_frame (ty_metric.rs:1929-2015) / schoen_affine_chart_frame
+ * (schoen_metric.rs:2202-2273) at each point: frames [n][3][8][re, im] in the layout of
+ * build_ty_frames / build_schoen_frames (ty_metric.rs:1472-1494, schoen_metric.rs:1758-1783),
+ * NaN rows where the chart is degenerate. */
+int cy3q_cicy_frames(int variety, int64_t n, const double *coords, const int32_t *patch,
+                     int device, double *frames);
+/* evaluate_section_basis_at_points and _derivs_at_points (ty_metric.rs:949-1035,
+ * schoen_metric.rs:1259-1333): section_values [n][2 n_basis], section_derivs
+ * [n][8][2 n_basis]. */
+int cy3q_cicy_sections(int64_t n, const double *coords, int64_t n_basis, const uint32_t *exps,
+                       int device, double *section_values, double *section_derivs);
+
 /* ---- The sigma-gradient kernel contract: AdamGradientKernel (gpu_adam.rs:1533-2194), the
  * object the reference's callers build once per (n_points, n_basis) and pass to
  * sigma_refine_analytic_gpu / sigma_refine_lbfgs_gpu (quintic.rs:6514, 6650, 7637). Static

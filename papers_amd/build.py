@@ -42,9 +42,11 @@ SOURCES = {
     "k_tridiag.hip": [],
     "k_yukawa.hip": [],
     "k_sigma8.hip": [],
+    "k_cicy.hip": ["-ffp-contract=off"],
     "solver.cpp": ["-x", "hip"],
     "galerkin.cpp": ["-x", "hip"],
     "batch.cpp": ["-x", "hip"],
+    "cicy.cpp": ["-x", "hip", "-ffp-contract=off"],
 }
 
 

@@ -54,6 +54,19 @@ void launch_compact_points(hipStream_t st, const double *cand, const int32_t *ac
                            const int64_t *pos, int64_t n_cand, int64_t out_offset,
                            int64_t n_max, double *pts);
 
+// Tian-Yau / Schoen set-up (k_cicy.hip). variety 0 = TY, 1 = Schoen. nrm: standard normals.
+void launch_cicy_span(hipStream_t st, int variety, const double *nrm, int64_t s0, int64_t n_cand,
+                      uint8_t *info);
+void launch_cicy_chain(hipStream_t st, int variety, const double *nrm, const int64_t *start,
+                       const uint8_t *info, int64_t n_lines, double *x, int32_t *flag);
+void launch_cicy_lanes(hipStream_t st, const double *nrm, int64_t n_lanes, double *x, int32_t *flag);
+void launch_cicy_post(hipStream_t st, int variety, const double *x, const int32_t *flag, int64_t n,
+                      double *coords, double *omega_sq, double *weight, int32_t *patch, int32_t *ok);
+void launch_cicy_frames(hipStream_t st, int variety, const double *coords, const int32_t *patch,
+                        int64_t n, double *frames);
+void launch_cicy_sections(hipStream_t st, const double *coords, int64_t n, const uint32_t *exps,
+                          int nb, double *sv, double *sd);
+
 // Per-point geometry (k_sections.hip).
 void launch_weights_logomega(hipStream_t st, const double *pts, int64_t n, double psi,
                              double *w, double *log_om, int32_t *chart_elim);

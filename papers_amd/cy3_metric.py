@@ -1,0 +1,239 @@
+"""The Tian-Yau and Schoen metric solvers (``Cy3MetricSolver``, route34/cy3_metric_unified.rs)
+on the GPU: sampling, Groebner-reduced section bases, section tables and tangent frames from
+``libcy3q.so`` (``cy3q_cicy_*``), then the Donaldson loop of :mod:`papers_amd.metric8` around
+the T-operator and sigma contracts of :mod:`papers_amd.donaldson_gpu`.
+
+* :func:`sample_points` is ``CicySampler::sample_batch`` (cicy_sampler.rs:591-617; with
+  ``mode="lanes"`` the GPU build's ``sample_batch_gpu``, :637-752) and
+  ``SchoenSampler::sample_points`` (route34/schoen_sampler.rs:482-498).
+* :func:`invariant_reduced_basis` is ``build_ty_invariant_reduced_basis`` (ty_metric.rs:873-895)
+  and ``build_schoen_invariant_reduced_basis`` (schoen_metric.rs:1185-1212).
+* :func:`affine_chart_frames` is ``build_ty_frames`` / ``build_schoen_frames``
+  (ty_metric.rs:1472-1494, schoen_metric.rs:1758-1783).
+* :func:`section_tables` is ``evaluate_section_basis_at_points`` and its derivative twin
+  (ty_metric.rs:949-1035).
+* :class:`TianYauSolver` and :class:`SchoenSolver` are the solvers of cy3_metric_unified.rs:226-320:
+  ``solve_metric(spec)`` runs ``solve_ty_metric`` (ty_metric.rs:461-788) or
+  ``solve_schoen_metric`` (schoen_metric.rs:782-1130) with the quotient weights (1/3, 1/9),
+  sigma at h = I, the damping and Tikhonov defaults, and returns a :class:`Cy3MetricResult`.
+
+The Tian-Yau sampler defaults to ``"lanes"``: the reference's determinism golden records
+(cy3_solver/tests/data/p5_10_determinism_golden.json) come from its GPU build, whose
+``sample_batch`` is the one-attempt batch sampler. ``"restarts"`` gives the CPU build's points.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import metric8 as M8
+from .donaldson_gpu import Cy3DonaldsonGpu, Cy3SigmaGpu
+from .quintic import _check, _load
+
+__all__ = ["CICY_SCHOEN", "CICY_TY", "Cy3MetricResult", "Cy3MetricSpec", "SchoenSolver",
+           "TianYauSolver", "affine_chart_frames", "invariant_reduced_basis", "sample_points",
+           "section_tables"]
+
+CICY_TY, CICY_SCHOEN = 0, 1
+_MODES = {"restarts": 0, "lanes": 1}
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = _load()
+    if not _bound:
+        vp, i64, ip, u64 = C.c_void_p, C.c_int64, C.c_int, C.c_uint64
+        L.cy3q_cicy_sample.argtypes = [ip, ip, u64, i64, ip, vp, vp, vp, vp, C.POINTER(i64),
+                                       C.POINTER(i64)]
+        L.cy3q_cicy_basis.argtypes = [ip, vp, vp, i64, C.POINTER(i64)]
+        L.cy3q_cicy_frames.argtypes = [ip, i64, vp, vp, ip, vp]
+        L.cy3q_cicy_sections.argtypes = [i64, vp, i64, vp, ip, vp, vp]
+        _bound = True
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def sample_points(variety: int, seed: int, n_target: int, mode: str = "restarts",
+                  device: int = 0) -> dict:
+    """Points with weights normalised to sum one (no quotient factor). Returns dict(coords
+    [n][16] (re, im of 8 coordinates), omega_sq [n], weight [n], patch [n][3], n_lines)."""
+    if mode not in _MODES:
+        raise ValueError(f"mode must be one of {sorted(_MODES)}")
+    n_target = int(n_target)
+    coords = np.zeros((n_target, 16))
+    om = np.zeros(n_target)
+    w = np.zeros(n_target)
+    patch = np.zeros((n_target, 3), np.int32)
+    n, nl = C.c_int64(0), C.c_int64(0)
+    _check(_lib().cy3q_cicy_sample(int(variety), _MODES[mode], int(seed) & (2**64 - 1), n_target,
+                                   int(device), _p(coords), _p(om), _p(w), _p(patch), C.byref(n),
+                                   C.byref(nl)))
+    k = n.value
+    return dict(coords=coords[:k].copy(), omega_sq=om[:k].copy(), weight=w[:k].copy(),
+                patch=patch[:k].copy(), n_lines=int(nl.value))
+
+
+def invariant_reduced_basis(variety: int, degrees) -> np.ndarray:
+    """[n_basis][8] exponents, in the reference's enumeration order."""
+    d = np.zeros(3, np.uint32)
+    d[: len(degrees)] = degrees
+    n = C.c_int64(0)
+    _check(_lib().cy3q_cicy_basis(int(variety), _p(d), None, 0, C.byref(n)))
+    out = np.zeros((max(n.value, 1), 8), np.uint32)
+    _check(_lib().cy3q_cicy_basis(int(variety), _p(d), _p(out), n.value, C.byref(n)))
+    return out[: n.value]
+
+
+def affine_chart_frames(variety: int, coords: np.ndarray, patch: np.ndarray,
+                        device: int = 0) -> np.ndarray:
+    """[n][48] frames (3 directions x 8 coordinates x (re, im)); NaN rows where degenerate."""
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 16)
+    pt = np.ascontiguousarray(patch, np.int32).reshape(-1, 3)
+    out = np.zeros((len(c), 48))
+    _check(_lib().cy3q_cicy_frames(int(variety), len(c), _p(c), _p(pt), int(device), _p(out)))
+    return out
+
+
+def section_tables(coords: np.ndarray, exps: np.ndarray, device: int = 0):
+    """(section_values [n][2 nb], section_derivs [n][8][2 nb])."""
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 16)
+    e = np.ascontiguousarray(exps, np.uint32).reshape(-1, 8)
+    n, nb = len(c), len(e)
+    sv = np.zeros((n, 2 * nb))
+    sd = np.zeros((n, 8, 2 * nb))
+    _check(_lib().cy3q_cicy_sections(n, _p(c), nb, _p(e), int(device), _p(sv), _p(sd)))
+    return sv, sd
+
+
+@dataclass
+class Cy3MetricSpec:
+    """Cy3MetricSpec (cy3_metric_unified.rs:30-58): variety "TianYau" with k, or "Schoen" with
+    (d_x, d_y, d_t)."""
+    variety: str
+    n_sample: int
+    max_iter: int
+    donaldson_tol: float
+    seed: int
+    k: int = 0
+    d_x: int = 0
+    d_y: int = 0
+    d_t: int = 0
+
+    @classmethod
+    def tian_yau(cls, k, n_sample, max_iter, donaldson_tol, seed):
+        return cls("TianYau", n_sample, max_iter, donaldson_tol, seed, k=k)
+
+    @classmethod
+    def schoen(cls, d_x, d_y, d_t, n_sample, max_iter, donaldson_tol, seed):
+        return cls("Schoen", n_sample, max_iter, donaldson_tol, seed, d_x=d_x, d_y=d_y, d_t=d_t)
+
+
+@dataclass
+class Cy3MetricResult:
+    """TyMetricResult / SchoenMetricResult (ty_metric.rs:352-427), the fields the callers read."""
+    variety: str
+    n_basis: int
+    n_points: int
+    iterations_run: int
+    final_donaldson_residual: float
+    final_sigma_residual: float
+    sigma_fs_identity: float
+    balanced_h_re: np.ndarray
+    balanced_h_im: np.ndarray
+    basis_monomials: np.ndarray
+    sigma_history: list = field(default_factory=list)
+    donaldson_history: list = field(default_factory=list)
+
+    def summary(self) -> dict:
+        """Cy3MetricSummary (cy3_metric_unified.rs:96-124)."""
+        return dict(variety=self.variety, n_basis=self.n_basis, n_points=self.n_points,
+                    iterations_run=self.iterations_run,
+                    final_donaldson_residual=self.final_donaldson_residual,
+                    final_sigma_residual=self.final_sigma_residual)
+
+
+def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, damping,
+           tikhonov, device: int) -> Cy3MetricResult:
+    pts = sample_points(variety, spec.seed, spec.n_sample, mode, device)
+    n = len(pts["weight"])
+    if n < spec.n_sample // 2:
+        raise RuntimeError(f"insufficient points: got {n}, requested {spec.n_sample}")
+    # apply_z3_quotient / apply_z3xz3_quotient (cicy_sampler.rs:758-763, schoen_sampler.rs:555-561)
+    w = pts["weight"] * (1.0 / (3.0 if variety == CICY_TY else 9.0))
+    exps = invariant_reduced_basis(variety, degrees)
+    nb = len(exps)
+    if nb == 0:
+        raise RuntimeError("empty section basis")
+    if not (math.isfinite(float(np.sum(w))) and float(np.sum(w)) > 0.0):
+        raise RuntimeError("non-positive weight sum")
+    sv, sd = section_tables(pts["coords"], exps, device)
+    frames = affine_chart_frames(variety, pts["coords"], pts["patch"], device)
+    top = Cy3DonaldsonGpu(n, nb, device)
+    sig = Cy3SigmaGpu(n, nb, device)
+    try:
+        top.upload_static(sv, w)
+        sig.upload_static(sv, sd, frames, pts["omega_sq"], w)
+        res = M8.solve_route34(top.t_operator_raw, sig.compute_sigma, np.eye(nb),
+                               np.zeros((nb, nb)), w, spec.max_iter, spec.donaldson_tol,
+                               damping=damping, tikhonov=tikhonov)
+    finally:
+        top.close()
+        sig.close()
+    hist = res["sigma_history"]
+    return Cy3MetricResult(variety=label, n_basis=nb, n_points=n,
+                           iterations_run=res["iterations_run"],
+                           final_donaldson_residual=res["final_donaldson_residual"],
+                           final_sigma_residual=hist[-1] if hist else float("nan"),
+                           sigma_fs_identity=res["sigma_fs_identity"],
+                           balanced_h_re=res["h_re"], balanced_h_im=res["h_im"],
+                           basis_monomials=exps, sigma_history=hist,
+                           donaldson_history=res["donaldson_history"])
+
+
+def _damping_mode(adaptive: bool):
+    # DonaldsonDampingMode::Adaptive { 0.3, 0.05, 1.0 } or Static(1.0)
+    return M8.AdaptiveDamping(0.3, 0.05, 1.0) if adaptive else None
+
+
+class TianYauSolver:
+    """TianYauSolver (cy3_metric_unified.rs:226-271). ``solve_metric(spec)`` with the defaults
+    of ``Cy3MetricSolver::solve_metric``: quotient applied, damping resolved by
+    resolve_ty_damping (adaptive for k >= 4, ty_metric.rs:247-270), no Tikhonov shift."""
+
+    label = "Tian-Yau Z/3"
+
+    def __init__(self, sampler: str = "lanes", device: int = 0):
+        self.sampler, self.device = sampler, device
+
+    def solve_metric(self, spec: Cy3MetricSpec, damping=None, tikhonov=0.0) -> Cy3MetricResult:
+        if spec.variety != "TianYau":
+            raise ValueError(f"spec mismatch: expected TianYau, got {spec.variety}")
+        if damping is None:
+            damping = _damping_mode(spec.k >= 4)
+        return _solve(CICY_TY, "TianYau", [spec.k], spec, self.sampler, damping, tikhonov,
+                      self.device)
+
+
+class SchoenSolver:
+    """SchoenSolver (cy3_metric_unified.rs:273-320): resolve_schoen_damping is adaptive when
+    d_x + d_y + d_t >= 10 (schoen_metric.rs:649-667)."""
+
+    label = "Schoen Z/3 x Z/3"
+
+    def __init__(self, device: int = 0):
+        self.device = device
+
+    def solve_metric(self, spec: Cy3MetricSpec, damping=None, tikhonov=0.0) -> Cy3MetricResult:
+        if spec.variety != "Schoen":
+            raise ValueError(f"spec mismatch: expected Schoen, got {spec.variety}")
+        if damping is None:
+            damping = _damping_mode(spec.d_x + spec.d_y + spec.d_t >= 10)
+        return _solve(CICY_SCHOEN, "Schoen", [spec.d_x, spec.d_y, spec.d_t], spec, "restarts",
+                      damping, tikhonov, self.device)
