@@ -397,6 +397,90 @@ def bench_route34(args) -> int:
     return 0
 
 
+def bench_cicy(args) -> int:
+    """The 8-coordinate callers end to end: the per-seed work of the reference's P5.10
+    Tian-Yau vs Schoen ensemble (bin/p5_10_ty_schoen_5sigma.rs defaults: k = 3, i.e. TY k = 3
+    with n_basis 87 and Schoen (3, 3, 1) with n_basis 27; 25,000 points; at most 100 Donaldson
+    iterations; tol 1e-6; sigma after every iteration). One step is one seed: both solves, each
+    sampling its points on the GPU (TY: the GPU build's batch sampler; Schoen: the sequential
+    sampler's exact stream), building the Groebner-reduced basis, sections and frames, and
+    running the Donaldson loop over the route34 T-operator and sigma contracts. Seeds are the
+    reference's SEEDS_20 list in order. `value` is seeds per second. The cpu_baseline runs the
+    oracle's restatement of the same two solves on a 2,500-point sample of one seed with the
+    iteration counts the GPU run took, scaled x10 to 25,000 points (every stage after
+    sampling is linear in the points)."""
+    import numpy as np
+    from papers_amd import cy3_metric as CM
+
+    seeds = [42, 100, 12345, 7, 99, 1, 2, 3, 4, 5, 137, 271, 314, 666, 1000, 2024, 4242,
+             0xDEAD, 0xBEEF, 0xCAFE]
+    n_pts = 25_000 if args.n_points == 1_000_000 else args.n_points
+    iters, tol = 100, 1e-6
+    n_seeds = max(1, min(args.steps, len(seeds)))
+
+    def one(seed):
+        ty = CM.TianYauSolver().solve_metric(CM.Cy3MetricSpec.tian_yau(3, n_pts, iters, tol, seed))
+        sc = CM.SchoenSolver().solve_metric(
+            CM.Cy3MetricSpec.schoen(3, 3, 1, n_pts, iters, tol, seed))
+        return ty, sc
+
+    one(seeds[0])  # warm-up: library load, first allocations
+    _log(f"cicy: {n_seeds} seeds, {n_pts} points, k=3")
+    t0 = time.perf_counter()
+    runs = [one(sd) for sd in seeds[:n_seeds]]
+    dt = time.perf_counter() - t0
+    stage = {}
+    for ty, sc in runs:
+        for r, tag in ((ty, "ty"), (sc, "schoen")):
+            for k_, v in r.stage_seconds.items():
+                stage[f"{tag}_{k_}"] = stage.get(f"{tag}_{k_}", 0.0) + v * 1e3 / n_seeds
+    iters_ty = [r[0].iterations_run for r in runs]
+    iters_sc = [r[1].iterations_run for r in runs]
+    _log(f"cicy: {dt:.2f} s for {n_seeds} seeds; TY iterations {iters_ty}, Schoen {iters_sc}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cicy_oracle as CO
+
+        sub = n_pts // 10
+        ty0, sc0 = runs[0]
+        t1 = time.perf_counter()
+        CO.solve(CO.TY, seeds[0], sub, (3,), ty0.iterations_run, 0.0)
+        CO.solve(CO.SCHOEN, seeds[0], sub, (3, 3, 1), sc0.iterations_run, 0.0)
+        t_cpu = (time.perf_counter() - t1) * (n_pts / sub)
+        cpu = {"value": 1.0 / t_cpu, "unit": "seeds/s", "cores": CO.O.lib().orc_num_threads(),
+               "kind": "port",
+               "sample": f"oracle solves of seed {seeds[0]} (TY {ty0.iterations_run} and Schoen "
+                         f"{sc0.iterations_run} iterations) on {sub} points, "
+                         f"{t_cpu * sub / n_pts:.2f} s, scaled x{n_pts // sub} to {n_pts} points"}
+        _log(f"cpu baseline: {t_cpu:.1f} s per seed (scaled)")
+    out = {
+        "metric": "P5.10 TY-vs-Schoen seeds/s: TY k=3 + Schoen (3,3,1), 25k pts, <=100 Donaldson iters",
+        "value": n_seeds / dt,
+        "unit": "seeds/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "steps": n_seeds,
+        "warmup": 1,
+        "ms_per_step": dt / n_seeds * 1e3,
+        "dtype": "f64",
+        "data": "synthetic: ChaCha8 line-intersection samples of the Tian-Yau and Schoen "
+                "threefolds, the reference's SEEDS_20",
+        "config": {"workload": f"P5.10 per-seed solves: TY k=3 (n_basis 87) and Schoen (3,3,1) "
+                               f"(n_basis 27), {n_pts} points each, max {iters} iterations, "
+                               f"tol {tol}, sigma every iteration",
+                   "n_points": n_pts, "seeds": n_seeds},
+        "iterations_ty": iters_ty,
+        "iterations_schoen": iters_sc,
+        "sigma_ty": [r[0].final_sigma_residual for r in runs],
+        "sigma_schoen": [r[1].final_sigma_residual for r in runs],
+        "stage_ms_per_seed": stage,
+        "vs_baseline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def bench_config1(args) -> int:
     """Plain-method workload: Fermat quintic, degree k = 3 (N = 35), 100,000 SZ points (20,000
     lines x 5 roots, seed 42), FS-Gram, 10 FP64 Donaldson T-operator iterations from h = I, then
@@ -646,7 +730,7 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload",
-                    choices=["tstep", "galerkin", "batch", "adjoint", "route34", "config1"],
+                    choices=["tstep", "galerkin", "batch", "adjoint", "route34", "config1", "cicy"],
                     default="tstep",
                     help="tstep: the headline T-operator benchmark; galerkin: the Laplacian and "
                          "Yukawa assembly on a balanced k=8 Dwork metric; batch: 32 Dwork "
@@ -654,7 +738,8 @@ def main() -> int:
                          "adjoint: sigma^2 gradient, finite-difference check and 200 Adam steps "
                          "at k=5, 400k points; route34: the 8-coordinate T-operator and sigma "
                          "contracts at n_basis 87, 200k points; config1: the plain k=3 protocol "
-                         "with sigma on a separate 50k sample, and its full-size CPU run")
+                         "with sigma on a separate 50k sample, and its full-size CPU run; cicy: "
+                         "the P5.10 Tian-Yau vs Schoen per-seed solves (k=3, 25k points)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy kernel of this many MiB (3 "
                          "launches): the known-size reference for scaling rocprofv3 "
@@ -688,6 +773,8 @@ def main() -> int:
         return bench_route34(args)
     if args.workload == "config1":
         return bench_config1(args)
+    if args.workload == "cicy":
+        return bench_cicy(args)
 
     import numpy as np
     import papers_amd as P

@@ -309,6 +309,35 @@ int build_syrk_segment_plan(cy3q_solver *s, SyrkPlan &pl, int64_t target_wgs, in
 // contiguous runs to (part, wave) slots of at most max_tiles each. With pad_repeat, a short
 // list is padded with repeats of its first tile (for kernels that read a static number of
 // slots; the reduction skips the repeats).
+// Host rows of `cols` doubles to device rows of pitch `ld` (zero padding after `cols`). A
+// pitched copy straight from pageable host memory goes row by row (about 9 us a row), so the
+// rows cross in one contiguous copy to a staging buffer and are padded on the device.
+int upload_rows_padded(cy3q_solver *s, double *dst, const double *src, int64_t rows,
+                       int64_t cols, int64_t ld) {
+  double *raw = nullptr;
+  ST_TRY(s->alloc(&raw, rows * cols));
+  HIP_TRY(hipMemcpyAsync(raw, src, sizeof(double) * rows * cols, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemsetAsync(dst, 0, sizeof(double) * rows * ld, s->st));
+  HIP_TRY(hipMemcpy2DAsync(dst, sizeof(double) * ld, raw, sizeof(double) * cols,
+                           sizeof(double) * cols, rows, hipMemcpyDeviceToDevice, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(raw);
+  return CY3Q_OK;
+}
+
+// The reverse: device rows of pitch `ld` to packed host rows of `cols` doubles.
+int download_rows_packed(cy3q_solver *s, double *dst, const double *src, int64_t rows,
+                         int64_t cols, int64_t ld) {
+  double *raw = nullptr;
+  ST_TRY(s->alloc(&raw, rows * cols));
+  HIP_TRY(hipMemcpy2DAsync(raw, sizeof(double) * cols, src, sizeof(double) * ld,
+                           sizeof(double) * cols, rows, hipMemcpyDeviceToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(dst, raw, sizeof(double) * rows * cols, hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  s->release(raw);
+  return CY3Q_OK;
+}
+
 int build_syrk_plan(cy3q_solver *s, SyrkPlan &pl, int tile, int waves, int cap_tiles,
                     int64_t target_wgs, int64_t min_chunk, int64_t kp, bool pad_repeat = false,
                     int fixed_tiles = 0) {
@@ -1792,13 +1821,9 @@ int cy3q_solver_get_sections(cy3q_solver *s, double *section_values, double *sec
   if (!s) return fail(CY3Q_ERR_ARG, "NULL solver");
   HIP_TRY(hipSetDevice(s->device));
   const int64_t two_n = 2 * (int64_t)s->nb;
-  if (section_values)
-    HIP_TRY(hipMemcpy2DAsync(section_values, two_n * sizeof(double), s->sv, s->m * sizeof(double),
-                             two_n * sizeof(double), s->n, hipMemcpyDeviceToHost, s->st));
+  if (section_values) ST_TRY(download_rows_packed(s, section_values, s->sv, s->n, two_n, s->m));
   if (section_derivs)
-    HIP_TRY(hipMemcpy2DAsync(section_derivs, two_n * sizeof(double), s->sd, s->m * sizeof(double),
-                             two_n * sizeof(double), s->n * 5, hipMemcpyDeviceToHost, s->st));
-  HIP_TRY(hipStreamSynchronize(s->st));
+    ST_TRY(download_rows_packed(s, section_derivs, s->sd, s->n * 5, two_n, s->m));
   return CY3Q_OK;
 }
 
@@ -1921,9 +1946,7 @@ int cy3q_top_upload_static(cy3q_top *t, const double *section_values, const doub
   HIP_TRY(hipSetDevice(s->device));
   const int64_t n = s->n, m = s->m, two_n = 2 * (int64_t)s->nb;
   // Row p of sv: the caller's 2 n_basis values, then zero padding to m columns.
-  HIP_TRY(hipMemsetAsync(s->sv, 0, sizeof(double) * n * m, s->st));
-  HIP_TRY(hipMemcpy2DAsync(s->sv, sizeof(double) * m, section_values, sizeof(double) * two_n,
-                           sizeof(double) * two_n, n, hipMemcpyHostToDevice, s->st));
+  ST_TRY(upload_rows_padded(s, s->sv, section_values, n, two_n, m));
   HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
   launch_valid_weights(s->st, s->w, n, s->wvalid);
   HIP_TRY(hipGetLastError());
@@ -2208,12 +2231,8 @@ int cy3q_adamgrad_upload_static_inputs(cy3q_adamgrad *g, const double *section_v
   HIP_TRY(hipSetDevice(s->device));
   const int64_t n = s->n, m = s->m, two_n = 2 * (int64_t)s->nb;
   // Rows padded from 2 N to m columns with zeros.
-  HIP_TRY(hipMemsetAsync(s->sv, 0, sizeof(double) * n * m, s->st));
-  HIP_TRY(hipMemsetAsync(s->sd, 0, sizeof(double) * n * 5 * m, s->st));
-  HIP_TRY(hipMemcpy2DAsync(s->sv, sizeof(double) * m, section_values, sizeof(double) * two_n,
-                           sizeof(double) * two_n, n, hipMemcpyHostToDevice, s->st));
-  HIP_TRY(hipMemcpy2DAsync(s->sd, sizeof(double) * m, section_derivs, sizeof(double) * two_n,
-                           sizeof(double) * two_n, n * 5, hipMemcpyHostToDevice, s->st));
+  ST_TRY(upload_rows_padded(s, s->sv, section_values, n, two_n, m));
+  ST_TRY(upload_rows_padded(s, s->sd, section_derivs, n * 5, two_n, m));
   HIP_TRY(hipMemcpyAsync(s->pts, points, sizeof(double) * 10 * n, hipMemcpyHostToDevice, s->st));
   HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
   HIP_TRY(hipMemcpyAsync(s->log_om, log_omega_sq, sizeof(double) * n, hipMemcpyHostToDevice,

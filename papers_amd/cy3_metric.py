@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -150,6 +151,7 @@ class Cy3MetricResult:
     basis_monomials: np.ndarray
     sigma_history: list = field(default_factory=list)
     donaldson_history: list = field(default_factory=list)
+    stage_seconds: dict = field(default_factory=dict)
 
     def summary(self) -> dict:
         """Cy3MetricSummary (cy3_metric_unified.rs:96-124)."""
@@ -161,7 +163,16 @@ class Cy3MetricResult:
 
 def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, damping,
            tikhonov, device: int) -> Cy3MetricResult:
+    clock = [time.perf_counter()]
+    stages = {}
+
+    def lap(name):
+        now = time.perf_counter()
+        stages[name] = now - clock[0]
+        clock[0] = now
+
     pts = sample_points(variety, spec.seed, spec.n_sample, mode, device)
+    lap("sample")
     n = len(pts["weight"])
     if n < spec.n_sample // 2:
         raise RuntimeError(f"insufficient points: got {n}, requested {spec.n_sample}")
@@ -173,16 +184,20 @@ def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, da
         raise RuntimeError("empty section basis")
     if not (math.isfinite(float(np.sum(w))) and float(np.sum(w)) > 0.0):
         raise RuntimeError("non-positive weight sum")
+    lap("basis")
     sv, sd = section_tables(pts["coords"], exps, device)
     frames = affine_chart_frames(variety, pts["coords"], pts["patch"], device)
+    lap("sections_frames")
     top = Cy3DonaldsonGpu(n, nb, device)
     sig = Cy3SigmaGpu(n, nb, device)
     try:
         top.upload_static(sv, w)
         sig.upload_static(sv, sd, frames, pts["omega_sq"], w)
+        lap("contracts_upload")
         res = M8.solve_route34(top.t_operator_raw, sig.compute_sigma, np.eye(nb),
                                np.zeros((nb, nb)), w, spec.max_iter, spec.donaldson_tol,
                                damping=damping, tikhonov=tikhonov)
+        lap("donaldson_loop")
     finally:
         top.close()
         sig.close()
@@ -194,7 +209,7 @@ def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, da
                            sigma_fs_identity=res["sigma_fs_identity"],
                            balanced_h_re=res["h_re"], balanced_h_im=res["h_im"],
                            basis_monomials=exps, sigma_history=hist,
-                           donaldson_history=res["donaldson_history"])
+                           donaldson_history=res["donaldson_history"], stage_seconds=stages)
 
 
 def _damping_mode(adaptive: bool):

@@ -257,7 +257,20 @@ def solve_route34(t_operator, sigma, h_re, h_im, weights, max_iter: int, tol: fl
     eff_max = max_iter
     if alpha0 < 1.0:
         eff_max = max_iter * max(int(math.ceil(2.0 / alpha0)), 2)
-    sig_fn = sigma if sigma is not None else (lambda a, b: float("nan"))
+    def sig_fn(a, b):
+        # The GPU sigma contract contracts the sections with W, W^H W = h, so it rejects an h
+        # that is not positive definite; the reference clamps K at 1e-30 and returns a (large)
+        # number for such an iterate (cy3_sigma_gpu.rs:80-107). sigma never steers the loop,
+        # so the iterate is recorded with sigma NaN and the loop goes on.
+        if sigma is None:
+            return float("nan")
+        try:
+            return sigma(a, b)
+        except Cy3qError as e:
+            if e.code != ERR_NOT_PD:
+                raise
+            return float("nan")
+
     out = {"sigma_fs_identity": sig_fn(h_re, h_im), "donaldson_history": [], "sigma_history": []}
     min_res, min_h, min_it, bad = math.inf, (h_re.copy(), h_im.copy()), 0, 0
     final_res, iters = math.inf, 0

@@ -135,6 +135,34 @@ def test_nan_residual_restores_snapshot(oracle):
     assert res["donaldson_history"][-1] == min(res["donaldson_history"][:2])
 
 
+def test_sigma_of_a_non_positive_definite_iterate_is_nan(oracle):
+    """The sigma contract rejects an h that is not positive definite (ERR_NOT_PD); the loop
+    records NaN for that iterate and runs on, with h and the residuals unchanged."""
+    from papers_amd.quintic import ERR_NOT_PD, Cy3qError
+    sv, w, nb = _problem(seed=4)
+    base = _top(oracle, sv, w)
+    calls = {"n": 0}
+
+    def sigma(hr, hi):
+        calls["n"] += 1
+        if calls["n"] == 3:
+            raise Cy3qError(ERR_NOT_PD, "h is not Hermitian positive definite")
+        return float(calls["n"])
+
+    res = M8.solve_route34(base, sigma, np.eye(nb), np.zeros((nb, nb)), w, max_iter=4, tol=0.0)
+    ref = M8.solve_route34(base, None, np.eye(nb), np.zeros((nb, nb)), w, max_iter=4, tol=0.0)
+    assert res["sigma_fs_identity"] == 1.0
+    assert np.isnan(res["sigma_history"][1]) and res["sigma_history"][2] == 4.0
+    assert res["donaldson_history"] == ref["donaldson_history"]
+    assert np.array_equal(res["h_re"], ref["h_re"])
+
+    def other(hr, hi):
+        raise Cy3qError(-2, "device failure")
+
+    with pytest.raises(Cy3qError):
+        M8.solve_route34(base, other, np.eye(nb), np.zeros((nb, nb)), w, max_iter=2, tol=0.0)
+
+
 def test_tikhonov_schedule_known_answers():
     # tikhonov_shift_resolution_rules (schoen_metric.rs and ty_metric.rs tests)
     assert M8.resolve_tikhonov(None) is None
