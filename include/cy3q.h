@@ -444,6 +444,16 @@ int cy3q_cicy_frames(int variety, int64_t n, const double *coords, const int32_t
 int cy3q_cicy_sections(int64_t n, const double *coords, int64_t n_basis, const uint32_t *exps,
                        int device, double *section_values, double *section_derivs);
 
+/* Sections, derivatives and frames of caller points (as cy3q_cicy_sections and cy3q_cicy_frames
+ * compute them) built on the device straight into the static buffers of a T-operator and a sigma
+ * contract, with the weights and |Omega|^2: the effect of build_ty_frames + the two
+ * upload_static calls of solve_ty_metric (ty_metric.rs:521-545) without passing the tables
+ * through host memory. No counterpart in the reference: its tables are host vectors. Both
+ * contracts must have been created for (n, n_basis) on the same device. */
+int cy3q_cicy_upload_contracts(int variety, int64_t n, const double *coords, const int32_t *patch,
+                               int64_t n_basis, const uint32_t *exps, const double *omega_sq,
+                               const double *weights, cy3q_top *top, cy3q_sig8 *sig);
+
 /* ---- The sigma-gradient kernel contract: AdamGradientKernel (gpu_adam.rs:1533-2194), the
  * object the reference's callers build once per (n_points, n_basis) and pass to
  * sigma_refine_analytic_gpu / sigma_refine_lbfgs_gpu (quintic.rs:6514, 6650, 7637). Static

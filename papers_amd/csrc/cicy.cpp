@@ -546,18 +546,21 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
 
 // The batch sampler of the reference's GPU build: max(4 n_target, 64) lines drawn up front,
 // 70 normals each, one damped Newton attempt per line. Lines are independent, so they run in
-// chunks (in line order) until n_target points are kept.
+// chunks (in line order) until n_target points are kept; the draws of lines past the last one
+// needed are never made (they only advance a generator the reference then drops).
 int sample_lanes(Call &c, uint64_t seed, int64_t n_target, Accepted &acc, int64_t *n_lines) {
   const int64_t nl = std::max<int64_t>(4 * n_target, 64);
-  const int64_t chunk = std::min<int64_t>(nl, (int64_t)1 << 20);
+  const int64_t cap = std::min<int64_t>(nl, (int64_t)1 << 20);
   NormalWindow nw(seed);
   double *d_nrm, *d_x;
   int32_t *d_flag;
-  C_TRY(c.alloc(&d_nrm, (size_t)chunk * 70));
-  C_TRY(c.alloc(&d_x, (size_t)chunk * 16));
-  C_TRY(c.alloc(&d_flag, (size_t)chunk));
-  for (int64_t l0 = 0; l0 < nl && acc.n < n_target; l0 += chunk) {
-    const int64_t m = std::min(chunk, nl - l0);
+  C_TRY(c.alloc(&d_nrm, (size_t)cap * 70));
+  C_TRY(c.alloc(&d_x, (size_t)cap * 16));
+  C_TRY(c.alloc(&d_flag, (size_t)cap));
+  for (int64_t l0 = 0; l0 < nl && acc.n < n_target;) {
+    // the remaining points plus slack (about 97% of lines give a point)
+    const int64_t want = (int64_t)((n_target - acc.n) * 1.08) + 64;
+    const int64_t m = std::min(std::min(cap, want), nl - l0);
     nw.drop_before(70 * l0);
     nw.ensure(70 * (l0 + m));
     C_HIP(hipMemcpyAsync(d_nrm, nw.buf.data() + (70 * l0 - nw.base), sizeof(double) * 70 * m,
@@ -566,6 +569,7 @@ int sample_lanes(Call &c, uint64_t seed, int64_t n_target, Accepted &acc, int64_
     C_HIP(hipGetLastError());
     int64_t used = 0;
     C_TRY(post_and_collect(c, CY3Q_CICY_TY, d_x, d_flag, m, n_target, acc, &used));
+    l0 += m;
   }
   *n_lines = nl;
   return CY3Q_OK;

@@ -2125,6 +2125,62 @@ int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im
   return CY3Q_OK;
 }
 
+// Sections, derivatives and frames of caller points evaluated on the device straight into the
+// static buffers of a T-operator and a sigma contract: the same arrays cy3q_cicy_sections and
+// cy3q_cicy_frames return, without the round trip through host memory.
+int cy3q_cicy_upload_contracts(int variety, int64_t n, const double *coords, const int32_t *patch,
+                               int64_t n_basis, const uint32_t *exps, const double *omega_sq,
+                               const double *weights, cy3q_top *top, cy3q_sig8 *sig) {
+  if (!coords || !patch || !exps || !omega_sq || !weights || !top || !sig)
+    return fail(CY3Q_ERR_ARG, "NULL argument");
+  if (variety != CY3Q_CICY_TY && variety != CY3Q_CICY_SCHOEN)
+    return fail(CY3Q_ERR_ARG, "variety must be CY3Q_CICY_TY or CY3Q_CICY_SCHOEN");
+  cy3q_solver *s = top->s, *s2 = sig->s;
+  if (s->n != n || s2->n != n || s->nb != n_basis || s2->nb != n_basis)
+    return fail(CY3Q_ERR_ARG, "contract sizes differ from n / n_basis");
+  if (s->device != s2->device) return fail(CY3Q_ERR_ARG, "contracts on different devices");
+  for (int64_t i = 0; i < 8 * n_basis; i++)
+    if (exps[i] > 64) return fail(CY3Q_ERR_ARG, "exponent above 64");
+  for (int64_t p = 0; p < n; p++) {
+    const int32_t *q = patch + 3 * p;
+    const bool good = variety == CY3Q_CICY_TY
+                          ? (q[0] >= 0 && q[0] < 4 && q[1] >= 0 && q[1] < 4)
+                          : (q[0] >= 0 && q[0] < 3 && q[1] >= 3 && q[1] < 6 && q[2] >= 6 && q[2] < 8);
+    if (!good) return fail(CY3Q_ERR_ARG, "patch index out of range at point " + std::to_string(p));
+  }
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t m = s->m, two_n = 2 * n_basis;
+  double *d_c = nullptr, *sv_raw = nullptr, *sd_raw = nullptr;
+  int32_t *d_p = nullptr;
+  uint32_t *d_e = nullptr;
+  ST_TRY(s->alloc(&d_c, n * 16));
+  ST_TRY(s->alloc(&d_p, n * 3));
+  ST_TRY(s->alloc(&d_e, n_basis * 8));
+  ST_TRY(s->alloc(&sv_raw, n * two_n));
+  ST_TRY(s->alloc(&sd_raw, n * 8 * two_n));
+  HIP_TRY(hipMemcpyAsync(d_c, coords, sizeof(double) * 16 * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(d_p, patch, sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(d_e, exps, sizeof(uint32_t) * 8 * n_basis, hipMemcpyHostToDevice, s->st));
+  launch_cicy_sections(s->st, d_c, n, d_e, (int)n_basis, sv_raw, sd_raw);
+  // T-operator: rows padded to m columns, weights and their validity.
+  HIP_TRY(hipMemsetAsync(s->sv, 0, sizeof(double) * n * m, s->st));
+  HIP_TRY(hipMemcpy2DAsync(s->sv, sizeof(double) * m, sv_raw, sizeof(double) * two_n,
+                           sizeof(double) * two_n, n, hipMemcpyDeviceToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  launch_valid_weights(s->st, s->w, n, s->wvalid);
+  // sigma: the nine packed rows per point, frames, |Omega|^2, weights.
+  launch_pack_rows9(s->st, sv_raw, sd_raw, n, two_n, s2->m, sig->x9);
+  launch_cicy_frames(s->st, variety, d_c, d_p, n, sig->frames);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(sig->om, omega_sq, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemcpyAsync(s2->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  for (void *q : {(void *)d_c, (void *)d_p, (void *)d_e, (void *)sv_raw, (void *)sd_raw}) s->release(q);
+  top->uploaded = true;
+  sig->uploaded = true;
+  return CY3Q_OK;
+}
+
 }  // extern "C"
 
 // ---- The sigma-gradient kernel contract: AdamGradientKernel (gpu_adam.rs:1533-2194) ----------

@@ -36,7 +36,7 @@ from .quintic import _check, _load
 
 __all__ = ["CICY_SCHOEN", "CICY_TY", "Cy3MetricResult", "Cy3MetricSpec", "SchoenSolver",
            "TianYauSolver", "affine_chart_frames", "invariant_reduced_basis", "sample_points",
-           "section_tables"]
+           "section_tables", "upload_contracts"]
 
 CICY_TY, CICY_SCHOEN = 0, 1
 _MODES = {"restarts": 0, "lanes": 1}
@@ -53,6 +53,7 @@ def _lib():
         L.cy3q_cicy_basis.argtypes = [ip, vp, vp, i64, C.POINTER(i64)]
         L.cy3q_cicy_frames.argtypes = [ip, i64, vp, vp, ip, vp]
         L.cy3q_cicy_sections.argtypes = [i64, vp, i64, vp, ip, vp, vp]
+        L.cy3q_cicy_upload_contracts.argtypes = [ip, i64, vp, vp, i64, vp, vp, vp, vp, vp]
         _bound = True
     return L
 
@@ -111,6 +112,21 @@ def section_tables(coords: np.ndarray, exps: np.ndarray, device: int = 0):
     sd = np.zeros((n, 8, 2 * nb))
     _check(_lib().cy3q_cicy_sections(n, _p(c), nb, _p(e), int(device), _p(sv), _p(sd)))
     return sv, sd
+
+
+def upload_contracts(variety: int, coords: np.ndarray, patch: np.ndarray, exps: np.ndarray,
+                     omega_sq: np.ndarray, weights: np.ndarray, top: Cy3DonaldsonGpu,
+                     sig: Cy3SigmaGpu):
+    """Sections, derivatives and frames built on the GPU straight into both contracts' static
+    buffers (cy3q_cicy_upload_contracts): the same inputs as section_tables + affine_chart_frames
+    + the two upload_static calls, without the tables passing through host memory."""
+    c = np.ascontiguousarray(coords, np.float64).reshape(-1, 16)
+    pt = np.ascontiguousarray(patch, np.int32).reshape(-1, 3)
+    e = np.ascontiguousarray(exps, np.uint32).reshape(-1, 8)
+    om = np.ascontiguousarray(omega_sq, np.float64)
+    w = np.ascontiguousarray(weights, np.float64)
+    _check(_lib().cy3q_cicy_upload_contracts(int(variety), len(c), _p(c), _p(pt), len(e), _p(e),
+                                             _p(om), _p(w), top._h, sig._h))
 
 
 @dataclass
@@ -185,15 +201,11 @@ def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, da
     if not (math.isfinite(float(np.sum(w))) and float(np.sum(w)) > 0.0):
         raise RuntimeError("non-positive weight sum")
     lap("basis")
-    sv, sd = section_tables(pts["coords"], exps, device)
-    frames = affine_chart_frames(variety, pts["coords"], pts["patch"], device)
-    lap("sections_frames")
     top = Cy3DonaldsonGpu(n, nb, device)
     sig = Cy3SigmaGpu(n, nb, device)
     try:
-        top.upload_static(sv, w)
-        sig.upload_static(sv, sd, frames, pts["omega_sq"], w)
-        lap("contracts_upload")
+        upload_contracts(variety, pts["coords"], pts["patch"], exps, pts["omega_sq"], w, top, sig)
+        lap("sections_frames_upload")
         res = M8.solve_route34(top.t_operator_raw, sig.compute_sigma, np.eye(nb),
                                np.zeros((nb, nb)), w, spec.max_iter, spec.donaldson_tol,
                                damping=damping, tikhonov=tikhonov)

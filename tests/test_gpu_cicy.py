@@ -98,3 +98,32 @@ def test_solver_reproduces_determinism_golden(cy3q, co, cand):
     assert abs(r.sigma_fs_identity / s_fs - 1) < 1e-12
     assert abs(s["final_sigma_residual"] / s_fin - 1) < 1e-8
     assert abs(s["final_donaldson_residual"] / res - 1) < 1e-7
+
+
+@pytest.mark.parametrize("variety,mode,deg", [(0, "lanes", [3]), (1, "restarts", [3, 3, 1])])
+def test_device_upload_equals_host_upload(cy3q, variety, mode, deg):
+    """cy3q_cicy_upload_contracts (tables built on the GPU into the contracts) gives the same
+    T-operator and sigma, bit for bit, as the host tables through upload_static."""
+    from papers_amd import cy3_metric as CM
+    from papers_amd.donaldson_gpu import Cy3DonaldsonGpu, Cy3SigmaGpu
+    print(f"device upload variety={variety}", flush=True)
+    pts = CM.sample_points(variety, 17, 2000, mode)
+    exps = CM.invariant_reduced_basis(variety, deg)
+    n, nb = len(pts["weight"]), len(exps)
+    w = pts["weight"] / 3.0
+    sv, sd = CM.section_tables(pts["coords"], exps)
+    fr = CM.affine_chart_frames(variety, pts["coords"], pts["patch"])
+    top_h, sig_h = Cy3DonaldsonGpu(n, nb), Cy3SigmaGpu(n, nb)
+    top_h.upload_static(sv, w)
+    sig_h.upload_static(sv, sd, fr, pts["omega_sq"], w)
+    top_d, sig_d = Cy3DonaldsonGpu(n, nb), Cy3SigmaGpu(n, nb)
+    CM.upload_contracts(variety, pts["coords"], pts["patch"], exps, pts["omega_sq"], w, top_d,
+                        sig_d)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((nb, nb)) + 1j * rng.standard_normal((nb, nb))
+    h = a.conj().T @ a / nb + np.eye(nb)
+    for x, y in zip(top_h.t_operator_raw(h.real, h.imag), top_d.t_operator_raw(h.real, h.imag)):
+        assert np.array_equal(x, y)
+    assert sig_h.compute_sigma(h.real, h.imag) == sig_d.compute_sigma(h.real, h.imag)
+    for o in (top_h, sig_h, top_d, sig_d):
+        o.close()
