@@ -11,8 +11,8 @@
 // W comes from one blocked elimination of the augmented matrix [T | I] in 32 x 32 complex tiles.
 // For each block column J:
 //   k_elim_panel  (1 workgroup): factors the diagonal tile and inverts it in the same 32-step
-//                 sweep (one barrier per column; the tile lives in registers and only the
-//                 pivot column and row go through LDS): W_JJ = L_JJ^-1.
+//                 sweep (one barrier per column; the tile lives in registers, one complex value
+//                 per element, and only the pivot column and row go through LDS): W_JJ = L_JJ^-1.
 //   k_elim_update (one workgroup per tile): L_IJ = A_IJ L_JJ^-H, then A_IK -= L_IJ L_KJ^H for
 //                 J < K <= I, and R_IK -= L_IJ W_JK for K <= J, where R is the updated right
 //                 half and W_JK = L_JJ^-1 R_JK (K < J) is formed in the same kernel. The tile
@@ -176,81 +176,70 @@ __device__ inline double rsqrt_f64(double x) {
 // Factors the diagonal tile of block column jblk (in LDS, identity outside the matrix) and
 // inverts it: W_JJ = E = L_JJ^-1 into einv and w. 256 threads; colb, rowb and e are LDS
 // scratch. A bad pivot sets status = ST_ERR_NOT_PD (later kernels return early).
+// Thread tid holds the elements (i, k) = (i0 + 8 u, k), i0 = tid / 32, k = tid % 32, u = 0..3,
+// each as one complex value: the left element A_ik up to step k, where column k is the pivot
+// column and is published, then the right element R_ik (still the identity's then: R_ik
+// changes only at steps j >= k). So every step updates each element by one complex product,
+// x -= d_ij b_k / d_jj, with b_k = conj(d_kj) (k > j) or r_jk (k <= j), with no selects.
+// The 32 steps are unrolled (static buffer, row and register indices): about 120 instructions
+// per wave and step. A step is a dependent chain (publish, barrier, read, 1/sqrt, update, next
+// publish) whose LDS round trips set its length, about 1,050 cycles on MI355X; each step
+// delivers the pivot column to all 1,024 elements as broadcast reads.
 __device__ __attribute__((always_inline)) inline void panel_core(const Cz (*tile)[TB + 1], int n, int jblk, double *__restrict__ einv,
                            double *__restrict__ w, int32_t *__restrict__ status,
                            Cz (*colb)[TB], Cz (*rowb)[TB], Cz (*e)[TB + 1]) {
   const int tid = threadIdx.x;
-  // Thread tid owns elements (i, k) = (tid / 32 + 8 u, tid % 32), u = 0..3, of the left tile
-  // A_JJ and of the right tile (starts as I), in registers.
   const int k = tid % TB, i0 = tid / TB;
-  Cz dv[4], rv[4];
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // rows i0 = 2 wv, 2 wv + 1
+  Cz x[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int i = i0 + 8 * u;
-    dv[u] = tile[i][k];
-    rv[u] = Cz{i == k ? 1.0 : 0.0, 0.0};
+  for (int u = 0; u < 4; u++) x[u] = tile[i0 + 8 * u][k];
+  // Pivots past the matrix (the last block column) belong to the identity padding: nothing
+  // couples them to the matrix, so their rows of E are unit rows and their steps are skipped.
+  const int jmax = n - jblk * TB < TB ? n - jblk * TB : TB;
+  for (int q = tid; q < TB * TB; q += 256) {
+    const int ii = q / TB, kk = q % TB;
+    e[ii][kk] = Cz{(ii == kk && ii >= jmax) ? 1.0 : 0.0, 0.0};
   }
-  for (int q = tid; q < TB * TB; q += 256) e[q / TB][q % TB] = Cz{0.0, 0.0};
-  // Step j: l_jj = sqrt(d_jj); row j of L^-1 is rr_j / l_jj; for i > j the trailing left part
-  // loses d_ij conj(d_kj) / d_jj (j < k <= i) and the right part loses d_ij rr_jk / d_jj
-  // (k <= j). The owners publish column j of d and row j of rr, then one barrier; every thread
-  // reads the pivot and its operands from those two short vectors and updates its registers.
-  // Buffer j & 1 is rewritten at step j + 2, after the barrier of step j + 1, so its readers
-  // at step j are done.
-  // The pivot loop runs as 4 passes of 8 unrolled steps (j = 8 jo + ji), which keeps the
-  // kernels that inline this at about 80 VGPRs with no scratch.
+  // Step j: l_jj = sqrt(d_jj); row j of L^-1 is r_j / l_jj; rows i > j lose d_ij b_k / d_jj.
+  // (Left elements above the diagonal change too; they are never read.) Buffer j & 1 is
+  // rewritten at step j + 2, after the barrier of step j + 1, so its readers at step j are done.
   bool ok = true;
-#pragma unroll 1
-  for (int jo = 0; jo < TB / 8; jo++)
 #pragma unroll
-  for (int ji = 0; ji < 8; ji++) {
-    const int j = 8 * jo + ji;
-    const int b = ji & 1;
-    if (k == j) {
+  for (int j = 0; j < TB; j++) {
+    if (j < jmax) {
+      const int b = j & 1, ji = j % 8, jo = j / 8;
+      if (k == j) {  // publish column j of the left tile, then hold the right element
 #pragma unroll
-      for (int u = 0; u < 4; u++) colb[b][i0 + 8 * u] = dv[u];
-    }
-    if (i0 == ji) {  // rv[jo], by a uniform branch (an indexed read would go through scratch)
-      if (jo == 0)
-        rowb[b][k] = rv[0];
-      else if (jo == 1)
-        rowb[b][k] = rv[1];
-      else if (jo == 2)
-        rowb[b][k] = rv[2];
-      else
-        rowb[b][k] = rv[3];
-    }
-    __syncthreads();
-    // All operand reads of the step are issued together, before the pivot test, so one LDS
-    // wait covers the step.
-    const double piv = colb[b][j].r;
-    const Cz ck = colb[b][k], rk = rowb[b][k], xe = rowb[b][tid & (TB - 1)];
-    Cz dl[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) dl[u] = colb[b][i0 + 8 * u];
-    // A bad pivot (the same for every thread) marks the factorisation failed; the remaining
-    // steps run on a unit pivot, without a branch, and their results are discarded.
-    ok = ok && piv > 0.0 && isfinite(piv);
-    const double inv_l = rsqrt_f64(ok ? piv : 1.0), ipiv = inv_l * inv_l;
-    if (tid <= j) e[j][tid] = Cz{xe.r * inv_l, xe.i * inv_l};
-    // Row i = i0 + 8 u changes only while i > j, and then in one of its two elements: the left
-    // one (i, k) if j < k <= i (it loses d_ij conj(d_kj) / d_jj), else the right one if k <= j
-    // (it loses d_ij rr_jk / d_jj). So each active element forms one complex product, and a
-    // wave skips a row group by a branch once both its rows (i0 = 2 w, 2 w + 1) are final: the
-    // step's issue shrinks with j instead of updating all eight elements through selects.
-    const bool lf = k > j;
-    const Cz bop = lf ? Cz{ck.r, -ck.i} : rk;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int i = i0 + 8 * u;
-      if (i > j) {
-        const Cz pp = cmul(dl[u], bop);
-        if (lf) {
-          if (k <= i) dv[u] = Cz{dv[u].r - pp.r * ipiv, dv[u].i - pp.i * ipiv};
-        } else {
-          rv[u] = Cz{rv[u].r - pp.r * ipiv, rv[u].i - pp.i * ipiv};
+        for (int u = 0; u < 4; u++) {
+          colb[b][i0 + 8 * u] = x[u];
+          x[u] = Cz{i0 + 8 * u == j ? 1.0 : 0.0, 0.0};
         }
       }
+      if (i0 == ji) rowb[b][k] = k <= j ? x[jo] : Cz{0.0, 0.0};  // row j of the right tile
+      __syncthreads();
+      // Row group u of this wave (rows 8 u + 2 w, 8 u + 2 w + 1) is final once
+      // 8 u + 2 w + 1 <= j: its d_ij is neither read nor used.
+      const bool lf = k > j;
+      const double piv = colb[b][j].r;
+      Cz bk = lf ? colb[b][k] : rowb[b][k];
+      Cz dl[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (8 * u + 2 * wv + 1 > j) dl[u] = colb[b][i0 + 8 * u];
+      // A bad pivot (the same for every thread) marks the factorisation failed; the remaining
+      // steps run on a unit pivot and their results are discarded.
+      ok = ok && piv > 0.0 && isfinite(piv);
+      const double inv_l = rsqrt_f64(ok ? piv : 1.0), ipiv = inv_l * inv_l;
+      if (i0 == ji && !lf) e[j][k] = Cz{bk.r * inv_l, bk.i * inv_l};
+      if (lf) bk.i = -bk.i;
+      const Cz bs = Cz{bk.r * ipiv, bk.i * ipiv};
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (8 * u + 2 * wv + 1 > j && i0 + 8 * u > j) {
+          x[u].r = fma(-dl[u].r, bs.r, fma(dl[u].i, bs.i, x[u].r));
+          x[u].i = fma(-dl[u].r, bs.i, fma(-dl[u].i, bs.r, x[u].i));
+        }
     }
   }
   __syncthreads();
