@@ -281,6 +281,14 @@ int cy3q_solver_laplacian_spectrum(cy3q_solver *s, uint32_t section_degree, int6
 int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double tol, int device,
                          double *eig, double *vecs, int64_t *sweeps);
 
+/* (T + shift I)^-1 for the 8-coordinate Donaldson loop, where the reference inverts the 2N x 2N
+ * real block of T (with its Tikhonov shift on the diagonal) by LU (pwos_math::linalg::invert
+ * at route34/ty_metric.rs:1241 and route34/schoen_metric.rs:1534). t, out: [n][n] complex,
+ * host memory; t Hermitian. Computed on `device` through the inverse Cholesky factor (W^H W
+ * with W = L^-1); CY3Q_ERR_NOT_PD (and out unspecified) when T + shift I is not numerically
+ * positive definite, where an LU may still invert it: callers keep the LU for that case. */
+int cy3q_hermitian_inverse(const double *t, int64_t n, double shift, int device, double *out);
+
 /* No counterpart in the reference. cy3q_hermitian_eigen without any device allocation, for
  * callers that cannot allocate (code under graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): d_a ([n][n] complex, read only), d_eig (n values, ascending) and d_vecs

@@ -585,6 +585,65 @@ int cy3q_hermitian_eigen(const double *a, int64_t n, int64_t max_sweeps, double 
   return CY3Q_OK;
 }
 
+// (T + shift I)^-1 of a Hermitian positive definite T through the inverse Cholesky factor
+// (k_hsolve.hip): W = L^-1, T + shift I = L L^H, inverse = W^H W. One workspace and stream per
+// device, grown to the largest n seen and kept until the process ends; calls on one device
+// are serialised by its mutex.
+namespace {
+struct InvWorkspace {
+  std::mutex mu;
+  hipStream_t st = nullptr;
+  int64_t cap = 0;
+  double *a = nullptr, *r = nullptr, *einv = nullptr, *w = nullptr, *g = nullptr;
+  int32_t *status = nullptr;
+};
+constexpr int kInvDevices = 64;
+InvWorkspace g_inv_ws[kInvDevices];
+}  // namespace
+
+int cy3q_hermitian_inverse(const double *t, int64_t n, double shift, int device, double *out) {
+  if (!t || !out || n <= 0 || n > 4096) return set_error(CY3Q_ERR_ARG, "bad matrix arguments");
+  if (device < 0 || device >= kInvDevices) return set_error(CY3Q_ERR_ARG, "bad device index");
+  if (!std::isfinite(shift)) return set_error(CY3Q_ERR_ARG, "shift must be finite");
+  InvWorkspace &ws = g_inv_ws[device];
+  std::lock_guard<std::mutex> lock(ws.mu);
+  G_HIP(hipSetDevice(device));
+  if (!ws.st) G_HIP(hipStreamCreate(&ws.st));
+  const int64_t nblk = (n + 31) / 32;
+  if (ws.cap < n) {
+    for (void *p : {(void *)ws.a, (void *)ws.r, (void *)ws.einv, (void *)ws.w, (void *)ws.g,
+                    (void *)ws.status})
+      if (p) G_HIP(hipFree(p));
+    ws.a = ws.r = ws.einv = ws.w = ws.g = nullptr;
+    ws.status = nullptr;
+    ws.cap = 0;
+    const size_t mat = sizeof(double) * 2 * n * n;
+    G_HIP(hipMalloc(&ws.a, mat));
+    G_HIP(hipMalloc(&ws.r, mat));
+    G_HIP(hipMalloc(&ws.w, mat));
+    G_HIP(hipMalloc(&ws.g, mat));
+    G_HIP(hipMalloc(&ws.einv, sizeof(double) * 2 * 32 * 32 * nblk));
+    G_HIP(hipMalloc(&ws.status, sizeof(int32_t)));
+    ws.cap = n;
+  }
+  const size_t bytes = sizeof(double) * 2 * n * n;
+  G_HIP(hipMemcpyAsync(ws.g, t, bytes, hipMemcpyHostToDevice, ws.st));
+  launch_elim_init(ws.st, ws.g, (int)n, ws.a, ws.r);
+  if (shift != 0.0) launch_add_diag(ws.st, reinterpret_cast<double2 *>(ws.a), (int)n, shift);
+  launch_fill_f64(ws.st, ws.w, 2 * n * n, 0.0);
+  launch_inverse_cholesky(ws.st, ws.a, ws.r, ws.einv, ws.w, (int)n, ws.status);
+  launch_gram_residual(ws.st, ws.w, (int)n, ws.g, ws.status, nullptr, nullptr, nullptr);
+  G_HIP(hipGetLastError());
+  int32_t st_h = -1;
+  G_HIP(hipMemcpyAsync(&st_h, ws.status, sizeof(st_h), hipMemcpyDeviceToHost, ws.st));
+  G_HIP(hipMemcpyAsync(out, ws.g, bytes, hipMemcpyDeviceToHost, ws.st));
+  G_HIP(hipStreamSynchronize(ws.st));
+  if (st_h != 0)
+    return set_error(CY3Q_ERR_NOT_PD, "cy3q_hermitian_inverse: the matrix (with its shift) is "
+                                      "not numerically positive definite");
+  return CY3Q_OK;
+}
+
 int cy3q_hermitian_eigen_noalloc(const double *d_a, int64_t n, int64_t max_sweeps, double tol,
                                  int device, double *d_eig, double *d_vecs, int64_t *sweeps) {
   if (!d_a || !d_eig || n <= 0) return set_error(CY3Q_ERR_ARG, "bad matrix arguments");

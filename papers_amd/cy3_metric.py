@@ -206,9 +206,12 @@ def _solve(variety: int, label: str, degrees, spec: Cy3MetricSpec, mode: str, da
     try:
         upload_contracts(variety, pts["coords"], pts["patch"], exps, pts["omega_sq"], w, top, sig)
         lap("sections_frames_upload")
+        # The loop's inverse on the GPU from n_basis 48 up (smaller blocks invert faster on
+        # the host than a round trip takes).
+        inverse = M8.gpu_inverse(device) if nb >= 48 else None
         res = M8.solve_route34(top.t_operator_raw, sig.compute_sigma, np.eye(nb),
                                np.zeros((nb, nb)), w, spec.max_iter, spec.donaldson_tol,
-                               damping=damping, tikhonov=tikhonov)
+                               damping=damping, tikhonov=tikhonov, inverse=inverse)
         lap("donaldson_loop")
     finally:
         top.close()

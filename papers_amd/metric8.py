@@ -27,7 +27,9 @@ import math
 import numpy as np
 import scipy.linalg as sla
 
-from .quintic import ERR_NOT_PD, Cy3qError
+import ctypes as C
+
+from .quintic import ERR_NOT_PD, Cy3qError, _check, _load
 
 __all__ = ["AdaptiveDamping", "GatingState", "TikhonovGating", "TikhonovShift",
            "auto_schoen_gated_tikhonov", "auto_schoen_tikhonov", "auto_ty_gated_tikhonov",
@@ -184,11 +186,37 @@ class AdaptiveDamping:
         self.last_residual = residual
 
 
+def gpu_inverse(device: int = 0):
+    """An `inverse` for the loops below: (T + shift I)^-1 of the Hermitian T = tr + i ti on the
+    GPU (cy3q_hermitian_inverse), as (Re, Im), or None when T + shift I is not numerically
+    positive definite (the loop then inverts the real block by LU, as the reference does)."""
+    L = _load()
+    if not getattr(L, "_cy3q_inv_bound", False):
+        L.cy3q_hermitian_inverse.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int,
+                                             C.c_void_p]
+        L._cy3q_inv_bound = True
+
+    def inverse(tr_, ti_, shift):
+        n = tr_.shape[0]
+        t = np.empty((n, n, 2))
+        t[..., 0], t[..., 1] = tr_, ti_
+        out = np.empty((n, n, 2))
+        rc = L.cy3q_hermitian_inverse(t.ctypes.data, n, float(shift), device, out.ctypes.data)
+        if rc == ERR_NOT_PD:
+            return None
+        _check(rc)
+        return out[..., 0], out[..., 1]
+
+    return inverse
+
+
 def donaldson_iteration_route34(t_operator, h_re, h_im, weights, damping: float = 1.0,
-                                tikhonov: float = 0.0):
+                                tikhonov: float = 0.0, inverse=None):
     """One iteration; returns (h_re, h_im, residual). On a non-finite weight sum, a singular or
     non-finite inverse, h comes back unchanged with residual NaN (the reference returns NaN
-    before its damped update)."""
+    before its damped update). inverse: None (LU of the 2N x 2N real block, as the reference)
+    or a callable(tr, ti, shift) -> (Re, Im) of (T + shift I)^-1, or None where it cannot
+    invert (then the LU runs), e.g. gpu_inverse(device)."""
     h_re = np.asarray(h_re, dtype=np.float64)
     h_im = np.asarray(h_im, dtype=np.float64)
     n = h_re.shape[0]
@@ -214,23 +242,32 @@ def donaldson_iteration_route34(t_operator, h_re, h_im, weights, damping: float 
     tr = float(np.trace(tr_))
     if abs(tr) > 1e-30:
         tr_, ti_ = tr_ * (n / tr), ti_ * (n / tr)
-    block = np.zeros((2 * n, 2 * n))
-    block[0::2, 0::2], block[0::2, 1::2] = tr_, -ti_
-    block[1::2, 0::2], block[1::2, 1::2] = ti_, tr_
-    if math.isfinite(tikhonov) and tikhonov > 0.0:
-        block[np.diag_indices(2 * n)] += tikhonov
-    try:
-        lu, piv = sla.lu_factor(block, check_finite=False)
-    except (ValueError, np.linalg.LinAlgError):
-        return h_re, h_im, float("nan")
-    d = np.abs(np.diag(lu))
-    if not np.all(np.isfinite(d)) or np.any(d < np.finfo(np.float64).tiny):
-        return h_re, h_im, float("nan")
-    inv = sla.lu_solve((lu, piv), np.eye(2 * n), check_finite=False)
-    if not np.all(np.isfinite(inv)):
-        return h_re, h_im, float("nan")
-    nr = 0.5 * (inv[0::2, 0::2] + inv[1::2, 1::2])
-    ni = 0.5 * (inv[1::2, 0::2] - inv[0::2, 1::2])
+    shift = tikhonov if math.isfinite(tikhonov) and tikhonov > 0.0 else 0.0
+    # A positive definite T + shift I: its inverse's real block has the unpacked parts
+    # nr = Re, ni = Im exactly.
+    got = inverse(tr_, ti_, shift) if inverse is not None else None
+    if got is not None:
+        nr, ni = got
+        if not (np.all(np.isfinite(nr)) and np.all(np.isfinite(ni))):
+            return h_re, h_im, float("nan")
+    else:
+        block = np.zeros((2 * n, 2 * n))
+        block[0::2, 0::2], block[0::2, 1::2] = tr_, -ti_
+        block[1::2, 0::2], block[1::2, 1::2] = ti_, tr_
+        if shift > 0.0:
+            block[np.diag_indices(2 * n)] += shift
+        try:
+            lu, piv = sla.lu_factor(block, check_finite=False)
+        except (ValueError, np.linalg.LinAlgError):
+            return h_re, h_im, float("nan")
+        d = np.abs(np.diag(lu))
+        if not np.all(np.isfinite(d)) or np.any(d < np.finfo(np.float64).tiny):
+            return h_re, h_im, float("nan")
+        inv = sla.lu_solve((lu, piv), np.eye(2 * n), check_finite=False)
+        if not np.all(np.isfinite(inv)):
+            return h_re, h_im, float("nan")
+        nr = 0.5 * (inv[0::2, 0::2] + inv[1::2, 1::2])
+        ni = 0.5 * (inv[1::2, 0::2] - inv[0::2, 1::2])
     tri = float(np.trace(nr))
     if abs(tri) > 1e-30:
         nr, ni = nr * (n / tri), ni * (n / tri)
@@ -240,14 +277,15 @@ def donaldson_iteration_route34(t_operator, h_re, h_im, weights, damping: float 
 
 
 def solve_route34(t_operator, sigma, h_re, h_im, weights, max_iter: int, tol: float,
-                  damping=None, tikhonov=0.0) -> dict:
+                  damping=None, tikhonov=0.0, inverse=None) -> dict:
     """The Donaldson loop of solve_ty_metric. damping: None or a float (static alpha, clamped to
     [1e-6, 1]) or an AdaptiveDamping. tikhonov: a constant shift (0 disables it) or a
     TikhonovShift; a schedule's lambda for an iteration is lambda_at(previous residual, first
     finite residual) while its gate is open, else 0 (ty_metric.rs:620-668). sigma:
     callable(h_re, h_im) or None. Returns a dict with h_re, h_im, iterations_run,
     final_donaldson_residual, donaldson_history, sigma_history, sigma_fs_identity and
-    tikhonov_history (the lambda of each iteration)."""
+    tikhonov_history (the lambda of each iteration). inverse: as in
+    donaldson_iteration_route34."""
     h_re = np.array(h_re, dtype=np.float64)
     h_im = np.array(h_im, dtype=np.float64)
     adaptive = damping if isinstance(damping, AdaptiveDamping) else None
@@ -288,7 +326,7 @@ def solve_route34(t_operator, sigma, h_re, h_im, weights, max_iter: int, tol: fl
             lam = 0.0
         out["tikhonov_history"].append(lam)
         h_re_n, h_im_n, r = donaldson_iteration_route34(t_operator, h_re, h_im, weights, alpha,
-                                                        lam)
+                                                        lam, inverse=inverse)
         if adaptive is not None:
             adaptive.update(r)
         if shift is not None and shift.gating.stall_band:

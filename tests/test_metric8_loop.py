@@ -221,7 +221,7 @@ def test_gated_tikhonov_closed_on_healthy_descent():
 def _scripted(monkeypatch, residuals):
     seen = []
 
-    def fake(top, h_re, h_im, w, damping=1.0, tikhonov=0.0):
+    def fake(top, h_re, h_im, w, damping=1.0, tikhonov=0.0, inverse=None):
         seen.append(tikhonov)
         return h_re, h_im, residuals[len(seen) - 1]
 
