@@ -421,6 +421,11 @@ int cy3q_sig8_compute_sigma(cy3q_sig8 *g, const double *h_re, const double *h_im
 #define CY3Q_CICY_RESTARTS 0 /* the sequential samplers: up to 4 Newton restarts per line */
 #define CY3Q_CICY_LANES 1    /* TY only: the batch sampler of the reference's GPU build */
 
+/* The samplers' normal stream: rand_distr 0.5 StandardNormal over rand_chacha 0.9 ChaCha8Rng
+ * seeded with seed_from_u64(seed) (cicy_sampler.rs:136-138, route34/schoen_sampler.rs:980-998),
+ * normals 0 .. count - 1 into out (host memory), drawn on `device` as the samplers draw them. */
+int cy3q_cicy_normals(uint64_t seed, int64_t count, int device, double *out);
+
 /* Sample n_target points with the reference's ChaCha8 + StandardNormal stream for `seed`.
  *   CY3Q_CICY_RESTARTS: CicySampler::sample_batch (cicy_sampler.rs:591-617, the CPU build) or
  *     SchoenSampler::sample_points (route34/schoen_sampler.rs:482-498);

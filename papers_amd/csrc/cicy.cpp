@@ -5,8 +5,10 @@
 // The random stream is the reference's: rand_chacha 0.9 ChaCha8Rng seeded through rand_core
 // 0.9's seed_from_u64 (eight PCG32 outputs as the key), read as u64 = lo | hi << 32, and
 // rand_distr 0.5's StandardNormal ziggurat on it (cy3/src/cicy_sampler.rs:136-138,
-// route34/schoen_sampler.rs:980-998). The host draws the normals (the ziggurat's rejection
-// loop makes the stream sequential) and the GPU does every line's Newton solves.
+// route34/schoen_sampler.rs:980-998). The GPU evaluates the ziggurat's first test at every
+// position of the keystream; the host follows the stream through the positions where it fails
+// (about 1 in 80, with the wedge and tail in its own exp and log) and the GPU gathers the
+// normals (DeviceNormals). The GPU does every line's Newton solves.
 //
 // The section bases keep the (k, k) / (d_x, d_y, d_t) monomials that are invariant under the
 // quotient group and lie outside the ideal of leading monomials of the reduced Groebner basis
@@ -95,29 +97,40 @@ class NormalStream {
       key_[i] = (xs >> rot) | (xs << ((32 - rot) & 31));
     }
   }
+  const uint32_t *key() const { return key_; }
 
-  double normal() {
+  // The normal that starts at u64 position e of the stream, given that its first test fails
+  // there (|u x_i| >= x_(i+1), u = 2 (bits >> 12) / 2^52 - 1, i = bits & 0xff): for i = 0 the
+  // tail beyond R (pairs of uniforms in (0, 1] until -2 log b >= (log a / R)^2), else the wedge
+  // test f_(i+1) + (f_i - f_(i+1)) v < exp(-x^2 / 2) with v from the next word. *words: the u64s
+  // it takes; a rejected wedge (*accepted false) takes 2, and the stream's next attempt starts
+  // after them.
+  void slow_at(uint64_t e, bool *accepted, double *value, int64_t *words) {
     const ZigTables &z = zig_tables();
-    for (;;) {
-      const uint64_t bits = next_u64();
-      const int i = (int)(bits & 0xff);
-      const double u = with_exponent(bits >> 12, 1) - 3.0;  // [-1, 1)
-      const double x = u * z.x[i];
-      if (std::fabs(x) < z.x[i + 1]) return x;
-      if (i == 0) {  // the tail beyond R
-        const double R = 3.654152885361008796;
-        double xt = 1.0, yt = 0.0;
-        while (-2.0 * yt < xt * xt) {
-          const double a = with_exponent(next_u64() >> 12, 0) - (1.0 - 0x1p-53);
-          const double b = with_exponent(next_u64() >> 12, 0) - (1.0 - 0x1p-53);
-          xt = std::log(a) / R;
-          yt = std::log(b);
-        }
-        return u < 0.0 ? xt - R : R - xt;
+    const uint64_t bits = u64_at(e);
+    const int i = (int)(bits & 0xff);
+    const double u = with_exponent(bits >> 12, 1) - 3.0;
+    const double x = u * z.x[i];
+    if (i == 0) {
+      const double R = 3.654152885361008796;
+      double xt = 1.0, yt = 0.0;
+      uint64_t q = e + 1;
+      while (-2.0 * yt < xt * xt) {
+        const double a = with_exponent(u64_at(q) >> 12, 0) - (1.0 - 0x1p-53);
+        const double b = with_exponent(u64_at(q + 1) >> 12, 0) - (1.0 - 0x1p-53);
+        q += 2;
+        xt = std::log(a) / R;
+        yt = std::log(b);
       }
-      const double un = (double)(next_u64() >> 11) * 0x1p-53;
-      if (z.f[i + 1] + (z.f[i] - z.f[i + 1]) * un < std::exp(-x * x / 2.0)) return x;
+      *accepted = true;
+      *value = u < 0.0 ? xt - R : R - xt;
+      *words = (int64_t)(q - e);
+      return;
     }
+    const double un = (double)(u64_at(e + 1) >> 11) * 0x1p-53;
+    *accepted = z.f[i + 1] + (z.f[i] - z.f[i + 1]) * un < std::exp(-x * x / 2.0);
+    *value = x;
+    *words = 2;
   }
 
  private:
@@ -129,14 +142,14 @@ class NormalStream {
   }
   static uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
 
-  // Four ChaCha8 blocks (counter, 64-bit, in words 12-13; stream 0 in words 14-15).
-  void refill() {
-    for (int b = 0; b < 4; b++) {
+  // u64 number q of the stream (block q / 8), with the last block kept.
+  uint64_t u64_at(uint64_t q) {
+    const uint64_t blk = q / 8;
+    if (blk != rblk_) {
       uint32_t s[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
       for (int i = 0; i < 8; i++) s[4 + i] = key_[i];
-      const uint64_t c = counter_ + b;
-      s[12] = (uint32_t)c;
-      s[13] = (uint32_t)(c >> 32);
+      s[12] = (uint32_t)blk;
+      s[13] = (uint32_t)(blk >> 32);
       s[14] = s[15] = 0;
       uint32_t x[16];
       std::memcpy(x, s, sizeof x);
@@ -146,44 +159,22 @@ class NormalStream {
         x[a] += x[bb]; x[d] ^= x[a]; x[d] = rotl(x[d], 8);
         x[cc] += x[d]; x[bb] ^= x[cc]; x[bb] = rotl(x[bb], 7);
       };
-      for (int r = 0; r < 4; r++) {  // 8 rounds = 4 column + diagonal double rounds
+      for (int r = 0; r < 4; r++) {
         qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
         qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
       }
-      for (int i = 0; i < 16; i++) buf_[16 * b + i] = x[i] + s[i];
+      for (int i = 0; i < 16; i++) rbuf_[i] = x[i] + s[i];
+      rblk_ = blk;
     }
-    counter_ += 4;
-    idx_ = 0;
-  }
-  uint64_t next_u64() {
-    if (idx_ >= 64) refill();
-    const uint64_t lo = buf_[idx_], hi = buf_[idx_ + 1];
-    idx_ += 2;
-    return lo | (hi << 32);
+    const int j = (int)(q % 8);
+    return (uint64_t)rbuf_[2 * j] | ((uint64_t)rbuf_[2 * j + 1] << 32);
   }
 
   uint32_t key_[8];
-  uint64_t counter_ = 0;
-  uint32_t buf_[64];
-  int idx_ = 64;
+  uint64_t rblk_ = ~0ull;
+  uint32_t rbuf_[16];
 };
 
-// A growing window of the normal stream: normals [base, base + size()) are held.
-struct NormalWindow {
-  NormalStream rng;
-  std::vector<double> buf;
-  int64_t base = 0;
-  explicit NormalWindow(uint64_t seed) : rng(seed) {}
-  void ensure(int64_t end) {  // hold normals up to index end (exclusive)
-    while (base + (int64_t)buf.size() < end) buf.push_back(rng.normal());
-  }
-  void drop_before(int64_t s) {
-    if (s <= base) return;
-    const int64_t k = std::min<int64_t>(s - base, (int64_t)buf.size());
-    buf.erase(buf.begin(), buf.begin() + k);
-    base += k;
-  }
-};
 
 // ---- Groebner bases over Q ------------------------------------------------------------------
 
@@ -426,6 +417,130 @@ struct Call {
   }
 };
 
+// The normal stream drawn on the device, in order: generate(count, d_out) writes the stream's
+// next `count` normals. Per pass over a range of u64 positions, k_zig_fast evaluates the
+// ziggurat's first test at every position (exact: a subtraction and a product of doubles) and
+// lists the positions where it fails (about 1.2%); the host follows the stream from the
+// current position: a position whose first test passes is a normal of one word, so the starts
+// run on until the next listed position, which the host evaluates itself (wedge or tail,
+// NormalStream::slow_at, with its exp and log); k_zig_gather then writes the normals.
+class DeviceNormals {
+ public:
+  explicit DeviceNormals(uint64_t seed) : host_(seed) {
+    std::memcpy(key_.k, host_.key(), sizeof key_.k);
+  }
+  int64_t generated() const { return generated_; }
+
+  int generate(Call &c, int64_t count, double *d_out) {
+    if (!tables_) {
+      upload_zig_x(c.st, zig_tables().x);
+      tables_ = true;
+    }
+    int64_t produced = 0;
+    while (produced < count) {
+      const int64_t need = count - produced;
+      const int64_t n_pos = need + need / 16 + 256;
+      C_TRY(grow(c, n_pos));
+      C_HIP(hipMemsetAsync(d_nexc_, 0, sizeof(unsigned long long), c.st));
+      launch_zig_fast(c.st, key_, pos_, n_pos, d_fast_, d_exc_, d_nexc_);
+      C_HIP(hipGetLastError());
+      unsigned long long n_exc = 0;
+      C_HIP(hipMemcpyAsync(&n_exc, d_nexc_, sizeof n_exc, hipMemcpyDeviceToHost, c.st));
+      C_HIP(hipStreamSynchronize(c.st));
+      exc_.resize((size_t)n_exc);
+      if (n_exc)
+        C_HIP(hipMemcpyAsync(exc_.data(), d_exc_, sizeof(int64_t) * n_exc, hipMemcpyDeviceToHost,
+                             c.st));
+      C_HIP(hipStreamSynchronize(c.st));
+      std::sort(exc_.begin(), exc_.end());
+      // Follow the stream through [pos_, pos_ + n_pos).
+      run_k_.clear();
+      run_p_.clear();
+      run_len_.clear();
+      ek_.clear();
+      ev_.clear();
+      const int64_t base = produced;
+      const uint64_t end = pos_ + (uint64_t)n_pos;
+      uint64_t p = pos_;
+      size_t ei = 0;
+      while (produced < count && p < end) {
+        while (ei < exc_.size() && (uint64_t)exc_[ei] < p) ei++;
+        const uint64_t e = ei < exc_.size() ? (uint64_t)exc_[ei] : end;
+        const int64_t r = std::min<int64_t>((int64_t)(e - p), count - produced);
+        if (r > 0) {
+          run_k_.push_back(produced - base);
+          run_p_.push_back((int64_t)(p - pos_));
+          run_len_.push_back(r);
+          produced += r;
+          p += (uint64_t)r;
+        }
+        if (produced == count || p >= end) break;
+        bool accepted;
+        double v;
+        int64_t words;
+        host_.slow_at(e, &accepted, &v, &words);
+        if (accepted) {
+          ek_.push_back(produced - base);
+          ev_.push_back(v);
+          produced++;
+        }
+        p = e + (uint64_t)words;
+      }
+      // Runs of one pass number at most its listed positions + 1.
+      const int64_t nr = (int64_t)run_k_.size(), ne = (int64_t)ek_.size();
+      int64_t *d_rk, *d_rp, *d_rl, *d_ek;
+      double *d_ev;
+      C_TRY(c.alloc(&d_rk, (size_t)nr));
+      C_TRY(c.alloc(&d_rp, (size_t)nr));
+      C_TRY(c.alloc(&d_rl, (size_t)nr));
+      C_TRY(c.alloc(&d_ek, (size_t)ne));
+      C_TRY(c.alloc(&d_ev, (size_t)ne));
+      if (nr) {
+        C_HIP(hipMemcpyAsync(d_rk, run_k_.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, c.st));
+        C_HIP(hipMemcpyAsync(d_rp, run_p_.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, c.st));
+        C_HIP(hipMemcpyAsync(d_rl, run_len_.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, c.st));
+      }
+      if (ne) {
+        C_HIP(hipMemcpyAsync(d_ek, ek_.data(), sizeof(int64_t) * ne, hipMemcpyHostToDevice, c.st));
+        C_HIP(hipMemcpyAsync(d_ev, ev_.data(), sizeof(double) * ne, hipMemcpyHostToDevice, c.st));
+      }
+      launch_zig_gather(c.st, d_fast_, d_rk, d_rp, d_rl, nr, d_ek, d_ev, ne, produced - base,
+                        d_out + base);
+      C_HIP(hipGetLastError());
+      for (void *q : {(void *)d_rk, (void *)d_rp, (void *)d_rl, (void *)d_ek, (void *)d_ev})
+        c.release(q);  // synchronises the stream, so the host vectors may change after this
+      pos_ = p;
+    }
+    generated_ += count;
+    return CY3Q_OK;
+  }
+
+ private:
+  int grow(Call &c, int64_t n_pos) {
+    if (n_pos <= cap_) return CY3Q_OK;
+    if (d_fast_) c.release(d_fast_);
+    if (d_exc_) c.release(d_exc_);
+    if (d_nexc_) c.release(d_nexc_);
+    C_TRY(c.alloc(&d_fast_, (size_t)n_pos));
+    C_TRY(c.alloc(&d_exc_, (size_t)n_pos));
+    C_TRY(c.alloc(&d_nexc_, 1));
+    cap_ = n_pos;
+    return CY3Q_OK;
+  }
+
+  NormalStream host_;
+  ZigKey key_;
+  uint64_t pos_ = 0;        // u64 position where the next normal starts
+  int64_t generated_ = 0;   // normals written so far
+  bool tables_ = false;
+  int64_t cap_ = 0;
+  double *d_fast_ = nullptr;
+  int64_t *d_exc_ = nullptr;
+  unsigned long long *d_nexc_ = nullptr;
+  std::vector<int64_t> exc_, run_k_, run_p_, run_len_, ek_;
+  std::vector<double> ev_;
+};
+
 constexpr int kLineNormals[2] = {64, 48};  // origin + one direction per hypersurface, 16 each
 constexpr int kRestartNormals[2] = {6, 4};  // one complex t per hypersurface
 
@@ -486,7 +601,11 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
   const int ln = kLineNormals[variety], rn = kRestartNormals[variety];
   const int64_t tail = ln + 4 * rn;
   const int64_t max_lines = std::max<int64_t>(n_target * 64, 1024);
-  NormalWindow nw(seed);
+  DeviceNormals dn(seed);
+  // The previous window's normals [prev_lo, prev_hi) stay on the device: the next window starts
+  // inside it (where the chain stopped), so its overlap is copied and only the rest is drawn.
+  double *d_prev = nullptr;
+  int64_t prev_lo = 0, prev_hi = 0;
   int64_t s = 0, lines = 0;
   while (acc.n < n_target && lines < max_lines) {
     // Lines per window: the expected remainder plus slack, at most 2^20 (the window then holds
@@ -495,15 +614,25 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
         (int64_t)1 << 20, std::max<int64_t>(256, (int64_t)((n_target - acc.n) * 1.15) + 64));
     const int64_t n_cand = want * (ln + rn) / 2;  // even starts in [s, s + 2 n_cand)
     const int64_t end = s + 2 * n_cand;
-    nw.drop_before(s);
-    nw.ensure(end + tail);
-    const int64_t held = end + tail - s;
+    // The window holds normals [s, hi), at least [s, end + tail) and everything drawn so far:
+    // [s, prev_hi) comes from the previous window (prev_hi is always the stream's end), the
+    // rest is drawn now.
+    const int64_t hi = std::max<int64_t>(end + tail, dn.generated());
     double *d_nrm;
     uint8_t *d_info;
-    C_TRY(c.alloc(&d_nrm, (size_t)held));
+    C_TRY(c.alloc(&d_nrm, (size_t)(hi - s)));
     C_TRY(c.alloc(&d_info, (size_t)n_cand));
-    C_HIP(hipMemcpyAsync(d_nrm, nw.buf.data() + (s - nw.base), sizeof(double) * held,
-                         hipMemcpyHostToDevice, c.st));
+    if (s < prev_lo || (d_prev && prev_hi != dn.generated()) || s > dn.generated())
+      return set_error(CY3Q_ERR_STATE, "sampler window does not continue the normal stream");
+    if (d_prev && prev_hi > s)
+      C_HIP(hipMemcpyAsync(d_nrm, d_prev + (s - prev_lo), sizeof(double) * (prev_hi - s),
+                           hipMemcpyDeviceToDevice, c.st));
+    if (hi > dn.generated())
+      C_TRY(dn.generate(c, hi - dn.generated(), d_nrm + (dn.generated() - s)));
+    if (d_prev) c.release(d_prev);
+    d_prev = d_nrm;
+    prev_lo = s;
+    prev_hi = hi;
     launch_cicy_span(c.st, variety, d_nrm, 0, n_cand, d_info);
     C_HIP(hipGetLastError());
     std::vector<uint8_t> info((size_t)n_cand);
@@ -536,7 +665,7 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
     C_TRY(post_and_collect(c, variety, d_x, d_flag, nl, n_target, acc, &used));
     lines += used;
     s = t;
-    for (void *p : {(void *)d_nrm, (void *)d_info, (void *)d_start, (void *)d_cinfo, (void *)d_x,
+    for (void *p : {(void *)d_info, (void *)d_start, (void *)d_cinfo, (void *)d_x,
                     (void *)d_flag})
       c.release(p);
   }
@@ -551,7 +680,7 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
 int sample_lanes(Call &c, uint64_t seed, int64_t n_target, Accepted &acc, int64_t *n_lines) {
   const int64_t nl = std::max<int64_t>(4 * n_target, 64);
   const int64_t cap = std::min<int64_t>(nl, (int64_t)1 << 20);
-  NormalWindow nw(seed);
+  DeviceNormals dn(seed);
   double *d_nrm, *d_x;
   int32_t *d_flag;
   C_TRY(c.alloc(&d_nrm, (size_t)cap * 70));
@@ -561,10 +690,7 @@ int sample_lanes(Call &c, uint64_t seed, int64_t n_target, Accepted &acc, int64_
     // the remaining points plus slack (about 97% of lines give a point)
     const int64_t want = (int64_t)((n_target - acc.n) * 1.08) + 64;
     const int64_t m = std::min(std::min(cap, want), nl - l0);
-    nw.drop_before(70 * l0);
-    nw.ensure(70 * (l0 + m));
-    C_HIP(hipMemcpyAsync(d_nrm, nw.buf.data() + (70 * l0 - nw.base), sizeof(double) * 70 * m,
-                         hipMemcpyHostToDevice, c.st));
+    C_TRY(dn.generate(c, 70 * m, d_nrm));  // normals [70 l0, 70 (l0 + m))
     launch_cicy_lanes(c.st, d_nrm, m, d_x, d_flag);
     C_HIP(hipGetLastError());
     int64_t used = 0;
@@ -627,6 +753,20 @@ int cy3q_cicy_basis(int variety, const uint32_t *degrees, uint32_t *out, int64_t
   if (cap < (int64_t)basis.size()) return set_error(CY3Q_ERR_ARG, "cap below the basis size");
   for (size_t a = 0; a < basis.size(); a++)
     for (int k = 0; k < 8; k++) out[8 * a + k] = basis[a][k];
+  return CY3Q_OK;
+}
+
+int cy3q_cicy_normals(uint64_t seed, int64_t count, int device, double *out) {
+  if (!out || count < 0) return set_error(CY3Q_ERR_ARG, "cy3q_cicy_normals: bad arguments");
+  if (count == 0) return CY3Q_OK;
+  Call c;
+  C_TRY(c.open(device));
+  double *d_out;
+  C_TRY(c.alloc(&d_out, (size_t)count));
+  DeviceNormals dn(seed);
+  C_TRY(dn.generate(c, count, d_out));
+  C_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * count, hipMemcpyDeviceToHost, c.st));
+  C_HIP(hipStreamSynchronize(c.st));
   return CY3Q_OK;
 }
 

@@ -60,6 +60,18 @@ void launch_cicy_span(hipStream_t st, int variety, const double *nrm, int64_t s0
 void launch_cicy_chain(hipStream_t st, int variety, const double *nrm, const int64_t *start,
                        const uint8_t *info, int64_t n_lines, double *x, int32_t *flag);
 void launch_cicy_lanes(hipStream_t st, const double *nrm, int64_t n_lanes, double *x, int32_t *flag);
+// The ChaCha8 + ziggurat normal stream (k_cicy.hip): key words of the keystream, the fast-path
+// values and failed positions of a range of u64 positions, and the gather of normals from runs.
+struct ZigKey {
+  uint32_t k[8];
+};
+void upload_zig_x(hipStream_t st, const double *x257);
+void launch_zig_fast(hipStream_t st, const ZigKey &key, uint64_t pos0, int64_t n_pos, double *fast,
+                     int64_t *exc, unsigned long long *n_exc);
+void launch_zig_gather(hipStream_t st, const double *fast, const int64_t *run_k,
+                       const int64_t *run_p, const int64_t *run_len, int64_t n_runs,
+                       const int64_t *exc_k, const double *exc_v, int64_t n_exc, int64_t count,
+                       double *out);
 void launch_cicy_post(hipStream_t st, int variety, const double *x, const int32_t *flag, int64_t n,
                       double *coords, double *omega_sq, double *weight, int32_t *patch, int32_t *ok);
 void launch_cicy_frames(hipStream_t st, int variety, const double *coords, const int32_t *patch,

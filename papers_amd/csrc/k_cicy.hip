@@ -7,10 +7,10 @@
 // (route34/schoen_metric.rs:2202-2273); the section tables (ty_metric.rs:897-1035,
 // schoen_metric.rs:1214-1333).
 //
-// The random draws come from the host as a stream of standard normals (the reference's
-// ChaCha8 + ziggurat stream). A line that starts at normal s takes its origin from
-// normals [s, s + 16), its directions from the next 16 per hypersurface, and restart a's initial
-// t from the 2 nh normals after those. Two ways of consuming the stream are reproduced:
+// The random draws are a stream of standard normals (the reference's ChaCha8 + ziggurat
+// stream, drawn by k_zig_fast / k_zig_gather below). A line that starts at normal s takes its
+// origin from normals [s, s + 16), its directions from the next 16 per hypersurface, and
+// restart a's initial t from the 2 nh normals after those. Two ways of consuming the stream are reproduced:
 //   - restarts (the reference's sequential CPU samplers): a line runs up to 4 Newton restarts
 //     and the next line starts after the restarts it used. Its start therefore depends on every
 //     line before it. k_cicy_span evaluates a line at every even start in a window at once
@@ -21,6 +21,12 @@
 //     normal 70 l and runs one damped Newton attempt (k_cicy_lanes).
 // k_cicy_post then does the patch rescale, the residue Omega and the pull-back weight of each
 // converged line, k_cicy_frames the implicit-function tangent frame of each accepted point.
+//
+// The normal stream itself is drawn here too: k_zig_fast evaluates, at every position of the
+// u64 keystream, the ziggurat's first test (a normal that starts there and takes one word), and
+// lists the positions where it fails; the host takes those (the wedge and tail use exp and log,
+// whose last bits are the host library's), follows the stream through them, and k_zig_gather
+// writes the normals from the resulting runs.
 //
 // Bit-for-bit agreement with the CPU restatement: every kernel here is compiled with
 // -ffp-contract=off and keeps the reference's operation order (num-complex products, naive
@@ -856,6 +862,119 @@ void launch_cicy_sections(hipStream_t st, const double *coords, int64_t n, const
   if (n <= 0 || nb <= 0) return;
   hipLaunchKernelGGL(k_cicy_sections, dim3(grid(n * nb, 256)), dim3(256), 0, st, coords, n, exps, nb,
                      sv, sd);
+}
+
+
+// ---- The normal stream ---------------------------------------------------------------------
+// ChaCha8 (rand_chacha 0.9): key words 4-11, a 64-bit block counter in words 12-13, stream 0 in
+// words 14-15, 8 rounds; u64 number q of the stream is words 2q and 2q + 1 of the keystream
+// (low word first), i.e. words 2 (q % 8), 2 (q % 8) + 1 of block q / 8.
+__constant__ double c_zig_x[257];
+
+void upload_zig_x(hipStream_t st, const double *x257) {
+  hipMemcpyToSymbolAsync(HIP_SYMBOL(c_zig_x), x257, sizeof(double) * 257, 0,
+                         hipMemcpyHostToDevice, st);
+}
+
+__device__ inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+__device__ inline void chacha8_block(const ZigKey &key, uint64_t counter, uint32_t (&out)[16]) {
+  uint32_t s[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+#pragma unroll
+  for (int i = 0; i < 8; i++) s[4 + i] = key.k[i];
+  s[12] = (uint32_t)counter;
+  s[13] = (uint32_t)(counter >> 32);
+  s[14] = s[15] = 0;
+  uint32_t x[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) x[i] = s[i];
+#define ZQR(a, b, c, d)                  \
+  x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 16); \
+  x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 12); \
+  x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 8);  \
+  x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 7);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    ZQR(0, 4, 8, 12) ZQR(1, 5, 9, 13) ZQR(2, 6, 10, 14) ZQR(3, 7, 11, 15)
+    ZQR(0, 5, 10, 15) ZQR(1, 6, 11, 12) ZQR(2, 7, 8, 13) ZQR(3, 4, 9, 14)
+  }
+#undef ZQR
+#pragma unroll
+  for (int i = 0; i < 16; i++) out[i] = x[i] + s[i];
+}
+
+// One thread per keystream block: positions q in [pos0, pos0 + n_pos) of that block. fast[q -
+// pos0] = u x_i with u = 2 (bits >> 12) / 2^52 - 1 in [-1, 1) and i = bits & 0xff (the value if
+// |u x_i| < x_(i+1)); the other positions are appended to exc (in no particular order).
+__global__ void __launch_bounds__(256) k_zig_fast(ZigKey key, uint64_t pos0, int64_t n_pos,
+                                                  double *__restrict__ fast,
+                                                  int64_t *__restrict__ exc,
+                                                  unsigned long long *__restrict__ n_exc) {
+  const uint64_t b0 = pos0 / 8;
+  const uint64_t blk = b0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int64_t)(blk * 8) >= (int64_t)(pos0 + n_pos)) return;
+  uint32_t w[16];
+  chacha8_block(key, blk, w);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const uint64_t q = blk * 8 + j;
+    if (q < pos0 || q >= pos0 + (uint64_t)n_pos) continue;
+    const uint64_t bits = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+    const int i = (int)(bits & 0xff);
+    const double u = __longlong_as_double((long long)((bits >> 12) | (1024ull << 52))) - 3.0;
+    const double x = u * c_zig_x[i];
+    fast[q - pos0] = x;
+    if (!(fabs(x) < c_zig_x[i + 1])) exc[atomicAdd(n_exc, 1ull)] = (int64_t)q;
+  }
+}
+
+// out[k] for k < count: in run r (run_k[r] <= k < run_k[r] + run_len[r]) the fast value at
+// position run_p[r] + (k - run_k[r]) (relative to the fast array), else the host's value
+// exc_v[e] for exc_k[e] == k. Runs and exc_k are sorted by k.
+__global__ void __launch_bounds__(256) k_zig_gather(const double *__restrict__ fast,
+                                                    const int64_t *__restrict__ run_k,
+                                                    const int64_t *__restrict__ run_p,
+                                                    const int64_t *__restrict__ run_len,
+                                                    int64_t n_runs,
+                                                    const int64_t *__restrict__ exc_k,
+                                                    const double *__restrict__ exc_v,
+                                                    int64_t n_exc, int64_t count,
+                                                    double *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  int64_t lo = 0, hi = n_runs;  // last run with run_k <= k
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (run_k[mid] <= k) lo = mid; else hi = mid;
+  }
+  if (n_runs > 0 && run_k[lo] <= k && k < run_k[lo] + run_len[lo]) {
+    out[k] = fast[run_p[lo] + (k - run_k[lo])];
+    return;
+  }
+  lo = 0;
+  hi = n_exc;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (exc_k[mid] <= k) lo = mid; else hi = mid;
+  }
+  out[k] = (n_exc > 0 && exc_k[lo] == k) ? exc_v[lo] : __builtin_nan("");
+}
+
+void launch_zig_fast(hipStream_t st, const ZigKey &key, uint64_t pos0, int64_t n_pos, double *fast,
+                     int64_t *exc, unsigned long long *n_exc) {
+  if (n_pos <= 0) return;
+  const int64_t nblk = (int64_t)((pos0 + n_pos + 7) / 8 - pos0 / 8);
+  hipLaunchKernelGGL(k_zig_fast, dim3(grid(nblk, 256)), dim3(256), 0, st, key, pos0, n_pos, fast,
+                     exc, n_exc);
+}
+
+void launch_zig_gather(hipStream_t st, const double *fast, const int64_t *run_k,
+                       const int64_t *run_p, const int64_t *run_len, int64_t n_runs,
+                       const int64_t *exc_k, const double *exc_v, int64_t n_exc, int64_t count,
+                       double *out) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_zig_gather, dim3(grid(count, 256)), dim3(256), 0, st, fast, run_k, run_p,
+                     run_len, n_runs, exc_k, exc_v, n_exc, count, out);
 }
 
 }  // namespace cy3q

@@ -51,6 +51,7 @@ def _lib():
         L.cy3q_cicy_sample.argtypes = [ip, ip, u64, i64, ip, vp, vp, vp, vp, C.POINTER(i64),
                                        C.POINTER(i64)]
         L.cy3q_cicy_basis.argtypes = [ip, vp, vp, i64, C.POINTER(i64)]
+        L.cy3q_cicy_normals.argtypes = [u64, i64, ip, vp]
         L.cy3q_cicy_frames.argtypes = [ip, i64, vp, vp, ip, vp]
         L.cy3q_cicy_sections.argtypes = [i64, vp, i64, vp, ip, vp, vp]
         L.cy3q_cicy_upload_contracts.argtypes = [ip, i64, vp, vp, i64, vp, vp, vp, vp, vp]
@@ -80,6 +81,14 @@ def sample_points(variety: int, seed: int, n_target: int, mode: str = "restarts"
     k = n.value
     return dict(coords=coords[:k].copy(), omega_sq=om[:k].copy(), weight=w[:k].copy(),
                 patch=patch[:k].copy(), n_lines=int(nl.value))
+
+
+def normal_stream(seed: int, count: int, device: int = 0) -> np.ndarray:
+    """Normals 0 .. count - 1 of the samplers' stream for `seed` (rand_distr StandardNormal
+    over ChaCha8Rng::seed_from_u64), drawn on the GPU as the samplers draw them."""
+    out = np.zeros(max(int(count), 0))
+    _check(_lib().cy3q_cicy_normals(int(seed) & (2**64 - 1), int(count), int(device), _p(out)))
+    return out
 
 
 def invariant_reduced_basis(variety: int, degrees) -> np.ndarray:

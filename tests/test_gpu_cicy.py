@@ -34,6 +34,17 @@ def _close(a, b, tol=1e-12):
     return np.abs(a - b).max() <= tol * max(1.0, np.abs(b).max())
 
 
+@pytest.mark.parametrize("seed,count", [(42, 1), (42, 70_000), (7, 2_000_000),
+                                        (2**40 + 3, 500_000)])
+def test_device_normal_stream_matches_oracle(cy3q, co, seed, count):
+    """The samplers' stream drawn on the device equals the oracle's sequential ziggurat bit for
+    bit; 2M normals hold a few hundred tail and several thousand wedge draws."""
+    from papers_amd import cy3_metric as CM
+    print(f"normal stream seed={seed} count={count}", flush=True)
+    got = CM.normal_stream(seed, count)
+    assert np.array_equal(got, co.std_normals(seed, count))
+
+
 @pytest.mark.parametrize("variety,mode,seed,n", [(0, "lanes", 42, 200), (0, "lanes", 9, 3000),
                                                  (0, "restarts", 42, 200), (0, "restarts", 5, 3000),
                                                  (1, "restarts", 42, 200), (1, "restarts", 11, 4000)])
