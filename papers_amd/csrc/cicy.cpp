@@ -105,9 +105,10 @@ class NormalStream {
   // test f_(i+1) + (f_i - f_(i+1)) v < exp(-x^2 / 2) with v from the next word. *words: the u64s
   // it takes; a rejected wedge (*accepted false) takes 2, and the stream's next attempt starts
   // after them.
-  void slow_at(uint64_t e, bool *accepted, double *value, int64_t *words) {
+  // bits, next: the words at e and e + 1 (only a tail reads further words).
+  void slow_at(uint64_t e, uint64_t bits, uint64_t next, bool *accepted, double *value,
+               int64_t *words) {
     const ZigTables &z = zig_tables();
-    const uint64_t bits = u64_at(e);
     const int i = (int)(bits & 0xff);
     const double u = with_exponent(bits >> 12, 1) - 3.0;
     const double x = u * z.x[i];
@@ -116,7 +117,7 @@ class NormalStream {
       double xt = 1.0, yt = 0.0;
       uint64_t q = e + 1;
       while (-2.0 * yt < xt * xt) {
-        const double a = with_exponent(u64_at(q) >> 12, 0) - (1.0 - 0x1p-53);
+        const double a = with_exponent((q == e + 1 ? next : u64_at(q)) >> 12, 0) - (1.0 - 0x1p-53);
         const double b = with_exponent(u64_at(q + 1) >> 12, 0) - (1.0 - 0x1p-53);
         q += 2;
         xt = std::log(a) / R;
@@ -127,7 +128,7 @@ class NormalStream {
       *words = (int64_t)(q - e);
       return;
     }
-    const double un = (double)(u64_at(e + 1) >> 11) * 0x1p-53;
+    const double un = (double)(next >> 11) * 0x1p-53;
     *accepted = z.f[i + 1] + (z.f[i] - z.f[i + 1]) * un < std::exp(-x * x / 2.0);
     *value = x;
     *words = 2;
@@ -383,38 +384,97 @@ int cached_lms(int variety, const std::vector<Exp> **out) {
 
 // ---- Device buffers of one call ---------------------------------------------------------------
 
+// Device blocks of the set-up calls, cached per device across calls: a sampler runs once per
+// solve, and allocating its buffers afresh after a solve has freed its own costs a few
+// milliseconds per call. Blocks come in power-of-two size classes (64 KiB up); at most 1 GiB
+// per device is kept, until the process ends.
+constexpr int kCacheDevices = 64;
+constexpr size_t kCacheLimit = (size_t)1 << 30;
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<size_t, void *> free_blocks[kCacheDevices];
+  size_t cached[kCacheDevices] = {};
+};
+BlockCache &block_cache() {
+  static BlockCache c;
+  return c;
+}
+size_t size_class(size_t bytes) {
+  size_t c = (size_t)1 << 16;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
 struct Call {
   hipStream_t st = nullptr;
-  std::vector<void *> mem;
+  int device = 0;
+  std::vector<std::pair<void *, size_t>> mem;  // live blocks and their size classes
   ~Call() {
     if (st) hipStreamSynchronize(st);
-    for (void *p : mem) hipFree(p);
+    for (auto &b : mem) give_back(b.first, b.second);
     if (st) hipStreamDestroy(st);
   }
-  int open(int device) {
-    C_HIP(hipSetDevice(device));
+  int open(int dev) {
+    if (dev < 0 || dev >= kCacheDevices) return set_error(CY3Q_ERR_ARG, "bad device index");
+    device = dev;
+    C_HIP(hipSetDevice(dev));
     C_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     return CY3Q_OK;
   }
   template <class T>
   int alloc(T **p, size_t count) {
+    const size_t cls = size_class(std::max<size_t>(count, 1) * sizeof(T));
     void *q = nullptr;
-    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess)
-      return set_error(CY3Q_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    mem.push_back(q);
+    {
+      BlockCache &bc = block_cache();
+      std::lock_guard<std::mutex> lock(bc.mu);
+      auto it = bc.free_blocks[device].find(cls);
+      if (it != bc.free_blocks[device].end()) {
+        q = it->second;
+        bc.free_blocks[device].erase(it);
+        bc.cached[device] -= cls;
+      }
+    }
+    if (!q) {
+      hipError_t e = hipMalloc(&q, cls);
+      if (e != hipSuccess)
+        return set_error(CY3Q_ERR_HIP,
+                         "hipMalloc(" + std::to_string(cls) + "): " + hipGetErrorString(e));
+    }
+    mem.emplace_back(q, cls);
     *p = (T *)q;
     return CY3Q_OK;
   }
   void release(void *p) {
-    auto it = std::find(mem.begin(), mem.end(), p);
+    auto it = std::find_if(mem.begin(), mem.end(),
+                           [p](const std::pair<void *, size_t> &b) { return b.first == p; });
     if (it != mem.end()) {
       hipStreamSynchronize(st);
-      hipFree(p);
+      give_back(it->first, it->second);
       mem.erase(it);
     }
   }
+
+ private:
+  void give_back(void *p, size_t cls) {
+    BlockCache &bc = block_cache();
+    std::lock_guard<std::mutex> lock(bc.mu);
+    if (bc.cached[device] + cls <= kCacheLimit) {
+      bc.free_blocks[device].emplace(cls, p);
+      bc.cached[device] += cls;
+    } else {
+      hipFree(p);
+    }
+  }
+};
+
+constexpr int kLineNormals[2] = {64, 48};  // origin + one direction per hypersurface, 16 each
+constexpr int kRestartNormals[2] = {6, 4};  // one complex t per hypersurface
+
+struct Accepted {
+  std::vector<double> coords, om, w;
+  std::vector<int32_t> patch;
+  int64_t n = 0;
 };
 
 // The normal stream drawn on the device, in order: generate(count, d_out) writes the stream's
@@ -440,19 +500,35 @@ class DeviceNormals {
     while (produced < count) {
       const int64_t need = count - produced;
       const int64_t n_pos = need + need / 16 + 256;
-      C_TRY(grow(c, n_pos));
-      C_HIP(hipMemsetAsync(d_nexc_, 0, sizeof(unsigned long long), c.st));
-      launch_zig_fast(c.st, key_, pos_, n_pos, d_fast_, d_exc_, d_nexc_);
-      C_HIP(hipGetLastError());
+      // The failed positions are about 1.2% of a pass; a pass whose list overflows its
+      // capacity runs again with room for every position.
+      int64_t exc_cap = n_pos / 16 + 1024;
       unsigned long long n_exc = 0;
-      C_HIP(hipMemcpyAsync(&n_exc, d_nexc_, sizeof n_exc, hipMemcpyDeviceToHost, c.st));
-      C_HIP(hipStreamSynchronize(c.st));
+      for (;;) {
+        C_TRY(grow(c, n_pos, exc_cap));
+        C_HIP(hipMemsetAsync(d_nexc_, 0, sizeof(unsigned long long), c.st));
+        launch_zig_fast(c.st, key_, pos_, n_pos, d_fast_, d_words_, d_exc_, exc_cap_, d_nexc_);
+        C_HIP(hipGetLastError());
+        C_HIP(hipMemcpyAsync(&n_exc, d_nexc_, sizeof n_exc, hipMemcpyDeviceToHost, c.st));
+        C_HIP(hipStreamSynchronize(c.st));
+        if ((int64_t)n_exc <= exc_cap_) break;
+        exc_cap = n_pos;
+      }
+      // The listed positions with their two words, in stream order.
       exc_.resize((size_t)n_exc);
-      if (n_exc)
+      exw_.resize(2 * (size_t)n_exc);
+      if (n_exc) {
+        launch_zig_exc_words(c.st, d_words_, d_exc_, (int64_t)n_exc, pos_, d_exw_);
+        C_HIP(hipGetLastError());
         C_HIP(hipMemcpyAsync(exc_.data(), d_exc_, sizeof(int64_t) * n_exc, hipMemcpyDeviceToHost,
                              c.st));
+        C_HIP(hipMemcpyAsync(exw_.data(), d_exw_, sizeof(uint64_t) * 2 * n_exc,
+                             hipMemcpyDeviceToHost, c.st));
+      }
       C_HIP(hipStreamSynchronize(c.st));
-      std::sort(exc_.begin(), exc_.end());
+      order_.resize((size_t)n_exc);
+      std::iota(order_.begin(), order_.end(), (size_t)0);
+      std::sort(order_.begin(), order_.end(), [&](size_t a, size_t b) { return exc_[a] < exc_[b]; });
       // Follow the stream through [pos_, pos_ + n_pos).
       run_k_.clear();
       run_p_.clear();
@@ -464,8 +540,8 @@ class DeviceNormals {
       uint64_t p = pos_;
       size_t ei = 0;
       while (produced < count && p < end) {
-        while (ei < exc_.size() && (uint64_t)exc_[ei] < p) ei++;
-        const uint64_t e = ei < exc_.size() ? (uint64_t)exc_[ei] : end;
+        while (ei < order_.size() && (uint64_t)exc_[order_[ei]] < p) ei++;
+        const uint64_t e = ei < order_.size() ? (uint64_t)exc_[order_[ei]] : end;
         const int64_t r = std::min<int64_t>((int64_t)(e - p), count - produced);
         if (r > 0) {
           run_k_.push_back(produced - base);
@@ -478,7 +554,8 @@ class DeviceNormals {
         bool accepted;
         double v;
         int64_t words;
-        host_.slow_at(e, &accepted, &v, &words);
+        const size_t oe = order_[ei];
+        host_.slow_at(e, exw_[2 * oe], exw_[2 * oe + 1], &accepted, &v, &words);
         if (accepted) {
           ek_.push_back(produced - base);
           ev_.push_back(v);
@@ -516,15 +593,22 @@ class DeviceNormals {
   }
 
  private:
-  int grow(Call &c, int64_t n_pos) {
-    if (n_pos <= cap_) return CY3Q_OK;
-    if (d_fast_) c.release(d_fast_);
-    if (d_exc_) c.release(d_exc_);
-    if (d_nexc_) c.release(d_nexc_);
-    C_TRY(c.alloc(&d_fast_, (size_t)n_pos));
-    C_TRY(c.alloc(&d_exc_, (size_t)n_pos));
-    C_TRY(c.alloc(&d_nexc_, 1));
-    cap_ = n_pos;
+  int grow(Call &c, int64_t n_pos, int64_t exc_cap) {
+    if (n_pos > cap_) {
+      if (d_fast_) c.release(d_fast_);
+      if (d_words_) c.release(d_words_);
+      C_TRY(c.alloc(&d_fast_, (size_t)n_pos));
+      C_TRY(c.alloc(&d_words_, (size_t)n_pos + 1));
+      cap_ = n_pos;
+    }
+    if (exc_cap > exc_cap_) {
+      if (d_exc_) c.release(d_exc_);
+      if (d_exw_) c.release(d_exw_);
+      C_TRY(c.alloc(&d_exc_, (size_t)exc_cap));
+      C_TRY(c.alloc(&d_exw_, 2 * (size_t)exc_cap));
+      exc_cap_ = exc_cap;
+    }
+    if (!d_nexc_) C_TRY(c.alloc(&d_nexc_, 1));
     return CY3Q_OK;
   }
 
@@ -533,22 +617,17 @@ class DeviceNormals {
   uint64_t pos_ = 0;        // u64 position where the next normal starts
   int64_t generated_ = 0;   // normals written so far
   bool tables_ = false;
-  int64_t cap_ = 0;
+  int64_t cap_ = 0, exc_cap_ = 0;
   double *d_fast_ = nullptr;
+  uint64_t *d_words_ = nullptr, *d_exw_ = nullptr;
   int64_t *d_exc_ = nullptr;
   unsigned long long *d_nexc_ = nullptr;
   std::vector<int64_t> exc_, run_k_, run_p_, run_len_, ek_;
+  std::vector<uint64_t> exw_;
+  std::vector<size_t> order_;
   std::vector<double> ev_;
 };
 
-constexpr int kLineNormals[2] = {64, 48};  // origin + one direction per hypersurface, 16 each
-constexpr int kRestartNormals[2] = {6, 4};  // one complex t per hypersurface
-
-struct Accepted {
-  std::vector<double> coords, om, w;
-  std::vector<int32_t> patch;
-  int64_t n = 0;
-};
 
 // Post-process `n` solved lines already on the device (x, flag) and append the accepted ones
 // (in line order) until `n_target`. Returns the number of lines consumed (up to the line that
@@ -679,7 +758,9 @@ int sample_restarts(Call &c, int variety, uint64_t seed, int64_t n_target, Accep
 // needed are never made (they only advance a generator the reference then drops).
 int sample_lanes(Call &c, uint64_t seed, int64_t n_target, Accepted &acc, int64_t *n_lines) {
   const int64_t nl = std::max<int64_t>(4 * n_target, 64);
-  const int64_t cap = std::min<int64_t>(nl, (int64_t)1 << 20);
+  // Chunks shrink as points accumulate, so the first one sizes the buffers.
+  const int64_t cap = std::min<int64_t>(
+      std::min<int64_t>(nl, (int64_t)1 << 20), (int64_t)(n_target * 1.08) + 64);
   DeviceNormals dn(seed);
   double *d_nrm, *d_x;
   int32_t *d_flag;

@@ -67,7 +67,10 @@ struct ZigKey {
 };
 void upload_zig_x(hipStream_t st, const double *x257);
 void launch_zig_fast(hipStream_t st, const ZigKey &key, uint64_t pos0, int64_t n_pos, double *fast,
-                     int64_t *exc, unsigned long long *n_exc);
+                     uint64_t *words, int64_t *exc, int64_t exc_cap,
+                     unsigned long long *n_exc);
+void launch_zig_exc_words(hipStream_t st, const uint64_t *words, const int64_t *exc, int64_t n,
+                          uint64_t pos0, uint64_t *out);
 void launch_zig_gather(hipStream_t st, const double *fast, const int64_t *run_k,
                        const int64_t *run_p, const int64_t *run_len, int64_t n_runs,
                        const int64_t *exc_k, const double *exc_v, int64_t n_exc, int64_t count,

@@ -903,28 +903,36 @@ __device__ inline void chacha8_block(const ZigKey &key, uint64_t counter, uint32
   for (int i = 0; i < 16; i++) out[i] = x[i] + s[i];
 }
 
-// One thread per keystream block: positions q in [pos0, pos0 + n_pos) of that block. fast[q -
-// pos0] = u x_i with u = 2 (bits >> 12) / 2^52 - 1 in [-1, 1) and i = bits & 0xff (the value if
-// |u x_i| < x_(i+1)); the other positions are appended to exc (in no particular order).
+// One thread per keystream block: positions q in [pos0, pos0 + n_pos] of that block; words[q -
+// pos0] is the u64 there (one past the range too, for k_zig_exc_words), and for q < pos0 + n_pos
+// fast[q - pos0] = u x_i with u = 2 (bits >> 12) / 2^52 - 1 in [-1, 1) and i = bits & 0xff (the
+// value if |u x_i| < x_(i+1)); the other positions are appended to exc (in no particular order;
+// n_exc counts them all, exc holds the first exc_cap).
 __global__ void __launch_bounds__(256) k_zig_fast(ZigKey key, uint64_t pos0, int64_t n_pos,
                                                   double *__restrict__ fast,
-                                                  int64_t *__restrict__ exc,
+                                                  uint64_t *__restrict__ words,
+                                                  int64_t *__restrict__ exc, int64_t exc_cap,
                                                   unsigned long long *__restrict__ n_exc) {
   const uint64_t b0 = pos0 / 8;
   const uint64_t blk = b0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if ((int64_t)(blk * 8) >= (int64_t)(pos0 + n_pos)) return;
+  if ((int64_t)(blk * 8) > (int64_t)(pos0 + n_pos)) return;
   uint32_t w[16];
   chacha8_block(key, blk, w);
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const uint64_t q = blk * 8 + j;
-    if (q < pos0 || q >= pos0 + (uint64_t)n_pos) continue;
+    if (q < pos0 || q > pos0 + (uint64_t)n_pos) continue;
     const uint64_t bits = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+    words[q - pos0] = bits;
+    if (q == pos0 + (uint64_t)n_pos) continue;  // the word after the range, for exc_words
     const int i = (int)(bits & 0xff);
     const double u = __longlong_as_double((long long)((bits >> 12) | (1024ull << 52))) - 3.0;
     const double x = u * c_zig_x[i];
     fast[q - pos0] = x;
-    if (!(fabs(x) < c_zig_x[i + 1])) exc[atomicAdd(n_exc, 1ull)] = (int64_t)q;
+    if (!(fabs(x) < c_zig_x[i + 1])) {
+      const unsigned long long slot = atomicAdd(n_exc, 1ull);
+      if (slot < (unsigned long long)exc_cap) exc[slot] = (int64_t)q;  // else counted only
+    }
   }
 }
 
@@ -960,12 +968,32 @@ __global__ void __launch_bounds__(256) k_zig_gather(const double *__restrict__ f
   out[k] = (n_exc > 0 && exc_k[lo] == k) ? exc_v[lo] : __builtin_nan("");
 }
 
+// The two words from each listed position on (the wedge's test reads both): out[2 e] and
+// out[2 e + 1] for exc[e] (words holds positions [pos0, pos0 + n_pos]).
+__global__ void __launch_bounds__(256) k_zig_exc_words(const uint64_t *__restrict__ words,
+                                                       const int64_t *__restrict__ exc, int64_t n,
+                                                       uint64_t pos0, uint64_t *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t r = exc[e] - (int64_t)pos0;
+  out[2 * e] = words[r];
+  out[2 * e + 1] = words[r + 1];
+}
+
 void launch_zig_fast(hipStream_t st, const ZigKey &key, uint64_t pos0, int64_t n_pos, double *fast,
-                     int64_t *exc, unsigned long long *n_exc) {
+                     uint64_t *words, int64_t *exc, int64_t exc_cap,
+                     unsigned long long *n_exc) {
   if (n_pos <= 0) return;
-  const int64_t nblk = (int64_t)((pos0 + n_pos + 7) / 8 - pos0 / 8);
+  const int64_t nblk = (int64_t)((pos0 + n_pos + 8) / 8 - pos0 / 8);  // positions through pos0 + n_pos
   hipLaunchKernelGGL(k_zig_fast, dim3(grid(nblk, 256)), dim3(256), 0, st, key, pos0, n_pos, fast,
-                     exc, n_exc);
+                     words, exc, exc_cap, n_exc);
+}
+
+void launch_zig_exc_words(hipStream_t st, const uint64_t *words, const int64_t *exc, int64_t n,
+                          uint64_t pos0, uint64_t *out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zig_exc_words, dim3(grid(n, 256)), dim3(256), 0, st, words, exc, n, pos0,
+                     out);
 }
 
 void launch_zig_gather(hipStream_t st, const double *fast, const int64_t *run_k,
