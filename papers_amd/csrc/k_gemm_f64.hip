@@ -266,53 +266,84 @@ __global__ void __launch_bounds__(512) k_dpass_f64(const double *__restrict__ x,
   issue_a(0);
   issue_b(0);
   if (n_stage > 1) issue_a(1);
-  for (int st = 0; st < n_stage; st++) {
-    asm volatile("" ::: "memory");
-    if (st + 1 < n_stage)
-      __builtin_amdgcn_s_waitcnt(0x72);  // vmcnt(2): X(st) and B(st) have landed
-    else
-      __builtin_amdgcn_s_waitcnt(0x70);  // vmcnt(0)
-    __syncthreads();                     // ... for every wave; B slot (st + 1) & 1 and X slot
-    asm volatile("" ::: "memory");       // (st + 2) % 3 are free
-    if (st + 1 < n_stage) issue_b(st + 1);
-    if (st + 2 < n_stage) issue_a(st + 2);
-    const double *sa = reinterpret_cast<const double *>(aring + (st % DG_ASLOTS) * DG_ABYTES);
-    const double *sb = reinterpret_cast<const double *>(bring + (st & 1) * bbytes);
-    // The active column tiles of this wave (cg >= st, cg < cg_hi) are the slots v with
-    // v_lo <= v < v_hi. All operands of a 4-deep k step are read before its MFMAs, and the next
-    // step's operands are read before the current step's MFMAs are issued.
-    const int v_lo = st > cg_lo + cm ? (st - cg_lo - cm + DG_OWN - 1) / DG_OWN : 0;
-    const int v_hi = ntile_r - cm > 0 ? (ntile_r - cm + DG_OWN - 1) / DG_OWN : 0;
-    auto load = [&](int kk, double (&a)[DG_PT], double (&b)[DG_NT]) {
-      const int kl = 4 * kk + (lane >> 4);
-#pragma unroll
-      for (int t = 0; t < DG_PT; t++)
-        a[t] = sa[(16 * (DG_PT * pg + t) + (lane & 15)) * DG_ALD + kl];
-#pragma unroll
-      for (int v = 0; v < DG_NT; v++) {
-        const int c = cm + DG_OWN * v < ntile_r ? cm + DG_OWN * v : ntile_r - 1;  // unused if idle
-        b[v] = sb[kl * bld + 16 * c + (lane & 15)];
-      }
-    };
-    double a0[DG_PT], b0[DG_NT], a1[DG_PT], b1[DG_NT];
-    load(0, a0, b0);
-#pragma unroll
-    for (int kk = 0; kk < DG_KS / 4; kk++) {
-      double(&a)[DG_PT] = (kk & 1) ? a1 : a0;
-      double(&b)[DG_NT] = (kk & 1) ? b1 : b0;
-      if (kk + 1 < DG_KS / 4) {
-        if (kk & 1)
-          load(kk + 1, a0, b0);
+  // Stages in epochs of this wave's first active column slot: at stage st the active slots are
+  // v_lo(st) <= v < v_hi with v_lo(st) = ceil((st - cg_lo - cm) / 4), so epoch E holds the stages
+  // st <= cg_lo + cm + 4 E not in an earlier epoch (and epoch 7 the rest, with no slot). E, and v_hi (this wave's number of column
+  // slots, dispatched once), are compile-time constants in each epoch's body, so the body has no
+  // tile guards and reads no B for finished slots.
+  const int v_hi = ntile_r - cm > 0 ? (ntile_r - cm + DG_OWN - 1) / DG_OWN : 0;
+  int st = 0;
+  auto run_all = [&](auto vhc) {
+    constexpr int VHI = decltype(vhc)::value;
+    auto run_epoch = [&](auto ec) {
+      constexpr int E = decltype(ec)::value;
+      // Epoch DG_NT: the stages past every slot of this wave (their B copies of the wave's
+      // tiles are skipped, so no slot may read them).
+      const int st_end = E == DG_NT ? n_stage
+                                    : (cg_lo + cm + 4 * E + 1 < n_stage ? cg_lo + cm + 4 * E + 1
+                                                                        : n_stage);
+      for (; st < st_end; st++) {
+        asm volatile("" ::: "memory");
+        if (st + 1 < n_stage)
+          __builtin_amdgcn_s_waitcnt(0x72);  // vmcnt(2): X(st) and B(st) have landed
         else
-          load(kk + 1, a1, b1);
-      }
-#pragma unroll
-      for (int v = 0; v < DG_NT; v++)
-        if (v >= v_lo && v < v_hi)
+          __builtin_amdgcn_s_waitcnt(0x70);  // vmcnt(0)
+        __syncthreads();                     // ... for every wave; B slot (st + 1) & 1 and X
+        asm volatile("" ::: "memory");       // slot (st + 2) % 3 are free
+        if (st + 1 < n_stage) issue_b(st + 1);
+        if (st + 2 < n_stage) issue_a(st + 2);
+        if (E >= VHI) continue;  // no active slot: only the copies and the barrier
+        const double *sa = reinterpret_cast<const double *>(aring + (st % DG_ASLOTS) * DG_ABYTES);
+        const double *sb = reinterpret_cast<const double *>(bring + (st & 1) * bbytes);
+        // All operands of a 4-deep k step are read before its MFMAs, and the next step's
+        // operands are read before the current step's MFMAs are issued.
+        auto load = [&](int kk, double (&a)[DG_PT], double (&b)[DG_NT]) {
+          const int kl = 4 * kk + (lane >> 4);
 #pragma unroll
           for (int t = 0; t < DG_PT; t++)
-            acc[t][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[v], acc[t][v], 0, 0, 0);
-    }
+            a[t] = sa[(16 * (DG_PT * pg + t) + (lane & 15)) * DG_ALD + kl];
+#pragma unroll
+          for (int v = E; v < VHI; v++) b[v] = sb[kl * bld + 16 * (cm + DG_OWN * v) + (lane & 15)];
+        };
+        double a0[DG_PT], b0[DG_NT], a1[DG_PT], b1[DG_NT];
+        load(0, a0, b0);
+#pragma unroll
+        for (int kk = 0; kk < DG_KS / 4; kk++) {
+          double(&a)[DG_PT] = (kk & 1) ? a1 : a0;
+          double(&b)[DG_NT] = (kk & 1) ? b1 : b0;
+          if (kk + 1 < DG_KS / 4) {
+            if (kk & 1)
+              load(kk + 1, a0, b0);
+            else
+              load(kk + 1, a1, b1);
+          }
+#pragma unroll
+          for (int v = E; v < VHI; v++)
+#pragma unroll
+            for (int t = 0; t < DG_PT; t++)
+              acc[t][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[v], acc[t][v], 0, 0, 0);
+        }
+      }
+    };
+    run_epoch(std::integral_constant<int, 0>{});
+    run_epoch(std::integral_constant<int, 1>{});
+    run_epoch(std::integral_constant<int, 2>{});
+    run_epoch(std::integral_constant<int, 3>{});
+    run_epoch(std::integral_constant<int, 4>{});
+    run_epoch(std::integral_constant<int, 5>{});
+    run_epoch(std::integral_constant<int, 6>{});
+    run_epoch(std::integral_constant<int, 7>{});
+  };
+  static_assert(DG_NT == 7, "epochs 0 .. 7, v_hi 0 .. 7");
+  switch (v_hi) {
+    case 0: run_all(std::integral_constant<int, 0>{}); break;
+    case 1: run_all(std::integral_constant<int, 1>{}); break;
+    case 2: run_all(std::integral_constant<int, 2>{}); break;
+    case 3: run_all(std::integral_constant<int, 3>{}); break;
+    case 4: run_all(std::integral_constant<int, 4>{}); break;
+    case 5: run_all(std::integral_constant<int, 5>{}); break;
+    case 6: run_all(std::integral_constant<int, 6>{}); break;
+    default: run_all(std::integral_constant<int, 7>{}); break;
   }
   // Row sums of squares: element e of lane l is point 16 (PT pg + t) + 4 e + l / 16.
   __syncthreads();  // the LDS slots are reused for the owner partials
