@@ -106,6 +106,22 @@ def test_t_operator_k6_quotient_matches_oracle(cy3q, oracle):
     assert abs(r - r_ref) <= 1e-9 * r_ref
 
 
+def test_k6_quotient_fp64_steps_match_oracle(cy3q, oracle):
+    """Three FP64 steps at M = 416 from the FS-Gram h: after the first step W is a general lower
+    triangular factor, so every column tile's finished stages (whose B columns the FP64 D pass
+    no longer copies) would show up here if a tile read them."""
+    print("k=6 quotient FP64 steps, 4001 points", flush=True)
+    P = _oracle_problem(oracle, 6, 4001, 21, quotient=True)
+    s = cy3q.QuinticSolver.from_points(6, P.pts, quotient=True)
+    s.orthonormalise_basis_fs_gram()
+    h = P.h
+    for it in range(3):
+        h, r_ref = oracle.donaldson_step(P.sv, P.w, h)
+        r = s.donaldson_step()
+        assert _rel(s.h_block, h) < 1e-11, it
+        assert abs(r - r_ref) <= 1e-9 * max(r_ref, 1e-300) + 1e-14, it
+
+
 def test_sigma_matches_oracle(cy3q, oracle):
     P = _oracle_problem(oracle, 3, 20000, 5)
     s = cy3q.QuinticSolver.from_points(3, P.pts)
